@@ -1,0 +1,63 @@
+/*
+ * peec_mi355x.h — entry points of libpeec_mi355x.so beyond the reference ABI of peec.h.
+ *
+ * None of these has a counterpart in OpenPEEC's peec.h; they expose the device-resident sweep
+ * (so a caller can time or batch frequency points without the host-array round trips of the
+ * fine-grained fill API) and a few conveniences for bindings that cannot pass a FILE*.
+ */
+#ifndef PEEC_MI355X_EXT_H
+#define PEEC_MI355X_EXT_H
+
+#include "peec.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* solver forms of the device sweep */
+#define PEEC_FORM_PLAIN 0   /* FP64 fill, FP64 LU (MFMA FP64 trailing update)                  */
+#define PEEC_FORM_MIXED 1   /* FP64 fill, LU with FP32 MFMA trailing update + FP64 refinement  */
+
+/* per-stage timings of device sweeps (accumulated over points; milliseconds) */
+typedef struct {
+	int    form;
+	double fill_ms, pot_ms, cinv_ms, asm_ms, lu_ms, solve_ms, total_ms;
+	double lu_gemm_ms;      /* trailing-update GEMM kernel time inside lu_ms */
+	double lu_gemm_flop;    /* algorithmic flops of those GEMMs (8 M N K each) */
+	int    lu_gemm_launches;
+	double fill_flop;       /* nominal fill flops (reference op count)        */
+	int    ising;           /* singular row of the last failure, or -1        */
+	int    refine_its;      /* refinement sweeps (mixed form)                 */
+	double berr;            /* worst backward error of the solutions (mixed)  */
+} peec_stats_t;
+
+/* Library identification: "libpeec_mi355x <version> gfx950". */
+const char *peec_mi355x_version(void);
+
+/* 1 when a HIP device is present (the library has no CPU compute path). */
+int peec_device_available(void);
+
+/* Deck from a file path (input_data on fopen(path)); returns input_data's code. */
+int peec_load_deck(const char *path, peec_t *p);
+
+/* Deck from a string (same grammar as a .peec file). */
+int peec_load_deck_string(const char *text, peec_t *p);
+
+/* Geometry + capacitive cells + MNA numbering, without any fill (the static fills of the
+ * reference's main are only needed for non-retarded sweeps).  fp_log may be NULL. */
+int peec_setup(peec_t *p, FILE *fp_log);
+
+/* Device-resident sweep of nfreq points starting at ifreq0 (freq_at indices).  zmat receives
+ * nfreq * nport * nport entries ([f][i][j]).  st may be NULL.  Returns 0, 1 (singular P),
+ * 2 (singular MNA matrix) or another non-zero code. */
+int peec_sweep(peec_t *p, int ifreq0, int nfreq, int form, d_complex_t *zmat, peec_stats_t *st);
+
+/* Sizes for bindings */
+size_t peec_sizeof_peec(void);
+size_t peec_sizeof_seg(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

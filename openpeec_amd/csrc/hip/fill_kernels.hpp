@@ -1,0 +1,36 @@
+// fill_kernels.hpp — launch interface of the dense fills (fill.hip) for the runtime layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "../peec_internal.h"
+
+namespace pe {
+
+struct polys {            // polygon pools of one packed cell set
+	const dedge_t *edge;
+	const dpnorm_t *pn;
+};
+
+// Arguments of one fill launch over an n x n symmetric matrix cut into nt x nt tiles of 16.
+struct fill_args {
+	const dcell_t *S;          // cells (rows and columns)
+	polys pS;
+	const dcell_t *SM;         // ground-plane images of S (nullptr when gp == 0)
+	polys pSM;
+	int n, nt;
+	long tile0;                // first tile of this launch (launches are chunked)
+	double kw;                 // wavenumber 2 pi f / c0 (0: static)
+	double coef;               // mu0 / 4 pi for Lp
+	int gp;                    // ground plane present
+	void *out;                 // n x n output, row-major, leading dimension ld
+	long ld;
+	const double *carea;       // fused P: per-cell length/area sums, else nullptr
+	double k4pe;               // 4 pi eps0 (fused P)
+};
+
+}  // namespace pe
+
+void fill_launch(hipStream_t s, int which, int km, const pe::fill_args &a);   // which: 0 Lp, 1 G/P
+void fill_one(hipStream_t s, int which, int km, const pe::fill_args &a, int i, int j, const int *csoff,
+              const int *csidx, double2 *res);
+void aggregate_launch(hipStream_t s, const double2 *G, long ldg, const int *csoff, const int *csidx,
+                      const double *carea, int nc, double k4pe, double2 *P, long ldp);

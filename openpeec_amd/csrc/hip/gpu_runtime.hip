@@ -1,0 +1,627 @@
+// gpu_runtime.hip — device mirror of a peec_t and the device-resident frequency sweep.
+//
+// A peec_t keeps its host arrays (malloc memory owned by the caller, freed by peec_free); the
+// device copies live here in a side table keyed by the peec_t address.  Per frequency point
+// the sweep runs entirely on the GPU:
+//   Lp fill -> charge-cell fill (P fused or aggregated) -> LU(P), C = P^-1 -> MNA assembly
+//   (-jw Lp block, jw C node block, sparse stamps) -> LU(A) -> all port right-hand sides ->
+//   port voltages (Z column), branch currents, cell charges.
+// Only the Z entries (and optional distributions) come back over PCIe.
+// Reference: solve.c:239-363 (sweep), mna.c:175-308 (assembly), potential.c:258-281 (C = P^-1).
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+#include <chrono>
+#include "../peec_internal.h"
+#include "../../../include/peec_mi355x.h"
+#include "hip_check.hpp"
+#include "fill_kernels.hpp"
+
+struct pset {   // one packed cell set on the device (cells and their images)
+	int n = 0, km = 0;
+	dcell_t *c = nullptr, *cm = nullptr;
+	dedge_t *e = nullptr, *em = nullptr;
+	dpnorm_t *pn = nullptr, *pnm = nullptr;
+};
+
+struct pgpu {
+	const peec_t *p = nullptr;
+	hipStream_t s = nullptr;
+	pset seg, chg;
+	int ncell = 0;
+	int *csoff = nullptr, *csidx = nullptr;
+	double *carea = nullptr;
+	bool cells_identity = false;
+	double2 *lp = nullptr;       // nseg x nseg
+	double2 *G = nullptr;        // nchg x nchg (non-identity aggregation only)
+	double2 *P = nullptr;        // ncell x ncell, factored in place by pgpu_cmat
+	double2 *C = nullptr;        // ncell x ncell
+	int *ppiv = nullptr, *pperm = nullptr;
+	double2 *cwork = nullptr;    // ncell x ncell solve workspace
+	// MNA
+	int n = 0;
+	double2 *A = nullptr;
+	int *apiv = nullptr, *aperm = nullptr;
+	double2 *B = nullptr, *Bw = nullptr;   // n x nport right-hand sides and workspace
+	int *cellrow = nullptr;      // [n] capacitive cell of node row r, or -1
+	int *d_st_ri = nullptr, *d_st_ci = nullptr;
+	double2 *d_st_v = nullptr;
+	int st_cap = 0;
+	double2 *qbuf = nullptr;     // ncell x nport cell charges
+};
+
+namespace {
+std::mutex g_mu;
+std::vector<pgpu *> g_tab;
+bool g_dev_init = false;
+
+template <typename T>
+T *dalloc(size_t n)
+{
+	T *p = nullptr;
+	if (n == 0) n = 1;
+	HIPCHK(hipMalloc((void **)&p, n * sizeof(T)));
+	return p;
+}
+
+void dfree(void *p)
+{
+	if (p) HIPCHK(hipFree(p));
+}
+
+void init_device()
+{
+	if (g_dev_init) return;
+	int ndev = 0;
+	if ((hipGetDeviceCount(&ndev) != hipSuccess) || (ndev <= 0)) {
+		fprintf(stderr, "*** libpeec_mi355x: no HIP device available (this library has no CPU path)\n");
+		fflush(stderr);
+		abort();
+	}
+	const char *env = getenv("PEEC_DEVICE");
+	const int dev = env ? atoi(env) : 0;
+	HIPCHK(hipSetDevice(dev));
+	g_dev_init = true;
+}
+
+void upload_set(pset &ps, const seg_t *src, int n, int gp, double gpz)
+{
+	ps.n = n;
+	ps.km = 0;
+	for (int i = 0; i < n; i++) {
+		const int k = (src[i].npv > 0) ? CELL_POLY : src[i].vol ? CELL_BAR : (src[i].wid > 0) ? CELL_RIBBON : CELL_WIRE;
+		ps.km |= 1 << k;
+	}
+	if (n <= 0) return;
+	for (int m = 0; m < (gp ? 2 : 1); m++) {
+		dcell_t *hc = nullptr;
+		dedge_t *he = nullptr;
+		dpnorm_t *hp = nullptr;
+		int ne = 0;
+		if (cells_pack(src, n, m, gpz, &hc, &he, &hp, &ne)) pgpu_fatal("cell packing (host allocation)");
+		dcell_t *dc = dalloc<dcell_t>(n);
+		dedge_t *de = dalloc<dedge_t>(ne);
+		dpnorm_t *dp = dalloc<dpnorm_t>(n);
+		HIPCHK(hipMemcpy(dc, hc, (size_t)n * sizeof(dcell_t), hipMemcpyHostToDevice));
+		if (ne > 0) HIPCHK(hipMemcpy(de, he, (size_t)ne * sizeof(dedge_t), hipMemcpyHostToDevice));
+		HIPCHK(hipMemcpy(dp, hp, (size_t)n * sizeof(dpnorm_t), hipMemcpyHostToDevice));
+		free(hc);
+		free(he);
+		free(hp);
+		if (m == 0) {
+			ps.c = dc;
+			ps.e = de;
+			ps.pn = dp;
+		}
+		else {
+			ps.cm = dc;
+			ps.em = de;
+			ps.pnm = dp;
+		}
+	}
+}
+
+void free_set(pset &ps)
+{
+	dfree(ps.c);
+	dfree(ps.cm);
+	dfree(ps.e);
+	dfree(ps.em);
+	dfree(ps.pn);
+	dfree(ps.pnm);
+	ps = pset();
+}
+
+pe::fill_args base_args(const pset &ps, int n, double kw, int gp, void *out, long ld)
+{
+	pe::fill_args a;
+	memset(&a, 0, sizeof(a));
+	a.S = ps.c;
+	a.pS.edge = ps.e;
+	a.pS.pn = ps.pn;
+	a.SM = ps.cm;
+	a.pSM.edge = ps.em;
+	a.pSM.pn = ps.pnm;
+	a.n = n;
+	a.nt = (n + 15) / 16;
+	a.kw = kw;
+	a.coef = MU0 / (4 * PI);
+	a.gp = gp;
+	a.out = out;
+	a.ld = ld;
+	a.k4pe = 4 * PI * EPS0;
+	return a;
+}
+
+// capacitive cells need pot_cells() on the host first
+void ensure_cells(pgpu *g)
+{
+	const peec_t *p = g->p;
+	if ((g->csoff != nullptr) || (p->ncell <= 0) || (p->csoff == nullptr)) return;
+	g->ncell = p->ncell;
+	g->csoff = dalloc<int>(p->ncell + 1);
+	g->csidx = dalloc<int>(p->nchg);
+	g->carea = dalloc<double>(p->ncell);
+	HIPCHK(hipMemcpy(g->csoff, p->csoff, (size_t)(p->ncell + 1) * sizeof(int), hipMemcpyHostToDevice));
+	HIPCHK(hipMemcpy(g->csidx, p->csidx, (size_t)p->nchg * sizeof(int), hipMemcpyHostToDevice));
+	HIPCHK(hipMemcpy(g->carea, p->carea, (size_t)p->ncell * sizeof(double), hipMemcpyHostToDevice));
+	bool ident = (p->ncell == p->nchg);
+	for (int h = 0; ident && (h < p->nchg); h++) ident = (p->cellof[h] == h);
+	g->cells_identity = ident;
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0)
+{
+	return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// ------------------------------------------------------------------------------------------
+// MNA assembly kernels
+
+// dense part: -jw Lp on the branch block, jw C on the node block, 0 elsewhere
+__global__ void k_assemble_dense(double2 *A, long lda, int n, int offS, int nseg, const double2 *lp, double omega,
+                                 const int *cellrow, const double2 *C, int ncell)
+{
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= (long)n * n) return;
+	const int r = (int)(e / n), c = (int)(e - (long)r * n);
+	double2 v = make_double2(0, 0);
+	if ((r >= offS) && (r < offS + nseg) && (c >= offS) && (c < offS + nseg)) {
+		const double2 x = lp[(long)(r - offS) * nseg + (c - offS)];
+		// d_mul((0, -w), x) = (0 x.r - (-w) x.i, 0 x.i + (-w) x.r)
+		v = make_double2((0 * x.x) - ((-omega) * x.y), (0 * x.y) + ((-omega) * x.x));
+	}
+	else if (cellrow != nullptr) {
+		const int cr = cellrow[r], cc = cellrow[c];
+		if ((cr >= 0) && (cc >= 0)) {
+			const double2 x = C[(long)cr * ncell + cc];
+			// d_mul((0, w), x)
+			v = make_double2((0 * x.x) - (omega * x.y), (0 * x.y) + (omega * x.x));
+		}
+	}
+	A[(long)r * lda + c] = make_double2(0 + v.x, 0 + v.y);
+}
+
+// sparse stamps (coalesced per entry in the reference's order): A += v
+__global__ void k_assemble_sparse(double2 *A, long lda, const int *ri, const int *ci, const double2 *val, int nst)
+{
+	const int e = blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= nst) return;
+	double2 *a = &A[(long)ri[e] * lda + ci[e]];
+	const double2 x = *a, v = val[e];
+	*a = make_double2(v.x + x.x, v.y + x.y);
+}
+
+__global__ void k_fill_identity(double2 *B, long ldb, int n)
+{
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= (long)n * n) return;
+	const int r = (int)(e / n), c = (int)(e - (long)r * n);
+	B[(long)r * ldb + c] = make_double2((r == c) ? 1.0 : 0.0, 0);
+}
+
+// port right-hand sides: column j has +1 at node row i1(j), -1 at i2(j)
+__global__ void k_port_rhs(double2 *B, int n, int nport, const int *i1, const int *i2)
+{
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= (long)n * nport) return;
+	const int r = (int)(e / nport), j = (int)(e - (long)r * nport);
+	double x = 0;
+	if (r == i1[j]) x = 0 + 1.0;
+	if (r == i2[j]) x = x - 1.0;
+	B[e] = make_double2(x, 0);
+}
+
+// q_m = sum_l C[m][l] v_l with v_l the node voltage of cell l (reference solve.c:352-358)
+__global__ void k_cell_charge(const double2 *C, int ncell, const int *rowofcell, const double2 *B, int nport,
+                              double2 *q)
+{
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= (long)ncell * nport) return;
+	const int m = (int)(e / nport), j = (int)(e - (long)m * nport);
+	double qr = 0, qi = 0;
+	for (int l = 0; l < ncell; l++) {
+		const int im = rowofcell[l];
+		const double2 v = (im < 0) ? make_double2(0, 0) : B[(long)im * nport + j];
+		const double2 c = C[(long)m * ncell + l];
+		qr = qr + ((c.x * v.x) - (c.y * v.y));
+		qi = qi + ((c.x * v.y) + (c.y * v.x));
+	}
+	q[e] = make_double2(qr, qi);
+}
+
+void ensure_mna(pgpu *g, const peec_t *p)
+{
+	if (g->A != nullptr) return;
+	const int n = p->nunknown;
+	g->n = n;
+	g->A = dalloc<double2>((size_t)n * n);
+	g->apiv = dalloc<int>(n);
+	g->aperm = dalloc<int>(n);
+	const int np = (p->nport > 0) ? p->nport : 1;
+	g->B = dalloc<double2>((size_t)n * np);
+	g->Bw = dalloc<double2>((size_t)n * np);
+	if (p->capacitance && (p->ncell > 0) && (p->offQ >= p->nunknown)) {
+		std::vector<int> cr(n, -1);
+		for (int i = 0; i < p->ncell; i++) {
+			const int m = p->nodemap[p->cellid[i]];
+			if (m >= 0) cr[m] = i;
+		}
+		g->cellrow = dalloc<int>(n);
+		HIPCHK(hipMemcpy(g->cellrow, cr.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+		g->qbuf = dalloc<double2>((size_t)p->ncell * np);
+	}
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+extern "C" {
+
+void pgpu_fatal(const char *what)
+{
+	fprintf(stderr, "*** libpeec_mi355x: %s\n", what);
+	fflush(stderr);
+	abort();
+}
+
+int pgpu_device_ok(void)
+{
+	int ndev = 0;
+	return (hipGetDeviceCount(&ndev) == hipSuccess) && (ndev > 0);
+}
+
+pgpu_t *pgpu_get(const peec_t *p)
+{
+	std::lock_guard<std::mutex> lk(g_mu);
+	for (pgpu *g : g_tab)
+		if (g->p == p) {
+			ensure_cells(g);
+			return g;
+		}
+	init_device();
+	pgpu *g = new pgpu();
+	g->p = p;
+	HIPCHK(hipStreamCreateWithFlags(&g->s, hipStreamNonBlocking));
+	upload_set(g->seg, p->seg, p->nseg, p->gp, p->gpz);
+	upload_set(g->chg, p->chg, p->nchg, p->gp, p->gpz);
+	ensure_cells(g);
+	g_tab.push_back(g);
+	return g;
+}
+
+void peec_gpu_release(peec_t *p)
+{
+	std::lock_guard<std::mutex> lk(g_mu);
+	for (size_t k = 0; k < g_tab.size(); k++) {
+		pgpu *g = g_tab[k];
+		if (g->p != p) continue;
+		HIPCHK(hipStreamSynchronize(g->s));
+		free_set(g->seg);
+		free_set(g->chg);
+		dfree(g->csoff);
+		dfree(g->csidx);
+		dfree(g->carea);
+		dfree(g->lp);
+		dfree(g->G);
+		dfree(g->P);
+		dfree(g->C);
+		dfree(g->ppiv);
+		dfree(g->pperm);
+		dfree(g->cwork);
+		dfree(g->A);
+		dfree(g->apiv);
+		dfree(g->aperm);
+		dfree(g->B);
+		dfree(g->Bw);
+		dfree(g->cellrow);
+		dfree(g->d_st_ri);
+		dfree(g->d_st_ci);
+		dfree(g->d_st_v);
+		dfree(g->qbuf);
+		HIPCHK(hipStreamDestroy(g->s));
+		delete g;
+		g_tab.erase(g_tab.begin() + (long)k);
+		return;
+	}
+}
+
+int pgpu_fill_lp(pgpu_t *g, double kw)
+{
+	const int n = g->seg.n;
+	if (n <= 0) return 0;
+	if (g->lp == nullptr) g->lp = dalloc<double2>((size_t)n * n);
+	pe::fill_args a = base_args(g->seg, n, kw, g->p->gp, g->lp, n);
+	fill_launch(g->s, 0, g->seg.km, a);
+	return 0;
+}
+
+int pgpu_fill_pot(pgpu_t *g, double kw)
+{
+	const peec_t *p = g->p;
+	const int nh = g->chg.n, nc = p->ncell;
+	if ((nh <= 0) || (nc <= 0)) return 0;
+	ensure_cells(g);
+	if (g->P == nullptr) g->P = dalloc<double2>((size_t)nc * nc);
+	if (g->cells_identity) {
+		pe::fill_args a = base_args(g->chg, nh, kw, p->gp, g->P, nc);
+		a.carea = g->carea;
+		fill_launch(g->s, 1, g->chg.km, a);
+	}
+	else {
+		if (g->G == nullptr) g->G = dalloc<double2>((size_t)nh * nh);
+		pe::fill_args a = base_args(g->chg, nh, kw, p->gp, g->G, nh);
+		fill_launch(g->s, 1, g->chg.km, a);
+		aggregate_launch(g->s, g->G, nh, g->csoff, g->csidx, g->carea, nc, 4 * PI * EPS0, g->P, nc);
+	}
+	return 0;
+}
+
+int pgpu_cmat(pgpu_t *g, int *ising)
+{
+	const int nc = g->p->ncell;
+	*ising = -1;
+	if (nc <= 0) return 0;
+	if (g->C == nullptr) {
+		g->C = dalloc<double2>((size_t)nc * nc);
+		g->ppiv = dalloc<int>(nc);
+		g->pperm = dalloc<int>(nc);
+		g->cwork = dalloc<double2>((size_t)nc * nc);
+	}
+	const int info = dlu_factor(g->s, nc, (d_complex_t *)g->P, nc, g->ppiv, g->pperm, nullptr);
+	if (info >= 0) {
+		*ising = info;
+		return 1;
+	}
+	const long tot = (long)nc * nc;
+	hipLaunchKernelGGL(k_fill_identity, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, g->s, g->C, (long)nc, nc);
+	dlu_solve_perm(g->s, nc, (const d_complex_t *)g->P, nc, g->pperm, (d_complex_t *)g->C, nc, nc,
+	               (d_complex_t *)g->cwork);
+	return 0;
+}
+
+void pgpu_lp_to_host(pgpu_t *g, d_complex_t *lp)
+{
+	const size_t n = (size_t)g->seg.n;
+	HIPCHK(hipMemcpyAsync(lp, g->lp, n * n * sizeof(double2), hipMemcpyDeviceToHost, g->s));
+	HIPCHK(hipStreamSynchronize(g->s));
+}
+
+void pgpu_cmat_to_host(pgpu_t *g, d_complex_t *cmat)
+{
+	const size_t n = (size_t)g->p->ncell;
+	HIPCHK(hipMemcpyAsync(cmat, g->C, n * n * sizeof(double2), hipMemcpyDeviceToHost, g->s));
+	HIPCHK(hipStreamSynchronize(g->s));
+}
+
+void pgpu_lp_from_host(pgpu_t *g, const d_complex_t *lp)
+{
+	const size_t n = (size_t)g->seg.n;
+	if (n == 0) return;
+	if (g->lp == nullptr) g->lp = dalloc<double2>(n * n);
+	HIPCHK(hipMemcpyAsync(g->lp, lp, n * n * sizeof(double2), hipMemcpyHostToDevice, g->s));
+	HIPCHK(hipStreamSynchronize(g->s));
+}
+
+void pgpu_cmat_from_host(pgpu_t *g, const d_complex_t *cmat)
+{
+	const size_t n = (size_t)g->p->ncell;
+	if (n == 0) return;
+	if (g->C == nullptr) {
+		g->C = dalloc<double2>(n * n);
+		g->ppiv = dalloc<int>(n);
+		g->pperm = dalloc<int>(n);
+		g->cwork = dalloc<double2>(n * n);
+	}
+	HIPCHK(hipMemcpyAsync(g->C, cmat, n * n * sizeof(double2), hipMemcpyHostToDevice, g->s));
+	HIPCHK(hipStreamSynchronize(g->s));
+}
+
+d_complex_t pgpu_lp_entry(pgpu_t *g, int i, int j, double kw)
+{
+	if (i > j) {
+		const int t = i;
+		i = j;
+		j = t;
+	}
+	double2 *res = dalloc<double2>(1);
+	pe::fill_args a = base_args(g->seg, g->seg.n, kw, g->p->gp, nullptr, 0);
+	fill_one(g->s, 0, g->seg.km, a, i, j, nullptr, nullptr, res);
+	double2 h;
+	HIPCHK(hipMemcpyAsync(&h, res, sizeof(h), hipMemcpyDeviceToHost, g->s));
+	HIPCHK(hipStreamSynchronize(g->s));
+	dfree(res);
+	return d_complex(h.x, h.y);
+}
+
+d_complex_t pgpu_pot_entry(pgpu_t *g, int i, int j, double kw)
+{
+	ensure_cells(g);
+	double2 *res = dalloc<double2>(1);
+	pe::fill_args a = base_args(g->chg, g->chg.n, kw, g->p->gp, nullptr, 0);
+	a.carea = g->carea;
+	fill_one(g->s, 1, g->chg.km, a, i, j, g->csoff, g->csidx, res);
+	double2 h;
+	HIPCHK(hipMemcpyAsync(&h, res, sizeof(h), hipMemcpyDeviceToHost, g->s));
+	HIPCHK(hipStreamSynchronize(g->s));
+	dfree(res);
+	return d_complex(h.x, h.y);
+}
+
+// ------------------------------------------------------------------------------------------
+int pgpu_sweep(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq, d_complex_t *zmat, d_complex_t *segi,
+               d_complex_t *cellq, pgpu_stats_t *st)
+{
+	const int n = p->nunknown, np = p->nport;
+	ensure_cells(g);
+	ensure_mna(g, p);
+	const bool usecap = p->capacitance && (p->ncell > 0) && (g->cellrow != nullptr);
+	std::vector<int> i1(np), i2(np);
+	for (int j = 0; j < np; j++) {
+		i1[j] = p->nodemap[p->port[j].n1];
+		i2[j] = p->nodemap[p->port[j].n2];
+	}
+	int *d_i1 = dalloc<int>(np), *d_i2 = dalloc<int>(np);
+	HIPCHK(hipMemcpy(d_i1, i1.data(), (size_t)np * sizeof(int), hipMemcpyHostToDevice));
+	HIPCHK(hipMemcpy(d_i2, i2.data(), (size_t)np * sizeof(int), hipMemcpyHostToDevice));
+	int *d_rowofcell = nullptr;
+	if (usecap) {
+		std::vector<int> rc(p->ncell);
+		for (int l = 0; l < p->ncell; l++) rc[l] = p->nodemap[p->cellid[l]];
+		d_rowofcell = dalloc<int>(p->ncell);
+		HIPCHK(hipMemcpy(d_rowofcell, rc.data(), (size_t)p->ncell * sizeof(int), hipMemcpyHostToDevice));
+	}
+	std::vector<double2> hB((size_t)n * np);
+	std::vector<double2> hq(usecap ? (size_t)p->ncell * np : 1);
+	hipEvent_t ev[8];
+	for (int k = 0; k < 8; k++) HIPCHK(hipEventCreate(&ev[k]));
+	int rc = 0;
+	// static sweeps reuse the setup fill (host p->lp / p->cmat) uploaded once
+	if (!p->retardation) {
+		if (p->nseg > 0) pgpu_lp_from_host(g, p->lp);
+		if (usecap) pgpu_cmat_from_host(g, p->cmat);
+	}
+	for (int f = 0; f < nfreq; f++) {
+		const int ifreq = ifreq0 + f;
+		const double freq = freq_at(p, ifreq);
+		const double kw = p->retardation ? (2 * PI * freq / C0) : 0;
+		const double omega = 2 * PI * freq;
+		const auto t0 = std::chrono::steady_clock::now();
+		HIPCHK(hipEventRecord(ev[0], g->s));
+		if (p->retardation) {
+			pgpu_fill_lp(g, kw);
+			HIPCHK(hipEventRecord(ev[1], g->s));
+			if (usecap) {
+				pgpu_fill_pot(g, kw);
+				HIPCHK(hipEventRecord(ev[2], g->s));
+				int ising = -1;
+				if (pgpu_cmat(g, &ising)) {
+					printf("*** singular potential coefficient matrix (row %d)\n", ising);
+					rc = 1;
+					break;
+				}
+			}
+			else {
+				HIPCHK(hipEventRecord(ev[2], g->s));
+			}
+		}
+		else {
+			HIPCHK(hipEventRecord(ev[1], g->s));
+			HIPCHK(hipEventRecord(ev[2], g->s));
+		}
+		HIPCHK(hipEventRecord(ev[3], g->s));
+		// assembly
+		stamps_t sp;
+		mna_stamps(p, freq, &sp);
+		if (sp.n > g->st_cap) {
+			dfree(g->d_st_ri);
+			dfree(g->d_st_ci);
+			dfree(g->d_st_v);
+			g->st_cap = sp.n;
+			g->d_st_ri = dalloc<int>(sp.n);
+			g->d_st_ci = dalloc<int>(sp.n);
+			g->d_st_v = dalloc<double2>(sp.n);
+		}
+		if (sp.n > 0) {
+			HIPCHK(hipMemcpyAsync(g->d_st_ri, sp.ri, (size_t)sp.n * sizeof(int), hipMemcpyHostToDevice, g->s));
+			HIPCHK(hipMemcpyAsync(g->d_st_ci, sp.ci, (size_t)sp.n * sizeof(int), hipMemcpyHostToDevice, g->s));
+			HIPCHK(hipMemcpyAsync(g->d_st_v, sp.val, (size_t)sp.n * sizeof(double2), hipMemcpyHostToDevice, g->s));
+		}
+		const long tot = (long)n * n;
+		hipLaunchKernelGGL(k_assemble_dense, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, g->s, g->A, (long)n, n,
+		                   p->offS, p->nseg, g->lp, omega, usecap ? g->cellrow : nullptr, g->C, p->ncell);
+		if (sp.n > 0)
+			hipLaunchKernelGGL(k_assemble_sparse, dim3((sp.n + 255) / 256), dim3(256), 0, g->s, g->A, (long)n,
+			                   g->d_st_ri, g->d_st_ci, g->d_st_v, sp.n);
+		HIPCHK(hipEventRecord(ev[4], g->s));
+		HIPCHK(hipStreamSynchronize(g->s));   // stamps host buffers are released below
+		mna_stamps_free(&sp);
+		// factor
+		const int info = dlu_factor(g->s, n, (d_complex_t *)g->A, n, g->apiv, g->aperm, st);
+		HIPCHK(hipEventRecord(ev[5], g->s));
+		if (info >= 0) {
+			printf("*** singular matrix at f = %.5e Hz (row %d : floating node or ideal voltage source loop?)\n", freq, info);
+			if (st) st->ising = info;
+			rc = 2;
+			break;
+		}
+		// all ports at once
+		const long tb = (long)n * np;
+		hipLaunchKernelGGL(k_port_rhs, dim3((unsigned)((tb + 255) / 256)), dim3(256), 0, g->s, g->B, n, np, d_i1, d_i2);
+		dlu_solve_perm(g->s, n, (const d_complex_t *)g->A, n, g->aperm, (d_complex_t *)g->B, np, np,
+		               (d_complex_t *)g->Bw);
+		if (usecap && (cellq != nullptr)) {
+			const long tq = (long)p->ncell * np;
+			hipLaunchKernelGGL(k_cell_charge, dim3((unsigned)((tq + 255) / 256)), dim3(256), 0, g->s, g->C, p->ncell,
+			                   d_rowofcell, g->B, np, g->qbuf);
+		}
+		HIPCHK(hipEventRecord(ev[6], g->s));
+		HIPCHK(hipMemcpyAsync(hB.data(), g->B, (size_t)n * np * sizeof(double2), hipMemcpyDeviceToHost, g->s));
+		if (usecap && (cellq != nullptr))
+			HIPCHK(hipMemcpyAsync(hq.data(), g->qbuf, (size_t)p->ncell * np * sizeof(double2), hipMemcpyDeviceToHost, g->s));
+		HIPCHK(hipStreamSynchronize(g->s));
+		for (int j = 0; j < np; j++) {
+			for (int i = 0; i < np; i++) {
+				const double2 v1 = (i1[i] < 0) ? make_double2(0, 0) : hB[(size_t)i1[i] * np + j];
+				const double2 v2 = (i2[i] < 0) ? make_double2(0, 0) : hB[(size_t)i2[i] * np + j];
+				zmat[((size_t)f * np + i) * np + j] = d_complex(v1.x - v2.x, v1.y - v2.y);
+			}
+			if (segi != nullptr)
+				for (int m = 0; m < p->nseg; m++) {
+					const double2 x = hB[(size_t)(p->offS + m) * np + j];
+					segi[((size_t)f * np + j) * p->nseg + m] = d_complex(x.x, x.y);
+				}
+			if (usecap && (cellq != nullptr))
+				for (int m = 0; m < p->ncell; m++) {
+					const double2 x = hq[(size_t)m * np + j];
+					cellq[((size_t)f * np + j) * p->ncell + m] = d_complex(x.x, x.y);
+				}
+		}
+		if (st) {
+			float a = 0;
+			HIPCHK(hipEventElapsedTime(&a, ev[0], ev[1]));
+			st->fill_ms += a;
+			HIPCHK(hipEventElapsedTime(&a, ev[1], ev[2]));
+			st->pot_ms += a;
+			HIPCHK(hipEventElapsedTime(&a, ev[2], ev[3]));
+			st->cinv_ms += a;
+			HIPCHK(hipEventElapsedTime(&a, ev[3], ev[4]));
+			st->asm_ms += a;
+			HIPCHK(hipEventElapsedTime(&a, ev[4], ev[5]));
+			st->lu_ms += a;
+			HIPCHK(hipEventElapsedTime(&a, ev[5], ev[6]));
+			st->solve_ms += a;
+			st->total_ms += ms_since(t0);
+		}
+	}
+	for (int k = 0; k < 8; k++) HIPCHK(hipEventDestroy(ev[k]));
+	dfree(d_i1);
+	dfree(d_i2);
+	dfree(d_rowofcell);
+	return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,532 @@
+// lu.hip — dense complex FP64 LU with partial pivoting on gfx950, and the triangular solves.
+//
+// Replaces lu_decomp / lu_solve (OpenPEEC src/lu.c:16-96).  Same contract: row-major matrix,
+// in-place Doolittle factors (unit L below the diagonal, U on and above), piv[k] = row swapped
+// with row k at step k, pivot = first row of maximal |re| + |im| in the column (ties to the
+// lowest index, lu.c:19-28), failure when that maximum is below 1e-30 (lu.c:30).
+//
+// Structure (right-looking, blocked by NB columns):
+//   k_panel     one cooperative launch per panel: the panel rows are split over P workgroups
+//               held in LDS; per column every workgroup publishes its local pivot candidate
+//               (value, row index, row contents) and the current diagonal row, one grid
+//               barrier, then every workgroup derives the same winner, swaps and eliminates.
+//               One grid barrier per column is the whole latency cost of partial pivoting.
+//   k_laswp     applies the panel's swaps to the columns left and right of the panel and to
+//               the running row permutation (used by the solves as one gather).
+//   k_trsm_lu   U12 = L11^-1 A12 (unit lower, LDS-staged).
+//   k_zgemm     A22 -= L21 U12 on FP64 MFMA (v_mfma_f64_16x16x4), complex as 4 real products.
+// The trailing update's summation order differs from the reference's rank-1 sequence, so the
+// factors agree with the reference to rounding, not bitwise; pivots agree except on near-ties.
+#include <hip/hip_runtime.h>
+#include <climits>
+#include <cstdio>
+#include "../peec_internal.h"
+#include "hip_check.hpp"
+
+namespace {
+
+constexpr int NB = 64;          // panel width (GEMM K)
+constexpr int PANEL_T = 256;    // threads per panel workgroup
+constexpr int PANEL_RMAX = 144; // panel rows per workgroup held in LDS (144 * 64 * 16 B = 144 KiB)
+constexpr int PANEL_PMAX = 256; // workgroups in a panel launch (at most one per CU)
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+	return make_double2((a.x * b.x) - (a.y * b.y), (a.x * b.y) + (a.y * b.x));
+}
+
+__device__ __forceinline__ double2 cinv(double2 z)
+{
+	const double n2 = (z.x * z.x) + (z.y * z.y);
+	if (n2 <= 0) return make_double2(0, 0);
+	return make_double2(z.x / n2, -z.y / n2);
+}
+
+// ------------------------------------------------------------------------------------------
+// grid barrier for cooperative launches: monotone counter, target = base + P * (step + 1)
+__device__ __forceinline__ void grid_barrier(unsigned *count, unsigned target)
+{
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		__threadfence();
+		atomicAdd(count, 1u);
+		while (__hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+			__builtin_amdgcn_s_sleep(1);
+		}
+		__threadfence();
+	}
+	__syncthreads();
+}
+
+// (value, index) arg-max with ties to the smaller index; value < 0 marks "no candidate"
+__device__ __forceinline__ void amax_merge(double &v, int &i, double ov, int oi)
+{
+	if ((ov > v) || ((ov == v) && (oi < i))) {
+		v = ov;
+		i = oi;
+	}
+}
+
+__device__ void block_amax(double &v, int &i, int &w_out, double *sv, int *si, int *sw, int w)
+{
+	for (int off = 32; off > 0; off >>= 1) {
+		const double ov = __shfl_xor(v, off);
+		const int oi = __shfl_xor(i, off);
+		const int ow = __shfl_xor(w, off);
+		if ((ov > v) || ((ov == v) && (oi < i))) {
+			v = ov;
+			i = oi;
+			w = ow;
+		}
+	}
+	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	if (lane == 0) {
+		sv[wave] = v;
+		si[wave] = i;
+		sw[wave] = w;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		for (int k = 1; k < (int)(blockDim.x >> 6); k++) {
+			if ((sv[k] > sv[0]) || ((sv[k] == sv[0]) && (si[k] < si[0]))) {
+				sv[0] = sv[k];
+				si[0] = si[k];
+				sw[0] = sw[k];
+			}
+		}
+	}
+	__syncthreads();
+	v = sv[0];
+	i = si[0];
+	w_out = sw[0];
+	__syncthreads();
+}
+
+struct panel_ws {
+	double  *pubv;      // [2][P]     local max |re|+|im|
+	int     *pubi;      // [2][P]     its global row
+	double2 *pubrow;    // [2][P][NB] that row's panel entries
+	double2 *diagrow;   // [2][NB]    current row kk
+	unsigned *bar;      // barrier counter
+	int     *info;      // singular row or -1
+};
+
+__global__ __launch_bounds__(PANEL_T) void k_panel(double2 *A, long lda, int n, int k0, int nb, int R, int *piv,
+                                                    panel_ws ws, unsigned bar_base)
+{
+	extern __shared__ double2 S[];   // [R][NB]
+	__shared__ double2 U[NB], D[NB], L[PANEL_RMAX];
+	__shared__ double sv[4];
+	__shared__ int si[4], sw[4];
+	const int P = gridDim.x, w = blockIdx.x, tid = threadIdx.x;
+	// a singular column in an earlier panel ends the factorisation: every workgroup reads the
+	// same flag (written before this launch) and leaves before the first barrier
+	if (*ws.info >= 0) return;
+	const int r0 = k0 + w * R;
+	const int rc = max(0, min(R, n - r0));
+
+	for (int e = tid; e < rc * nb; e += PANEL_T) {
+		const int r = e / nb, c = e - r * nb;
+		S[r * NB + c] = A[(long)(r0 + r) * lda + k0 + c];
+	}
+	__syncthreads();
+
+	for (int jj = 0; jj < nb; jj++) {
+		const int kk = k0 + jj, par = jj & 1;
+		double bv = -1;
+		int bi = INT_MAX;
+		for (int r = tid; r < rc; r += PANEL_T) {
+			const int g = r0 + r;
+			if (g >= kk) {
+				const double2 a = S[r * NB + jj];
+				amax_merge(bv, bi, fabs(a.x) + fabs(a.y), g);
+			}
+		}
+		int dummy;
+		block_amax(bv, bi, dummy, sv, si, sw, w);
+		if (tid == 0) {
+			ws.pubv[par * P + w] = bv;
+			ws.pubi[par * P + w] = bi;
+		}
+		if ((bv >= 0) && (tid < nb)) ws.pubrow[((long)par * P + w) * NB + tid] = S[(bi - r0) * NB + tid];
+		if ((kk >= r0) && (kk < r0 + rc) && (tid < nb)) ws.diagrow[par * NB + tid] = S[(kk - r0) * NB + tid];
+		grid_barrier(ws.bar, bar_base + (unsigned)(P * (jj + 1)));
+
+		double gv = -1;
+		int gi = INT_MAX, gw = -1;
+		for (int q = tid; q < P; q += PANEL_T) {
+			const double qv = ws.pubv[par * P + q];
+			const int qi = ws.pubi[par * P + q];
+			if ((qv > gv) || ((qv == gv) && (qi < gi))) {
+				gv = qv;
+				gi = qi;
+				gw = q;
+			}
+		}
+		block_amax(gv, gi, gw, sv, si, sw, gw);
+		if (!(gv >= 1e-30)) {
+			if ((w == 0) && (tid == 0)) {
+				piv[kk] = (gi == INT_MAX) ? kk : gi;
+				*ws.info = kk;
+			}
+			return;
+		}
+		if (tid < nb) {
+			U[tid] = ws.pubrow[((long)par * P + gw) * NB + tid];
+			D[tid] = ws.diagrow[par * NB + tid];
+		}
+		if ((w == 0) && (tid == 0)) piv[kk] = gi;
+		__syncthreads();
+		if (tid < nb) {
+			if ((gi != kk) && (gi >= r0) && (gi < r0 + rc)) S[(gi - r0) * NB + tid] = D[tid];
+			if ((kk >= r0) && (kk < r0 + rc)) S[(kk - r0) * NB + tid] = U[tid];
+		}
+		__syncthreads();
+		const double2 pinv = cinv(U[jj]);
+		const int rfirst = max(0, kk + 1 - r0);
+		for (int r = rfirst + tid; r < rc; r += PANEL_T) {
+			const double2 l = cmul(S[r * NB + jj], pinv);
+			S[r * NB + jj] = l;
+			L[r] = l;
+		}
+		__syncthreads();
+		const int ncol = nb - jj - 1;
+		if (ncol > 0) {
+			const int nrow = rc - rfirst;
+			for (int e = tid; e < nrow * ncol; e += PANEL_T) {
+				const int r = rfirst + e / ncol, c = jj + 1 + (e % ncol);
+				const double2 p = cmul(L[r], U[c]);
+				double2 a = S[r * NB + c];
+				a.x = a.x - p.x;
+				a.y = a.y - p.y;
+				S[r * NB + c] = a;
+			}
+		}
+		__syncthreads();
+	}
+	for (int e = tid; e < rc * nb; e += PANEL_T) {
+		const int r = e / nb, c = e - r * nb;
+		A[(long)(r0 + r) * lda + k0 + c] = S[r * NB + c];
+	}
+}
+
+// apply piv[k0 .. k0+nb) to every column outside the panel, and to the running permutation
+__global__ void k_laswp(double2 *A, long lda, int n, int k0, int nb, const int *piv, int *perm, const int *info)
+{
+	if (*info >= 0) return;
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	const int nout = n - nb;
+	if (t < nout) {
+		const int j = (t < k0) ? t : (t + nb);
+		for (int kk = k0; kk < k0 + nb; kk++) {
+			const int p = piv[kk];
+			if (p != kk) {
+				const double2 x = A[(long)kk * lda + j];
+				A[(long)kk * lda + j] = A[(long)p * lda + j];
+				A[(long)p * lda + j] = x;
+			}
+		}
+	}
+	else if ((t == nout) && (perm != nullptr)) {
+		for (int kk = k0; kk < k0 + nb; kk++) {
+			const int p = piv[kk];
+			if (p != kk) {
+				const int x = perm[kk];
+				perm[kk] = perm[p];
+				perm[p] = x;
+			}
+		}
+	}
+}
+
+// X <- T^-1 X for an nb x nb triangle T (unit lower if LOWER, else non-unit upper) at T[0][0]
+// and the nb x ncol block X; one thread per column, T and X staged in LDS.
+template <bool LOWER>
+__global__ __launch_bounds__(64) void k_trsm(const double2 *T, long ldt, double2 *X, long ldx, int nb, int ncol)
+{
+	__shared__ double2 St[NB * NB];
+	__shared__ double2 Sx[NB * 64];
+	const int tid = threadIdx.x;
+	const int j = blockIdx.x * 64 + tid;
+	for (int e = tid; e < nb * nb; e += 64) {
+		const int r = e / nb, c = e - r * nb;
+		St[r * NB + c] = T[(long)r * ldt + c];
+	}
+	if (j < ncol)
+		for (int r = 0; r < nb; r++) Sx[r * 64 + tid] = X[(long)r * ldx + j];
+	__syncthreads();
+	if (j >= ncol) return;
+	if (LOWER) {
+		for (int r = 1; r < nb; r++) {
+			double2 s = Sx[r * 64 + tid];
+			for (int q = 0; q < r; q++) {
+				const double2 p = cmul(St[r * NB + q], Sx[q * 64 + tid]);
+				s.x -= p.x;
+				s.y -= p.y;
+			}
+			Sx[r * 64 + tid] = s;
+		}
+	}
+	else {
+		for (int r = nb - 1; r >= 0; r--) {
+			double2 s = Sx[r * 64 + tid];
+			for (int q = r + 1; q < nb; q++) {
+				const double2 p = cmul(St[r * NB + q], Sx[q * 64 + tid]);
+				s.x -= p.x;
+				s.y -= p.y;
+			}
+			Sx[r * 64 + tid] = cmul(s, cinv(St[r * NB + r]));
+		}
+	}
+	for (int r = 0; r < nb; r++) X[(long)r * ldx + j] = Sx[r * 64 + tid];
+}
+
+// C[M x N] -= A[M x K] B[K x N] on FP64 MFMA.  Wave tile 32 x 64 (2 x 4 blocks of 16 x 16),
+// workgroup 2 x 2 waves = 64 x 128.  v_mfma_f64_16x16x4 lane layout (checked on gfx950 with an
+// in-bounds probe): A row = lane%16, k = lane/16; B col = lane%16, k = lane/16;
+// D register v of lane l holds row lane/16 + 4 v, col lane%16.
+__global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, const double2 *__restrict__ A, long lda,
+                                                const double2 *__restrict__ B, long ldb, double2 *__restrict__ C,
+                                                long ldc)
+{
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int row0 = blockIdx.y * 64 + (wave >> 1) * 32;
+	const int col0 = blockIdx.x * 128 + (wave & 1) * 64;
+	const int li = lane & 15, lk = lane >> 4;
+	d4 ar[2][4], ai[2][4];
+	for (int a = 0; a < 2; a++)
+		for (int b = 0; b < 4; b++) {
+			ar[a][b] = d4{0, 0, 0, 0};
+			ai[a][b] = d4{0, 0, 0, 0};
+		}
+	for (int k = 0; k < K; k += 4) {
+		const int kk = k + lk;
+		double2 fa[2], fb[4];
+		for (int rb = 0; rb < 2; rb++) {
+			const int r = row0 + rb * 16 + li;
+			fa[rb] = ((r < M) && (kk < K)) ? A[(long)r * lda + kk] : make_double2(0, 0);
+		}
+		for (int cb = 0; cb < 4; cb++) {
+			const int c = col0 + cb * 16 + li;
+			fb[cb] = ((kk < K) && (c < N)) ? B[(long)kk * ldb + c] : make_double2(0, 0);
+		}
+		for (int rb = 0; rb < 2; rb++) {
+			for (int cb = 0; cb < 4; cb++) {
+				ar[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[rb].x, fb[cb].x, ar[rb][cb], 0, 0, 0);
+				ar[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(-fa[rb].y, fb[cb].y, ar[rb][cb], 0, 0, 0);
+				ai[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[rb].x, fb[cb].y, ai[rb][cb], 0, 0, 0);
+				ai[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[rb].y, fb[cb].x, ai[rb][cb], 0, 0, 0);
+			}
+		}
+	}
+	for (int rb = 0; rb < 2; rb++) {
+		for (int cb = 0; cb < 4; cb++) {
+			const int c = col0 + cb * 16 + li;
+			if (c >= N) continue;
+			for (int v = 0; v < 4; v++) {
+				const int r = row0 + rb * 16 + lk + 4 * v;
+				if (r >= M) continue;
+				double2 x = C[(long)r * ldc + c];
+				x.x -= ar[rb][cb][v];
+				x.y -= ai[rb][cb][v];
+				C[(long)r * ldc + c] = x;
+			}
+		}
+	}
+}
+
+__global__ void k_iota(int *p, int n)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) p[i] = i;
+}
+
+// dst[k][:] = src[perm[k]][:]
+__global__ void k_gather_rows(const double2 *src, long lds, double2 *dst, long ldd, const int *perm, int n, int ncol)
+{
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= (long)n * ncol) return;
+	const int k = (int)(e / ncol), j = (int)(e - (long)k * ncol);
+	dst[(long)k * ldd + j] = src[(long)perm[k] * lds + j];
+}
+
+__global__ void k_copy_rows(const double2 *src, long lds, double2 *dst, long ldd, int n, int ncol)
+{
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= (long)n * ncol) return;
+	const int k = (int)(e / ncol), j = (int)(e - (long)k * ncol);
+	dst[(long)k * ldd + j] = src[(long)k * lds + j];
+}
+
+struct lu_work {
+	size_t cap = 0;
+	char *base = nullptr;
+	panel_ws ws{};
+	int *perm = nullptr;
+	int nperm = 0;
+	int ncu = 0;
+	bool attr = false;
+};
+lu_work g_lw;
+
+void ensure_ws(int n)
+{
+	if (!g_lw.attr) {
+		HIPCHK(hipFuncSetAttribute((const void *)k_panel, hipFuncAttributeMaxDynamicSharedMemorySize,
+		                           PANEL_RMAX * NB * (int)sizeof(double2)));
+		int dev = 0;
+		HIPCHK(hipGetDevice(&dev));
+		HIPCHK(hipDeviceGetAttribute(&g_lw.ncu, hipDeviceAttributeMultiprocessorCount, dev));
+		g_lw.attr = true;
+	}
+	const size_t need = 2 * PANEL_PMAX * sizeof(double) + 2 * PANEL_PMAX * sizeof(int) +
+	                    2 * (size_t)PANEL_PMAX * NB * sizeof(double2) + 2 * NB * sizeof(double2) + 256 + 256;
+	if (g_lw.base == nullptr) {
+		HIPCHK(hipMalloc((void **)&g_lw.base, need));
+		char *p = g_lw.base;
+		g_lw.ws.pubrow = (double2 *)p;
+		p += 2 * (size_t)PANEL_PMAX * NB * sizeof(double2);
+		g_lw.ws.diagrow = (double2 *)p;
+		p += 2 * NB * sizeof(double2);
+		g_lw.ws.pubv = (double *)p;
+		p += 2 * PANEL_PMAX * sizeof(double);
+		g_lw.ws.pubi = (int *)p;
+		p += 2 * PANEL_PMAX * sizeof(int);
+		g_lw.ws.bar = (unsigned *)p;
+		p += 256;
+		g_lw.ws.info = (int *)p;
+	}
+	if (g_lw.nperm < n) {
+		if (g_lw.perm) HIPCHK(hipFree(g_lw.perm));
+		HIPCHK(hipMalloc((void **)&g_lw.perm, (size_t)n * sizeof(int)));
+		g_lw.nperm = n;
+	}
+}
+
+void launch_gemm(hipStream_t s, int M, int N, int K, const double2 *A, long lda, const double2 *B, long ldb,
+                 double2 *C, long ldc)
+{
+	if ((M <= 0) || (N <= 0) || (K <= 0)) return;
+	dim3 grid((N + 127) / 128, (M + 63) / 64);
+	hipLaunchKernelGGL(k_zgemm, grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc);
+}
+
+}  // namespace
+
+extern "C" void dlu_zgemm_sub(void *stream, int M, int N, int K, const d_complex_t *A, long lda,
+                              const d_complex_t *B, long ldb, d_complex_t *C, long ldc)
+{
+	launch_gemm((hipStream_t)stream, M, N, K, (const double2 *)A, lda, (const double2 *)B, ldb, (double2 *)C, ldc);
+}
+
+
+extern "C" int dlu_factor(void *stream, int n, d_complex_t *dA_, int lda, int *dpiv, int *dperm, pgpu_stats_t *st)
+{
+	hipStream_t s = (hipStream_t)stream;
+	double2 *A = (double2 *)dA_;
+	if (n <= 0) return -1;
+	ensure_ws(n);
+	int *perm = (dperm != nullptr) ? dperm : g_lw.perm;
+	HIPCHK(hipMemsetAsync(g_lw.ws.bar, 0, sizeof(unsigned), s));
+	HIPCHK(hipMemsetAsync(g_lw.ws.info, 0xff, sizeof(int), s));   // -1
+	hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, s, perm, n);
+	hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, s, dpiv, n);
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	if (st) {
+		HIPCHK(hipEventCreate(&e0));
+		HIPCHK(hipEventCreate(&e1));
+	}
+	unsigned bar_base = 0;
+	const int pmax = (g_lw.ncu < PANEL_PMAX) ? g_lw.ncu : PANEL_PMAX;
+	const long ldal = lda;
+	for (int k0 = 0; k0 < n; k0 += NB) {
+		const int nb = (n - k0 < NB) ? (n - k0) : NB;
+		const int m = n - k0;
+		int P = (m + PANEL_RMAX - 1) / PANEL_RMAX;
+		if (P < 1) P = 1;
+		if (P > pmax) {
+			fprintf(stderr, "*** dlu_factor: n = %d exceeds the panel capacity (%d rows)\n", n, pmax * PANEL_RMAX);
+			return 0;
+		}
+		int R = (m + P - 1) / P;
+		int *pivp = dpiv;
+		int nn = n, kk0 = k0, nbb = nb;
+		panel_ws ws = g_lw.ws;
+		unsigned bb = bar_base;
+		double2 *Ap = A;
+		long ldp = ldal;
+		void *kargs[] = {(void *)&Ap, (void *)&ldp, (void *)&nn, (void *)&kk0, (void *)&nbb,
+		                 (void *)&R, (void *)&pivp, (void *)&ws, (void *)&bb};
+		HIPCHK(hipLaunchCooperativeKernel((const void *)k_panel, dim3(P), dim3(PANEL_T), kargs,
+		                                  (unsigned)(R * NB * sizeof(double2)), s));
+		bar_base += (unsigned)(P * nb);
+		const int nout = n - nb;
+		hipLaunchKernelGGL(k_laswp, dim3((nout + 1 + 255) / 256), dim3(256), 0, s, A, ldal, n, k0, nb, dpiv, perm,
+		                   (const int *)g_lw.ws.info);
+		const int nr = n - k0 - nb;
+		if (nr > 0) {
+			double2 *A11 = A + (long)k0 * lda + k0;
+			double2 *A12 = A + (long)k0 * lda + k0 + nb;
+			double2 *A21 = A + (long)(k0 + nb) * lda + k0;
+			double2 *A22 = A + (long)(k0 + nb) * lda + k0 + nb;
+			hipLaunchKernelGGL(k_trsm<true>, dim3((nr + 63) / 64), dim3(64), 0, s, A11, ldal, A12, ldal, nb, nr);
+			if (st) HIPCHK(hipEventRecord(e0, s));
+			launch_gemm(s, nr, nr, nb, A21, ldal, A12, ldal, A22, ldal);
+			if (st) {
+				HIPCHK(hipEventRecord(e1, s));
+				HIPCHK(hipEventSynchronize(e1));
+				float ms = 0;
+				HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+				st->lu_gemm_ms += ms;
+				st->lu_gemm_flop += 8.0 * nr * (double)nr * nb;
+				st->lu_gemm_launches++;
+			}
+		}
+	}
+	if (st) {
+		HIPCHK(hipEventDestroy(e0));
+		HIPCHK(hipEventDestroy(e1));
+	}
+	int info = -1;
+	HIPCHK(hipMemcpyAsync(&info, g_lw.ws.info, sizeof(int), hipMemcpyDeviceToHost, s));
+	HIPCHK(hipStreamSynchronize(s));
+	return info;
+}
+
+// B <- A^-1 B using the factors and the row permutation recorded by the last dlu_factor on
+// this matrix (perm must be the one returned with these factors: dperm).
+extern "C" void dlu_solve_perm(void *stream, int n, const d_complex_t *dA_, int lda, const int *dperm,
+                               d_complex_t *dB_, int ldb, int nrhs, d_complex_t *dwork)
+{
+	hipStream_t s = (hipStream_t)stream;
+	const double2 *A = (const double2 *)dA_;
+	double2 *B = (double2 *)dB_;
+	double2 *W = (double2 *)dwork;
+	const long tot = (long)n * nrhs;
+	hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, B, (long)ldb, W, (long)nrhs,
+	                   dperm, n, nrhs);
+	hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, W, (long)nrhs, B, (long)ldb, n,
+	                   nrhs);
+	// forward: unit lower, block rows of NB
+	for (int r0 = 0; r0 < n; r0 += NB) {
+		const int nb = (n - r0 < NB) ? (n - r0) : NB;
+		hipLaunchKernelGGL(k_trsm<true>, dim3((nrhs + 63) / 64), dim3(64), 0, s, A + (long)r0 * lda + r0, (long)lda,
+		                   B + (long)r0 * ldb, (long)ldb, nb, nrhs);
+		const int rest = n - r0 - nb;
+		if (rest > 0)
+			launch_gemm(s, rest, nrhs, nb, A + (long)(r0 + nb) * lda + r0, lda, B + (long)r0 * ldb, ldb,
+			            B + (long)(r0 + nb) * ldb, ldb);
+	}
+	// backward: upper, from the last block row
+	const int nblk = (n + NB - 1) / NB;
+	for (int b = nblk - 1; b >= 0; b--) {
+		const int r0 = b * NB;
+		const int nb = (n - r0 < NB) ? (n - r0) : NB;
+		hipLaunchKernelGGL(k_trsm<false>, dim3((nrhs + 63) / 64), dim3(64), 0, s, A + (long)r0 * lda + r0, (long)lda,
+		                   B + (long)r0 * ldb, (long)ldb, nb, nrhs);
+		if (r0 > 0)
+			launch_gemm(s, r0, nrhs, nb, A + r0, lda, B + (long)r0 * ldb, ldb, B, ldb);
+	}
+}

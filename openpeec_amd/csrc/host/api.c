@@ -1,0 +1,370 @@
+/*
+ * api.c — the peec.h entry points of the hot path, backed by the device (no CPU compute path):
+ * lp_fill / pot_fill / lp_entry / pot_entry (partial.c:371-473, potential.c:124-303),
+ * pot_cells (potential.c:29-114, 168-175) and solve (solve.c:171-430), plus the extension
+ * entry points of peec_mi355x.h.  Host arrays keep the reference's ownership: p->lp and
+ * p->cmat are malloc'ed here on first use and released by peec_free.
+ */
+#include "peec.h"
+#include "peec_mi355x.h"
+#include "peec_internal.h"
+
+/* ---------------------------------------------------------------- capacitive cells ---- */
+
+/* cells = distinct nodes of the charge cells in order of first appearance; carea = sum of the
+ * sub-cell lengths (reference potential.c:29-62) */
+static int build_cells(peec_t *p)
+{
+	const int nh = p->nchg;
+	p->cellof = (int *)malloc((size_t)nh * sizeof(int));
+	p->cellid = (int *)malloc((size_t)nh * sizeof(int));
+	if ((p->cellof == NULL) || (p->cellid == NULL)) return 1;
+	/* first-appearance numbering with a node -> cell lookup table */
+	int maxnode = 0;
+	for (int h = 0; h < nh; h++)
+		if (p->chgnode[h] > maxnode) maxnode = p->chgnode[h];
+	int *slot = (int *)malloc((size_t)(maxnode + 1) * sizeof(int));
+	if (slot == NULL) return 1;
+	for (int i = 0; i <= maxnode; i++) slot[i] = -1;
+	p->ncell = 0;
+	for (int h = 0; h < nh; h++) {
+		const int node = p->chgnode[h];
+		if (slot[node] < 0) {
+			slot[node] = p->ncell;
+			p->cellid[p->ncell++] = node;
+		}
+		p->cellof[h] = slot[node];
+	}
+	free(slot);
+	p->carea = (double *)calloc((size_t)p->ncell, sizeof(double));
+	if (p->carea == NULL) return 1;
+	for (int h = 0; h < nh; h++) p->carea[p->cellof[h]] += p->chg[h].len;
+	return 0;
+}
+
+/* cell -> sub-cell CSR (ascending sub-cell index) and length-weighted centroids
+ * (reference potential.c:72-114) */
+static int build_cellsub(peec_t *p)
+{
+	const int nh = p->nchg, nc = p->ncell;
+	p->csoff = (int *)calloc((size_t)(nc + 1), sizeof(int));
+	p->csidx = (int *)malloc((size_t)(nh > 0 ? nh : 1) * sizeof(int));
+	p->cpt = (double *)malloc((size_t)(nc > 0 ? nc : 1) * 3 * sizeof(double));
+	if ((p->csoff == NULL) || (p->csidx == NULL) || (p->cpt == NULL)) return 1;
+	for (int h = 0; h < nh; h++) p->csoff[p->cellof[h] + 1]++;
+	for (int i = 0; i < nc; i++) p->csoff[i + 1] += p->csoff[i];
+	int *fill = (int *)malloc((size_t)(nc + 1) * sizeof(int));
+	if (fill == NULL) return 1;
+	memcpy(fill, p->csoff, (size_t)(nc + 1) * sizeof(int));
+	for (int h = 0; h < nh; h++) p->csidx[fill[p->cellof[h]]++] = h;
+	free(fill);
+	for (int i = 0; i < nc; i++) {
+		double w = 0, x[3] = {0, 0, 0};
+		for (int e = p->csoff[i]; e < p->csoff[i + 1]; e++) {
+			const seg_t *s = &p->chg[p->csidx[e]];
+			for (int c = 0; c < 3; c++) x[c] += s->len * 0.5 * (s->x1[c] + s->x2[c]);
+			w += s->len;
+		}
+		for (int c = 0; c < 3; c++) p->cpt[(3 * i) + c] = (w > 0) ? (x[c] / w) : 0;
+	}
+	return 0;
+}
+
+int pot_cells(peec_t *p)
+{
+	if (p->cellof != NULL) return 0;
+	if (p->nchg <= 0) return 1;
+	if (build_cells(p)) return 1;
+	return build_cellsub(p);
+}
+
+/* ---------------------------------------------------------------- fills ---------------- */
+
+void lp_fill(peec_t *p, double f, FILE *fp_log)
+{
+	const int n = p->nseg;
+	if (n <= 0) return;
+	if (p->lp == NULL) {
+		p->lp = (d_complex_t *)malloc((size_t)n * n * sizeof(d_complex_t));
+		if (p->lp == NULL) pgpu_fatal("host allocation of p->lp failed");
+	}
+	const double kw = p->retardation ? (2 * PI * f / C0) : 0;
+	pgpu_t *g = pgpu_get(p);
+	pgpu_fill_lp(g, kw);
+	pgpu_lp_to_host(g, p->lp);
+
+	/* one-time report of unphysical coupling |k| > 1 (overlapping conductors) */
+	if (p->lpwarn) return;
+	p->lpwarn = 1;
+	int nwarn = 0;
+	for (int i = 0; (i < n) && (nwarn < 10); i++) {
+		for (int j = i + 1; (j < n) && (nwarn < 10); j++) {
+			const double m = fabs(p->lp[(size_t)i * n + j].r);
+			const double s = sqrt(fabs(p->lp[(size_t)i * n + i].r * p->lp[(size_t)j * n + j].r));
+			if (m > s) {
+				if (fp_log) fprintf(fp_log, "*** warning : |k| > 1 between segment %d and %d (overlapping conductors?)\n", i, j);
+				nwarn++;
+			}
+		}
+	}
+}
+
+int pot_fill(peec_t *p, double f, FILE *fp_log)
+{
+	const char errmem[] = "*** memory allocation error (potential)";
+	if (!p->capacitance || (p->nchg <= 0)) return 0;
+	if (pot_cells(p)) {
+		printf("%s\n", errmem);
+		return 1;
+	}
+	if (p->compress) {
+		if (!p->clogged) {
+			p->clogged = 1;
+			printf("PEEC: capacitive cells = %d (compressed : charge unknowns)\n", p->ncell);
+			if (fp_log) {
+				fprintf(fp_log, "PEEC: capacitive cells = %d (compressed : charge unknowns)\n", p->ncell);
+				fflush(fp_log);
+			}
+		}
+		return 0;
+	}
+	const int nc = p->ncell;
+	const double kw = p->retardation ? (2 * PI * f / C0) : 0;
+	pgpu_t *g = pgpu_get(p);
+	pgpu_fill_pot(g, kw);
+	int ising = -1;
+	if (pgpu_cmat(g, &ising)) {
+		printf("*** singular potential coefficient matrix (row %d)\n", ising);
+		if (fp_log) fprintf(fp_log, "*** singular potential coefficient matrix (row %d)\n", ising);
+		return 1;
+	}
+	if (p->cmat == NULL) {
+		p->cmat = (d_complex_t *)malloc((size_t)nc * nc * sizeof(d_complex_t));
+		if (p->cmat == NULL) {
+			printf("%s\n", errmem);
+			return 1;
+		}
+	}
+	pgpu_cmat_to_host(g, p->cmat);
+	if (!p->clogged) {
+		p->clogged = 1;
+		double ctotal = 0;
+		for (int i = 0; i < nc; i++)
+			for (int j = 0; j < nc; j++) ctotal += p->cmat[(size_t)i * nc + j].r;
+		printf("PEEC: capacitive cells = %d, total capacitance = %.6e F\n", nc, ctotal);
+		if (fp_log) {
+			fprintf(fp_log, "PEEC: capacitive cells = %d, total capacitance = %.6e F\n", nc, ctotal);
+			fflush(fp_log);
+		}
+	}
+	return 0;
+}
+
+d_complex_t lp_entry(const peec_t *p, int i, int j, double kw)
+{
+	return pgpu_lp_entry(pgpu_get(p), i, j, kw);
+}
+
+d_complex_t pot_entry(const peec_t *p, int i, int j, double kw)
+{
+	return pgpu_pot_entry(pgpu_get(p), i, j, kw);
+}
+
+double lp_self(double l, double a)
+{
+	return MU0 / (4 * PI) * neumann_self(l, a);
+}
+
+/* ---------------------------------------------------------------- sweep ---------------- */
+
+/* S = sqrt(z0_j / z0_i) (Z - Z0)(Z + Z0)^-1 on the nport x nport host block (solve.c:127-168);
+ * the port block is tiny, so it is eliminated here with the reference's pivoting rule. */
+static int z_to_s(peec_t *p, int ifreq)
+{
+	const int np = p->nport;
+	d_complex_t *m = (d_complex_t *)malloc((size_t)np * np * sizeof(d_complex_t));
+	d_complex_t *inv = (d_complex_t *)malloc((size_t)np * np * sizeof(d_complex_t));
+	int *piv = (int *)malloc((size_t)np * sizeof(int));
+	int rc = 0;
+	for (int i = 0; i < np; i++)
+		for (int j = 0; j < np; j++) {
+			d_complex_t v = p->zmat[ZIDX(p, ifreq, i, j)];
+			if (i == j) v = d_add(v, d_complex(p->port[i].z0, 0));
+			m[(size_t)i * np + j] = v;
+		}
+	for (int k = 0; k < np; k++) {
+		int ip = k;
+		double amax = fabs(m[(size_t)k * np + k].r) + fabs(m[(size_t)k * np + k].i);
+		for (int i = k + 1; i < np; i++) {
+			const double v = fabs(m[(size_t)i * np + k].r) + fabs(m[(size_t)i * np + k].i);
+			if (v > amax) {
+				amax = v;
+				ip = i;
+			}
+		}
+		piv[k] = ip;
+		if (amax < 1e-30) {
+			rc = 1;
+			goto out;
+		}
+		if (ip != k)
+			for (int j = 0; j < np; j++) {
+				const d_complex_t t = m[(size_t)k * np + j];
+				m[(size_t)k * np + j] = m[(size_t)ip * np + j];
+				m[(size_t)ip * np + j] = t;
+			}
+		const d_complex_t pinv = d_inv(m[(size_t)k * np + k]);
+		for (int i = k + 1; i < np; i++) {
+			const d_complex_t l = d_mul(m[(size_t)i * np + k], pinv);
+			m[(size_t)i * np + k] = l;
+			for (int j = k + 1; j < np; j++)
+				m[(size_t)i * np + j] = d_sub(m[(size_t)i * np + j], d_mul(l, m[(size_t)k * np + j]));
+		}
+	}
+	for (int c = 0; c < np; c++) {
+		d_complex_t *b = &inv[(size_t)c * np];   /* column c, stored contiguously */
+		for (int i = 0; i < np; i++) b[i] = d_complex(i == c ? 1 : 0, 0);
+		for (int k = 0; k < np; k++)
+			if (piv[k] != k) {
+				const d_complex_t t = b[k];
+				b[k] = b[piv[k]];
+				b[piv[k]] = t;
+			}
+		for (int i = 1; i < np; i++) {
+			d_complex_t s = b[i];
+			for (int j = 0; j < i; j++) s = d_sub(s, d_mul(m[(size_t)i * np + j], b[j]));
+			b[i] = s;
+		}
+		for (int i = np - 1; i >= 0; i--) {
+			d_complex_t s = b[i];
+			for (int j = i + 1; j < np; j++) s = d_sub(s, d_mul(m[(size_t)i * np + j], b[j]));
+			b[i] = d_mul(s, d_inv(m[(size_t)i * np + i]));
+		}
+	}
+	for (int i = 0; i < np; i++)
+		for (int j = 0; j < np; j++) {
+			d_complex_t s = d_complex(0, 0);
+			for (int k = 0; k < np; k++) {
+				d_complex_t nk = p->zmat[ZIDX(p, ifreq, i, k)];
+				if (i == k) nk = d_sub(nk, d_complex(p->port[i].z0, 0));
+				s = d_add(s, d_mul(nk, inv[(size_t)j * np + k]));
+			}
+			p->smat[ZIDX(p, ifreq, i, j)] = d_rmul(sqrt(p->port[j].z0 / p->port[i].z0), s);
+		}
+out:
+	free(m);
+	free(inv);
+	free(piv);
+	return rc;
+}
+
+int solve(peec_t *p, FILE *fp_log)
+{
+	const int np = p->nport;
+	if (p->compress) {
+		printf("%s\n", "*** compression = 1 : the compressed (H-matrix) solve is not available in this build");
+		return 1;
+	}
+	if (p->pw || (p->nsrc > 0)) {
+		printf("%s\n", "*** planewave / independent sources are not available in this build");
+		return 1;
+	}
+	const size_t nmat = (size_t)p->nfreq * np * np;
+	p->zin = (d_complex_t *)malloc((size_t)np * p->nfreq * sizeof(d_complex_t));
+	p->zmat = (d_complex_t *)malloc(nmat * sizeof(d_complex_t));
+	p->smat = (d_complex_t *)malloc(nmat * sizeof(d_complex_t));
+	if ((p->dist || (p->ffnth > 0)) && (p->nseg > 0))
+		p->segi = (d_complex_t *)malloc((size_t)p->nfreq * np * p->nseg * sizeof(d_complex_t));
+	if (p->dist && p->capacitance && (p->ncell > 0))
+		p->cellq = (d_complex_t *)malloc((size_t)p->nfreq * np * p->ncell * sizeof(d_complex_t));
+	if ((p->zin == NULL) || (p->zmat == NULL) || (p->smat == NULL)) {
+		printf("%s\n", "*** memory allocation error (matrix)");
+		return 1;
+	}
+	pgpu_t *g = pgpu_get(p);
+	for (int ifreq = 0; ifreq < p->nfreq; ifreq++) {
+		const double f = freq_at(p, ifreq);
+		const int rc = pgpu_sweep(g, p, ifreq, 1, &p->zmat[ZIDX(p, ifreq, 0, 0)],
+			p->segi ? &p->segi[DIDX(p, ifreq, 0, 0)] : NULL,
+			p->cellq ? &p->cellq[QIDX(p, ifreq, 0, 0)] : NULL, NULL);
+		if (rc == 1) {
+			if (fp_log) fprintf(fp_log, "*** singular potential coefficient matrix at f = %.5e Hz\n", f);
+			return 1;
+		}
+		if (rc) {
+			if (fp_log) fprintf(fp_log, "*** singular matrix at f = %.5e Hz\n", f);
+			return 1;
+		}
+		for (int j = 0; j < np; j++) p->zin[(size_t)j * p->nfreq + ifreq] = p->zmat[ZIDX(p, ifreq, j, j)];
+		if (z_to_s(p, ifreq)) {
+			printf("*** singular (Z + Z0) at f = %.5e Hz\n", f);
+			if (fp_log) fprintf(fp_log, "*** singular (Z + Z0) at f = %.5e Hz\n", f);
+			return 1;
+		}
+	}
+	return 0;
+}
+
+/* ---------------------------------------------------------------- extensions ----------- */
+
+const char *peec_mi355x_version(void)
+{
+	return "libpeec_mi355x 0.1 gfx950";
+}
+
+int peec_device_available(void)
+{
+	return pgpu_device_ok();
+}
+
+int peec_load_deck(const char *path, peec_t *p)
+{
+	FILE *fp = fopen(path, "r");
+	if (fp == NULL) {
+		printf("*** %s open error\n", path);
+		memset(p, 0, sizeof(peec_t));
+		return 1;
+	}
+	const int rc = input_data(fp, p);
+	fclose(fp);
+	return rc;
+}
+
+int peec_load_deck_string(const char *text, peec_t *p)
+{
+	FILE *fp = fmemopen((void *)text, strlen(text), "r");
+	if (fp == NULL) return 1;
+	const int rc = input_data(fp, p);
+	fclose(fp);
+	return rc;
+}
+
+int peec_setup(peec_t *p, FILE *fp_log)
+{
+	if (wire_build(p, fp_log)) return 1;
+	if (!p->retardation) {
+		if (!p->compress) lp_fill(p, 0, fp_log);
+		if (pot_fill(p, 0, fp_log)) return 1;
+	}
+	else if (p->capacitance && (p->nchg > 0) && pot_cells(p)) {
+		return 1;
+	}
+	return mna_numbering(p, fp_log);
+}
+
+int peec_sweep(peec_t *p, int ifreq0, int nfreq, int form, d_complex_t *zmat, peec_stats_t *st)
+{
+	if (st) {
+		memset(st, 0, sizeof(*st));
+		st->form = form;
+		st->ising = -1;
+	}
+	if (form != PEEC_FORM_PLAIN) {
+		printf("%s\n", "*** peec_sweep: unknown solver form");
+		return 4;
+	}
+	return pgpu_sweep(pgpu_get(p), p, ifreq0, nfreq, zmat, NULL, NULL, st);
+}
+
+size_t peec_sizeof_peec(void) { return sizeof(peec_t); }
+size_t peec_sizeof_seg(void) { return sizeof(seg_t); }

@@ -1,0 +1,123 @@
+/*
+ * peec_internal.h — internal C interface between the C host code (deck, geometry, MNA, sweep
+ * driver, ABI wrappers) and the HIP runtime layer (device mirror of a peec_t, fill kernels,
+ * dense LU, assembly).  Not part of the public ABI.
+ */
+#ifndef PEEC_MI355X_INTERNAL_H
+#define PEEC_MI355X_INTERNAL_H
+
+#include <stddef.h>
+#include "peec.h"
+#include "peec_mi355x.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------------------- */
+/* Packed cell record as the fill kernels see it (host-built, 256 bytes).                    */
+/* Every derived quantity is computed on the host with the reference's own expression, so a  */
+/* kernel reading t[] gets bit-for-bit the value the reference recomputes inline.            */
+typedef struct {
+	double x1[3];     /*  0 */
+	double x2[3];     /*  3 */
+	double d[3];      /*  6  x2 - x1                      */
+	double t[3];      /*  9  (x2 - x1) / len              */
+	double c[3];      /* 12  0.5 (x1 + x2)                */
+	double wv[3];     /* 15  ribbon transverse unit vector */
+	double nv[3];     /* 18  t x wv                        */
+	double len;       /* 21 */
+	double wid;       /* 22 */
+	double thick;     /* 23 */
+	double aL;        /* 24 */
+	double aP;        /* 25 */
+	double papex[3];  /* 26 */
+	int    npv;       /* 29 (4 ints in 2 doubles) */
+	int    poff;      /*    offset into the polygon pools */
+	int    vol;       /* 30 */
+	int    kind;      /*    CELL_WIRE / CELL_RIBBON / CELL_BAR / CELL_POLY */
+	double pad;       /* 31 */
+} dcell_t;
+
+#define CELL_WIRE   0
+#define CELL_RIBBON 1
+#define CELL_BAR    2
+#define CELL_POLY   3
+
+/* per-edge polygon data (edge k from pv[k] to pv[k+1]) for the Wilton edge sum */
+typedef struct {
+	double a[3];      /* start vertex                    */
+	double sh[3];     /* unit edge vector                */
+	double mh[3];     /* sh x nh (in-plane edge normal)  */
+	double el;        /* edge length                     */
+	double b[3];      /* end vertex                      */
+	double at;        /* area of fan triangle (papex, a, b) */
+} dedge_t;
+
+/* per-polygon unit normal (nh) and Newell norm n2 (n2 <= 0: degenerate) */
+typedef struct {
+	double nh[3];
+	double n2;
+} dpnorm_t;
+
+/* host-side packing (fill_pack.c) */
+int cells_pack(const seg_t *src, int n, int mirror, double gpz,
+	dcell_t **cells, dedge_t **edges, dpnorm_t **pnorm, int *nedge);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Device mirror of one peec_t (gpu_runtime.hip).                                            */
+typedef struct pgpu pgpu_t;
+
+pgpu_t *pgpu_get(const peec_t *p);          /* find or create; uploads cells on creation */
+void    peec_gpu_release(peec_t *p);        /* drop the mirror of p (called by peec_free) */
+int     pgpu_device_ok(void);               /* 1 if a HIP device is usable                */
+void    pgpu_fatal(const char *what);       /* print and abort: no host fallback exists   */
+
+/* Fill (results stay on the device; *_to_host copies them into the peec_t arrays).
+ * Return 0 on success. */
+int  pgpu_fill_lp(pgpu_t *g, double kw);
+int  pgpu_fill_pot(pgpu_t *g, double kw);   /* P (aggregated, scaled) into the device P buffer */
+int  pgpu_cmat(pgpu_t *g, int *ising);      /* C = P^-1 on the device; *ising = -1 or row     */
+void pgpu_lp_to_host(pgpu_t *g, d_complex_t *lp);
+void pgpu_cmat_to_host(pgpu_t *g, d_complex_t *cmat);
+void pgpu_lp_from_host(pgpu_t *g, const d_complex_t *lp);
+void pgpu_cmat_from_host(pgpu_t *g, const d_complex_t *cmat);
+
+/* single entries (lp_entry / pot_entry through the same device code) */
+d_complex_t pgpu_lp_entry(pgpu_t *g, int i, int j, double kw);
+d_complex_t pgpu_pot_entry(pgpu_t *g, int i, int j, double kw);
+
+/* sweep of frequency points, device resident: fills, assembles, factors, solves every port.
+ * Writes zmat (nport x nport per point), and optionally segi/cellq rows.  Returns 0 or an error
+ * code (1 singular P, 2 singular MNA). */
+typedef peec_stats_t pgpu_stats_t;
+
+int pgpu_sweep(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq, d_complex_t *zmat,
+	d_complex_t *segi, d_complex_t *cellq, pgpu_stats_t *st);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Dense complex LU on device buffers (lu.hip).  Matrices are row-major, leading dim lda.   */
+/* factor in place; dpiv[k] as lu_decomp; dperm (may be NULL) receives the row permutation
+ * (row k of the factors is original row dperm[k]).  Returns -1 or the singular step. */
+int  dlu_factor(void *stream, int n, d_complex_t *dA, int lda, int *dpiv, int *dperm, pgpu_stats_t *st);
+/* B <- A^-1 B (nrhs columns, row-major, leading dim ldb); dwork holds n * nrhs entries */
+void dlu_solve_perm(void *stream, int n, const d_complex_t *dA, int lda, const int *dperm,
+	d_complex_t *dB, int ldb, int nrhs, d_complex_t *dwork);
+/* C[M x N] -= A[M x K] B[K x N] (complex FP64 MFMA) */
+void dlu_zgemm_sub(void *stream, int M, int N, int K, const d_complex_t *A, long lda,
+	const d_complex_t *B, long ldb, d_complex_t *C, long ldc);
+
+/* MNA sparse stamps (host, mna_host.c), coalesced in the reference's summation order */
+typedef struct {
+	int n;                   /* number of distinct entries */
+	int *ri, *ci;
+	d_complex_t *val;
+} stamps_t;
+void mna_stamps(const peec_t *p, double f, stamps_t *st);
+void mna_stamps_free(stamps_t *st);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
