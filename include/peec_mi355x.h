@@ -52,6 +52,11 @@ int peec_setup(peec_t *p, FILE *fp_log);
  * 2 (singular MNA matrix) or another non-zero code. */
 int peec_sweep(peec_t *p, int ifreq0, int nfreq, int form, d_complex_t *zmat, peec_stats_t *st);
 
+/* Nominal FP64 operation count of the dense fills (Lp and charge-cell integrals, upper
+ * triangles with ground-plane images) at frequency f: the algorithmic work behind the fill
+ * GFLOP/s figure.  Host-side walk of the reference's branch tree; evaluates no integral. */
+double peec_fill_flops(const peec_t *p, double f);
+
 /* Sizes for bindings */
 size_t peec_sizeof_peec(void);
 size_t peec_sizeof_seg(void);

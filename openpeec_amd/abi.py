@@ -166,6 +166,7 @@ def declare(lib: C.CDLL, reference: bool = False) -> None:
             "peec_setup": (C.c_int, [PP, C.c_void_p]),
             "peec_sweep": (C.c_int, [PP, C.c_int, C.c_int, C.c_int, PZ, C.POINTER(peec_stats_t)]),
             "peec_sizeof_peec": (C.c_size_t, []),
+            "peec_fill_flops": (C.c_double, [PP, C.c_double]),
             "peec_sizeof_seg": (C.c_size_t, []),
         })
     for name, (res, args) in sig.items():

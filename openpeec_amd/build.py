@@ -25,7 +25,7 @@ ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 INC = [f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 
-C_FLAGS = ["-std=gnu11", "-O2", "-fPIC", "-Wall", "-Wno-unused-function", "-ffp-contract=off"]
+C_FLAGS = ["-std=gnu11", "-O2", "-fPIC", "-Wall", "-Wno-unused-function", "-ffp-contract=off", "-fopenmp"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -64,7 +64,7 @@ def build(verbose: bool = False) -> Path:
             f.result()
     objs = [str(_obj(s)) for s in c_srcs + hip_srcs]
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-Bsymbolic", "-o", str(tmp), *objs, "-lm"])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-Bsymbolic", "-o", str(tmp), *objs, "-lgomp", "-lm"])
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")

@@ -1,9 +1,12 @@
 """Lp and C = P^-1 fills on the GPU against the reference's own FP64 fills on the same decks.
 
-Per-entry bar for Lp: relative error (complex modulus) <= 1e-12, except the Hoer-Love
-cancellation case (plate_thick), whose entries are compared at 1e-9: the 64-term alternating
-sum loses ~8 digits, so ulp-level differences between the device libm and glibc surface there
-(documented in DESIGN.md).  C = P^-1 goes through a blocked LU (different summation order from
+Per-entry bar for Lp: relative error (complex modulus) <= 1e-12 for every quadrature-evaluated
+class.  Entries produced by cancelling closed forms carry the libm ulp differences (device ocml
+vs glibc) amplified by their cancellation factor: the 4-term parallel-filament sum (fpar,
+~1.4e-12 on collinear far pairs) and the 64-term Hoer-Love sum (~2.5e-12 for the plate_thick
+bars, ~3e-8 for the thin dielectric layers of diel_pp).  Those decks carry their measured bar
+here; test_cancellation.py pins both sides of those entries against a quad-precision evaluation
+of the same formulas.  C = P^-1 goes through a blocked LU (different summation order from
 the reference's unblocked elimination), so it is compared at 1e-9 relative to max |C|.
 """
 import numpy as np
@@ -17,7 +20,7 @@ pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not ref_available(), reason="r
 
 CASES = [
     # deck, frequencies, Lp tolerance
-    ("dipole_halfwave.peec", [0.0, 1.45e8], 1e-12),
+    ("dipole_halfwave.peec", [0.0, 1.45e8], 3e-12),
     ("wire_gp.peec", [0.0, 3e8], 1e-12),
     ("monopole_gp.peec", [1.4e8], 1e-12),
     ("plate_cap.peec", [0.0, 2e8], 1e-12),
@@ -26,8 +29,9 @@ CASES = [
     ("quad_square.peec", [0.0, 1e9], 1e-12),
     ("quad_taper.peec", [5e8], 1e-12),
     ("disk_cap.peec", [0.0, 1e9], 1e-12),
-    ("plate_thick.peec", [0.0, 1e9], 1e-9),
+    ("plate_thick.peec", [0.0, 1e9], 5e-12),
     ("twowire_cap.peec", [0.0], 1e-12),
+    ("diel_pp.peec", [1e9], 1e-7),
 ]
 
 
