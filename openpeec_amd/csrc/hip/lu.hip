@@ -27,7 +27,7 @@ namespace {
 
 constexpr int NB = 64;          // panel width (GEMM K)
 constexpr int PANEL_T = 256;    // threads per panel workgroup
-constexpr int PANEL_RMAX = 144; // panel rows per workgroup held in LDS (144 * 64 * 16 B = 144 KiB)
+constexpr int PANEL_RMAX = 144; // panel rows per workgroup held in LDS (144 * 65 * 16 B = 146 KiB)
 constexpr int PANEL_PMAX = 256; // workgroups in a panel launch (at most one per CU)
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -69,41 +69,6 @@ __device__ __forceinline__ void amax_merge(double &v, int &i, double ov, int oi)
 	}
 }
 
-__device__ void block_amax(double &v, int &i, int &w_out, double *sv, int *si, int *sw, int w)
-{
-	for (int off = 32; off > 0; off >>= 1) {
-		const double ov = __shfl_xor(v, off);
-		const int oi = __shfl_xor(i, off);
-		const int ow = __shfl_xor(w, off);
-		if ((ov > v) || ((ov == v) && (oi < i))) {
-			v = ov;
-			i = oi;
-			w = ow;
-		}
-	}
-	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-	if (lane == 0) {
-		sv[wave] = v;
-		si[wave] = i;
-		sw[wave] = w;
-	}
-	__syncthreads();
-	if (threadIdx.x == 0) {
-		for (int k = 1; k < (int)(blockDim.x >> 6); k++) {
-			if ((sv[k] > sv[0]) || ((sv[k] == sv[0]) && (si[k] < si[0]))) {
-				sv[0] = sv[k];
-				si[0] = si[k];
-				sw[0] = sw[k];
-			}
-		}
-	}
-	__syncthreads();
-	v = sv[0];
-	i = si[0];
-	w_out = sw[0];
-	__syncthreads();
-}
-
 struct panel_ws {
 	double  *pubv;      // [2][P]     local max |re|+|im|
 	int     *pubi;      // [2][P]     its global row
@@ -113,14 +78,21 @@ struct panel_ws {
 	int     *info;      // singular row or -1
 };
 
+constexpr int SROW = NB + 1;    // padded LDS row stride (complex): spreads row-per-thread access over banks
+
+// Panel factorisation.  Thread t of workgroup w owns panel row r0 + t (rows held in LDS with a
+// padded stride); per column: wave 0 finds and publishes the local candidate, one grid barrier,
+// wave 0 merges the P candidates and fetches the pivot row and the current diagonal row, then
+// every thread swaps/eliminates its own row.  Four workgroup barriers and two global round
+// trips per column.
 __global__ __launch_bounds__(PANEL_T) void k_panel(double2 *A, long lda, int n, int k0, int nb, int R, int *piv,
                                                     panel_ws ws, unsigned bar_base)
 {
-	extern __shared__ double2 S[];   // [R][NB]
-	__shared__ double2 U[NB], D[NB], L[PANEL_RMAX];
-	__shared__ double sv[4];
-	__shared__ int si[4], sw[4];
+	extern __shared__ double2 S[];   // [R][SROW]
+	__shared__ double2 U[NB], D[NB];
+	__shared__ int s_gi, s_stop;
 	const int P = gridDim.x, w = blockIdx.x, tid = threadIdx.x;
+	const int wave = tid >> 6, lane = tid & 63;
 	// a singular column in an earlier panel ends the factorisation: every workgroup reads the
 	// same flag (written before this launch) and leaves before the first barrier
 	if (*ws.info >= 0) return;
@@ -129,86 +101,100 @@ __global__ __launch_bounds__(PANEL_T) void k_panel(double2 *A, long lda, int n, 
 
 	for (int e = tid; e < rc * nb; e += PANEL_T) {
 		const int r = e / nb, c = e - r * nb;
-		S[r * NB + c] = A[(long)(r0 + r) * lda + k0 + c];
+		S[r * SROW + c] = A[(long)(r0 + r) * lda + k0 + c];
 	}
 	__syncthreads();
 
 	for (int jj = 0; jj < nb; jj++) {
 		const int kk = k0 + jj, par = jj & 1;
-		double bv = -1;
-		int bi = INT_MAX;
-		for (int r = tid; r < rc; r += PANEL_T) {
-			const int g = r0 + r;
-			if (g >= kk) {
-				const double2 a = S[r * NB + jj];
-				amax_merge(bv, bi, fabs(a.x) + fabs(a.y), g);
+		if (wave == 0) {
+			double bv = -1;
+			int bi = INT_MAX;
+			for (int r = lane; r < rc; r += 64) {
+				const int g = r0 + r;
+				if (g >= kk) {
+					const double2 a = S[r * SROW + jj];
+					amax_merge(bv, bi, fabs(a.x) + fabs(a.y), g);
+				}
 			}
+			for (int off = 32; off > 0; off >>= 1) {
+				const double ov = __shfl_xor(bv, off);
+				const int oi = __shfl_xor(bi, off);
+				amax_merge(bv, bi, ov, oi);
+			}
+			if (lane == 0) {
+				ws.pubv[par * P + w] = bv;
+				ws.pubi[par * P + w] = bi;
+			}
+			if ((bv >= 0) && (lane < nb)) ws.pubrow[((long)par * P + w) * NB + lane] = S[(bi - r0) * SROW + lane];
+			if ((kk >= r0) && (kk < r0 + rc) && (lane < nb)) ws.diagrow[par * NB + lane] = S[(kk - r0) * SROW + lane];
 		}
-		int dummy;
-		block_amax(bv, bi, dummy, sv, si, sw, w);
-		if (tid == 0) {
-			ws.pubv[par * P + w] = bv;
-			ws.pubi[par * P + w] = bi;
-		}
-		if ((bv >= 0) && (tid < nb)) ws.pubrow[((long)par * P + w) * NB + tid] = S[(bi - r0) * NB + tid];
-		if ((kk >= r0) && (kk < r0 + rc) && (tid < nb)) ws.diagrow[par * NB + tid] = S[(kk - r0) * NB + tid];
 		grid_barrier(ws.bar, bar_base + (unsigned)(P * (jj + 1)));
-
-		double gv = -1;
-		int gi = INT_MAX, gw = -1;
-		for (int q = tid; q < P; q += PANEL_T) {
-			const double qv = ws.pubv[par * P + q];
-			const int qi = ws.pubi[par * P + q];
-			if ((qv > gv) || ((qv == gv) && (qi < gi))) {
-				gv = qv;
-				gi = qi;
-				gw = q;
+		if (wave == 0) {
+			double gv = -1;
+			int gi = INT_MAX, gw = 0;
+			for (int q = lane; q < P; q += 64) {
+				const double qv = ws.pubv[par * P + q];
+				const int qi = ws.pubi[par * P + q];
+				if ((qv > gv) || ((qv == gv) && (qi < gi))) {
+					gv = qv;
+					gi = qi;
+					gw = q;
+				}
+			}
+			for (int off = 32; off > 0; off >>= 1) {
+				const double ov = __shfl_xor(gv, off);
+				const int oi = __shfl_xor(gi, off);
+				const int ow = __shfl_xor(gw, off);
+				if ((ov > gv) || ((ov == gv) && (oi < gi))) {
+					gv = ov;
+					gi = oi;
+					gw = ow;
+				}
+			}
+			const int stop = !(gv >= 1e-30);
+			if (!stop && (lane < nb)) {
+				U[lane] = ws.pubrow[((long)par * P + gw) * NB + lane];
+				D[lane] = ws.diagrow[par * NB + lane];
+			}
+			if (lane == 0) {
+				s_gi = gi;
+				s_stop = stop;
+				if (w == 0) {
+					piv[kk] = stop ? ((gi == INT_MAX) ? kk : gi) : gi;
+					if (stop) *ws.info = kk;
+				}
 			}
 		}
-		block_amax(gv, gi, gw, sv, si, sw, gw);
-		if (!(gv >= 1e-30)) {
-			if ((w == 0) && (tid == 0)) {
-				piv[kk] = (gi == INT_MAX) ? kk : gi;
-				*ws.info = kk;
+		__syncthreads();
+		if (s_stop) return;
+		const int gi = s_gi;
+		if (tid < rc) {
+			double2 *row = &S[tid * SROW];
+			const int g = r0 + tid;
+			if (g == kk) {
+				for (int c = 0; c < nb; c++) row[c] = U[c];
 			}
-			return;
-		}
-		if (tid < nb) {
-			U[tid] = ws.pubrow[((long)par * P + gw) * NB + tid];
-			D[tid] = ws.diagrow[par * NB + tid];
-		}
-		if ((w == 0) && (tid == 0)) piv[kk] = gi;
-		__syncthreads();
-		if (tid < nb) {
-			if ((gi != kk) && (gi >= r0) && (gi < r0 + rc)) S[(gi - r0) * NB + tid] = D[tid];
-			if ((kk >= r0) && (kk < r0 + rc)) S[(kk - r0) * NB + tid] = U[tid];
-		}
-		__syncthreads();
-		const double2 pinv = cinv(U[jj]);
-		const int rfirst = max(0, kk + 1 - r0);
-		for (int r = rfirst + tid; r < rc; r += PANEL_T) {
-			const double2 l = cmul(S[r * NB + jj], pinv);
-			S[r * NB + jj] = l;
-			L[r] = l;
-		}
-		__syncthreads();
-		const int ncol = nb - jj - 1;
-		if (ncol > 0) {
-			const int nrow = rc - rfirst;
-			for (int e = tid; e < nrow * ncol; e += PANEL_T) {
-				const int r = rfirst + e / ncol, c = jj + 1 + (e % ncol);
-				const double2 p = cmul(L[r], U[c]);
-				double2 a = S[r * NB + c];
-				a.x = a.x - p.x;
-				a.y = a.y - p.y;
-				S[r * NB + c] = a;
+			else if (g == gi) {
+				for (int c = 0; c < nb; c++) row[c] = D[c];
+			}
+			if (g > kk) {
+				const double2 l = cmul(row[jj], cinv(U[jj]));
+				row[jj] = l;
+				for (int c = jj + 1; c < nb; c++) {
+					const double2 p = cmul(l, U[c]);
+					double2 a = row[c];
+					a.x = a.x - p.x;
+					a.y = a.y - p.y;
+					row[c] = a;
+				}
 			}
 		}
 		__syncthreads();
 	}
 	for (int e = tid; e < rc * nb; e += PANEL_T) {
 		const int r = e / nb, c = e - r * nb;
-		A[(long)(r0 + r) * lda + k0 + c] = S[r * NB + c];
+		A[(long)(r0 + r) * lda + k0 + c] = S[r * SROW + c];
 	}
 }
 
@@ -260,24 +246,24 @@ __global__ __launch_bounds__(64) void k_trsm(const double2 *T, long ldt, double2
 	if (j >= ncol) return;
 	if (LOWER) {
 		for (int r = 1; r < nb; r++) {
-			double2 s = Sx[r * 64 + tid];
+			double2 sacc = Sx[r * 64 + tid];
 			for (int q = 0; q < r; q++) {
 				const double2 p = cmul(St[r * NB + q], Sx[q * 64 + tid]);
-				s.x -= p.x;
-				s.y -= p.y;
+				sacc.x -= p.x;
+				sacc.y -= p.y;
 			}
-			Sx[r * 64 + tid] = s;
+			Sx[r * 64 + tid] = sacc;
 		}
 	}
 	else {
 		for (int r = nb - 1; r >= 0; r--) {
-			double2 s = Sx[r * 64 + tid];
+			double2 sacc = Sx[r * 64 + tid];
 			for (int q = r + 1; q < nb; q++) {
 				const double2 p = cmul(St[r * NB + q], Sx[q * 64 + tid]);
-				s.x -= p.x;
-				s.y -= p.y;
+				sacc.x -= p.x;
+				sacc.y -= p.y;
 			}
-			Sx[r * 64 + tid] = cmul(s, cinv(St[r * NB + r]));
+			Sx[r * 64 + tid] = cmul(sacc, cinv(St[r * NB + r]));
 		}
 	}
 	for (int r = 0; r < nb; r++) X[(long)r * ldx + j] = Sx[r * 64 + tid];
@@ -285,9 +271,11 @@ __global__ __launch_bounds__(64) void k_trsm(const double2 *T, long ldt, double2
 
 // C[M x N] -= A[M x K] B[K x N] on FP64 MFMA.  Wave tile 32 x 64 (2 x 4 blocks of 16 x 16),
 // workgroup 2 x 2 waves = 64 x 128.  v_mfma_f64_16x16x4 lane layout (checked on gfx950 with an
-// in-bounds probe): A row = lane%16, k = lane/16; B col = lane%16, k = lane/16;
-// D register v of lane l holds row lane/16 + 4 v, col lane%16.
-__global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, const double2 *__restrict__ A, long lda,
+// in-bounds probe): A row = lane%16, k = lane/16; B col = lane%16, k = lane/16; D register v of
+// lane l holds row lane/16 + 4 v, col lane%16.  Complex = 4 real MFMAs into (re, im)
+// accumulators.  The fragments of step k+4 are loaded before the MFMAs of step k, so each
+// wave keeps its loads in flight behind 64 MFMAs.
+__global__ __launch_bounds__(256, 2) void k_zgemm(int M, int N, int K, const double2 *__restrict__ A, long lda,
                                                 const double2 *__restrict__ B, long ldb, double2 *__restrict__ C,
                                                 long ldc)
 {
@@ -295,36 +283,75 @@ __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, const double
 	const int row0 = blockIdx.y * 64 + (wave >> 1) * 32;
 	const int col0 = blockIdx.x * 128 + (wave & 1) * 64;
 	const int li = lane & 15, lk = lane >> 4;
+	const double2 *arow[2];
+	const double2 *bcol[4];
+	bool aok[2], bok[4];
+#pragma unroll
+	for (int rb = 0; rb < 2; rb++) {
+		const int r = row0 + rb * 16 + li;
+		aok[rb] = r < M;
+		arow[rb] = A + (long)(aok[rb] ? r : 0) * lda;
+	}
+#pragma unroll
+	for (int cb = 0; cb < 4; cb++) {
+		const int c = col0 + cb * 16 + li;
+		bok[cb] = c < N;
+		bcol[cb] = B + (bok[cb] ? c : 0);
+	}
 	d4 ar[2][4], ai[2][4];
+#pragma unroll
 	for (int a = 0; a < 2; a++)
+#pragma unroll
 		for (int b = 0; b < 4; b++) {
 			ar[a][b] = d4{0, 0, 0, 0};
 			ai[a][b] = d4{0, 0, 0, 0};
 		}
-	for (int k = 0; k < K; k += 4) {
-		const int kk = k + lk;
-		double2 fa[2], fb[4];
-		for (int rb = 0; rb < 2; rb++) {
-			const int r = row0 + rb * 16 + li;
-			fa[rb] = ((r < M) && (kk < K)) ? A[(long)r * lda + kk] : make_double2(0, 0);
-		}
-		for (int cb = 0; cb < 4; cb++) {
-			const int c = col0 + cb * 16 + li;
-			fb[cb] = ((kk < K) && (c < N)) ? B[(long)kk * ldb + c] : make_double2(0, 0);
-		}
-		for (int rb = 0; rb < 2; rb++) {
-			for (int cb = 0; cb < 4; cb++) {
-				ar[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[rb].x, fb[cb].x, ar[rb][cb], 0, 0, 0);
-				ar[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(-fa[rb].y, fb[cb].y, ar[rb][cb], 0, 0, 0);
-				ai[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[rb].x, fb[cb].y, ai[rb][cb], 0, 0, 0);
-				ai[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[rb].y, fb[cb].x, ai[rb][cb], 0, 0, 0);
-			}
-		}
+	double2 fa[2], fb[4];
+	{
+		const int kk = lk;
+#pragma unroll
+		for (int rb = 0; rb < 2; rb++) fa[rb] = (aok[rb] && (kk < K)) ? arow[rb][kk] : make_double2(0, 0);
+#pragma unroll
+		for (int cb = 0; cb < 4; cb++) fb[cb] = (bok[cb] && (kk < K)) ? bcol[cb][(long)kk * ldb] : make_double2(0, 0);
 	}
+#pragma unroll 1
+	for (int k = 0; k < K; k += 4) {
+		double2 na[2], nb4[4];
+		const int kk = k + 4 + lk;
+#pragma unroll
+		for (int rb = 0; rb < 2; rb++) na[rb] = (aok[rb] && (kk < K)) ? arow[rb][kk] : make_double2(0, 0);
+#pragma unroll
+		for (int cb = 0; cb < 4; cb++) nb4[cb] = (bok[cb] && (kk < K)) ? bcol[cb][(long)kk * ldb] : make_double2(0, 0);
+		// four passes over the 8 blocks so that the two MFMAs feeding one accumulator are 8
+		// instructions apart (no back-to-back dependency)
+#pragma unroll
+		for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+			for (int cb = 0; cb < 4; cb++) ar[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[rb].x, fb[cb].x, ar[rb][cb], 0, 0, 0);
+#pragma unroll
+		for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+			for (int cb = 0; cb < 4; cb++) ai[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[rb].x, fb[cb].y, ai[rb][cb], 0, 0, 0);
+#pragma unroll
+		for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+			for (int cb = 0; cb < 4; cb++) ar[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(-fa[rb].y, fb[cb].y, ar[rb][cb], 0, 0, 0);
+#pragma unroll
+		for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+			for (int cb = 0; cb < 4; cb++) ai[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[rb].y, fb[cb].x, ai[rb][cb], 0, 0, 0);
+#pragma unroll
+		for (int q = 0; q < 2; q++) fa[q] = na[q];
+#pragma unroll
+		for (int q = 0; q < 4; q++) fb[q] = nb4[q];
+	}
+#pragma unroll
 	for (int rb = 0; rb < 2; rb++) {
+#pragma unroll
 		for (int cb = 0; cb < 4; cb++) {
 			const int c = col0 + cb * 16 + li;
 			if (c >= N) continue;
+#pragma unroll
 			for (int v = 0; v < 4; v++) {
 				const int r = row0 + rb * 16 + lk + 4 * v;
 				if (r >= M) continue;
@@ -333,6 +360,85 @@ __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, const double
 				x.y -= ai[rb][cb][v];
 				C[(long)r * ldc + c] = x;
 			}
+		}
+	}
+}
+
+// Few right-hand sides (nrhs <= 4): triangular solve of one NB x NB diagonal block by a single
+// wavefront, lane r holding row r (its NB triangle entries in registers) and x[r]; the column
+// sweep broadcasts x[q] with a lane shuffle.  Unit lower when LOWER, else non-unit upper.
+template <bool LOWER>
+__global__ __launch_bounds__(64) void k_trsv(const double2 *__restrict__ T, long ldt, double2 *X, long ldx, int nb, int nrhs)
+{
+	const int r = threadIdx.x;
+	double2 t[NB];
+#pragma unroll
+	for (int q = 0; q < NB; q++) t[q] = ((r < nb) && (q < nb)) ? T[(long)r * ldt + q] : make_double2(0, 0);
+	const double2 dinv = (r < nb) ? cinv(T[(long)r * ldt + r]) : make_double2(0, 0);
+	for (int j = 0; j < nrhs; j++) {
+		double2 x = (r < nb) ? X[(long)r * ldx + j] : make_double2(0, 0);
+		if (LOWER) {
+#pragma unroll
+			for (int q = 0; q < NB; q++) {
+				const double2 xq = make_double2(__shfl(x.x, q), __shfl(x.y, q));
+				if (r > q) {
+					const double2 p = cmul(t[q], xq);
+					x.x -= p.x;
+					x.y -= p.y;
+				}
+			}
+		}
+		else {
+#pragma unroll
+			for (int q = NB - 1; q >= 0; q--) {
+				if (r == q) x = cmul(x, dinv);
+				const double2 xq = make_double2(__shfl(x.x, q), __shfl(x.y, q));
+				if (r < q) {
+					const double2 p = cmul(t[q], xq);
+					x.x -= p.x;
+					x.y -= p.y;
+				}
+			}
+		}
+		if (r < nb) X[(long)r * ldx + j] = x;
+	}
+}
+
+// Y[M x nrhs] -= A[M x K] X[K x nrhs] for K <= NB and nrhs <= 4: 16 lanes per row, 16 rows per
+// 256-thread workgroup, partial dot products reduced by shuffles.
+__global__ __launch_bounds__(256) void k_gemv(int M, int K, int nrhs, const double2 *__restrict__ A, long lda,
+                                               const double2 *__restrict__ Xv, long ldx, double2 *Y, long ldy)
+{
+	__shared__ double2 xs[NB * 4];
+	for (int e = threadIdx.x; e < K * nrhs; e += 256) {
+		const int k = e / nrhs, j = e - k * nrhs;
+		xs[k * 4 + j] = Xv[(long)k * ldx + j];
+	}
+	__syncthreads();
+	const int row = blockIdx.x * 16 + (threadIdx.x >> 4);
+	const int sub = threadIdx.x & 15;
+	double2 acc[4];
+	for (int j = 0; j < 4; j++) acc[j] = make_double2(0, 0);
+	if (row < M) {
+		for (int k = sub; k < K; k += 16) {
+			const double2 a = A[(long)row * lda + k];
+			for (int j = 0; j < nrhs; j++) {
+				const double2 p = cmul(a, xs[k * 4 + j]);
+				acc[j].x += p.x;
+				acc[j].y += p.y;
+			}
+		}
+	}
+	for (int j = 0; j < nrhs; j++) {
+		for (int off = 8; off > 0; off >>= 1) {
+			acc[j].x += __shfl_xor(acc[j].x, off);
+			acc[j].y += __shfl_xor(acc[j].y, off);
+		}
+		if ((row < M) && (sub == 0)) {
+			double2 y = Y[(long)row * ldy + j];
+			y.x -= acc[j].x;
+			y.y -= acc[j].y;
+			Y[(long)row * ldy + j] = y;
 		}
 	}
 }
@@ -375,7 +481,7 @@ void ensure_ws(int n)
 {
 	if (!g_lw.attr) {
 		HIPCHK(hipFuncSetAttribute((const void *)k_panel, hipFuncAttributeMaxDynamicSharedMemorySize,
-		                           PANEL_RMAX * NB * (int)sizeof(double2)));
+		                           PANEL_RMAX * SROW * (int)sizeof(double2)));
 		int dev = 0;
 		HIPCHK(hipGetDevice(&dev));
 		HIPCHK(hipDeviceGetAttribute(&g_lw.ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -460,7 +566,7 @@ extern "C" int dlu_factor(void *stream, int n, d_complex_t *dA_, int lda, int *d
 		void *kargs[] = {(void *)&Ap, (void *)&ldp, (void *)&nn, (void *)&kk0, (void *)&nbb,
 		                 (void *)&R, (void *)&pivp, (void *)&ws, (void *)&bb};
 		HIPCHK(hipLaunchCooperativeKernel((const void *)k_panel, dim3(P), dim3(PANEL_T), kargs,
-		                                  (unsigned)(R * NB * sizeof(double2)), s));
+		                                  (unsigned)(R * SROW * sizeof(double2)), s));
 		bar_base += (unsigned)(P * nb);
 		const int nout = n - nb;
 		hipLaunchKernelGGL(k_laswp, dim3((nout + 1 + 255) / 256), dim3(256), 0, s, A, ldal, n, k0, nb, dpiv, perm,
@@ -509,13 +615,19 @@ extern "C" void dlu_solve_perm(void *stream, int n, const d_complex_t *dA_, int 
 	                   dperm, n, nrhs);
 	hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, W, (long)nrhs, B, (long)ldb, n,
 	                   nrhs);
+	const bool few = (nrhs <= 4);
 	// forward: unit lower, block rows of NB
 	for (int r0 = 0; r0 < n; r0 += NB) {
 		const int nb = (n - r0 < NB) ? (n - r0) : NB;
-		hipLaunchKernelGGL(k_trsm<true>, dim3((nrhs + 63) / 64), dim3(64), 0, s, A + (long)r0 * lda + r0, (long)lda,
-		                   B + (long)r0 * ldb, (long)ldb, nb, nrhs);
+		const double2 *T = A + (long)r0 * lda + r0;
+		if (few) hipLaunchKernelGGL(k_trsv<true>, dim3(1), dim3(64), 0, s, T, (long)lda, B + (long)r0 * ldb, (long)ldb, nb, nrhs);
+		else hipLaunchKernelGGL(k_trsm<true>, dim3((nrhs + 63) / 64), dim3(64), 0, s, T, (long)lda, B + (long)r0 * ldb, (long)ldb, nb, nrhs);
 		const int rest = n - r0 - nb;
-		if (rest > 0)
+		if (rest <= 0) continue;
+		if (few)
+			hipLaunchKernelGGL(k_gemv, dim3((rest + 15) / 16), dim3(256), 0, s, rest, nb, nrhs, A + (long)(r0 + nb) * lda + r0,
+			                   (long)lda, B + (long)r0 * ldb, (long)ldb, B + (long)(r0 + nb) * ldb, (long)ldb);
+		else
 			launch_gemm(s, rest, nrhs, nb, A + (long)(r0 + nb) * lda + r0, lda, B + (long)r0 * ldb, ldb,
 			            B + (long)(r0 + nb) * ldb, ldb);
 	}
@@ -524,9 +636,14 @@ extern "C" void dlu_solve_perm(void *stream, int n, const d_complex_t *dA_, int 
 	for (int b = nblk - 1; b >= 0; b--) {
 		const int r0 = b * NB;
 		const int nb = (n - r0 < NB) ? (n - r0) : NB;
-		hipLaunchKernelGGL(k_trsm<false>, dim3((nrhs + 63) / 64), dim3(64), 0, s, A + (long)r0 * lda + r0, (long)lda,
-		                   B + (long)r0 * ldb, (long)ldb, nb, nrhs);
-		if (r0 > 0)
+		const double2 *T = A + (long)r0 * lda + r0;
+		if (few) hipLaunchKernelGGL(k_trsv<false>, dim3(1), dim3(64), 0, s, T, (long)lda, B + (long)r0 * ldb, (long)ldb, nb, nrhs);
+		else hipLaunchKernelGGL(k_trsm<false>, dim3((nrhs + 63) / 64), dim3(64), 0, s, T, (long)lda, B + (long)r0 * ldb, (long)ldb, nb, nrhs);
+		if (r0 <= 0) continue;
+		if (few)
+			hipLaunchKernelGGL(k_gemv, dim3((r0 + 15) / 16), dim3(256), 0, s, r0, nb, nrhs, A + r0, (long)lda,
+			                   B + (long)r0 * ldb, (long)ldb, B, (long)ldb);
+		else
 			launch_gemm(s, r0, nrhs, nb, A + r0, lda, B + (long)r0 * ldb, ldb, B, ldb);
 	}
 }

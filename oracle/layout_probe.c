@@ -14,7 +14,6 @@ int main(void)
 {
 	SZ(d_complex_t); SZ(rc_t); SZ(ind_t); SZ(mut_t); SZ(src_t); SZ(port_t);
 	SZ(wire_t); SZ(panel_t); SZ(plate_t); SZ(diel_t); SZ(seg_t); SZ(peec_t);
-	SZ(mna_sparse_t); SZ(hblock_t);
 	OF(seg_t, n1); OF(seg_t, x1); OF(seg_t, len); OF(seg_t, shape); OF(seg_t, wid);
 	OF(seg_t, wv); OF(seg_t, vol); OF(seg_t, npv); OF(seg_t, papex); OF(seg_t, pv);
 	OF(seg_t, aL); OF(seg_t, aP); OF(seg_t, area); OF(seg_t, res); OF(seg_t, diel);
