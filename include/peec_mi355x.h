@@ -29,6 +29,8 @@ typedef struct {
 	int    ising;           /* singular row of the last failure, or -1        */
 	int    refine_its;      /* refinement sweeps (mixed form)                 */
 	double berr;            /* worst backward error of the solutions (mixed)  */
+	int    n_lu;            /* factorisations performed (mixed form)          */
+	long   gmres_its;       /* GMRES iterations summed over systems (mixed)   */
 } peec_stats_t;
 
 /* Library identification: "libpeec_mi355x <version> gfx950". */
@@ -51,6 +53,13 @@ int peec_setup(peec_t *p, FILE *fp_log);
  * nfreq * nport * nport entries ([f][i][j]).  st may be NULL.  Returns 0, 1 (singular P),
  * 2 (singular MNA matrix) or another non-zero code. */
 int peec_sweep(peec_t *p, int ifreq0, int nfreq, int form, d_complex_t *zmat, peec_stats_t *st);
+
+/* As peec_sweep, with the mixed form's batch size (frequencies advanced in lock-step sharing one
+ * preconditioner; 1..8).  The mixed form keeps its preconditioner factorisation between calls. */
+int peec_sweep_ex(peec_t *p, int ifreq0, int nfreq, int form, int batch, d_complex_t *zmat, peec_stats_t *st);
+
+/* Drop the mixed form's cached preconditioner (the next mixed sweep factorises afresh). */
+void peec_sweep_reset(void);
 
 /* Nominal FP64 operation count of the dense fills (Lp and charge-cell integrals, upper
  * triangles with ground-plane images) at frequency f: the algorithmic work behind the fill

@@ -129,12 +129,12 @@ class Peec:
             return p.f0 * (p.f1 / p.f0) ** (ifreq / (p.nfreq - 1))
         return p.f0 + (p.f1 - p.f0) * ifreq / (p.nfreq - 1)
 
-    def sweep(self, ifreq0: int, nfreq: int, form: int = PEEC_FORM_PLAIN):
+    def sweep(self, ifreq0: int, nfreq: int, form: int = PEEC_FORM_PLAIN, batch: int = 4):
         np_ = self.p.nport
         z = np.zeros((nfreq, np_, np_), dtype=np.complex128)
         st = abi.peec_stats_t()
         zp = z.ctypes.data_as(C.POINTER(abi.d_complex_t))
-        rc = self.lib.peec_sweep(C.byref(self.p), ifreq0, nfreq, form, zp, C.byref(st))
+        rc = self.lib.peec_sweep_ex(C.byref(self.p), ifreq0, nfreq, form, batch, zp, C.byref(st))
         if rc != 0:
             raise RuntimeError(f"peec_sweep failed (code {rc})")
         return z, st

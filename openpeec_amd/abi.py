@@ -120,7 +120,7 @@ class peec_stats_t(C.Structure):
                 ("solve_ms", C.c_double), ("total_ms", C.c_double), ("lu_gemm_ms", C.c_double),
                 ("lu_gemm_flop", C.c_double), ("lu_gemm_launches", C.c_int),
                 ("fill_flop", C.c_double), ("ising", C.c_int), ("refine_its", C.c_int),
-                ("berr", C.c_double)]
+                ("berr", C.c_double), ("n_lu", C.c_int), ("gmres_its", C.c_long)]
 
 
 def declare(lib: C.CDLL, reference: bool = False) -> None:
@@ -165,6 +165,8 @@ def declare(lib: C.CDLL, reference: bool = False) -> None:
             "peec_load_deck_string": (C.c_int, [C.c_char_p, PP]),
             "peec_setup": (C.c_int, [PP, C.c_void_p]),
             "peec_sweep": (C.c_int, [PP, C.c_int, C.c_int, C.c_int, PZ, C.POINTER(peec_stats_t)]),
+            "peec_sweep_ex": (C.c_int, [PP, C.c_int, C.c_int, C.c_int, C.c_int, PZ, C.POINTER(peec_stats_t)]),
+            "peec_sweep_reset": (None, []),
             "peec_sizeof_peec": (C.c_size_t, []),
             "peec_fill_flops": (C.c_double, [PP, C.c_double]),
             "peec_sizeof_seg": (C.c_size_t, []),

@@ -281,6 +281,10 @@ void ensure_mna(pgpu *g, const peec_t *p)
 // ------------------------------------------------------------------------------------------
 extern "C" {
 
+void *pgpu_stream(pgpu_t *g) { return (void *)g->s; }
+const double2 *pgpu_lp_dev(pgpu_t *g) { return g->lp; }
+const double2 *pgpu_p_dev(pgpu_t *g) { return g->P; }
+
 void pgpu_fatal(const char *what)
 {
 	fprintf(stderr, "*** libpeec_mi355x: %s\n", what);

@@ -352,18 +352,27 @@ int peec_setup(peec_t *p, FILE *fp_log)
 	return mna_numbering(p, fp_log);
 }
 
-int peec_sweep(peec_t *p, int ifreq0, int nfreq, int form, d_complex_t *zmat, peec_stats_t *st)
+int peec_sweep_ex(peec_t *p, int ifreq0, int nfreq, int form, int batch, d_complex_t *zmat, peec_stats_t *st)
 {
 	if (st) {
 		memset(st, 0, sizeof(*st));
 		st->form = form;
 		st->ising = -1;
 	}
-	if (form != PEEC_FORM_PLAIN) {
-		printf("%s\n", "*** peec_sweep: unknown solver form");
-		return 4;
-	}
-	return pgpu_sweep(pgpu_get(p), p, ifreq0, nfreq, zmat, NULL, NULL, st);
+	if (form == PEEC_FORM_PLAIN) return pgpu_sweep(pgpu_get(p), p, ifreq0, nfreq, zmat, NULL, NULL, st);
+	if (form == PEEC_FORM_MIXED) return pgpu_sweep_fast(pgpu_get(p), p, ifreq0, nfreq, zmat, st, batch);
+	printf("%s\n", "*** peec_sweep: unknown solver form");
+	return 4;
+}
+
+int peec_sweep(peec_t *p, int ifreq0, int nfreq, int form, d_complex_t *zmat, peec_stats_t *st)
+{
+	return peec_sweep_ex(p, ifreq0, nfreq, form, 4, zmat, st);
+}
+
+void peec_sweep_reset(void)
+{
+	pgpu_fast_reset();
 }
 
 size_t peec_sizeof_peec(void) { return sizeof(peec_t); }

@@ -150,12 +150,14 @@ static void stamp(acc_t *a, int i, int j, d_complex_t v)
 	o->n++;
 }
 
-void mna_stamps(const peec_t *p, double f, stamps_t *st)
+/* stamps of assemble(skiplp = 1) (mna.c:175-297); charge_off >= 0 adds the charge-unknown
+ * couplings of the compressed formulation with the charges numbered from charge_off */
+static void stamps_any(const peec_t *p, double f, stamps_t *st, int charge_off)
 {
 	const double omega = 2 * PI * f;
 	const int *map = p->nodemap;
 	acc_t a;
-	acc_init(&a, p->nunknown, p->nnode + 2 * p->nres + 2 * p->ncap + 5 * p->nind + 5 * p->nseg + 4 * p->nsrc + 3 * p->ncell, st);
+	acc_init(&a, p->nunknown + p->ncell, p->nnode + 2 * p->nres + 2 * p->ncap + 5 * p->nind + 5 * p->nseg + 4 * p->nsrc + 3 * p->ncell, st);
 
 	if (p->gmin > 0)
 		for (int i = 0; i < p->nnode; i++) stamp(&a, i, i, d_complex(p->gmin, 0));
@@ -212,16 +214,28 @@ void mna_stamps(const peec_t *p, double f, stamps_t *st)
 		stamp(&a, rk, rk, d_rmul(-1, zint));
 	}
 	/* compressed capacitive form: KCL + jw q and charge-row -v couplings (mna.c:271-283) */
-	if (p->capacitance && (p->ncell > 0) && (p->offQ < p->nunknown)) {
+	if (p->capacitance && (p->ncell > 0) && (charge_off >= 0)) {
 		for (int i = 0; i < p->ncell; i++) {
 			const int mi = map[p->cellid[i]];
-			const int ri = p->offQ + i;
+			const int ri = charge_off + i;
 			stamp(&a, mi, ri, d_complex(0, omega));
 			stamp(&a, ri, mi, d_complex(-1, 0));
 		}
 	}
 	free(a.key);
 	free(a.slot);
+}
+
+void mna_stamps(const peec_t *p, double f, stamps_t *st)
+{
+	stamps_any(p, f, st, (p->offQ < p->nunknown) ? p->offQ : -1);
+}
+
+/* the dense numbering extended by one charge unknown per capacitive cell after the last
+ * branch current (charges at p->nunknown ...): the system of the accelerated sweep */
+void mna_stamps_charge(const peec_t *p, double f, stamps_t *st)
+{
+	stamps_any(p, f, st, p->nunknown);
 }
 
 void mna_stamps_free(stamps_t *st)

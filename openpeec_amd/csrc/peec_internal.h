@@ -95,6 +95,12 @@ typedef peec_stats_t pgpu_stats_t;
 int pgpu_sweep(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq, d_complex_t *zmat,
 	d_complex_t *segi, d_complex_t *cellq, pgpu_stats_t *st);
 
+/* accelerated sweep (sweep_fast.hip): charge-unknown system, stale FP32-stored LU as GMRES
+ * preconditioner, frequencies in lock-step batches of `batch` systems */
+int  pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq, d_complex_t *zmat,
+	pgpu_stats_t *st, int batch);
+void pgpu_fast_reset(void);
+
 /* ---------------------------------------------------------------------------------------- */
 /* Dense complex LU on device buffers (lu.hip).  Matrices are row-major, leading dim lda.   */
 /* factor in place; dpiv[k] as lu_decomp; dperm (may be NULL) receives the row permutation
