@@ -2,11 +2,17 @@
 //
 // Same physics and the same FP64 fills as the plain sweep; the solve changes:
 //   * charge-unknown formulation (the reference's compressed-path system, mna.c:86-107, written
-//     densely): unknowns [nodes | L | V | branch currents | cell charges], P enters as a block,
-//     so no C = P^-1 is formed per frequency;
+//     with dense kernel blocks): unknowns [nodes | L | V | branch currents | cell charges], P
+//     enters as a block, so no C = P^-1 is formed per frequency.  The product A x is applied
+//     block-wise as the reference's mna_apply does (mna.c:338-364): sparse stamps (CSR) plus
+//     -jw Lp on the branch-current block plus P on the charge block, so each GMRES product
+//     streams Lp and P once (1.4 GB on the 64 x 64 plate) instead of a dense n^2 matrix;
 //   * one LU of A(f_ref) serves a run of frequencies as the right preconditioner of FP64
-//     restarted GMRES (the reference's acceleration = 1 scheme, solve.c:47-64 / iterative.c:
-//     230-252), factors stored in FP32 (half the bytes per preconditioner application);
+//     flexible GMRES (the reference's acceleration = 1 scheme, solve.c:47-64 / iterative.c:
+//     230-252); its off-diagonal blocks are stored in FP32, its diagonal blocks as FP64
+//     inverses.  Flexible GMRES keeps the preconditioned directions, so the preconditioner may
+//     be applied inexactly (FP32 data, any summation order) without limiting the final
+//     residual;
 //   * frequencies advance in lock-step batches of F systems sharing the preconditioner, so each
 //     latency-bound triangular sweep serves F right-hand sides at once;
 //   * GMRES runs to a relative residual of 1e-10 (ACCEL_TOL, solve.c:21), checked on the true
@@ -49,18 +55,6 @@ __device__ __forceinline__ double2 zinv(double2 z)
 	return make_double2(z.x / n2, -z.y / n2);
 }
 
-__device__ __forceinline__ void grid_sync(unsigned *count, unsigned target)
-{
-	__syncthreads();
-	if (threadIdx.x == 0) {
-		__threadfence();
-		atomicAdd(count, 1u);
-		while (__hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) __builtin_amdgcn_s_sleep(1);
-		__threadfence();
-	}
-	__syncthreads();
-}
-
 // ---------------------------------------------------------------- assembly
 
 // dense blocks of the charge-unknown system: -jw Lp at [offS, offS+nseg)^2, P at [offQ, offQ+nc)^2
@@ -98,31 +92,118 @@ __global__ void k_demote(const double2 *src, float2 *dst, long count)
 	dst[e] = make_float2((float)v.x, (float)v.y);
 }
 
+// Inverses of the TB x TB diagonal blocks of the packed LU factors (unit lower when LOWER,
+// else upper), FP64.  One workgroup per block; thread c solves T y = e_c in LDS.  Rows past
+// the end of a short last block act as identity.
+template <bool LOWER>
+__global__ __launch_bounds__(64) void k_diaginv(const double2 *A, long lda, int n, double2 *dinv)
+{
+	__shared__ double2 T[TB * TB];
+	__shared__ double2 Y[TB * TB];
+	const int b = blockIdx.x, c = threadIdx.x;
+	const int r0 = b * TB, nb = min(TB, n - r0);
+	for (int r = 0; r < TB; r++) {
+		double2 v = make_double2((r == c) ? 1.0 : 0.0, 0);
+		if ((r < nb) && (c < nb)) v = A[(long)(r0 + r) * lda + r0 + c];
+		T[r * TB + c] = v;
+	}
+	__syncthreads();
+	if (LOWER) {
+		for (int r = 0; r < TB; r++) {
+			double2 acc = make_double2((r == c) ? 1.0 : 0.0, 0);
+			for (int q = 0; q < r; q++) {
+				const double2 p = zmul(T[r * TB + q], Y[q * TB + c]);
+				acc.x -= p.x;
+				acc.y -= p.y;
+			}
+			Y[r * TB + c] = acc;
+		}
+	}
+	else {
+		for (int r = TB - 1; r >= 0; r--) {
+			double2 acc = make_double2((r == c) ? 1.0 : 0.0, 0);
+			for (int q = r + 1; q < TB; q++) {
+				const double2 p = zmul(T[r * TB + q], Y[q * TB + c]);
+				acc.x -= p.x;
+				acc.y -= p.y;
+			}
+			Y[r * TB + c] = zmul(acc, zinv(T[r * TB + r]));
+		}
+	}
+	__syncthreads();
+	for (int r = 0; r < TB; r++) dinv[((long)b * TB + r) * TB + c] = Y[r * TB + c];
+}
+
 // ---------------------------------------------------------------- batched dense products
 
-struct mats {
-	const double2 *a[FMAX];
+// One system of the batch: CSR stamps and its dense kernel blocks.
+struct slot {
+	const int *rp;          // n + 1
+	const int *ci;
+	const double2 *val;
+	const double2 *lp;      // nseg x nseg (kernel Lp, scaled by -jw on the fly)
+	const double2 *P;       // nc x nc
+	double omega;
+};
+struct slots {
+	slot s[FMAX];
 };
 
-// Y[:, f] = A_f X[:, f]  (vectors stored n x F, row-major); 64 lanes per row, 4 rows per block
-__global__ __launch_bounds__(256) void k_gemv(mats M, int n, int F, const double2 *__restrict__ X, double2 *__restrict__ Y)
+// Y[:, f] = A_f X[:, f] (vectors n x F row-major): one wave per row; the dense part is summed
+// lane-strided over the block's columns, two 16-byte loads in flight per lane.
+__global__ __launch_bounds__(256) void k_matvec(slots S, int n, int F, int offS, int nseg, int offQ, int nc,
+                                                const double2 *__restrict__ X, double2 *__restrict__ Y)
 {
 	const int f = blockIdx.y;
 	const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
 	const int lane = threadIdx.x & 63;
 	if (row >= n) return;
-	const double2 *a = M.a[f] + (long)row * n;
+	const slot &sl = S.s[f];
 	double sr = 0, si = 0;
-	for (int c = lane; c < n; c += 64) {
-		const double2 p = zmul(a[c], X[(long)c * F + f]);
-		sr += p.x;
-		si += p.y;
+	const double2 *a = nullptr;
+	int c0 = 0, nn = 0;
+	const bool lrow = (row >= offS) && (row < offS + nseg);
+	if (lrow) {
+		a = sl.lp + (long)(row - offS) * nseg;
+		c0 = offS;
+		nn = nseg;
 	}
-	for (int off = 32; off > 0; off >>= 1) {
-		sr += __shfl_xor(sr, off);
-		si += __shfl_xor(si, off);
+	else if ((row >= offQ) && (row < offQ + nc)) {
+		a = sl.P + (long)(row - offQ) * nc;
+		c0 = offQ;
+		nn = nc;
 	}
-	if (lane == 0) Y[(long)row * F + f] = make_double2(sr, si);
+	if (a) {
+		int c = lane;
+		for (; c + 64 < nn; c += 128) {
+			const double2 m0 = a[c], m1 = a[c + 64];
+			const double2 x0 = X[(long)(c0 + c) * F + f], x1 = X[(long)(c0 + c + 64) * F + f];
+			const double2 p0 = zmul(m0, x0), p1 = zmul(m1, x1);
+			sr += p0.x;
+			si += p0.y;
+			sr += p1.x;
+			si += p1.y;
+		}
+		if (c < nn) {
+			const double2 p = zmul(a[c], X[(long)(c0 + c) * F + f]);
+			sr += p.x;
+			si += p.y;
+		}
+		for (int off = 32; off > 0; off >>= 1) {
+			sr += __shfl_xor(sr, off);
+			si += __shfl_xor(si, off);
+		}
+	}
+	if (lane == 0) {
+		double2 y = make_double2(sr, si);
+		if (lrow) y = make_double2(sl.omega * si, -sl.omega * sr);   // -jw * sum
+		for (int e = sl.rp[row]; e < sl.rp[row + 1]; e++) {
+			const double2 p = zmul(sl.val[e], X[(long)sl.ci[e] * F + f]);
+			y.x += p.x;
+			y.y += p.y;
+		}
+		Y[(long)row * F + f] = y;
+	}
 }
 
 // Y = B - Y (residual after a product), all columns
@@ -227,6 +308,18 @@ __global__ void k_add(double2 *X, const double2 *D, long cnt)
 	X[e] = make_double2(X[e].x + D[e].x, X[e].y + D[e].y);
 }
 
+__global__ void k_zero(double2 *X, long cnt)
+{
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e < cnt) X[e] = make_double2(0, 0);
+}
+
+__global__ void k_zero_u32(unsigned *x, int cnt)
+{
+	const int e = blockIdx.x * blockDim.x + threadIdx.x;
+	if (e < cnt) x[e] = 0;
+}
+
 // dst[k][f] = src[perm[k]][f]
 __global__ void k_permute(const double2 *src, double2 *dst, const int *perm, int n, int F)
 {
@@ -237,88 +330,112 @@ __global__ void k_permute(const double2 *src, double2 *dst, const int *perm, int
 }
 
 // ---------------------------------------------------------------- preconditioner sweeps
-// X <- T^-1 X for the FP32-stored LU factors (unit lower when LOWER, else upper), X n x F in
-// FP64.  Cooperative: one grid barrier per TB-row block.  Every workgroup solves the current
-// diagonal block redundantly in LDS (wave 0, lane r = row r), the owner of that block row
-// stores it, then each workgroup updates the block rows it owns (block row i belongs to
-// workgroup i % G) that are still to be solved.
+// X <- T^-1 X for the packed LU factors (unit lower when LOWER, else upper), X n x F in FP64.
+// Left-looking by TB-row blocks: a workgroup takes the next block in solve order from a ticket
+// counter, accumulates the products with every block already solved as each one's flag is
+// published, applies the FP64 inverse of its diagonal block and publishes its own flag.  A
+// workgroup waits only for blocks whose tickets were taken before its own, by workgroups that
+// are already running, so every wait ends without needing co-resident workgroups.
+// Thread (r, q): row r of the block, column quarter q (16 columns).
 template <bool LOWER>
-__global__ __launch_bounds__(256) void k_tri(const float2 *__restrict__ M, long ld, int n, double2 *X, int F,
-                                              unsigned *bar, unsigned base)
+__global__ __launch_bounds__(256) void k_trisolve(const float2 *__restrict__ M, long ld, int n, const double2 *__restrict__ dinv,
+                                                  double2 *X, int F, unsigned *ctl, unsigned epoch)
 {
-	__shared__ double2 xb[TB * FMAX];
-	__shared__ float2 Tb[TB * (TB + 1)];
-	const int G = gridDim.x, w = blockIdx.x, tid = threadIdx.x;
+	__shared__ double2 xs[TB * FMAX];
+	__shared__ double2 ys[TB * FMAX];
+	__shared__ int sb;
+	unsigned *ticket = ctl;
+	unsigned *flags = ctl + 1;
 	const int nblk = (n + TB - 1) / TB;
-	for (int s = 0; s < nblk; s++) {
-		const int b = LOWER ? s : (nblk - 1 - s);
-		const int r0 = b * TB;
-		const int nb = min(TB, n - r0);
-		for (int e = tid; e < nb * nb; e += 256) {
-			const int r = e / nb, c = e - r * nb;
-			Tb[r * (TB + 1) + c] = M[(long)(r0 + r) * ld + r0 + c];
+	const int tid = threadIdx.x, r = tid >> 2, q = tid & 3;
+	if (tid == 0) sb = (int)atomicAdd(ticket, 1u);
+	__syncthreads();
+	const int t = sb;
+	const int b = LOWER ? t : (nblk - 1 - t);
+	const int r0 = b * TB, nb = min(TB, n - r0);
+	double ar[FMAX], ai[FMAX];
+#pragma unroll
+	for (int f = 0; f < FMAX; f++) ar[f] = ai[f] = 0;
+	for (int s = 0; s < t; s++) {
+		const int j = LOWER ? s : (nblk - 1 - s);
+		const int c0 = j * TB, ncj = min(TB, n - c0);
+		if (tid == 0) {
+			while (__hip_atomic_load(&flags[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) __builtin_amdgcn_s_sleep(1);
+			__threadfence();
 		}
-		for (int e = tid; e < nb * F; e += 256) xb[e] = X[(long)r0 * F + e];
 		__syncthreads();
-		if (tid < 64) {
-			const int r = tid;
-			for (int f = 0; f < F; f++) {
-				double2 x = (r < nb) ? xb[r * F + f] : make_double2(0, 0);
-				if (LOWER) {
-					for (int q = 0; q < nb; q++) {
-						const double2 xq = make_double2(__shfl(x.x, q), __shfl(x.y, q));
-						if (r > q) {
-							const float2 t = Tb[r * (TB + 1) + q];
-							const double2 p = zmul(make_double2(t.x, t.y), xq);
-							x.x -= p.x;
-							x.y -= p.y;
-						}
+		for (int e = tid; e < ncj * F; e += 256) xs[e] = X[(long)c0 * F + e];
+		__syncthreads();
+		if (r < nb) {
+			const float2 *mrow = M + (long)(r0 + r) * ld + c0 + q * 16;
+			const int cend = min(16, ncj - q * 16);
+			for (int cc = 0; cc < cend; cc++) {
+				const float2 m = mrow[cc];
+				const double mr = m.x, mi = m.y;
+				const double2 *xc = &xs[(q * 16 + cc) * F];
+#pragma unroll
+				for (int f = 0; f < FMAX; f++) {
+					if (f < F) {
+						const double2 x = xc[f];
+						ar[f] += (mr * x.x) - (mi * x.y);
+						ai[f] += (mr * x.y) + (mi * x.x);
 					}
 				}
-				else {
-					for (int q = nb - 1; q >= 0; q--) {
-						if (r == q) {
-							const float2 t = Tb[r * (TB + 1) + r];
-							x = zmul(x, zinv(make_double2(t.x, t.y)));
-						}
-						const double2 xq = make_double2(__shfl(x.x, q), __shfl(x.y, q));
-						if (r < q) {
-							const float2 t = Tb[r * (TB + 1) + q];
-							const double2 p = zmul(make_double2(t.x, t.y), xq);
-							x.x -= p.x;
-							x.y -= p.y;
-						}
-					}
-				}
-				if (r < nb) xb[r * F + f] = x;
 			}
 		}
 		__syncthreads();
-		// update own block rows still to be solved
-		for (int i = w; i < nblk; i += G) {
-			if (LOWER ? (i <= b) : (i >= b)) continue;
-			const int q0 = i * TB;
-			const int nq = min(TB, n - q0);
-			for (int e = tid; e < nq * F; e += 256) {
-				const int r = e / F, f = e - r * F;
-				const float2 *mrow = M + (long)(q0 + r) * ld + r0;
-				double sr = 0, si = 0;
-				for (int c = 0; c < nb; c++) {
-					const float2 t = mrow[c];
-					const double2 xv = xb[c * F + f];
-					sr += ((double)t.x * xv.x) - ((double)t.y * xv.y);
-					si += ((double)t.x * xv.y) + ((double)t.y * xv.x);
+	}
+	// combine the four column quarters of each row, form the right-hand side of the block
+#pragma unroll
+	for (int f = 0; f < FMAX; f++) {
+		ar[f] += __shfl_xor(ar[f], 1);
+		ai[f] += __shfl_xor(ai[f], 1);
+		ar[f] += __shfl_xor(ar[f], 2);
+		ai[f] += __shfl_xor(ai[f], 2);
+	}
+	if (q == 0) {
+		for (int f = 0; f < F; f++) {
+			double2 y = make_double2(0, 0);
+			if (r < nb) {
+				const double2 x = X[(long)(r0 + r) * F + f];
+				y = make_double2(x.x - ar[f], x.y - ai[f]);
+			}
+			ys[r * F + f] = y;
+		}
+	}
+	__syncthreads();
+	// x_b = Dinv_b y
+	double br[FMAX], bi[FMAX];
+#pragma unroll
+	for (int f = 0; f < FMAX; f++) br[f] = bi[f] = 0;
+	{
+		const double2 *drow = dinv + ((long)b * TB + r) * TB + q * 16;
+		for (int cc = 0; cc < 16; cc++) {
+			const double2 d = drow[cc];
+			const double2 *yc = &ys[(q * 16 + cc) * F];
+#pragma unroll
+			for (int f = 0; f < FMAX; f++) {
+				if (f < F) {
+					const double2 y = yc[f];
+					br[f] += (d.x * y.x) - (d.y * y.y);
+					bi[f] += (d.x * y.y) + (d.y * y.x);
 				}
-				double2 *xo = &X[(long)(q0 + r) * F + f];
-				*xo = make_double2(xo->x - sr, xo->y - si);
 			}
 		}
-		grid_sync(bar, base + (unsigned)(G * (s + 1)));
-		// the solved block goes back to X only now: before the barrier other workgroups may still
-		// be reading its unsolved right-hand side
-		if ((b % G) == w)
-			for (int e = tid; e < nb * F; e += 256) X[(long)r0 * F + e] = xb[e];
-		__syncthreads();
+	}
+#pragma unroll
+	for (int f = 0; f < FMAX; f++) {
+		br[f] += __shfl_xor(br[f], 1);
+		bi[f] += __shfl_xor(bi[f], 1);
+		br[f] += __shfl_xor(br[f], 2);
+		bi[f] += __shfl_xor(bi[f], 2);
+	}
+	if ((q == 0) && (r < nb))
+		for (int f = 0; f < F; f++) X[(long)(r0 + r) * F + f] = make_double2(br[f], bi[f]);
+	__syncthreads();
+	if (tid == 0) {
+		__threadfence();
+		__hip_atomic_store(&flags[b], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 	}
 }
 
@@ -337,41 +454,43 @@ unsigned blocks(long n, int t = 256)
 
 // ---------------------------------------------------------------- host-side context
 
+struct slotbuf {
+	double2 *lp = nullptr, *P = nullptr;
+	int *rp = nullptr, *ci = nullptr, *ri = nullptr;
+	double2 *val = nullptr;
+	int cap = 0, nnz = 0;
+	double omega = 0;
+};
+
 struct fast {
-	int n = 0, nc = 0, F = 0;
-	double2 *A[FMAX] = {};
+	int n = 0, nc = 0, F = 0, nseg = 0;
+	slotbuf sl[FMAX];
 	double2 *luw = nullptr;   // FP64 factorisation workspace
 	float2 *M = nullptr;      // FP32 factors
+	double2 *dinvL = nullptr, *dinvU = nullptr;
 	int *piv = nullptr, *perm = nullptr;
 	double fref = -1;
-	double2 *V = nullptr;     // (MKRYLOV + 1) x n x F
+	double2 *V = nullptr;     // (MKRYLOV + 1) x n x F Arnoldi basis
+	double2 *Z = nullptr;     // MKRYLOV x n x F preconditioned directions
 	double2 *W = nullptr, *Xv = nullptr, *Bv = nullptr, *R = nullptr, *T1 = nullptr;
 	double2 *partial = nullptr, *h = nullptr, *yd = nullptr;
 	double *scale = nullptr;
-	unsigned *bar = nullptr;
-	unsigned barbase = 0;
-	int ncu = 0;
-	int *st_ri = nullptr, *st_ci = nullptr;
-	double2 *st_v = nullptr;
-	int st_cap = 0;
-	double2 *lp = nullptr, *P = nullptr;
+	unsigned *ctl = nullptr;  // ticket + per-block flags of the triangular sweeps
+	unsigned epoch = 0;
 };
 
 void tri_solve(fast &c, hipStream_t s, double2 *X, int F)
 {
 	const int nblk = (c.n + TB - 1) / TB;
-	int G = std::min(c.ncu, std::max(1, nblk));
-	if (G > 128) G = 128;
 	for (int pass = 0; pass < 2; pass++) {
-		const float2 *Mp = c.M;
-		long ld = c.n;
-		int nn = c.n, FF = F;
-		unsigned *bp = c.bar;
-		unsigned bb = c.barbase;
-		void *args[] = {(void *)&Mp, (void *)&ld, (void *)&nn, (void *)&X, (void *)&FF, (void *)&bp, (void *)&bb};
-		const void *fn = pass == 0 ? (const void *)k_tri<true> : (const void *)k_tri<false>;
-		HIPCHK(hipLaunchCooperativeKernel(fn, dim3(G), dim3(256), args, 0, s));
-		c.barbase += (unsigned)(G * nblk);
+		c.epoch++;
+		hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, c.ctl, 1);
+		if (pass == 0)
+			hipLaunchKernelGGL(k_trisolve<true>, dim3(nblk), dim3(256), 0, s, (const float2 *)c.M, (long)c.n, c.n,
+			                   (const double2 *)c.dinvL, X, F, c.ctl, c.epoch);
+		else
+			hipLaunchKernelGGL(k_trisolve<false>, dim3(nblk), dim3(256), 0, s, (const float2 *)c.M, (long)c.n, c.n,
+			                   (const double2 *)c.dinvU, X, F, c.ctl, c.epoch);
 	}
 }
 
@@ -382,11 +501,16 @@ void precond(fast &c, hipStream_t s, const double2 *v, double2 *z, int F)
 	tri_solve(c, s, z, F);
 }
 
+int g_offS = 0, g_offQ = 0;
+
 void matvec(fast &c, hipStream_t s, const double2 *x, double2 *y, int F)
 {
-	mats m;
-	for (int f = 0; f < FMAX; f++) m.a[f] = (f < F) ? c.A[f] : nullptr;
-	hipLaunchKernelGGL(k_gemv, dim3((c.n + 3) / 4, F), dim3(256), 0, s, m, c.n, F, x, y);
+	slots S;
+	for (int f = 0; f < FMAX; f++) {
+		const slotbuf &b = c.sl[(f < F) ? f : 0];
+		S.s[f] = slot{b.rp, b.ci, b.val, b.lp, b.P, b.omega};
+	}
+	hipLaunchKernelGGL(k_matvec, dim3((c.n + 3) / 4, F), dim3(256), 0, s, S, c.n, F, g_offS, c.nseg, g_offQ, c.nc, x, y);
 }
 
 // h[k][f] = V_k^H W for k < nk, copied to host
@@ -400,8 +524,8 @@ void dots(fast &c, hipStream_t s, const double2 *V, long vstride, int nk, const 
 	HIPCHK(hipStreamSynchronize(s));
 }
 
-// batched restarted GMRES on F systems A_f x_f = b_f (b in c.Bv, x in c.Xv); returns per-system
-// iteration counts, -1 for a system that did not converge
+// batched restarted flexible GMRES on F systems A_f x_f = b_f (b in c.Bv, x in c.Xv); returns
+// per-system iteration counts, -1 for a system that did not converge
 std::vector<int> gmres_batch(fast &c, hipStream_t s, int F, int nactive)
 {
 	const int n = c.n;
@@ -414,7 +538,7 @@ std::vector<int> gmres_batch(fast &c, hipStream_t s, int F, int nactive)
 	std::vector<double2> hh;
 	dots(c, s, c.Bv, 0, 1, c.Bv, F, hh);
 	for (int f = 0; f < F; f++) bnorm[f] = std::sqrt(hh[f].x);
-	HIPCHK(hipMemsetAsync(c.Xv, 0, nf * sizeof(double2), s));
+	hipLaunchKernelGGL(k_zero, dim3(blocks(nf)), dim3(256), 0, s, c.Xv, nf);
 	for (int cyc = 0; cyc < MCYCLES; cyc++) {
 		// r = b - A x
 		matvec(c, s, c.Xv, c.R, F);
@@ -441,8 +565,9 @@ std::vector<int> gmres_batch(fast &c, hipStream_t s, int F, int nactive)
 		int j;
 		for (j = 0; j < MKRYLOV; j++) {
 			double2 *vj = c.V + (long)j * vstride;
-			precond(c, s, vj, c.T1, F);
-			matvec(c, s, c.T1, c.W, F);
+			double2 *zj = c.Z + (long)j * vstride;
+			precond(c, s, vj, zj, F);
+			matvec(c, s, zj, c.W, F);
 			std::vector<double2> h1, h2;
 			dots(c, s, c.V, vstride, j + 1, c.W, F, h1);
 			HIPCHK(hipMemcpyAsync(c.h, h1.data(), h1.size() * sizeof(double2), hipMemcpyHostToDevice, s));
@@ -488,7 +613,7 @@ std::vector<int> gmres_batch(fast &c, hipStream_t s, int F, int nactive)
 				break;
 			}
 		}
-		// y = H^-1 g per system; x += M^-1 V y
+		// y = H^-1 g per system; x += Z y
 		const int kmax = std::min(j, MKRYLOV);
 		std::vector<double2> yv((size_t)kmax * F, make_double2(0, 0));
 		for (int f = 0; f < F; f++) {
@@ -504,8 +629,7 @@ std::vector<int> gmres_batch(fast &c, hipStream_t s, int F, int nactive)
 		}
 		if (kmax > 0) {
 			HIPCHK(hipMemcpyAsync(c.yd, yv.data(), yv.size() * sizeof(double2), hipMemcpyHostToDevice, s));
-			hipLaunchKernelGGL(k_combine, dim3(blocks(nf)), dim3(256), 0, s, c.V, vstride, kmax, c.yd, c.T1, n, F);
-			precond(c, s, c.T1, c.W, F);
+			hipLaunchKernelGGL(k_combine, dim3(blocks(nf)), dim3(256), 0, s, c.Z, vstride, kmax, c.yd, c.W, n, F);
 			hipLaunchKernelGGL(k_add, dim3(blocks(nf)), dim3(256), 0, s, c.Xv, c.W, nf);
 		}
 	}
@@ -519,6 +643,55 @@ std::vector<int> gmres_batch(fast &c, hipStream_t s, int F, int nactive)
 }
 
 fast g_fast;
+
+void free_fast(fast &c)
+{
+	for (int f = 0; f < FMAX; f++) {
+		slotbuf &b = c.sl[f];
+		void *bb[] = {b.lp, b.P, b.rp, b.ci, b.ri, b.val};
+		for (void *p : bb)
+			if (p) HIPCHK(hipFree(p));
+	}
+	void *bufs[] = {c.luw, c.M, c.dinvL, c.dinvU, c.piv, c.perm, c.V, c.Z, c.W, c.Xv, c.Bv, c.R, c.T1, c.partial, c.h, c.yd, c.scale, c.ctl};
+	for (void *b : bufs)
+		if (b) HIPCHK(hipFree(b));
+	c = fast();
+}
+
+// stamps of one slot -> device triplets sorted by row (stable) + CSR row pointer
+void upload_stamps(slotbuf &b, const stamps_t &sp, int n, hipStream_t s)
+{
+	std::vector<int> cnt(n + 1, 0);
+	for (int e = 0; e < sp.n; e++) cnt[sp.ri[e] + 1]++;
+	for (int i = 0; i < n; i++) cnt[i + 1] += cnt[i];
+	std::vector<int> pos(cnt.begin(), cnt.end() - 1), ri(sp.n), ci(sp.n);
+	std::vector<double2> val(sp.n);
+	for (int e = 0; e < sp.n; e++) {
+		const int k = pos[sp.ri[e]]++;
+		ri[k] = sp.ri[e];
+		ci[k] = sp.ci[e];
+		val[k] = make_double2(sp.val[e].r, sp.val[e].i);
+	}
+	if (sp.n > b.cap) {
+		if (b.ci) {
+			HIPCHK(hipFree(b.ci));
+			HIPCHK(hipFree(b.ri));
+			HIPCHK(hipFree(b.val));
+		}
+		b.cap = sp.n;
+		b.ci = dal<int>(sp.n);
+		b.ri = dal<int>(sp.n);
+		b.val = dal<double2>(sp.n);
+	}
+	b.nnz = sp.n;
+	HIPCHK(hipMemcpyAsync(b.rp, cnt.data(), (size_t)(n + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+	if (sp.n > 0) {
+		HIPCHK(hipMemcpyAsync(b.ri, ri.data(), (size_t)sp.n * sizeof(int), hipMemcpyHostToDevice, s));
+		HIPCHK(hipMemcpyAsync(b.ci, ci.data(), (size_t)sp.n * sizeof(int), hipMemcpyHostToDevice, s));
+		HIPCHK(hipMemcpyAsync(b.val, val.data(), (size_t)sp.n * sizeof(double2), hipMemcpyHostToDevice, s));
+	}
+	HIPCHK(hipStreamSynchronize(s));
+}
 
 }  // namespace
 
@@ -542,23 +715,27 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 	const int n = p->nunknown + nc;
 	const int offQ = p->nunknown;
 	const int F = std::max(1, std::min(batch, FMAX));
-	if ((c.n != n) || (c.F < F)) {
-		for (int f = 0; f < FMAX; f++)
-			if (c.A[f]) HIPCHK(hipFree(c.A[f]));
-		void *bufs[] = {c.luw, c.M, c.piv, c.perm, c.V, c.W, c.Xv, c.Bv, c.R, c.T1, c.partial, c.h, c.yd, c.scale, c.bar};
-		for (void *b : bufs)
-			if (b) HIPCHK(hipFree(b));
-		c = fast();
+	if ((c.n != n) || (c.F < F) || (c.nseg != p->nseg)) {
+		free_fast(c);
 		c.n = n;
 		c.nc = nc;
 		c.F = F;
-		for (int f = 0; f < F; f++) c.A[f] = dal<double2>((size_t)n * n);
+		c.nseg = p->nseg;
+		for (int f = 0; f < F; f++) {
+			c.sl[f].lp = dal<double2>((size_t)p->nseg * p->nseg);
+			c.sl[f].P = dal<double2>((size_t)nc * nc);
+			c.sl[f].rp = dal<int>(n + 1);
+		}
+		const int nblk = (n + TB - 1) / TB;
 		c.luw = dal<double2>((size_t)n * n);
 		c.M = dal<float2>((size_t)n * n);
+		c.dinvL = dal<double2>((size_t)nblk * TB * TB);
+		c.dinvU = dal<double2>((size_t)nblk * TB * TB);
 		c.piv = dal<int>(n);
 		c.perm = dal<int>(n);
 		const long nf = (long)n * FMAX;
 		c.V = dal<double2>((size_t)(MKRYLOV + 1) * nf);
+		c.Z = dal<double2>((size_t)MKRYLOV * nf);
 		c.W = dal<double2>(nf);
 		c.Xv = dal<double2>(nf);
 		c.Bv = dal<double2>(nf);
@@ -568,13 +745,12 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 		c.h = dal<double2>((size_t)(MKRYLOV + 1) * FMAX);
 		c.yd = dal<double2>((size_t)(MKRYLOV + 1) * FMAX);
 		c.scale = dal<double>(FMAX);
-		c.bar = dal<unsigned>(64);
-		int dev = 0;
-		HIPCHK(hipGetDevice(&dev));
-		HIPCHK(hipDeviceGetAttribute(&c.ncu, hipDeviceAttributeMultiprocessorCount, dev));
+		c.ctl = dal<unsigned>(1 + nblk);
+		hipLaunchKernelGGL(k_zero_u32, dim3(blocks(1 + nblk, 64)), dim3(64), 0, s, c.ctl, 1 + nblk);
+		c.epoch = 0;
 	}
-	HIPCHK(hipMemsetAsync(c.bar, 0, sizeof(unsigned), s));
-	c.barbase = 0;
+	g_offS = p->offS;
+	g_offQ = offQ;
 	std::vector<int> i1(np), i2(np);
 	for (int j = 0; j < np; j++) {
 		i1[j] = p->nodemap[p->port[j].n1];
@@ -585,49 +761,44 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 	for (int f0 = 0; (f0 < nfreq) && !rc; f0 += F) {
 		const int nb = std::min(F, nfreq - f0);
 		const auto t0 = std::chrono::steady_clock::now();
-		// fill + assemble each system of the batch
+		// fill each system of the batch into its slot
 		for (int b = 0; b < nb; b++) {
 			const double freq = freq_at(p, ifreq0 + f0 + b);
 			const double kw = p->retardation ? (2 * PI * freq / C0) : 0;
-			const double omega = 2 * PI * freq;
-			if (p->retardation || (f0 + b == 0)) {
-				pgpu_fill_lp(g, kw);
-				if (usecap) pgpu_fill_pot(g, kw);
-			}
+			slotbuf &sb = c.sl[b];
+			sb.omega = 2 * PI * freq;
+			pgpu_fill_lp(g, kw);
+			if (usecap) pgpu_fill_pot(g, kw);
+			HIPCHK(hipMemcpyAsync(sb.lp, pgpu_lp_dev(g), (size_t)p->nseg * p->nseg * sizeof(double2), hipMemcpyDeviceToDevice, s));
+			if (nc) HIPCHK(hipMemcpyAsync(sb.P, pgpu_p_dev(g), (size_t)nc * nc * sizeof(double2), hipMemcpyDeviceToDevice, s));
 			stamps_t sp;
 			mna_stamps_charge(p, freq, &sp);
-			if (sp.n > c.st_cap) {
-				if (c.st_ri) {
-					HIPCHK(hipFree(c.st_ri));
-					HIPCHK(hipFree(c.st_ci));
-					HIPCHK(hipFree(c.st_v));
-				}
-				c.st_cap = sp.n;
-				c.st_ri = dal<int>(sp.n);
-				c.st_ci = dal<int>(sp.n);
-				c.st_v = dal<double2>(sp.n);
-			}
-			HIPCHK(hipMemcpyAsync(c.st_ri, sp.ri, (size_t)sp.n * sizeof(int), hipMemcpyHostToDevice, s));
-			HIPCHK(hipMemcpyAsync(c.st_ci, sp.ci, (size_t)sp.n * sizeof(int), hipMemcpyHostToDevice, s));
-			HIPCHK(hipMemcpyAsync(c.st_v, sp.val, (size_t)sp.n * sizeof(double2), hipMemcpyHostToDevice, s));
-			hipLaunchKernelGGL(k_assemble_aug, dim3(blocks((long)n * n)), dim3(256), 0, s, c.A[b], n, p->offS, p->nseg,
-			                   pgpu_lp_dev(g), omega, offQ, nc, pgpu_p_dev(g));
-			if (sp.n > 0) hipLaunchKernelGGL(k_stamp, dim3(blocks(sp.n)), dim3(256), 0, s, c.A[b], (long)n, c.st_ri, c.st_ci, c.st_v, sp.n);
-			HIPCHK(hipStreamSynchronize(s));
+			upload_stamps(sb, sp, n, s);
 			mna_stamps_free(&sp);
 		}
 		const auto t1 = std::chrono::steady_clock::now();
 		if (st) st->fill_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
-		// (re)factor the preconditioner when there is none yet
+		// (re)factor the preconditioner at the frequency of slot b
 		auto refactor = [&](int b) -> int {
-			HIPCHK(hipMemcpyAsync(c.luw, c.A[b], (size_t)n * n * sizeof(double2), hipMemcpyDeviceToDevice, s));
+			const auto ta = std::chrono::steady_clock::now();
+			const slotbuf &sb = c.sl[b];
+			hipLaunchKernelGGL(k_assemble_aug, dim3(blocks((long)n * n)), dim3(256), 0, s, c.luw, n, p->offS, p->nseg,
+			                   (const double2 *)sb.lp, sb.omega, offQ, nc, (const double2 *)sb.P);
+			if (sb.nnz > 0)
+				hipLaunchKernelGGL(k_stamp, dim3(blocks(sb.nnz)), dim3(256), 0, s, c.luw, (long)n, sb.ri, sb.ci,
+				                   (const double2 *)sb.val, sb.nnz);
 			const int info = dlu_factor(s, n, (d_complex_t *)c.luw, n, c.piv, c.perm, nullptr);
 			if (info >= 0) return 2;
 			hipLaunchKernelGGL(k_demote, dim3(blocks((long)n * n)), dim3(256), 0, s, c.luw, c.M, (long)n * n);
+			const int nblk = (n + TB - 1) / TB;
+			hipLaunchKernelGGL(k_diaginv<true>, dim3(nblk), dim3(64), 0, s, (const double2 *)c.luw, (long)n, n, c.dinvL);
+			hipLaunchKernelGGL(k_diaginv<false>, dim3(nblk), dim3(64), 0, s, (const double2 *)c.luw, (long)n, n, c.dinvU);
+			HIPCHK(hipStreamSynchronize(s));
 			c.fref = freq_at(p, ifreq0 + f0 + b);
-			HIPCHK(hipMemsetAsync(c.bar, 0, sizeof(unsigned), s));
-			c.barbase = 0;
-			if (st) st->n_lu++;
+			if (st) {
+				st->n_lu++;
+				st->lu_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ta).count();
+			}
 			return 0;
 		};
 		if (c.fref < 0) {
@@ -642,13 +813,13 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 				if (i2[j] >= 0) hb[(size_t)i2[j] * F + b].x -= 1;
 			}
 			HIPCHK(hipMemcpyAsync(c.Bv, hb.data(), hb.size() * sizeof(double2), hipMemcpyHostToDevice, s));
-			int Fb = F;
-			std::vector<int> its = gmres_batch(c, s, Fb, nb);
+			const auto tg = std::chrono::steady_clock::now();
+			std::vector<int> its = gmres_batch(c, s, F, nb);
 			for (int b = 0; b < nb; b++) {
 				if (its[b] < 0) {
-					// stale preconditioner: refactor at this frequency and solve this system alone
+					// stale preconditioner: refactor at this frequency and solve the batch again
 					if ((rc = refactor(b))) break;
-					its = gmres_batch(c, s, Fb, nb);
+					its = gmres_batch(c, s, F, nb);
 					if (its[b] < 0) {
 						rc = 4;
 						break;
@@ -656,6 +827,7 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 				}
 				if (st) st->gmres_its += its[b];
 			}
+			if (st) st->solve_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tg).count();
 			HIPCHK(hipMemcpyAsync(hx.data(), c.Xv, hx.size() * sizeof(double2), hipMemcpyDeviceToHost, s));
 			HIPCHK(hipStreamSynchronize(s));
 			for (int b = 0; b < nb; b++)
