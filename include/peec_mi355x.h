@@ -31,6 +31,8 @@ typedef struct {
 	double berr;            /* worst backward error of the solutions (mixed)  */
 	int    n_lu;            /* factorisations performed (mixed form)          */
 	long   gmres_its;       /* GMRES iterations summed over systems (mixed)   */
+	double kf_setup_ms;     /* k-factored fill preparation (mixed), in fill_ms */
+	long   kf_direct_pairs; /* pairs the k-factored fill evaluates directly   */
 } peec_stats_t;
 
 /* Library identification: "libpeec_mi355x <version> gfx950". */
@@ -65,6 +67,12 @@ void peec_sweep_reset(void);
  * triangles with ground-plane images) at frequency f: the algorithmic work behind the fill
  * GFLOP/s figure.  Host-side walk of the reference's branch tree; evaluates no integral. */
 double peec_fill_flops(const peec_t *p, double f);
+
+/* Lp (nseg x nseg) and P (ncell x ncell, the potential coefficients before inversion) at f,
+ * filled on the device and copied to host buffers (P may be NULL).  mode 0: element-by-element
+ * evaluation (the plain form's fill); mode 1: the k-factored fill of the mixed form, prepared
+ * for the deck's whole frequency band.  Returns 0, or 1 when mode 1 cannot be prepared. */
+int peec_fill_matrices(peec_t *p, double f, int mode, d_complex_t *lp, d_complex_t *P);
 
 /* Sizes for bindings */
 size_t peec_sizeof_peec(void);

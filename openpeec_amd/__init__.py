@@ -117,6 +117,17 @@ class Peec:
         n = self.p.ncell
         return cview(self.p.cmat, (n, n)).copy()
 
+    def fill_matrices(self, f: float, mode: int = 0):
+        """(Lp, P) at f from the device fill: mode 0 element by element, 1 k-factored."""
+        ns, nc = self.p.nseg, self.p.ncell if self.p.capacitance else 0
+        lp = np.zeros((ns, ns), dtype=np.complex128)
+        P = np.zeros((nc, nc), dtype=np.complex128)
+        rc = self.lib.peec_fill_matrices(C.byref(self.p), f, mode, lp.ctypes.data_as(C.POINTER(abi.d_complex_t)),
+                                         P.ctypes.data_as(C.POINTER(abi.d_complex_t)) if nc else None)
+        if rc != 0:
+            raise RuntimeError(f"peec_fill_matrices failed (code {rc})")
+        return lp, P
+
     def kw(self, f: float) -> float:
         return 2 * np.pi * f / 2.99792458e8 if self.p.retardation else 0.0
 

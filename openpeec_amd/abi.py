@@ -120,7 +120,8 @@ class peec_stats_t(C.Structure):
                 ("solve_ms", C.c_double), ("total_ms", C.c_double), ("lu_gemm_ms", C.c_double),
                 ("lu_gemm_flop", C.c_double), ("lu_gemm_launches", C.c_int),
                 ("fill_flop", C.c_double), ("ising", C.c_int), ("refine_its", C.c_int),
-                ("berr", C.c_double), ("n_lu", C.c_int), ("gmres_its", C.c_long)]
+                ("berr", C.c_double), ("n_lu", C.c_int), ("gmres_its", C.c_long),
+                ("kf_setup_ms", C.c_double), ("kf_direct_pairs", C.c_long)]
 
 
 def declare(lib: C.CDLL, reference: bool = False) -> None:
@@ -169,6 +170,7 @@ def declare(lib: C.CDLL, reference: bool = False) -> None:
             "peec_sweep_reset": (None, []),
             "peec_sizeof_peec": (C.c_size_t, []),
             "peec_fill_flops": (C.c_double, [PP, C.c_double]),
+            "peec_fill_matrices": (C.c_int, [PP, C.c_double, C.c_int, PZ, PZ]),
             "peec_sizeof_seg": (C.c_size_t, []),
         })
     for name, (res, args) in sig.items():

@@ -142,6 +142,35 @@ __global__ void k_aggregate(const double2 *G, long ldg, const int *csoff, const 
 	P[(long)c1 * ldp + c2] = make_double2(s * sr, s * si);
 }
 
+// listed (i <= j) pairs at kw: the pairs a k-factored fill evaluates directly each frequency
+template <int KM>
+__global__ __launch_bounds__(256) void k_fill_list(fill_args a, int which, const int2 *list, int cnt)
+{
+	const int e = blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= cnt) return;
+	const int i = list[e].x, j = list[e].y;
+	double2 *out = (double2 *)a.out;
+	if (which == 0) {
+		const cplx v = lp_entry<KM>(a.S, a.pS, a.SM, a.pSM, i, j, a.kw, a.coef, a.gp);
+		out[(long)i * a.ld + j] = make_double2(v.r, v.i);
+		if (i != j) out[(long)j * a.ld + i] = make_double2(v.r, v.i);
+		return;
+	}
+	const cplx v = ig_entry<KM>(a.S, a.pS, a.SM, a.pSM, i, j, a.kw, a.gp);
+	if (a.carea == nullptr) {
+		out[(long)i * a.ld + j] = make_double2(v.r, v.i);
+		if (i != j) out[(long)j * a.ld + i] = make_double2(v.r, v.i);
+		return;
+	}
+	const double ai = a.carea[i], aj = a.carea[j];
+	const double sij = 1 / (a.k4pe * ai * aj);
+	out[(long)i * a.ld + j] = make_double2(sij * (0 + v.r), sij * (0 + v.i));
+	if (i != j) {
+		const double sji = 1 / (a.k4pe * aj * ai);
+		out[(long)j * a.ld + i] = make_double2(sji * (0 + v.r), sji * (0 + v.i));
+	}
+}
+
 // single entries for lp_entry / pot_entry calls through the ABI
 template <int KM>
 __global__ void k_one_lp(fill_args a, int i, int j, double2 *res)
@@ -215,6 +244,24 @@ void fill_launch(hipStream_t s, int which, int km, const pe::fill_args &a)
 	case pe::KM_RIBBON | pe::KM_BAR: launch_fill<pe::KM_RIBBON | pe::KM_BAR>(s, which, a); break;
 	case pe::KM_WIRE | pe::KM_RIBBON | pe::KM_BAR: launch_fill<pe::KM_WIRE | pe::KM_RIBBON | pe::KM_BAR>(s, which, a); break;
 	default: launch_fill<15>(s, which, a); break;
+	}
+	HIPCHK(hipGetLastError());
+}
+
+void fill_list_launch(hipStream_t s, int which, int km, const pe::fill_args &a, const int2 *list, int cnt)
+{
+	if (cnt <= 0) return;
+	const dim3 g((unsigned)((cnt + 255) / 256)), b(256);
+	switch (km_class(km)) {
+	case pe::KM_WIRE: hipLaunchKernelGGL(pe::k_fill_list<pe::KM_WIRE>, g, b, 0, s, a, which, list, cnt); break;
+	case pe::KM_RIBBON: hipLaunchKernelGGL(pe::k_fill_list<pe::KM_RIBBON>, g, b, 0, s, a, which, list, cnt); break;
+	case pe::KM_RIBBON | pe::KM_BAR:
+		hipLaunchKernelGGL(pe::k_fill_list<pe::KM_RIBBON | pe::KM_BAR>, g, b, 0, s, a, which, list, cnt);
+		break;
+	case pe::KM_WIRE | pe::KM_RIBBON | pe::KM_BAR:
+		hipLaunchKernelGGL(pe::k_fill_list<pe::KM_WIRE | pe::KM_RIBBON | pe::KM_BAR>, g, b, 0, s, a, which, list, cnt);
+		break;
+	default: hipLaunchKernelGGL(pe::k_fill_list<15>, g, b, 0, s, a, which, list, cnt); break;
 	}
 	HIPCHK(hipGetLastError());
 }

@@ -34,3 +34,12 @@ void fill_one(hipStream_t s, int which, int km, const pe::fill_args &a, int i, i
               const int *csidx, double2 *res);
 void aggregate_launch(hipStream_t s, const double2 *G, long ldg, const int *csoff, const int *csidx,
                       const double *carea, int nc, double k4pe, double2 *P, long ldp);
+void fill_list_launch(hipStream_t s, int which, int km, const pe::fill_args &a, const int2 *list, int cnt);
+
+// k-factored fill (kfill.hip)
+long kf_slots(int n);
+int kf_record_doubles(void);
+void kf_rho_launch(hipStream_t s, const dcell_t *C, const dedge_t *edge, int n, double *rho);
+void kf_setup_launch(hipStream_t s, int which, int km, const pe::fill_args &f, double klo, double khi, const double *rho,
+                     double *rec, int T, int2 *list, unsigned *nlist);
+void kf_eval_launch(hipStream_t s, int which, const pe::fill_args &f, const double *rec, int T);

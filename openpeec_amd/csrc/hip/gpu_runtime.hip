@@ -51,6 +51,15 @@ struct pgpu {
 	double2 *d_st_v = nullptr;
 	int st_cap = 0;
 	double2 *qbuf = nullptr;     // ncell x nport cell charges
+	// k-factored fill over a wavenumber band (kfill.hip)
+	bool kf_ready = false;
+	double kf_lo = 0, kf_hi = 0;
+	double *kf_rec_lp = nullptr, *kf_rec_ig = nullptr;
+	int kf_T = 0;
+	int2 *kf_list_lp = nullptr, *kf_list_ig = nullptr;
+	unsigned *kf_cnt = nullptr;
+	int kf_nd_lp = 0, kf_nd_ig = 0;
+	double *kf_rho_seg = nullptr, *kf_rho_chg = nullptr;
 };
 
 namespace {
@@ -330,6 +339,13 @@ void peec_gpu_release(peec_t *p)
 		dfree(g->csidx);
 		dfree(g->carea);
 		dfree(g->lp);
+		dfree(g->kf_rec_lp);
+		dfree(g->kf_rec_ig);
+		dfree(g->kf_list_lp);
+		dfree(g->kf_list_ig);
+		dfree(g->kf_cnt);
+		dfree(g->kf_rho_seg);
+		dfree(g->kf_rho_chg);
 		dfree(g->G);
 		dfree(g->P);
 		dfree(g->C);
@@ -382,6 +398,107 @@ int pgpu_fill_pot(pgpu_t *g, double kw)
 		aggregate_launch(g->s, g->G, nh, g->csoff, g->csidx, g->carea, nc, 4 * PI * EPS0, g->P, nc);
 	}
 	return 0;
+}
+
+// Prepare the k-factored fill for wavenumbers in [klo, khi] (reused while a later band lies
+// inside it).  Returns 0 when ready, 1 when the records would not fit the memory budget (the
+// caller then fills directly).
+int pgpu_kf_setup(pgpu_t *g, double klo, double khi)
+{
+	const peec_t *p = g->p;
+	if (g->kf_ready && (klo >= g->kf_lo) && (khi <= g->kf_hi)) return 0;
+	const int ns = g->seg.n, nh = g->chg.n;
+	const int T = p->gp ? 2 : 1;
+	const long sl = kf_slots(ns), sh = (nh > 0) ? kf_slots(nh) : 0;
+	const double bytes = (double)T * kf_record_doubles() * 8.0 * (double)(sl + sh) + 8.0 * (double)(sl + sh);
+	if (bytes > 64e9) return 1;
+	dfree(g->kf_rec_lp);
+	dfree(g->kf_rec_ig);
+	dfree(g->kf_list_lp);
+	dfree(g->kf_list_ig);
+	g->kf_rec_lp = g->kf_rec_ig = nullptr;
+	g->kf_list_lp = g->kf_list_ig = nullptr;
+	if (g->kf_cnt == nullptr) g->kf_cnt = dalloc<unsigned>(2);
+	if (g->kf_rho_seg == nullptr) g->kf_rho_seg = dalloc<double>(ns);
+	HIPCHK(hipMemsetAsync(g->kf_cnt, 0, 2 * sizeof(unsigned), g->s));
+	g->kf_T = T;
+	if (ns > 0) {
+		g->kf_rec_lp = dalloc<double>((size_t)T * kf_record_doubles() * sl);
+		g->kf_list_lp = dalloc<int2>(sl);
+		kf_rho_launch(g->s, g->seg.c, g->seg.e, ns, g->kf_rho_seg);
+		pe::fill_args a = base_args(g->seg, ns, khi, p->gp, nullptr, ns);
+		kf_setup_launch(g->s, 0, g->seg.km, a, klo, khi, g->kf_rho_seg, g->kf_rec_lp, T, g->kf_list_lp, g->kf_cnt);
+	}
+	if (nh > 0 && p->ncell > 0) {
+		ensure_cells(g);
+		if (g->kf_rho_chg == nullptr) g->kf_rho_chg = dalloc<double>(nh);
+		g->kf_rec_ig = dalloc<double>((size_t)T * kf_record_doubles() * sh);
+		g->kf_list_ig = dalloc<int2>(sh);
+		kf_rho_launch(g->s, g->chg.c, g->chg.e, nh, g->kf_rho_chg);
+		pe::fill_args a = base_args(g->chg, nh, khi, p->gp, nullptr, nh);
+		kf_setup_launch(g->s, 1, g->chg.km, a, klo, khi, g->kf_rho_chg, g->kf_rec_ig, T, g->kf_list_ig, g->kf_cnt + 1);
+	}
+	unsigned cnt[2] = {0, 0};
+	HIPCHK(hipMemcpyAsync(cnt, g->kf_cnt, sizeof(cnt), hipMemcpyDeviceToHost, g->s));
+	HIPCHK(hipStreamSynchronize(g->s));
+	g->kf_nd_lp = (int)cnt[0];
+	g->kf_nd_ig = (int)cnt[1];
+	g->kf_lo = klo;
+	g->kf_hi = khi;
+	g->kf_ready = true;
+	return 0;
+}
+
+// Lp (nseg x nseg) and P (ncell x ncell) at kw from the records into caller buffers
+void pgpu_kf_fill(pgpu_t *g, double kw, d_complex_t *dlp, d_complex_t *dP)
+{
+	double2 *lp_out = (double2 *)dlp, *p_out = (double2 *)dP;
+	const peec_t *p = g->p;
+	const int ns = g->seg.n, nh = g->chg.n, nc = p->ncell;
+	if (ns > 0) {
+		pe::fill_args a = base_args(g->seg, ns, kw, p->gp, lp_out, ns);
+		kf_eval_launch(g->s, 0, a, g->kf_rec_lp, g->kf_T);
+		fill_list_launch(g->s, 0, g->seg.km, a, g->kf_list_lp, g->kf_nd_lp);
+	}
+	if ((nh > 0) && (nc > 0) && p_out) {
+		if (g->cells_identity) {
+			pe::fill_args a = base_args(g->chg, nh, kw, p->gp, p_out, nc);
+			a.carea = g->carea;
+			kf_eval_launch(g->s, 1, a, g->kf_rec_ig, g->kf_T);
+			fill_list_launch(g->s, 1, g->chg.km, a, g->kf_list_ig, g->kf_nd_ig);
+		}
+		else {
+			if (g->G == nullptr) g->G = dalloc<double2>((size_t)nh * nh);
+			pe::fill_args a = base_args(g->chg, nh, kw, p->gp, g->G, nh);
+			kf_eval_launch(g->s, 1, a, g->kf_rec_ig, g->kf_T);
+			fill_list_launch(g->s, 1, g->chg.km, a, g->kf_list_ig, g->kf_nd_ig);
+			aggregate_launch(g->s, g->G, nh, g->csoff, g->csidx, g->carea, nc, 4 * PI * EPS0, p_out, nc);
+		}
+	}
+}
+
+int pgpu_kf_direct_pairs(pgpu_t *g)
+{
+	return g->kf_nd_lp + g->kf_nd_ig;
+}
+
+void pgpu_fill_to_host(pgpu_t *g, double kw, int mode, d_complex_t *lp, d_complex_t *P)
+{
+	const peec_t *p = g->p;
+	const int ns = g->seg.n, nc = p->ncell;
+	const bool cap = p->capacitance && (g->chg.n > 0) && (nc > 0);
+	if (ns > 0 && g->lp == nullptr) g->lp = dalloc<double2>((size_t)ns * ns);
+	if (cap && g->P == nullptr) g->P = dalloc<double2>((size_t)nc * nc);
+	if (mode == 1) {
+		pgpu_kf_fill(g, kw, (d_complex_t *)g->lp, cap ? (d_complex_t *)g->P : nullptr);
+	}
+	else {
+		pgpu_fill_lp(g, kw);
+		if (cap) pgpu_fill_pot(g, kw);
+	}
+	if (ns > 0) HIPCHK(hipMemcpyAsync(lp, g->lp, (size_t)ns * ns * sizeof(double2), hipMemcpyDeviceToHost, g->s));
+	if (cap && P) HIPCHK(hipMemcpyAsync(P, g->P, (size_t)nc * nc * sizeof(double2), hipMemcpyDeviceToHost, g->s));
+	HIPCHK(hipStreamSynchronize(g->s));
 }
 
 int pgpu_cmat(pgpu_t *g, int *ising)

@@ -27,6 +27,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 #include "../peec_internal.h"
@@ -151,13 +152,17 @@ struct slots {
 
 // Y[:, f] = A_f X[:, f] (vectors n x F row-major): one wave per row; the dense part is summed
 // lane-strided over the block's columns, two 16-byte loads in flight per lane.
-__global__ __launch_bounds__(256) void k_matvec(slots S, int n, int F, int offS, int nseg, int offQ, int nc,
+__global__ __launch_bounds__(256) void k_matvec(slots S, int n, int F, int nact, int offS, int nseg, int offQ, int nc,
                                                 const double2 *__restrict__ X, double2 *__restrict__ Y)
 {
 	const int f = blockIdx.y;
 	const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
 	const int lane = threadIdx.x & 63;
 	if (row >= n) return;
+	if (f >= nact) {   // slot without a system in this batch: its buffers hold nothing to read
+		if (lane == 0) Y[(long)row * F + f] = make_double2(0, 0);
+		return;
+	}
 	const slot &sl = S.s[f];
 	double sr = 0, si = 0;
 	const double2 *a = nullptr;
@@ -503,14 +508,16 @@ void precond(fast &c, hipStream_t s, const double2 *v, double2 *z, int F)
 
 int g_offS = 0, g_offQ = 0;
 
-void matvec(fast &c, hipStream_t s, const double2 *x, double2 *y, int F)
+// y = A_f x for the first nact systems of the batch (columns f >= nact are set to zero)
+void matvec(fast &c, hipStream_t s, const double2 *x, double2 *y, int F, int nact)
 {
 	slots S;
 	for (int f = 0; f < FMAX; f++) {
-		const slotbuf &b = c.sl[(f < F) ? f : 0];
+		const slotbuf &b = c.sl[(f < nact) ? f : 0];
 		S.s[f] = slot{b.rp, b.ci, b.val, b.lp, b.P, b.omega};
 	}
-	hipLaunchKernelGGL(k_matvec, dim3((c.n + 3) / 4, F), dim3(256), 0, s, S, c.n, F, g_offS, c.nseg, g_offQ, c.nc, x, y);
+	hipLaunchKernelGGL(k_matvec, dim3((c.n + 3) / 4, F), dim3(256), 0, s, S, c.n, F, nact, g_offS, c.nseg, g_offQ, c.nc,
+	                   x, y);
 }
 
 // h[k][f] = V_k^H W for k < nk, copied to host
@@ -541,7 +548,7 @@ std::vector<int> gmres_batch(fast &c, hipStream_t s, int F, int nactive)
 	hipLaunchKernelGGL(k_zero, dim3(blocks(nf)), dim3(256), 0, s, c.Xv, nf);
 	for (int cyc = 0; cyc < MCYCLES; cyc++) {
 		// r = b - A x
-		matvec(c, s, c.Xv, c.R, F);
+		matvec(c, s, c.Xv, c.R, F, nactive);
 		hipLaunchKernelGGL(k_resid, dim3(blocks(nf)), dim3(256), 0, s, c.Bv, c.R, nf);
 		dots(c, s, c.R, 0, 1, c.R, F, hh);
 		std::vector<double> beta(F);
@@ -567,7 +574,7 @@ std::vector<int> gmres_batch(fast &c, hipStream_t s, int F, int nactive)
 			double2 *vj = c.V + (long)j * vstride;
 			double2 *zj = c.Z + (long)j * vstride;
 			precond(c, s, vj, zj, F);
-			matvec(c, s, zj, c.W, F);
+			matvec(c, s, zj, c.W, F, nactive);
 			std::vector<double2> h1, h2;
 			dots(c, s, c.V, vstride, j + 1, c.W, F, h1);
 			HIPCHK(hipMemcpyAsync(c.h, h1.data(), h1.size() * sizeof(double2), hipMemcpyHostToDevice, s));
@@ -634,7 +641,7 @@ std::vector<int> gmres_batch(fast &c, hipStream_t s, int F, int nactive)
 		}
 	}
 	// final true-residual check
-	matvec(c, s, c.Xv, c.R, F);
+	matvec(c, s, c.Xv, c.R, F, nactive);
 	hipLaunchKernelGGL(k_resid, dim3(blocks(nf)), dim3(256), 0, s, c.Bv, c.R, nf);
 	dots(c, s, c.R, 0, 1, c.R, F, hh);
 	for (int f = 0; f < F; f++)
@@ -703,6 +710,13 @@ const double2 *pgpu_p_dev(pgpu_t *g);
 void mna_stamps_charge(const peec_t *p, double f, stamps_t *st);
 }
 
+// PEEC_KF=0 in the environment switches the k-factored fill off (element-by-element fills)
+static bool pgpu_kf_enabled()
+{
+	const char *e = getenv("PEEC_KF");
+	return !(e && e[0] == '0');
+}
+
 // Sweep nfreq points from ifreq0 with the accelerated form.  zmat as pgpu_sweep.
 extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq, d_complex_t *zmat, pgpu_stats_t *st,
                                int batch)
@@ -725,6 +739,7 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 			c.sl[f].lp = dal<double2>((size_t)p->nseg * p->nseg);
 			c.sl[f].P = dal<double2>((size_t)nc * nc);
 			c.sl[f].rp = dal<int>(n + 1);
+			hipLaunchKernelGGL(k_zero_u32, dim3(blocks(n + 1, 64)), dim3(64), 0, s, (unsigned *)c.sl[f].rp, n + 1);
 		}
 		const int nblk = (n + TB - 1) / TB;
 		c.luw = dal<double2>((size_t)n * n);
@@ -758,6 +773,25 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 	}
 	std::vector<double2> hx((size_t)n * F);
 	int rc = 0;
+	// k-factored fill over the band of the deck's sweep (and of this call)
+	bool kf = false;
+	if (p->retardation && pgpu_kf_enabled()) {
+		double lo = 1e300, hi = 0;
+		const int ends[4] = {0, (p->nfreq > 0) ? p->nfreq - 1 : 0, ifreq0, ifreq0 + nfreq - 1};
+		for (int e : ends) {
+			const double fe = freq_at(p, e);
+			lo = std::min(lo, fe);
+			hi = std::max(hi, fe);
+		}
+		const auto ta = std::chrono::steady_clock::now();
+		kf = (pgpu_kf_setup(g, 2 * PI * lo / C0, 2 * PI * hi / C0) == 0);
+		const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ta).count();
+		if (st) {
+			st->kf_setup_ms += ms;
+			st->fill_ms += ms;
+			st->kf_direct_pairs = kf ? pgpu_kf_direct_pairs(g) : -1;
+		}
+	}
 	for (int f0 = 0; (f0 < nfreq) && !rc; f0 += F) {
 		const int nb = std::min(F, nfreq - f0);
 		const auto t0 = std::chrono::steady_clock::now();
@@ -767,10 +801,15 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 			const double kw = p->retardation ? (2 * PI * freq / C0) : 0;
 			slotbuf &sb = c.sl[b];
 			sb.omega = 2 * PI * freq;
-			pgpu_fill_lp(g, kw);
-			if (usecap) pgpu_fill_pot(g, kw);
-			HIPCHK(hipMemcpyAsync(sb.lp, pgpu_lp_dev(g), (size_t)p->nseg * p->nseg * sizeof(double2), hipMemcpyDeviceToDevice, s));
-			if (nc) HIPCHK(hipMemcpyAsync(sb.P, pgpu_p_dev(g), (size_t)nc * nc * sizeof(double2), hipMemcpyDeviceToDevice, s));
+			if (kf) {
+				pgpu_kf_fill(g, kw, (d_complex_t *)sb.lp, nc ? (d_complex_t *)sb.P : nullptr);
+			}
+			else {
+				pgpu_fill_lp(g, kw);
+				if (usecap) pgpu_fill_pot(g, kw);
+				HIPCHK(hipMemcpyAsync(sb.lp, pgpu_lp_dev(g), (size_t)p->nseg * p->nseg * sizeof(double2), hipMemcpyDeviceToDevice, s));
+				if (nc) HIPCHK(hipMemcpyAsync(sb.P, pgpu_p_dev(g), (size_t)nc * nc * sizeof(double2), hipMemcpyDeviceToDevice, s));
+			}
 			stamps_t sp;
 			mna_stamps_charge(p, freq, &sp);
 			upload_stamps(sb, sp, n, s);

@@ -375,5 +375,32 @@ void peec_sweep_reset(void)
 	pgpu_fast_reset();
 }
 
+/* wavenumber band of the deck's sweep together with f */
+static void deck_band(const peec_t *p, double f, double *klo, double *khi)
+{
+	double lo = f, hi = f;
+	const double fs[2] = {freq_at(p, 0), freq_at(p, (p->nfreq > 0) ? p->nfreq - 1 : 0)};
+	for (int i = 0; i < 2; i++) {
+		if (fs[i] < lo) lo = fs[i];
+		if (fs[i] > hi) hi = fs[i];
+	}
+	*klo = 2 * PI * lo / C0;
+	*khi = 2 * PI * hi / C0;
+}
+
+int peec_fill_matrices(peec_t *p, double f, int mode, d_complex_t *lp, d_complex_t *P)
+{
+	if (p->capacitance && (p->nchg > 0) && pot_cells(p)) return 2;
+	pgpu_t *g = pgpu_get(p);
+	const double kw = p->retardation ? (2 * PI * f / C0) : 0;
+	if (mode == 1) {
+		double klo, khi;
+		deck_band(p, f, &klo, &khi);
+		if (!p->retardation || pgpu_kf_setup(g, klo, khi)) return 1;
+	}
+	pgpu_fill_to_host(g, kw, mode, lp, P);
+	return 0;
+}
+
 size_t peec_sizeof_peec(void) { return sizeof(peec_t); }
 size_t peec_sizeof_seg(void) { return sizeof(seg_t); }

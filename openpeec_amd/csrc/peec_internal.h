@@ -83,6 +83,15 @@ void pgpu_cmat_to_host(pgpu_t *g, d_complex_t *cmat);
 void pgpu_lp_from_host(pgpu_t *g, const d_complex_t *lp);
 void pgpu_cmat_from_host(pgpu_t *g, const d_complex_t *cmat);
 
+/* k-factored fill (kfill.hip): records for wavenumbers in [klo, khi], then Lp / P at any kw
+ * of the band straight into device buffers.  setup returns 1 when the records exceed the
+ * memory budget. */
+int  pgpu_kf_setup(pgpu_t *g, double klo, double khi);
+void pgpu_kf_fill(pgpu_t *g, double kw, d_complex_t *dlp, d_complex_t *dP);
+int  pgpu_kf_direct_pairs(pgpu_t *g);
+/* Lp and P at kw (mode 0 element code, mode 1 k-factored, set up beforehand) to host buffers */
+void pgpu_fill_to_host(pgpu_t *g, double kw, int mode, d_complex_t *lp, d_complex_t *P);
+
 /* single entries (lp_entry / pot_entry through the same device code) */
 d_complex_t pgpu_lp_entry(pgpu_t *g, int i, int j, double kw);
 d_complex_t pgpu_pot_entry(pgpu_t *g, int i, int j, double kw);

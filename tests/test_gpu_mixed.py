@@ -45,3 +45,16 @@ def test_mixed_matches_plain_on_plate(lib):
     assert _rel(zm, zp) <= 1e-6
     assert st.n_lu <= 3
     m.close()
+
+
+def test_mixed_short_first_batch(lib):
+    """A sweep whose first batch holds fewer systems than the batch size: the unused slots carry
+    no system and must not be read (one frequency with batch 8)."""
+    text = decks.dipole_deck()
+    lib.peec_sweep_reset()
+    m = Peec.from_deck_text(text, lib)
+    m.setup()
+    z, _ = m.sweep(7, 1, PEEC_FORM_MIXED, 8)
+    zo = oracle_zmat(text, 7, 1)
+    assert _rel(z, zo) <= 1e-6
+    m.close()
