@@ -4,15 +4,23 @@
 Workload (the precision configuration): copper plate 100 x 100 mm, 35 um, 64 x 64 grid
 (8320 ribbon branches, 4225 charge cells, 12545 MNA unknowns), retardation and capacitance
 on, corner-to-corner port, sweep 10 MHz - 3 GHz in 201 points.  A step is one frequency point
-of that sweep, device resident from the cell arrays to Z(f): Lp fill, charge-cell fill -> P,
-LU(P) and C = P^-1, MNA assembly, LU of the 12545 x 12545 complex matrix, port solve.
+of that sweep, device resident from the cell arrays to Z(f).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--form plain|mixed] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--form mixed|plain] [--batch B] [--no-cpu]
 
-Prints one JSON line.  value = K / (wall time of K steps); the roofline block is measured on
-one extra instrumented step (HIP events on the library's stream around every trailing-update
-GEMM launch); cpu_baseline times the oracle (the reference's arithmetic restated in C, OpenMP
-on all host cores) on a bounded sample and extrapolates to one frequency point.
+Forms.  mixed (default): the sweep's points 0..K-1 in one accelerated sweep -- k-factored Lp/P
+fill, charge-unknown system, one FP64 LU of A(f_ref) inverted into FP32 factors, FP64 flexible
+GMRES to a 1e-10 relative residual in lock-step batches of B frequencies; the LU, the
+inversion and the fill preparation run inside the timed region (cold start).  plain: every
+point independently (fill, P LU and inverse, assembly, FP64 LU, solve), K points spread over
+the sweep.  Either way value = K / (wall time of the K points).
+
+Zin of the mixed form is compared with the plain FP64 form on three of the timed points (the
+metric is quoted at Zin rel err <= 1e-6); the plain form's own rate on those points is reported
+beside it.  The roofline block is the LU trailing-update GEMM (k_zgemm, FP64 MFMA) timed with HIP
+events on the library's stream around every launch; the GMRES matvec is reported in GB/s the
+same way.  cpu_baseline times the oracle (the reference's arithmetic restated in C, OpenMP on
+all host cores) on a bounded sample and extrapolates to one frequency point.
 """
 from __future__ import annotations
 
@@ -37,9 +45,10 @@ PEAK_FP64_VECTOR = 78.6   # TFLOP/s, MI355X FP64 vector
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=None, help="points timed (default: 201 mixed, 3 plain)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--form", choices=["plain", "mixed"], default="plain")
+    ap.add_argument("--form", choices=["plain", "mixed"], default="mixed")
+    ap.add_argument("--batch", type=int, default=4, help="mixed form: frequencies per lock-step batch (1..8)")
     ap.add_argument("--grid", type=int, default=64, help="plate grid (64 = the precision workload)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--no-check", action="store_true", help="skip the small-deck oracle check")
@@ -123,6 +132,18 @@ def small_check(lib) -> dict:
     return {"deck": "plate 12x12, 10 MHz and 3 GHz", "zin_max_rel_err_vs_oracle": err}
 
 
+def plain_points(m, idx):
+    """Plain FP64 form at the given points: Zin per point and the elapsed time."""
+    from openpeec_amd import PEEC_FORM_PLAIN
+    zs = []
+    st = None
+    t0 = time.perf_counter()
+    for i in idx:
+        z, st = m.sweep(i, 1, PEEC_FORM_PLAIN)
+        zs.append(z[0, 0, 0])
+    return np.array(zs), time.perf_counter() - t0, st
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -134,65 +155,110 @@ def main():
     lib = load_library(build_if_missing=True)
     if not lib.peec_device_available():
         raise SystemExit("no HIP device: the library has no CPU path")
-    form = PEEC_FORM_PLAIN if args.form == "plain" else PEEC_FORM_MIXED
     text = decks.plate_deck(n=args.grid)
     m = Peec.from_deck_text(text, lib)
     m.setup()
     nf = m.p.nfreq
-    idx = [int(round(x)) for x in np.linspace(0, nf - 1, args.steps + 2)[1:-1]] if args.steps > 0 else []
-    for w in range(args.warmup):
-        m.sweep(nf - 1 - w, 1, form)
-    zs = []
-    t0 = time.perf_counter()
-    for i in idx:
-        z, _ = m.sweep(i, 1, form)
-        zs.append(z[0, 0, 0])
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    # one instrumented point at the top frequency for the stage split and the roofline
-    z_top, st = m.sweep(nf - 1, 1, form)
-    fill_flop = lib.peec_fill_flops(C.byref(m.p), m.freq(nf - 1))
-    fill_ms = st.fill_ms + st.pot_ms
-    gemm_avg_ms = st.lu_gemm_ms / max(1, st.lu_gemm_launches)
-    gemm_avg_flop = st.lu_gemm_flop / max(1, st.lu_gemm_launches)
-    achieved = (gemm_avg_flop / (gemm_avg_ms * 1e-3)) / 1e12 if gemm_avg_ms > 0 else 0.0
     n = m.p.nunknown
-    result = {
-        "metric": METRIC,
-        "value": (len(idx) / elapsed) if elapsed > 0 else 0.0,
-        "unit": "freq points/s",
-        "n_gpus": 1,
-        "steps": len(idx),
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / max(1, len(idx)),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if form == PEEC_FORM_PLAIN else "f64 (LU trailing update f32)",
-        "data": "synthetic deck: plate geometry generated in-process (no dataset)",
-        "config": {
-            "workload": f"copper plate 100x100 mm, 35 um, {args.grid}x{args.grid} grid, retarded + capacitive, "
-                        f"corner-to-corner port, 10 MHz-3 GHz sweep; one step = one frequency point",
-            "unknowns": n, "branches": m.p.nseg, "charge_cells": m.p.ncell,
-            "form": args.form, "frequencies_timed_hz": [m.freq(i) for i in idx],
-        },
-        "stages_ms_top_freq": {
-            "lp_fill": st.fill_ms, "charge_fill": st.pot_ms, "p_lu_and_inverse": st.cinv_ms,
-            "mna_assembly": st.asm_ms, "mna_lu": st.lu_ms, "port_solve": st.solve_ms, "total": st.total_ms,
-        },
-        "fill_gflops": fill_flop / (fill_ms * 1e-3) / 1e9 if fill_ms > 0 else None,
-        "fill_frac_of_fp64_vector_peak": (fill_flop / (fill_ms * 1e-3) / 1e12) / PEAK_FP64_VECTOR if fill_ms > 0 else None,
-        "roofline": {
-            "kernel": "k_zgemm (LU trailing update, FP64 MFMA)",
-            "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP64_MFMA, "traffic": None,
-            "launches": st.lu_gemm_launches, "avg_launch_ms": gemm_avg_ms, "avg_launch_flop": gemm_avg_flop,
-        },
-        "zin_top_freq": [float(z_top[0, 0, 0].real), float(z_top[0, 0, 0].imag)],
-    }
+    mixed = args.form == "mixed"
+    steps = args.steps if args.steps is not None else (nf if mixed else 3)
+    steps = max(1, min(steps, nf))
+    result = {"metric": METRIC}
+    if mixed:
+        # warm-up (kernel loading, allocations), then a cold sweep: preconditioner and fill
+        # records are dropped so the timed region rebuilds them
+        if args.warmup > 0:
+            m.sweep(0, min(args.warmup, nf), PEEC_FORM_MIXED, args.batch)
+        lib.peec_sweep_reset()
+        t0 = time.perf_counter()
+        z, st = m.sweep(0, steps, PEEC_FORM_MIXED, args.batch)
+        elapsed = time.perf_counter() - t0
+        idx = sorted({0, steps // 2, steps - 1})
+        zm = np.array([z[i, 0, 0] for i in idx])
+        zp, t_plain, stp = plain_points(m, idx)
+        fill_flop_top = lib.peec_fill_flops(C.byref(m.p), m.freq(idx[-1]))
+        plain_fill_ms = stp.fill_ms + stp.pot_ms
+        err = float(np.max(np.abs(zm - zp) / np.abs(zp)))
+        gemm_ms = st.lu_gemm_ms / max(1, st.lu_gemm_launches)
+        gemm_flop = st.lu_gemm_flop / max(1, st.lu_gemm_launches)
+        achieved = (gemm_flop / (gemm_ms * 1e-3)) / 1e12 if gemm_ms > 0 else 0.0
+        mv_gbs = (st.mv_bytes / (st.mv_ms * 1e-3)) / 1e9 if st.mv_ms > 0 else None
+        fill_flop = lib.peec_fill_flops(C.byref(m.p), m.freq(steps - 1))
+        result.update({
+            "value": steps / elapsed, "unit": "freq points/s", "n_gpus": 1, "steps": steps,
+            "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / steps, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64 (GMRES, residuals, fill); preconditioner factors stored f32",
+            "data": "synthetic deck: plate geometry generated in-process (no dataset)",
+            "config": {
+                "workload": f"copper plate 100x100 mm, 35 um, {args.grid}x{args.grid} grid, retarded + capacitive, "
+                            f"corner-to-corner port, 10 MHz-3 GHz sweep of {nf} points; one step = one frequency point",
+                "unknowns": n, "charge_unknowns": m.p.ncell, "branches": m.p.nseg, "form": "mixed",
+                "batch": args.batch, "points": [m.freq(0), m.freq(steps - 1)],
+            },
+            "zin_check": {"points_hz": [m.freq(i) for i in idx], "max_rel_err_vs_plain_fp64": err,
+                          "bar": 1e-6, "ok": err <= 1e-6},
+            "plain_fp64_points_per_s": len(idx) / t_plain,
+            "speedup_vs_plain_fp64": (steps / elapsed) / (len(idx) / t_plain),
+            "stages_ms": {"fill_incl_kf_setup": st.fill_ms, "kf_setup": st.kf_setup_ms,
+                          "lu_and_inverse": st.lu_ms, "gmres": st.solve_ms, "total": st.total_ms},
+            "n_lu": st.n_lu, "gmres_its_per_point": st.gmres_its / steps, "kf_direct_pairs": st.kf_direct_pairs,
+            "gmres_matvec": {"GB_per_s": mv_gbs, "launches": st.mv_launches,
+                             "avg_launch_ms": st.mv_ms / max(1, st.mv_launches),
+                             "frac_of_hbm_peak": (mv_gbs / 8000.0) if mv_gbs else None},
+            "fill_nominal_gflop_per_point": fill_flop / 1e9,
+            "plain_fill": {"gflops": fill_flop_top / (plain_fill_ms * 1e-3) / 1e9,
+                           "frac_of_fp64_vector_peak": fill_flop_top / (plain_fill_ms * 1e-3) / 1e12 / PEAK_FP64_VECTOR,
+                           "note": "element-by-element FP64 fill of the plain form at the last checked point"},
+            "kfactored_fill_ms_per_point": (st.fill_ms - st.kf_setup_ms) / steps,
+            "roofline": {
+                "kernel": "k_zgemm (LU trailing update of A(f_ref), FP64 MFMA)",
+                "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP64_MFMA, "traffic": None,
+                "launches": st.lu_gemm_launches, "avg_launch_ms": gemm_ms, "avg_launch_flop": gemm_flop,
+            },
+        })
+    else:
+        form = PEEC_FORM_PLAIN
+        idx = [int(round(x)) for x in np.linspace(0, nf - 1, steps + 2)[1:-1]]
+        for w in range(args.warmup):
+            m.sweep(nf - 1 - w, 1, form)
+        t0 = time.perf_counter()
+        for i in idx:
+            m.sweep(i, 1, form)
+        elapsed = time.perf_counter() - t0
+        z_top, st = m.sweep(nf - 1, 1, form)
+        fill_flop = lib.peec_fill_flops(C.byref(m.p), m.freq(nf - 1))
+        fill_ms = st.fill_ms + st.pot_ms
+        gemm_avg_ms = st.lu_gemm_ms / max(1, st.lu_gemm_launches)
+        gemm_avg_flop = st.lu_gemm_flop / max(1, st.lu_gemm_launches)
+        achieved = (gemm_avg_flop / (gemm_avg_ms * 1e-3)) / 1e12 if gemm_avg_ms > 0 else 0.0
+        result.update({
+            "value": len(idx) / elapsed, "unit": "freq points/s", "n_gpus": 1, "steps": len(idx),
+            "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / len(idx), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic deck: plate geometry generated in-process (no dataset)",
+            "config": {
+                "workload": f"copper plate 100x100 mm, 35 um, {args.grid}x{args.grid} grid, retarded + capacitive, "
+                            f"corner-to-corner port, 10 MHz-3 GHz sweep; one step = one frequency point",
+                "unknowns": n, "branches": m.p.nseg, "charge_cells": m.p.ncell, "form": "plain",
+                "frequencies_timed_hz": [m.freq(i) for i in idx],
+            },
+            "stages_ms_top_freq": {
+                "lp_fill": st.fill_ms, "charge_fill": st.pot_ms, "p_lu_and_inverse": st.cinv_ms,
+                "mna_assembly": st.asm_ms, "mna_lu": st.lu_ms, "port_solve": st.solve_ms, "total": st.total_ms,
+            },
+            "fill_gflops": fill_flop / (fill_ms * 1e-3) / 1e9 if fill_ms > 0 else None,
+            "fill_frac_of_fp64_vector_peak": (fill_flop / (fill_ms * 1e-3) / 1e12) / PEAK_FP64_VECTOR if fill_ms > 0 else None,
+            "roofline": {
+                "kernel": "k_zgemm (LU trailing update, FP64 MFMA)",
+                "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP64_MFMA, "traffic": None,
+                "launches": st.lu_gemm_launches, "avg_launch_ms": gemm_avg_ms, "avg_launch_flop": gemm_avg_flop,
+            },
+        })
     if not args.no_check:
-        result["zin_check"] = small_check(lib)
-    m_cpu = None
+        result["small_deck_check"] = small_check(lib)
     if not args.no_cpu:
         sys.path.insert(0, str(ROOT / "tests"))
         from oracle_py import oracle_peec

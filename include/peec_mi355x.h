@@ -33,6 +33,9 @@ typedef struct {
 	long   gmres_its;       /* GMRES iterations summed over systems (mixed)   */
 	double kf_setup_ms;     /* k-factored fill preparation (mixed), in fill_ms */
 	long   kf_direct_pairs; /* pairs the k-factored fill evaluates directly   */
+	double mv_ms;           /* GMRES products A x: summed kernel time (mixed) */
+	double mv_bytes;        /* their algorithmic HBM bytes (matrix blocks read) */
+	long   mv_launches;
 } peec_stats_t;
 
 /* Library identification: "libpeec_mi355x <version> gfx950". */
@@ -60,7 +63,8 @@ int peec_sweep(peec_t *p, int ifreq0, int nfreq, int form, d_complex_t *zmat, pe
  * preconditioner; 1..8).  The mixed form keeps its preconditioner factorisation between calls. */
 int peec_sweep_ex(peec_t *p, int ifreq0, int nfreq, int form, int batch, d_complex_t *zmat, peec_stats_t *st);
 
-/* Drop the mixed form's cached preconditioner (the next mixed sweep factorises afresh). */
+/* Drop the mixed form's cached preconditioner and k-factored fill records (the next mixed sweep
+ * factorises and prepares afresh). */
 void peec_sweep_reset(void);
 
 /* Nominal FP64 operation count of the dense fills (Lp and charge-cell integrals, upper

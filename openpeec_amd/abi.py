@@ -121,7 +121,8 @@ class peec_stats_t(C.Structure):
                 ("lu_gemm_flop", C.c_double), ("lu_gemm_launches", C.c_int),
                 ("fill_flop", C.c_double), ("ising", C.c_int), ("refine_its", C.c_int),
                 ("berr", C.c_double), ("n_lu", C.c_int), ("gmres_its", C.c_long),
-                ("kf_setup_ms", C.c_double), ("kf_direct_pairs", C.c_long)]
+                ("kf_setup_ms", C.c_double), ("kf_direct_pairs", C.c_long),
+                ("mv_ms", C.c_double), ("mv_bytes", C.c_double), ("mv_launches", C.c_long)]
 
 
 def declare(lib: C.CDLL, reference: bool = False) -> None:

@@ -477,6 +477,12 @@ void pgpu_kf_fill(pgpu_t *g, double kw, d_complex_t *dlp, d_complex_t *dP)
 	}
 }
 
+void pgpu_kf_reset_all(void)
+{
+	std::lock_guard<std::mutex> lk(g_mu);
+	for (pgpu *g : g_tab) g->kf_ready = false;
+}
+
 int pgpu_kf_direct_pairs(pgpu_t *g)
 {
 	return g->kf_nd_lp + g->kf_nd_ig;

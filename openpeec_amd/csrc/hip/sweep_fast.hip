@@ -9,12 +9,13 @@
 //     streams Lp and P once (1.4 GB on the 64 x 64 plate) instead of a dense n^2 matrix;
 //   * one LU of A(f_ref) serves a run of frequencies as the right preconditioner of FP64
 //     flexible GMRES (the reference's acceleration = 1 scheme, solve.c:47-64 / iterative.c:
-//     230-252); its off-diagonal blocks are stored in FP32, its diagonal blocks as FP64
-//     inverses.  Flexible GMRES keeps the preconditioned directions, so the preconditioner may
-//     be applied inexactly (FP32 data, any summation order) without limiting the final
-//     residual;
+//     230-252).  The factors are inverted once (recursive blocking on MFMA GEMMs) and the two
+//     inverses stored in FP32, so applying the preconditioner is two bandwidth-bound triangular
+//     matrix-vector products instead of two chains of dependent substitution steps.  Flexible
+//     GMRES keeps the preconditioned directions, so the preconditioner may be applied inexactly
+//     (FP32 data, any summation order) without limiting the final residual;
 //   * frequencies advance in lock-step batches of F systems sharing the preconditioner, so each
-//     latency-bound triangular sweep serves F right-hand sides at once;
+//     pass over the inverses serves F right-hand sides at once;
 //   * GMRES runs to a relative residual of 1e-10 (ACCEL_TOL, solve.c:21), checked on the true
 //     residual b - A x in FP64; a system that fails to converge within the restart budget
 //     triggers a fresh factorisation at its own frequency (as accel_rhs does).
@@ -41,6 +42,7 @@ constexpr int MKRYLOV = 40;    // GMRES restart length
 constexpr int MCYCLES = 6;     // restarts before refactorising
 constexpr int TB = 64;         // triangular-solve block
 constexpr double TOL = 1e-10;  // relative residual
+constexpr int REFACTOR_MIN_ITS = 24;   // batches below this many iterations never trigger a refactor
 
 typedef std::complex<double> zc;
 
@@ -150,64 +152,125 @@ struct slots {
 	slot s[FMAX];
 };
 
-// Y[:, f] = A_f X[:, f] (vectors n x F row-major): one wave per row; the dense part is summed
-// lane-strided over the block's columns, two 16-byte loads in flight per lane.
+// Y[:, f] = A_f X[:, f] for f < nact (vectors n x F row-major; columns f >= nact are zeroed):
+// one wave per pair of rows serving every system of the batch.  A lane reads the F entries of X
+// for its column as one contiguous run and uses them for both rows; the dense block rows of the
+// F systems stream lane-strided over the block's columns.  The two rows of a wave lie in the same
+// dense block except where a block starts; such a pair is handled row by row.
+struct rowref {
+	bool dense, lrow;
+	int c0, nn;
+	long roff;
+};
+
+__device__ __forceinline__ rowref row_of(int row, int offS, int nseg, int offQ, int nc)
+{
+	rowref r;
+	r.lrow = (row >= offS) && (row < offS + nseg);
+	const bool q = (row >= offQ) && (row < offQ + nc);
+	r.dense = r.lrow || q;
+	r.c0 = r.lrow ? offS : offQ;
+	r.nn = r.lrow ? nseg : (q ? nc : 0);
+	r.roff = r.lrow ? (long)(row - offS) * nseg : (q ? (long)(row - offQ) * nc : 0);
+	return r;
+}
+
+template <int NR>
+__device__ __forceinline__ void dense_rows(const slots &S, int nact, int F, const rowref *rr, const double2 *__restrict__ X,
+                                           int lane, double (&sr)[2][FMAX], double (&si)[2][FMAX])
+{
+	const double2 *a[NR][FMAX];
+#pragma unroll
+	for (int k = 0; k < NR; k++)
+#pragma unroll
+		for (int f = 0; f < FMAX; f++)
+			a[k][f] = ((f < nact) ? (rr[k].lrow ? S.s[f].lp : S.s[f].P) : S.s[0].lp) + rr[k].roff;
+	const int c0 = rr[0].c0, nn = rr[0].nn;
+	for (int c = lane; c < nn; c += 64) {
+		const double2 *xc = X + (long)(c0 + c) * F;
+#pragma unroll
+		for (int f = 0; f < FMAX; f++) {
+			if (f < nact) {
+				const double2 x = xc[f];
+#pragma unroll
+				for (int k = 0; k < NR; k++) {
+					const double2 p = zmul(a[k][f][c], x);
+					sr[k][f] += p.x;
+					si[k][f] += p.y;
+				}
+			}
+		}
+	}
+}
+
 __global__ __launch_bounds__(256) void k_matvec(slots S, int n, int F, int nact, int offS, int nseg, int offQ, int nc,
                                                 const double2 *__restrict__ X, double2 *__restrict__ Y)
 {
-	const int f = blockIdx.y;
-	const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+	const int row0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 2;
 	const int lane = threadIdx.x & 63;
-	if (row >= n) return;
-	if (f >= nact) {   // slot without a system in this batch: its buffers hold nothing to read
-		if (lane == 0) Y[(long)row * F + f] = make_double2(0, 0);
-		return;
+	if (row0 >= n) return;
+	const int nr = min(2, n - row0);
+	rowref rr[2];
+	rr[0] = row_of(row0, offS, nseg, offQ, nc);
+	rr[1] = (nr > 1) ? row_of(row0 + 1, offS, nseg, offQ, nc) : rr[0];
+	double sr[2][FMAX], si[2][FMAX];
+#pragma unroll
+	for (int k = 0; k < 2; k++)
+#pragma unroll
+		for (int f = 0; f < FMAX; f++) sr[k][f] = si[k][f] = 0;
+	const bool paired = (nr == 2) && rr[0].dense && rr[1].dense && (rr[0].lrow == rr[1].lrow);
+	if (paired) {
+		dense_rows<2>(S, nact, F, rr, X, lane, sr, si);
 	}
-	const slot &sl = S.s[f];
-	double sr = 0, si = 0;
-	const double2 *a = nullptr;
-	int c0 = 0, nn = 0;
-	const bool lrow = (row >= offS) && (row < offS + nseg);
-	if (lrow) {
-		a = sl.lp + (long)(row - offS) * nseg;
-		c0 = offS;
-		nn = nseg;
-	}
-	else if ((row >= offQ) && (row < offQ + nc)) {
-		a = sl.P + (long)(row - offQ) * nc;
-		c0 = offQ;
-		nn = nc;
-	}
-	if (a) {
-		int c = lane;
-		for (; c + 64 < nn; c += 128) {
-			const double2 m0 = a[c], m1 = a[c + 64];
-			const double2 x0 = X[(long)(c0 + c) * F + f], x1 = X[(long)(c0 + c + 64) * F + f];
-			const double2 p0 = zmul(m0, x0), p1 = zmul(m1, x1);
-			sr += p0.x;
-			si += p0.y;
-			sr += p1.x;
-			si += p1.y;
-		}
-		if (c < nn) {
-			const double2 p = zmul(a[c], X[(long)(c0 + c) * F + f]);
-			sr += p.x;
-			si += p.y;
-		}
-		for (int off = 32; off > 0; off >>= 1) {
-			sr += __shfl_xor(sr, off);
-			si += __shfl_xor(si, off);
+	else {
+		for (int k = 0; k < nr; k++) {
+			if (!rr[k].dense) continue;
+			double tr[2][FMAX], ti[2][FMAX];
+#pragma unroll
+			for (int f = 0; f < FMAX; f++) tr[0][f] = ti[0][f] = tr[1][f] = ti[1][f] = 0;
+			dense_rows<1>(S, nact, F, &rr[k], X, lane, tr, ti);
+#pragma unroll
+			for (int f = 0; f < FMAX; f++) {
+				sr[k][f] = tr[0][f];
+				si[k][f] = ti[0][f];
+			}
 		}
 	}
-	if (lane == 0) {
-		double2 y = make_double2(sr, si);
-		if (lrow) y = make_double2(sl.omega * si, -sl.omega * sr);   // -jw * sum
-		for (int e = sl.rp[row]; e < sl.rp[row + 1]; e++) {
-			const double2 p = zmul(sl.val[e], X[(long)sl.ci[e] * F + f]);
-			y.x += p.x;
-			y.y += p.y;
+#pragma unroll
+	for (int k = 0; k < 2; k++)
+#pragma unroll
+		for (int f = 0; f < FMAX; f++) {
+			if (f >= nact) continue;
+			for (int off = 32; off > 0; off >>= 1) {
+				sr[k][f] += __shfl_xor(sr[k][f], off);
+				si[k][f] += __shfl_xor(si[k][f], off);
+			}
 		}
-		Y[(long)row * F + f] = y;
+	if (lane < 2 * F) {
+		const int k = lane / F, f = lane - k * F;
+		if (k < nr) {
+			const int row = row0 + k;
+			double2 y = make_double2(0, 0);
+			if (f < nact) {
+				const slot &sl = S.s[f];
+				double tr = 0, ti = 0;
+#pragma unroll
+				for (int kk = 0; kk < 2; kk++)
+#pragma unroll
+					for (int g = 0; g < FMAX; g++)
+						if ((kk == k) && (g == f)) {
+							tr = sr[kk][g];
+							ti = si[kk][g];
+						}
+				y = rr[k].lrow ? make_double2(sl.omega * ti, -sl.omega * tr) : make_double2(tr, ti);   // Lp rows: -jw * sum
+				for (int e = sl.rp[row]; e < sl.rp[row + 1]; e++) {
+					const double2 p = zmul(sl.val[e], X[(long)sl.ci[e] * F + f]);
+					y.x += p.x;
+					y.y += p.y;
+				}
+			}
+			Y[(long)row * F + f] = y;
+		}
 	}
 }
 
@@ -334,113 +397,98 @@ __global__ void k_permute(const double2 *src, double2 *dst, const int *perm, int
 	dst[e] = src[(long)perm[k] * F + f];
 }
 
-// ---------------------------------------------------------------- preconditioner sweeps
-// X <- T^-1 X for the packed LU factors (unit lower when LOWER, else upper), X n x F in FP64.
-// Left-looking by TB-row blocks: a workgroup takes the next block in solve order from a ticket
-// counter, accumulates the products with every block already solved as each one's flag is
-// published, applies the FP64 inverse of its diagonal block and publishes its own flag.  A
-// workgroup waits only for blocks whose tickets were taken before its own, by workgroups that
-// are already running, so every wait ends without needing co-resident workgroups.
-// Thread (r, q): row r of the block, column quarter q (16 columns).
-template <bool LOWER>
-__global__ __launch_bounds__(256) void k_trisolve(const float2 *__restrict__ M, long ld, int n, const double2 *__restrict__ dinv,
-                                                  double2 *X, int F, unsigned *ctl, unsigned epoch)
+// ---------------------------------------------------------------- preconditioner
+// The preconditioner is applied through explicit inverses of the two factors, Linv (unit lower)
+// and Uinv (upper), stored in FP32: z = Uinv (Linv (P v)).  Two triangular matrix-vector
+// products with F right-hand sides stream each inverse once (half of n^2 FP32 entries each) and
+// have no sequential dependence, unlike substitution, whose 64-row steps each wait for the
+// previous one.  The inverses are built once per factorisation by recursive blocking (below).
+
+// X[r][c] *= s over a rows x cols block (s = 0 clears, s = -1 negates)
+__global__ void k_scale_block(double2 *X, long ld, int rows, int cols, double sc)
 {
-	__shared__ double2 xs[TB * FMAX];
-	__shared__ double2 ys[TB * FMAX];
-	__shared__ int sb;
-	unsigned *ticket = ctl;
-	unsigned *flags = ctl + 1;
-	const int nblk = (n + TB - 1) / TB;
-	const int tid = threadIdx.x, r = tid >> 2, q = tid & 3;
-	if (tid == 0) sb = (int)atomicAdd(ticket, 1u);
-	__syncthreads();
-	const int t = sb;
-	const int b = LOWER ? t : (nblk - 1 - t);
-	const int r0 = b * TB, nb = min(TB, n - r0);
-	double ar[FMAX], ai[FMAX];
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= (long)rows * cols) return;
+	const int r = (int)(e / cols), c = (int)(e - (long)r * cols);
+	double2 *x = &X[(long)r * ld + c];
+	*x = (sc == 0) ? make_double2(0, 0) : make_double2(sc * x->x, sc * x->y);
+}
+
+// Out[r][c] = D[r][c] for an n x n corner of a TB x TB block
+__global__ void k_put_block(const double2 *D, double2 *Out, long ld, int n)
+{
+	const int e = blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= n * n) return;
+	const int r = e / n, c = e - r * n;
+	Out[(long)r * ld + c] = D[r * TB + c];
+}
+
+// Y[:, f] = Mt X[:, f] for a triangular FP32 matrix (lower when LOWER), X and Y n x F in FP64.
+// A workgroup owns 16 rows (4 per wave) and walks the 256-column chunks its rows touch; each
+// chunk of X is staged in LDS once and reused by the 16 rows; a lane takes 4 columns of the
+// chunk, so 16 matrix loads per lane are in flight between two barriers.
+constexpr int TG_CH = 256;
+template <bool LOWER>
+__global__ __launch_bounds__(256) void k_tgemv(const float2 *__restrict__ Mt, long ld, int n, const double2 *__restrict__ X,
+                                               double2 *__restrict__ Y, int F)
+{
+	__shared__ double2 xs[TG_CH * FMAX];
+	const int r0 = blockIdx.x * 16;
+	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	const int cbeg = LOWER ? 0 : ((r0 / TG_CH) * TG_CH);
+	const int cend = LOWER ? min(n, r0 + 16) : n;
+	double ar[4][FMAX], ai[4][FMAX];
 #pragma unroll
-	for (int f = 0; f < FMAX; f++) ar[f] = ai[f] = 0;
-	for (int s = 0; s < t; s++) {
-		const int j = LOWER ? s : (nblk - 1 - s);
-		const int c0 = j * TB, ncj = min(TB, n - c0);
-		if (tid == 0) {
-			while (__hip_atomic_load(&flags[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) __builtin_amdgcn_s_sleep(1);
-			__threadfence();
+	for (int q = 0; q < 4; q++)
+#pragma unroll
+		for (int f = 0; f < FMAX; f++) ar[q][f] = ai[q][f] = 0;
+	for (int c0 = cbeg; c0 < cend; c0 += TG_CH) {
+		const int nc = min(TG_CH, n - c0);
+		__syncthreads();
+		for (int e = threadIdx.x; e < nc * F; e += 256) xs[e] = X[(long)c0 * F + e];
+		__syncthreads();
+		float2 m[4][4];
+#pragma unroll
+		for (int q = 0; q < 4; q++) {
+			const int row = r0 + wave * 4 + q;
+#pragma unroll
+			for (int u = 0; u < 4; u++) {
+				const int c = lane + 64 * u;
+				m[q][u] = ((row < n) && (c < nc)) ? Mt[(long)row * ld + c0 + c] : make_float2(0, 0);
+			}
 		}
-		__syncthreads();
-		for (int e = tid; e < ncj * F; e += 256) xs[e] = X[(long)c0 * F + e];
-		__syncthreads();
-		if (r < nb) {
-			const float2 *mrow = M + (long)(r0 + r) * ld + c0 + q * 16;
-			const int cend = min(16, ncj - q * 16);
-			for (int cc = 0; cc < cend; cc++) {
-				const float2 m = mrow[cc];
-				const double mr = m.x, mi = m.y;
-				const double2 *xc = &xs[(q * 16 + cc) * F];
 #pragma unroll
-				for (int f = 0; f < FMAX; f++) {
-					if (f < F) {
-						const double2 x = xc[f];
-						ar[f] += (mr * x.x) - (mi * x.y);
-						ai[f] += (mr * x.y) + (mi * x.x);
+		for (int u = 0; u < 4; u++) {
+			const int c = lane + 64 * u;
+			if (c >= nc) continue;
+			const double2 *xc = &xs[c * F];
+#pragma unroll
+			for (int f = 0; f < FMAX; f++) {
+				if (f < F) {
+					const double2 x = xc[f];
+#pragma unroll
+					for (int q = 0; q < 4; q++) {
+						const double mr = m[q][u].x, mi = m[q][u].y;
+						ar[q][f] += (mr * x.x) - (mi * x.y);
+						ai[q][f] += (mr * x.y) + (mi * x.x);
 					}
 				}
 			}
 		}
-		__syncthreads();
 	}
-	// combine the four column quarters of each row, form the right-hand side of the block
 #pragma unroll
-	for (int f = 0; f < FMAX; f++) {
-		ar[f] += __shfl_xor(ar[f], 1);
-		ai[f] += __shfl_xor(ai[f], 1);
-		ar[f] += __shfl_xor(ar[f], 2);
-		ai[f] += __shfl_xor(ai[f], 2);
-	}
-	if (q == 0) {
-		for (int f = 0; f < F; f++) {
-			double2 y = make_double2(0, 0);
-			if (r < nb) {
-				const double2 x = X[(long)(r0 + r) * F + f];
-				y = make_double2(x.x - ar[f], x.y - ai[f]);
+	for (int q = 0; q < 4; q++) {
+		const int row = r0 + wave * 4 + q;
+#pragma unroll
+		for (int f = 0; f < FMAX; f++) {
+			if (f >= F) continue;
+			double sr = ar[q][f], si = ai[q][f];
+			for (int off = 32; off > 0; off >>= 1) {
+				sr += __shfl_xor(sr, off);
+				si += __shfl_xor(si, off);
 			}
-			ys[r * F + f] = y;
+			if ((lane == 0) && (row < n)) Y[(long)row * F + f] = make_double2(sr, si);
 		}
-	}
-	__syncthreads();
-	// x_b = Dinv_b y
-	double br[FMAX], bi[FMAX];
-#pragma unroll
-	for (int f = 0; f < FMAX; f++) br[f] = bi[f] = 0;
-	{
-		const double2 *drow = dinv + ((long)b * TB + r) * TB + q * 16;
-		for (int cc = 0; cc < 16; cc++) {
-			const double2 d = drow[cc];
-			const double2 *yc = &ys[(q * 16 + cc) * F];
-#pragma unroll
-			for (int f = 0; f < FMAX; f++) {
-				if (f < F) {
-					const double2 y = yc[f];
-					br[f] += (d.x * y.x) - (d.y * y.y);
-					bi[f] += (d.x * y.y) + (d.y * y.x);
-				}
-			}
-		}
-	}
-#pragma unroll
-	for (int f = 0; f < FMAX; f++) {
-		br[f] += __shfl_xor(br[f], 1);
-		bi[f] += __shfl_xor(bi[f], 1);
-		br[f] += __shfl_xor(br[f], 2);
-		bi[f] += __shfl_xor(bi[f], 2);
-	}
-	if ((q == 0) && (r < nb))
-		for (int f = 0; f < F; f++) X[(long)(r0 + r) * F + f] = make_double2(br[f], bi[f]);
-	__syncthreads();
-	if (tid == 0) {
-		__threadfence();
-		__hip_atomic_store(&flags[b], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 	}
 }
 
@@ -471,53 +519,145 @@ struct fast {
 	int n = 0, nc = 0, F = 0, nseg = 0;
 	slotbuf sl[FMAX];
 	double2 *luw = nullptr;   // FP64 factorisation workspace
-	float2 *M = nullptr;      // FP32 factors
+	double2 *inv = nullptr;   // FP64 inverse of one factor (built, then demoted)
+	double2 *scr = nullptr;   // GEMM scratch of the recursive inversion
+	float2 *ML = nullptr, *MU = nullptr;   // FP32 Linv, Uinv
 	double2 *dinvL = nullptr, *dinvU = nullptr;
 	int *piv = nullptr, *perm = nullptr;
 	double fref = -1;
+	bool stale = false;       // refactor before the next batch
+	int its_ref = -1;         // lock-step iterations of the first batch after a factorisation
 	double2 *V = nullptr;     // (MKRYLOV + 1) x n x F Arnoldi basis
 	double2 *Z = nullptr;     // MKRYLOV x n x F preconditioned directions
 	double2 *W = nullptr, *Xv = nullptr, *Bv = nullptr, *R = nullptr, *T1 = nullptr;
 	double2 *partial = nullptr, *h = nullptr, *yd = nullptr;
 	double *scale = nullptr;
-	unsigned *ctl = nullptr;  // ticket + per-block flags of the triangular sweeps
-	unsigned epoch = 0;
 };
 
-void tri_solve(fast &c, hipStream_t s, double2 *X, int F)
+void scale_block(hipStream_t s, double2 *X, long ld, int rows, int cols, double sc)
 {
-	const int nblk = (c.n + TB - 1) / TB;
-	for (int pass = 0; pass < 2; pass++) {
-		c.epoch++;
-		hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, c.ctl, 1);
-		if (pass == 0)
-			hipLaunchKernelGGL(k_trisolve<true>, dim3(nblk), dim3(256), 0, s, (const float2 *)c.M, (long)c.n, c.n,
-			                   (const double2 *)c.dinvL, X, F, c.ctl, c.epoch);
-		else
-			hipLaunchKernelGGL(k_trisolve<false>, dim3(nblk), dim3(256), 0, s, (const float2 *)c.M, (long)c.n, c.n,
-			                   (const double2 *)c.dinvU, X, F, c.ctl, c.epoch);
+	if ((rows <= 0) || (cols <= 0)) return;
+	hipLaunchKernelGGL(k_scale_block, dim3(blocks((long)rows * cols)), dim3(256), 0, s, X, ld, rows, cols, sc);
+}
+
+// Out[i0.., i0..] (n x n, leading dimension ld, zero on entry) = inverse of the unit-lower (lower)
+// or upper part of the packed factor T over the same rows/columns.  Halves split on TB
+// boundaries; with T = [T11 0; T21 T22] the off-diagonal block of the inverse is
+// -inv(T22) T21 inv(T11) (lower) and -inv(T11) T12 inv(T22) (upper), two GEMMs on MFMA.
+// Leaves are the diagonal-block inverses.  scr holds at least ceil(n/2)^2 + TB n entries.
+void tri_inv(hipStream_t s, bool lower, const double2 *T, long ld, double2 *Out, int i0, int n, const double2 *dinv,
+             double2 *scr)
+{
+	if (n <= TB) {
+		hipLaunchKernelGGL(k_put_block, dim3(blocks((long)n * n)), dim3(256), 0, s, dinv + (long)(i0 / TB) * TB * TB,
+		                   Out + (long)i0 * ld + i0, ld, n);
+		return;
+	}
+	const int n1 = ((n / 2 + TB - 1) / TB) * TB, n2 = n - n1;
+	tri_inv(s, lower, T, ld, Out, i0, n1, dinv, scr);
+	tri_inv(s, lower, T, ld, Out, i0 + n1, n2, dinv, scr);
+	const double2 *O11 = Out + (long)i0 * ld + i0;
+	const double2 *O22 = Out + (long)(i0 + n1) * ld + (i0 + n1);
+	if (lower) {
+		// C = -T21 inv(T11) (n2 x n1); Out21 = -(inv(T22) C), then negated
+		scale_block(s, scr, n1, n2, n1, 0);
+		dlu_zgemm_sub(s, n2, n1, n1, (const d_complex_t *)(T + (long)(i0 + n1) * ld + i0), ld, (const d_complex_t *)O11, ld,
+		              (d_complex_t *)scr, n1);
+		double2 *O21 = Out + (long)(i0 + n1) * ld + i0;
+		dlu_zgemm_sub(s, n2, n1, n2, (const d_complex_t *)O22, ld, (const d_complex_t *)scr, n1, (d_complex_t *)O21, ld);
+		scale_block(s, O21, ld, n2, n1, -1);
+	}
+	else {
+		// C = -T12 inv(T22) (n1 x n2); Out12 = -(inv(T11) C), then negated
+		scale_block(s, scr, n2, n1, n2, 0);
+		dlu_zgemm_sub(s, n1, n2, n2, (const d_complex_t *)(T + (long)i0 * ld + (i0 + n1)), ld, (const d_complex_t *)O22, ld,
+		              (d_complex_t *)scr, n2);
+		double2 *O12 = Out + (long)i0 * ld + (i0 + n1);
+		dlu_zgemm_sub(s, n1, n2, n1, (const d_complex_t *)O11, ld, (const d_complex_t *)scr, n2, (d_complex_t *)O12, ld);
+		scale_block(s, O12, ld, n1, n2, -1);
 	}
 }
 
-// z = M^-1 v (permutation gather, then the two sweeps), v and z n x F
+// both FP32 inverses from the FP64 factors in c.luw
+void build_inverses(fast &c, hipStream_t s)
+{
+	const int n = c.n, nblk = (n + TB - 1) / TB;
+	hipLaunchKernelGGL(k_diaginv<true>, dim3(nblk), dim3(64), 0, s, (const double2 *)c.luw, (long)n, n, c.dinvL);
+	hipLaunchKernelGGL(k_diaginv<false>, dim3(nblk), dim3(64), 0, s, (const double2 *)c.luw, (long)n, n, c.dinvU);
+	for (int pass = 0; pass < 2; pass++) {
+		const bool lower = (pass == 0);
+		hipLaunchKernelGGL(k_zero, dim3(blocks((long)n * n)), dim3(256), 0, s, c.inv, (long)n * n);
+		tri_inv(s, lower, c.luw, n, c.inv, 0, n, lower ? c.dinvL : c.dinvU, c.scr);
+		hipLaunchKernelGGL(k_demote, dim3(blocks((long)n * n)), dim3(256), 0, s, (const double2 *)c.inv, lower ? c.ML : c.MU,
+		                   (long)n * n);
+	}
+}
+
+// z = M^-1 v = Uinv (Linv (P v)), v and z n x F
 void precond(fast &c, hipStream_t s, const double2 *v, double2 *z, int F)
 {
 	hipLaunchKernelGGL(k_permute, dim3(blocks((long)c.n * F)), dim3(256), 0, s, v, z, c.perm, c.n, F);
-	tri_solve(c, s, z, F);
+	const unsigned g = (unsigned)((c.n + 15) / 16);
+	hipLaunchKernelGGL(k_tgemv<true>, dim3(g), dim3(256), 0, s, (const float2 *)c.ML, (long)c.n, c.n, (const double2 *)z, c.T1, F);
+	hipLaunchKernelGGL(k_tgemv<false>, dim3(g), dim3(256), 0, s, (const float2 *)c.MU, (long)c.n, c.n, (const double2 *)c.T1, z, F);
 }
 
 int g_offS = 0, g_offQ = 0;
 
+// kernel timing of the GMRES products (HIP events on the sweep's stream), read back when the
+// stream is next synchronised
+struct mvtimer {
+	bool on = false;
+	std::vector<hipEvent_t> ev;
+	int used = 0;
+	double bytes = 0;
+	void reset()
+	{
+		used = 0;
+		bytes = 0;
+	}
+} g_mvt;
+
 // y = A_f x for the first nact systems of the batch (columns f >= nact are set to zero)
 void matvec(fast &c, hipStream_t s, const double2 *x, double2 *y, int F, int nact)
 {
+	if (g_mvt.on) {
+		if ((int)g_mvt.ev.size() < g_mvt.used + 2) {
+			hipEvent_t e0, e1;
+			HIPCHK(hipEventCreate(&e0));
+			HIPCHK(hipEventCreate(&e1));
+			g_mvt.ev.push_back(e0);
+			g_mvt.ev.push_back(e1);
+		}
+		HIPCHK(hipEventRecord(g_mvt.ev[g_mvt.used], s));
+	}
 	slots S;
 	for (int f = 0; f < FMAX; f++) {
 		const slotbuf &b = c.sl[(f < nact) ? f : 0];
 		S.s[f] = slot{b.rp, b.ci, b.val, b.lp, b.P, b.omega};
 	}
-	hipLaunchKernelGGL(k_matvec, dim3((c.n + 3) / 4, F), dim3(256), 0, s, S, c.n, F, nact, g_offS, c.nseg, g_offQ, c.nc,
-	                   x, y);
+	hipLaunchKernelGGL(k_matvec, dim3((c.n + 7) / 8), dim3(256), 0, s, S, c.n, F, nact, g_offS, c.nseg, g_offQ, c.nc, x, y);
+	if (g_mvt.on) {
+		HIPCHK(hipEventRecord(g_mvt.ev[g_mvt.used + 1], s));
+		g_mvt.used += 2;
+		// matrix blocks of the nact systems (16 B per entry) + x, y and the CSR stamps
+		double b = 16.0 * nact * ((double)c.nseg * c.nseg + (double)c.nc * c.nc) + 32.0 * c.n * F;
+		for (int f = 0; f < nact; f++) b += 4.0 * (c.n + 1) + 20.0 * c.sl[f].nnz;
+		g_mvt.bytes += b;
+	}
+}
+
+// sum the recorded product timings into st (stream already synchronised by the caller)
+void mv_collect(pgpu_stats_t *st)
+{
+	for (int k = 0; k + 1 < g_mvt.used; k += 2) {
+		float ms = 0;
+		HIPCHK(hipEventElapsedTime(&ms, g_mvt.ev[k], g_mvt.ev[k + 1]));
+		st->mv_ms += ms;
+		st->mv_launches++;
+	}
+	st->mv_bytes += g_mvt.bytes;
+	g_mvt.reset();
 }
 
 // h[k][f] = V_k^H W for k < nk, copied to host
@@ -659,7 +799,7 @@ void free_fast(fast &c)
 		for (void *p : bb)
 			if (p) HIPCHK(hipFree(p));
 	}
-	void *bufs[] = {c.luw, c.M, c.dinvL, c.dinvU, c.piv, c.perm, c.V, c.Z, c.W, c.Xv, c.Bv, c.R, c.T1, c.partial, c.h, c.yd, c.scale, c.ctl};
+	void *bufs[] = {c.luw, c.inv, c.scr, c.ML, c.MU, c.dinvL, c.dinvU, c.piv, c.perm, c.V, c.Z, c.W, c.Xv, c.Bv, c.R, c.T1, c.partial, c.h, c.yd, c.scale};
 	for (void *b : bufs)
 		if (b) HIPCHK(hipFree(b));
 	c = fast();
@@ -710,6 +850,9 @@ const double2 *pgpu_p_dev(pgpu_t *g);
 void mna_stamps_charge(const peec_t *p, double f, stamps_t *st);
 }
 
+// PEEC_TRACE=1 prints the iteration counts of every batch to stderr
+static const bool g_trace = getenv("PEEC_TRACE") && getenv("PEEC_TRACE")[0] == '1';
+
 // PEEC_KF=0 in the environment switches the k-factored fill off (element-by-element fills)
 static bool pgpu_kf_enabled()
 {
@@ -743,7 +886,10 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 		}
 		const int nblk = (n + TB - 1) / TB;
 		c.luw = dal<double2>((size_t)n * n);
-		c.M = dal<float2>((size_t)n * n);
+		c.inv = dal<double2>((size_t)n * n);
+		c.scr = dal<double2>((size_t)((n + 1) / 2 + TB) * ((n + 1) / 2 + TB));
+		c.ML = dal<float2>((size_t)n * n);
+		c.MU = dal<float2>((size_t)n * n);
 		c.dinvL = dal<double2>((size_t)nblk * TB * TB);
 		c.dinvU = dal<double2>((size_t)nblk * TB * TB);
 		c.piv = dal<int>(n);
@@ -760,9 +906,6 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 		c.h = dal<double2>((size_t)(MKRYLOV + 1) * FMAX);
 		c.yd = dal<double2>((size_t)(MKRYLOV + 1) * FMAX);
 		c.scale = dal<double>(FMAX);
-		c.ctl = dal<unsigned>(1 + nblk);
-		hipLaunchKernelGGL(k_zero_u32, dim3(blocks(1 + nblk, 64)), dim3(64), 0, s, c.ctl, 1 + nblk);
-		c.epoch = 0;
 	}
 	g_offS = p->offS;
 	g_offQ = offQ;
@@ -826,12 +969,9 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 			if (sb.nnz > 0)
 				hipLaunchKernelGGL(k_stamp, dim3(blocks(sb.nnz)), dim3(256), 0, s, c.luw, (long)n, sb.ri, sb.ci,
 				                   (const double2 *)sb.val, sb.nnz);
-			const int info = dlu_factor(s, n, (d_complex_t *)c.luw, n, c.piv, c.perm, nullptr);
+			const int info = dlu_factor(s, n, (d_complex_t *)c.luw, n, c.piv, c.perm, st);
 			if (info >= 0) return 2;
-			hipLaunchKernelGGL(k_demote, dim3(blocks((long)n * n)), dim3(256), 0, s, c.luw, c.M, (long)n * n);
-			const int nblk = (n + TB - 1) / TB;
-			hipLaunchKernelGGL(k_diaginv<true>, dim3(nblk), dim3(64), 0, s, (const double2 *)c.luw, (long)n, n, c.dinvL);
-			hipLaunchKernelGGL(k_diaginv<false>, dim3(nblk), dim3(64), 0, s, (const double2 *)c.luw, (long)n, n, c.dinvU);
+			build_inverses(c, s);
 			HIPCHK(hipStreamSynchronize(s));
 			c.fref = freq_at(p, ifreq0 + f0 + b);
 			if (st) {
@@ -840,10 +980,16 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 			}
 			return 0;
 		};
-		if (c.fref < 0) {
-			rc = refactor(0);
+		// a fresh factorisation when there is none, or when the last batch needed markedly more
+		// iterations than the first batch after the last factorisation (the preconditioner has
+		// drifted too far from the current frequencies): at the middle of this batch
+		if ((c.fref < 0) || c.stale) {
+			rc = refactor(nb / 2);
 			if (rc) break;
+			c.stale = false;
+			c.its_ref = -1;
 		}
+		int its_batch = 0;
 		// every port: batched GMRES over the nb systems
 		for (int j = 0; (j < np) && !rc; j++) {
 			std::vector<double2> hb((size_t)n * F, make_double2(0, 0));
@@ -853,18 +999,28 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 			}
 			HIPCHK(hipMemcpyAsync(c.Bv, hb.data(), hb.size() * sizeof(double2), hipMemcpyHostToDevice, s));
 			const auto tg = std::chrono::steady_clock::now();
+			g_mvt.on = (st != nullptr);
+			g_mvt.reset();
 			std::vector<int> its = gmres_batch(c, s, F, nb);
+			if (st) mv_collect(st);
 			for (int b = 0; b < nb; b++) {
 				if (its[b] < 0) {
 					// stale preconditioner: refactor at this frequency and solve the batch again
 					if ((rc = refactor(b))) break;
 					its = gmres_batch(c, s, F, nb);
+					if (st) mv_collect(st);
 					if (its[b] < 0) {
 						rc = 4;
 						break;
 					}
 				}
 				if (st) st->gmres_its += its[b];
+				its_batch = std::max(its_batch, its[b]);
+			}
+			if (g_trace) {
+				fprintf(stderr, "[sweep_fast] points %d..%d port %d its", ifreq0 + f0, ifreq0 + f0 + nb - 1, j);
+				for (int b = 0; b < nb; b++) fprintf(stderr, " %d", its[b]);
+				fprintf(stderr, " (ref %.4g Hz)\n", c.fref);
 			}
 			if (st) st->solve_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tg).count();
 			HIPCHK(hipMemcpyAsync(hx.data(), c.Xv, hx.size() * sizeof(double2), hipMemcpyDeviceToHost, s));
@@ -876,6 +1032,8 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 					zmat[((size_t)(f0 + b) * np + i) * np + j] = d_complex(v1.x - v2.x, v1.y - v2.y);
 				}
 		}
+		if (c.its_ref < 0) c.its_ref = its_batch;
+		else if (its_batch > std::max(REFACTOR_MIN_ITS, 2 * c.its_ref)) c.stale = true;
 		if (st) st->total_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 	}
 	return rc;
@@ -885,4 +1043,6 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 extern "C" void pgpu_fast_reset(void)
 {
 	g_fast.fref = -1;
+	g_fast.stale = false;
+	g_fast.its_ref = -1;
 }

@@ -373,6 +373,7 @@ int peec_sweep(peec_t *p, int ifreq0, int nfreq, int form, d_complex_t *zmat, pe
 void peec_sweep_reset(void)
 {
 	pgpu_fast_reset();
+	pgpu_kf_reset_all();
 }
 
 /* wavenumber band of the deck's sweep together with f */

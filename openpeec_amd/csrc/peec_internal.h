@@ -89,6 +89,7 @@ void pgpu_cmat_from_host(pgpu_t *g, const d_complex_t *cmat);
 int  pgpu_kf_setup(pgpu_t *g, double klo, double khi);
 void pgpu_kf_fill(pgpu_t *g, double kw, d_complex_t *dlp, d_complex_t *dP);
 int  pgpu_kf_direct_pairs(pgpu_t *g);
+void pgpu_kf_reset_all(void);     /* every device mirror prepares its records afresh */
 /* Lp and P at kw (mode 0 element code, mode 1 k-factored, set up beforehand) to host buffers */
 void pgpu_fill_to_host(pgpu_t *g, double kw, int mode, d_complex_t *lp, d_complex_t *P);
 
