@@ -242,9 +242,54 @@ d_complex_t zint_round(double len, double a, double sigma, double freq);
 d_complex_t zint_bar(double len, double area, double perim, double sigma, double freq);
 d_complex_t zint_seg(const seg_t *s, double freq);
 
+/* ---- H-matrix compression (reference peec.h:277-296); device resident ---- */
+struct hmat_t;
+typedef d_complex_t (*hmat_entry_fn)(void *ctx, int i, int j, double kw);
+/* Entries are evaluated on the GPU: fn must be the library's own lp_entry / pot_entry kernels
+ * (hmat_build / hmat_build_pot pass them); any other callback is refused (NULL returned). */
+struct hmat_t *hmat_build_gen(int n, const double *pts, hmat_entry_fn fn, void *ctx, double kw,
+	double tol, const char *label, FILE *fp_log);                                /* hmatrix.c:403 */
+struct hmat_t *hmat_build(peec_t *p, double f, FILE *fp_log);                   /* hmatrix.c:616 */
+struct hmat_t *hmat_build_pot(peec_t *p, double f, FILE *fp_log);               /* potential.c:158 */
+void hmat_free(struct hmat_t *h);                                               /* hmatrix.c:636 */
+void hmat_matvec(const struct hmat_t *h, const d_complex_t *x, d_complex_t *y); /* hmatrix.c:680 */
+int  hmat_nleaf(const struct hmat_t *h);                                        /* hmatrix.c:807 */
+const d_complex_t *hmat_leaf_block(const struct hmat_t *h, int r, const int **idx, int *m); /* hmatrix.c:812 */
+double hmat_memory_mb(const struct hmat_t *h);                                  /* hmatrix.c:656 */
+double hmat_dense_mb(int n);                                                    /* hmatrix.c:675 */
+
 /* ---- MNA (reference peec.h:319-352) ---- */
+typedef struct {
+	int *ri, *ci;
+	d_complex_t *val;
+	int  nnz, cap;
+} mna_sparse_t;                                                                 /* peec.h:322-326 */
+
+typedef struct {
+	int off, n;
+	const struct hmat_t *h;
+	d_complex_t scale;
+} hblock_t;                                                                     /* peec.h:334-338 */
+
+#define HBLK_MAX 2
+
 int  mna_numbering(peec_t *p, FILE *fp_log);                                    /* mna.c:16  */
+void mna_sparse(const peec_t *p, double f, mna_sparse_t *sp);                   /* mna.c:317 */
+void mna_sparse_free(mna_sparse_t *sp);                                         /* mna.c:326 */
+void mna_apply(const peec_t *p, const mna_sparse_t *sp, const hblock_t *blk, int nblk,
+	const d_complex_t *x, d_complex_t *y, d_complex_t *work);                     /* mna.c:345 */
 void mna_rhs_port(const peec_t *p, int iport, d_complex_t *b);                  /* mna.c:367 */
+
+/* ---- leaf-block + circuit-Schur preconditioner (reference peec.h:358-364); device resident ---- */
+struct precond_t;
+struct precond_t *pc_build(const peec_t *p, const mna_sparse_t *sp, const hblock_t *blk, int nblk); /* precond.c:139 */
+void pc_free(struct precond_t *pc);                                             /* precond.c:78  */
+void pc_apply(const struct precond_t *pc, d_complex_t *b);                      /* precond.c:401 */
+double pc_memory_mb(const struct precond_t *pc);                                /* precond.c:435 */
+
+/* ---- iterative solvers (reference peec.h:366-371) ---- */
+int  gmres_hmat(const peec_t *p, const mna_sparse_t *sp, const hblock_t *blk, int nblk,
+	const struct precond_t *pc, d_complex_t *b, double tol);                      /* iterative.c:285 */
 
 /* ---- dense solver (reference peec.h:354-356) ---- */
 int  lu_decomp(int n, d_complex_t *a, int *piv);                                /* lu.c:16 */

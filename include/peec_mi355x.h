@@ -78,6 +78,17 @@ double peec_fill_flops(const peec_t *p, double f);
  * for the deck's whole frequency band.  Returns 0, or 1 when mode 1 cannot be prepared. */
 int peec_fill_matrices(peec_t *p, double f, int mode, d_complex_t *lp, d_complex_t *P);
 
+/* Block structure of a compressed matrix (for tests): counts = {far blocks, near blocks,
+ * leaves, n}; perm (n) the cluster order; ranks (far blocks) the ACA ranks, -1 where a far block
+ * is stored dense.  perm / ranks may be NULL.  Returns 0, or 1 for a NULL matrix. */
+struct hmat_t;
+int hmat_info(const struct hmat_t *h, int *counts, int *perm, int *ranks);
+
+/* The H-matrix block partition of n points (host only, no device): counts = {far blocks, near
+ * blocks, leaves, n}; perm (n); far / near receive (r0, m, c0, nn) per block when non-NULL
+ * (call first with NULL to size them).  Returns 0, or 1 when the leaves do not tile [0, n). */
+int hmat_tree_info(int n, const double *pts, int *counts, int *perm, int *far, int *near);
+
 /* Sizes for bindings */
 size_t peec_sizeof_peec(void);
 size_t peec_sizeof_seg(void);

@@ -114,6 +114,15 @@ class peec_t(C.Structure):
     ]
 
 
+class mna_sparse_t(C.Structure):
+    _fields_ = [("ri", C.POINTER(C.c_int)), ("ci", C.POINTER(C.c_int)), ("val", C.POINTER(d_complex_t)),
+                ("nnz", C.c_int), ("cap", C.c_int)]
+
+
+class hblock_t(C.Structure):
+    _fields_ = [("off", C.c_int), ("n", C.c_int), ("h", C.c_void_p), ("scale", d_complex_t)]
+
+
 class peec_stats_t(C.Structure):
     _fields_ = [("form", C.c_int), ("fill_ms", C.c_double), ("pot_ms", C.c_double),
                 ("cinv_ms", C.c_double), ("asm_ms", C.c_double), ("lu_ms", C.c_double),
@@ -143,6 +152,25 @@ def declare(lib: C.CDLL, reference: bool = False) -> None:
         "solve": (C.c_int, [PP, C.c_void_p]),
         "lu_decomp": (C.c_int, [C.c_int, PZ, C.POINTER(C.c_int)]),
         "lu_solve": (None, [C.c_int, PZ, C.POINTER(C.c_int), PZ]),
+        "hmat_build": (C.c_void_p, [PP, C.c_double, C.c_void_p]),
+        "hmat_build_pot": (C.c_void_p, [PP, C.c_double, C.c_void_p]),
+        "hmat_build_gen": (C.c_void_p, [C.c_int, C.POINTER(C.c_double), C.c_void_p, C.c_void_p, C.c_double,
+                                         C.c_double, C.c_char_p, C.c_void_p]),
+        "hmat_free": (None, [C.c_void_p]),
+        "hmat_matvec": (None, [C.c_void_p, PZ, PZ]),
+        "hmat_nleaf": (C.c_int, [C.c_void_p]),
+        "hmat_leaf_block": (PZ, [C.c_void_p, C.c_int, C.POINTER(C.POINTER(C.c_int)), C.POINTER(C.c_int)]),
+        "hmat_memory_mb": (C.c_double, [C.c_void_p]),
+        "hmat_dense_mb": (C.c_double, [C.c_int]),
+        "mna_sparse": (None, [PP, C.c_double, C.POINTER(mna_sparse_t)]),
+        "mna_sparse_free": (None, [C.POINTER(mna_sparse_t)]),
+        "mna_apply": (None, [PP, C.POINTER(mna_sparse_t), C.POINTER(hblock_t), C.c_int, PZ, PZ, PZ]),
+        "pc_build": (C.c_void_p, [PP, C.POINTER(mna_sparse_t), C.POINTER(hblock_t), C.c_int]),
+        "pc_free": (None, [C.c_void_p]),
+        "pc_apply": (None, [C.c_void_p, PZ]),
+        "pc_memory_mb": (C.c_double, [C.c_void_p]),
+        "gmres_hmat": (C.c_int, [PP, C.POINTER(mna_sparse_t), C.POINTER(hblock_t), C.c_int, C.c_void_p, PZ,
+                                 C.c_double]),
         "neumann_self": (C.c_double, [C.c_double, C.c_double]),
         "neumann_pair": (C.c_double, [PS, PS, C.c_double, C.c_double]),
         "neumann_self_k": (d_complex_t, [PS, C.c_double, C.c_double]),
@@ -172,6 +200,9 @@ def declare(lib: C.CDLL, reference: bool = False) -> None:
             "peec_sizeof_peec": (C.c_size_t, []),
             "peec_fill_flops": (C.c_double, [PP, C.c_double]),
             "peec_fill_matrices": (C.c_int, [PP, C.c_double, C.c_int, PZ, PZ]),
+            "hmat_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+            "hmat_tree_info": (C.c_int, [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int), C.POINTER(C.c_int)]),
             "peec_sizeof_seg": (C.c_size_t, []),
         })
     for name, (res, args) in sig.items():

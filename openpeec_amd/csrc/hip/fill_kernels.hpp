@@ -43,3 +43,8 @@ void kf_rho_launch(hipStream_t s, const dcell_t *C, const dedge_t *edge, int n, 
 void kf_setup_launch(hipStream_t s, int which, int km, const pe::fill_args &f, double klo, double khi, const double *rho,
                      double *rec, int T, int2 *list, unsigned *nlist);
 void kf_eval_launch(hipStream_t s, int which, const pe::fill_args &f, const double *rec, int T);
+
+// cell set of a device mirror as fill arguments (gpu_runtime.hip); which 0 current cells, 1
+// charge cells (carea set, sub-cell CSR returned)
+pe::fill_args pgpu_set_args(pgpu_t *g, int which, double kw, int *km, const int **csoff, const int **csidx);
+hipStream_t pgpu_hip_stream(pgpu_t *g);

@@ -287,6 +287,24 @@ void ensure_mna(pgpu *g, const peec_t *p)
 
 }  // namespace
 
+// fill arguments of one cell set (0: current cells for Lp, 1: charge cells) for other modules
+pe::fill_args pgpu_set_args(pgpu_t *g, int which, double kw, int *km, const int **csoff, const int **csidx)
+{
+	const pset &ps = (which == 0) ? g->seg : g->chg;
+	if (which == 1) ensure_cells(g);
+	pe::fill_args a = base_args(ps, ps.n, kw, g->p->gp, nullptr, 0);
+	if (which == 1) a.carea = g->carea;
+	if (km) *km = ps.km;
+	if (csoff) *csoff = g->csoff;
+	if (csidx) *csidx = g->csidx;
+	return a;
+}
+
+hipStream_t pgpu_hip_stream(pgpu_t *g)
+{
+	return g->s;
+}
+
 // ------------------------------------------------------------------------------------------
 extern "C" {
 

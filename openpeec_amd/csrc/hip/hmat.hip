@@ -1,0 +1,779 @@
+// hmat.hip — H-matrix compression of Lp and P on the device (compression = 1).
+//
+// The block structure (cluster tree, far/near blocks, leaf-row grouping) is the reference's
+// (hmatrix.c:43-227, 483-586) and is built on the host (hmat_host.c).  Everything that touches
+// matrix entries runs here:
+//   * ACA, one 256-thread workgroup per far block (k_aca): the reference's partial-pivot
+//     adaptive cross approximation with its restarts from unrepresented rows (aca_block,
+//     hmatrix.c:250-395), step for step; the entry evaluations of a residual row/column and
+//     the norms are spread over the workgroup, reductions run in a fixed tree order, pivot
+//     searches keep the reference's first-extremum rule;
+//   * near blocks and far blocks that did not compress are evaluated densely (k_hm_dense);
+//   * y = H x in two stages as the reference's hmat_matvec (hmatrix.c:680-778): t_b = V_b^T x
+//     per far block, then one workgroup per leaf row cluster adds U_b t_b and the dense rows of
+//     its blocks in the registered order, so every output row is written by one workgroup.
+// Entries come from the same device code as the dense fills (lp_entry / the aggregated
+// pot_entry rule, fill_entries.hpp), so compressed and dense paths see identical numbers.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+#include "elements.hpp"
+#include "fill_entries.hpp"
+#include "fill_kernels.hpp"
+#include "hip_check.hpp"
+#include "hmat_dev.hpp"
+
+namespace pe {
+
+// one entry of the kernel matrix in original numbering: Lp (kind 0) or P (kind 1, summed over
+// the sub-cells of both capacitive cells, pot_entry potential.c:124-150)
+template <int KM>
+__device__ cplx hm_entry(const hent &E, int i, int j)
+{
+	if (E.kind == 0) {
+		if (i > j) {
+			const int t = i;
+			i = j;
+			j = t;
+		}
+		return lp_entry<KM>(E.f.S, E.f.pS, E.f.SM, E.f.pSM, i, j, E.f.kw, E.f.coef, E.f.gp);
+	}
+	double sr = 0, si = 0;
+	for (int p = E.csoff[i]; p < E.csoff[i + 1]; p++) {
+		const int h1 = E.csidx[p];
+		for (int q = E.csoff[j]; q < E.csoff[j + 1]; q++) {
+			const int h2 = E.csidx[q];
+			const int lo = (h1 <= h2) ? h1 : h2, hi = (h1 <= h2) ? h2 : h1;
+			const cplx v = ig_entry<KM>(E.f.S, E.f.pS, E.f.SM, E.f.pSM, lo, hi, E.f.kw, E.f.gp);
+			sr = sr + v.r;
+			si = si + v.i;
+		}
+	}
+	const double s = 1 / (E.f.k4pe * E.f.carea[i] * E.f.carea[j]);
+	return mk(s * sr, s * si);
+}
+
+constexpr int AT = 256;
+
+__device__ __forceinline__ double wg_sum(double x, double *red)
+{
+	const int t = threadIdx.x;
+	red[t] = x;
+	__syncthreads();
+	for (int s = AT / 2; s > 0; s >>= 1) {
+		if (t < s) red[t] += red[t + s];
+		__syncthreads();
+	}
+	const double r = red[0];
+	__syncthreads();
+	return r;
+}
+
+__device__ __forceinline__ double wg_max(double x, double *red)
+{
+	const int t = threadIdx.x;
+	red[t] = x;
+	__syncthreads();
+	for (int s = AT / 2; s > 0; s >>= 1) {
+		if ((t < s) && (red[t + s] > red[t])) red[t] = red[t + s];
+		__syncthreads();
+	}
+	const double r = red[0];
+	__syncthreads();
+	return r;
+}
+
+// largest value, ties to the smaller index; (0, -1) when nothing positive was seen.  Each thread
+// scanned its indices in increasing order keeping only strictly larger values, so the result
+// is the first index of the maximum, as a serial scan with ">" would find.
+__device__ __forceinline__ void wg_argmax(double &v, int &idx, double *rv, int *ri)
+{
+	const int t = threadIdx.x;
+	rv[t] = v;
+	ri[t] = idx;
+	__syncthreads();
+	for (int s = AT / 2; s > 0; s >>= 1) {
+		if (t < s) {
+			const double ov = rv[t + s];
+			const int oi = ri[t + s];
+			if ((oi >= 0) && ((ri[t] < 0) || (ov > rv[t]) || ((ov == rv[t]) && (oi < ri[t])))) {
+				rv[t] = ov;
+				ri[t] = oi;
+			}
+		}
+		__syncthreads();
+	}
+	v = rv[0];
+	idx = ri[0];
+	__syncthreads();
+}
+
+// unused row with the smallest representation (first index among equals), if at most thr
+// (aca_unrepresented, hmatrix.c:233-248)
+__device__ int unrepresented(const int *rused, const double *urep, int m, double thr, double *rv, int *ri)
+{
+	double best = 0;
+	int nxt = -1;
+	for (int i = threadIdx.x; i < m; i += AT) {
+		if (rused[i]) continue;
+		if ((nxt < 0) || (urep[i] < best)) {
+			best = urep[i];
+			nxt = i;
+		}
+	}
+	const int t = threadIdx.x;
+	rv[t] = best;
+	ri[t] = nxt;
+	__syncthreads();
+	for (int s = AT / 2; s > 0; s >>= 1) {
+		if (t < s) {
+			const double ov = rv[t + s];
+			const int oi = ri[t + s];
+			if ((oi >= 0) && ((ri[t] < 0) || (ov < rv[t]) || ((ov == rv[t]) && (oi < ri[t])))) {
+				rv[t] = ov;
+				ri[t] = oi;
+			}
+		}
+		__syncthreads();
+	}
+	best = rv[0];
+	nxt = ri[0];
+	__syncthreads();
+	return ((nxt >= 0) && (best <= thr)) ? nxt : -1;
+}
+
+// a - u v  (d_sub(a, d_mul(u, v)))
+__device__ __forceinline__ void sub_uv(double &ar, double &ai, double2 u, double2 v)
+{
+	const double pr = (u.x * v.x) - (u.y * v.y);
+	const double pi = (u.x * v.y) + (u.y * v.x);
+	ar = ar - pr;
+	ai = ai - pi;
+}
+
+template <int KM>
+__global__ __launch_bounds__(AT) void k_aca(aca_args a)
+{
+	__shared__ double rv[AT];
+	__shared__ int ri[AT];
+	__shared__ double2 cdu[HM_MAXIT], cdv[HM_MAXIT];
+	__shared__ double sh_cross;
+	const int b = blockIdx.x, tid = threadIdx.x;
+	const int wave = tid >> 6, lane = tid & 63;
+	const hfar B = a.far[b];
+	const int r0 = B.r0, m = B.m, c0 = B.c0, nn = B.nn;
+	const int kmax = (m < nn) ? m : nn;
+	const int lim = (kmax < HM_MAXIT) ? kmax : HM_MAXIT;
+	double2 *U = a.scr + a.soff[b];         // [lim][m]
+	double2 *V = U + (long)m * lim;         // [lim][nn]
+	double *urep = a.urep + a.moff[b];
+	int *rused = a.rused + a.moff[b];
+	for (int i = tid; i < m; i += AT) {
+		urep[i] = 0;
+		rused[i] = 0;
+	}
+	__syncthreads();
+	double fro2 = 0, urmax = 0;
+	int k = 0, irow = 0, probe = 0;
+	while (k < lim) {
+		// residual row  v = A(irow, :) - sum_l u_l[irow] v_l
+		double amax = 0, rn2 = 0;
+		int jmax = -1;
+		const int gi = a.perm[r0 + irow];
+		for (int j = tid; j < nn; j += AT) {
+			const cplx e = hm_entry<KM>(a.E, gi, a.perm[c0 + j]);
+			double ar = e.r, ai = e.i;
+			for (int l = 0; l < k; l++) sub_uv(ar, ai, U[(long)l * m + irow], V[(long)l * nn + j]);
+			V[(long)k * nn + j] = make_double2(ar, ai);
+			const double t = sqrt((ar * ar) + (ai * ai));
+			rn2 += t * t;
+			if (t > amax) {
+				amax = t;
+				jmax = j;
+			}
+		}
+		rn2 = wg_sum(rn2, rv);
+		wg_argmax(amax, jmax, rv, ri);
+		if (tid == 0) rused[irow] = 1;
+		__syncthreads();
+		if ((jmax < 0) || (amax <= 0) || (sqrt(rn2) <= a.tol * sqrt(fro2))) {
+			if (++probe > HM_PROBE) break;
+			const int nxt = unrepresented(rused, urep, m, a.tol * urmax, rv, ri);
+			if (nxt < 0) break;
+			irow = nxt;
+			continue;
+		}
+		probe = 0;
+		// normalise the row by its pivot (d_div)
+		const double2 piv = V[(long)k * nn + jmax];
+		__syncthreads();
+		const double den = (piv.x * piv.x) + (piv.y * piv.y);
+		for (int j = tid; j < nn; j += AT) {
+			const double2 x = V[(long)k * nn + j];
+			V[(long)k * nn + j] = ((fabs(piv.x) <= 0) && (fabs(piv.y) <= 0))
+			                          ? make_double2(0, 0)
+			                          : make_double2(((x.x * piv.x) + (x.y * piv.y)) / den, ((x.y * piv.x) - (x.x * piv.y)) / den);
+		}
+		__syncthreads();
+		// residual column  u = A(:, jmax) - sum_l u_l v_l[jmax]
+		const int gj = a.perm[c0 + jmax];
+		for (int i = tid; i < m; i += AT) {
+			const cplx e = hm_entry<KM>(a.E, a.perm[r0 + i], gj);
+			double ar = e.r, ai = e.i;
+			for (int l = 0; l < k; l++) sub_uv(ar, ai, U[(long)l * m + i], V[(long)l * nn + jmax]);
+			U[(long)k * m + i] = make_double2(ar, ai);
+		}
+		__syncthreads();
+		// norms of u_k, v_k and the cross terms of ||A_k||_F^2
+		double un = 0, vn = 0;
+		for (int i = tid; i < m; i += AT) {
+			const double2 x = U[(long)k * m + i];
+			un += (x.x * x.x) + (x.y * x.y);
+		}
+		for (int j = tid; j < nn; j += AT) {
+			const double2 x = V[(long)k * nn + j];
+			vn += (x.x * x.x) + (x.y * x.y);
+		}
+		un = sqrt(wg_sum(un, rv));
+		vn = sqrt(wg_sum(vn, rv));
+		for (int l = wave; l < k; l += AT / 64) {
+			double dur = 0, dui = 0, dvr = 0, dvi = 0;
+			for (int i = lane; i < m; i += 64) {
+				const double2 p = U[(long)l * m + i], q = U[(long)k * m + i];   // conj(p) q
+				dur += (p.x * q.x) + (p.y * q.y);
+				dui += (p.x * q.y) - (p.y * q.x);
+			}
+			for (int j = lane; j < nn; j += 64) {
+				const double2 p = V[(long)l * nn + j], q = V[(long)k * nn + j];
+				dvr += (p.x * q.x) + (p.y * q.y);
+				dvi += (p.x * q.y) - (p.y * q.x);
+			}
+			for (int off = 32; off > 0; off >>= 1) {
+				dur += __shfl_xor(dur, off);
+				dui += __shfl_xor(dui, off);
+				dvr += __shfl_xor(dvr, off);
+				dvi += __shfl_xor(dvi, off);
+			}
+			if (lane == 0) {
+				cdu[l] = make_double2(dur, dui);
+				cdv[l] = make_double2(dvr, dvi);
+			}
+		}
+		__syncthreads();
+		if (tid == 0) {
+			double cross = 0;
+			for (int l = 0; l < k; l++) {
+				const double2 p = cdu[l], q = cdv[l];
+				const double pr = (p.x * q.x) - (p.y * q.y), pi = (p.x * q.y) + (p.y * q.x);
+				cross += 2 * sqrt((pr * pr) + (pi * pi));
+			}
+			sh_cross = cross;
+		}
+		__syncthreads();
+		fro2 += (un * un * vn * vn) + sh_cross;
+		k++;
+		// how strongly each row appears in the approximation (reducible blocks)
+		double lmax = 0;
+		for (int i = tid; i < m; i += AT) {
+			const double2 x = U[(long)(k - 1) * m + i];
+			const double t = sqrt((x.x * x.x) + (x.y * x.y));
+			if (t > urep[i]) urep[i] = t;
+			if (t > lmax) lmax = t;
+		}
+		lmax = wg_max(lmax, rv);
+		if (lmax > urmax) urmax = lmax;
+		// next pivot row: largest unused entry of u_k
+		int inext = -1;
+		if ((un * vn) > a.tol * sqrt((fro2 > 0) ? fro2 : 1)) {
+			double umax = 0;
+			for (int i = tid; i < m; i += AT) {
+				if (rused[i]) continue;
+				const double2 x = U[(long)(k - 1) * m + i];
+				const double t = sqrt((x.x * x.x) + (x.y * x.y));
+				if (t > umax) {
+					umax = t;
+					inext = i;
+				}
+			}
+			wg_argmax(umax, inext, rv, ri);
+		}
+		if (inext < 0) {
+			if (++probe > HM_PROBE) break;
+			inext = unrepresented(rused, urep, m, a.tol * urmax, rv, ri);
+			if (inext < 0) break;
+		}
+		irow = inext;
+	}
+	if (tid == 0) a.rank[b] = ((k <= 0) || (k >= lim)) ? -1 : k;
+}
+
+// dense blocks (near, and far blocks ACA could not compress): D[i][j] = A(perm[r0+i], perm[c0+j])
+template <int KM>
+__global__ __launch_bounds__(AT) void k_hm_dense(hent E, const int *perm, const hdense *blk, double2 *D)
+{
+	const hdense B = blk[blockIdx.x];
+	const long tot = (long)B.m * B.nn;
+	for (long e = threadIdx.x; e < tot; e += AT) {
+		const int i = (int)(e / B.nn), j = (int)(e - (long)i * B.nn);
+		const cplx v = hm_entry<KM>(E, perm[B.r0 + i], perm[B.c0 + j]);
+		D[B.off + e] = make_double2(v.r, v.i);
+	}
+}
+
+// compact the ACA factors of block b from the scratch into the U / V pools
+__global__ void k_aca_compact(const hfar *far, const long *soff, const double2 *scr, double2 *Upool, double2 *Vpool)
+{
+	const hfar B = far[blockIdx.x];
+	if (B.rank <= 0) return;
+	const int kmax = (B.m < B.nn) ? B.m : B.nn;
+	const int lim = (kmax < HM_MAXIT) ? kmax : HM_MAXIT;
+	const double2 *U = scr + soff[blockIdx.x];
+	const double2 *V = U + (long)B.m * lim;
+	for (long e = threadIdx.x; e < (long)B.rank * B.m; e += blockDim.x) Upool[B.uo + e] = U[e];
+	for (long e = threadIdx.x; e < (long)B.rank * B.nn; e += blockDim.x) Vpool[B.vo + e] = V[e];
+}
+
+// xp[j] = x[perm[j]]
+__global__ void k_hm_gather(const int *perm, const double2 *x, double2 *xp, int n)
+{
+	const int e = blockIdx.x * blockDim.x + threadIdx.x;
+	if (e < n) xp[e] = x[perm[e]];
+}
+
+// stage 1: ft[ftoff_b + l] = v_l^T xp over the block's columns, one wave per rank
+__global__ __launch_bounds__(AT) void k_hm_vt(const hfar *far, const double2 *Vpool, const double2 *xp, double2 *ft)
+{
+	const hfar B = far[blockIdx.x];
+	if (B.rank <= 0) return;
+	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	for (int l = wave; l < B.rank; l += AT / 64) {
+		const double2 *v = Vpool + B.vo + (long)l * B.nn;
+		double sr = 0, si = 0;
+		for (int j = lane; j < B.nn; j += 64) {
+			const double2 a = v[j], x = xp[B.c0 + j];
+			sr += (a.x * x.x) - (a.y * x.y);
+			si += (a.x * x.y) + (a.y * x.x);
+		}
+		for (int off = 32; off > 0; off >>= 1) {
+			sr += __shfl_xor(sr, off);
+			si += __shfl_xor(si, off);
+		}
+		if (lane == 0) ft[B.fo + l] = make_double2(sr, si);
+	}
+}
+
+// stage 2: one workgroup per leaf row cluster; a wave per row; blocks in registered order
+__global__ __launch_bounds__(AT) void k_hm_leaf(hleafs L, const hfar *far, const hdense *near, const double2 *Upool,
+                                                const double2 *Dpool, const double2 *ft, const double2 *xp, const int *perm,
+                                                double2 *y)
+{
+	const int r = blockIdx.x;
+	const int i0 = L.lo[r], i1 = L.hi[r];
+	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	for (int i = i0 + wave; i < i1; i += AT / 64) {
+		double sr = 0, si = 0;
+		for (int e = L.rfoff[r]; e < L.rfoff[r + 1]; e++) {
+			const hfar B = far[L.rfidx[e]];
+			if (B.rank > 0) {
+				for (int l = lane; l < B.rank; l += 64) {
+					const double2 u = Upool[B.uo + (long)l * B.m + (i - B.r0)], t = ft[B.fo + l];
+					sr += (u.x * t.x) - (u.y * t.y);
+					si += (u.x * t.y) + (u.y * t.x);
+				}
+			}
+			else {
+				const double2 *row = Dpool + B.uo + (long)(i - B.r0) * B.nn;
+				for (int j = lane; j < B.nn; j += 64) {
+					const double2 a = row[j], x = xp[B.c0 + j];
+					sr += (a.x * x.x) - (a.y * x.y);
+					si += (a.x * x.y) + (a.y * x.x);
+				}
+			}
+		}
+		for (int e = L.rnoff[r]; e < L.rnoff[r + 1]; e++) {
+			const hdense B = near[L.rnidx[e]];
+			const double2 *row = Dpool + B.off + (long)(i - B.r0) * B.nn;
+			for (int j = lane; j < B.nn; j += 64) {
+				const double2 a = row[j], x = xp[B.c0 + j];
+				sr += (a.x * x.x) - (a.y * x.y);
+				si += (a.x * x.y) + (a.y * x.x);
+			}
+		}
+		for (int off = 32; off > 0; off >>= 1) {
+			sr += __shfl_xor(sr, off);
+			si += __shfl_xor(si, off);
+		}
+		if (lane == 0) y[perm[i]] = make_double2(sr, si);
+	}
+}
+
+}  // namespace pe
+
+// ------------------------------------------------------------------------------------------
+// host side
+
+namespace {
+
+template <typename T>
+T *dal(size_t n)
+{
+	T *p = nullptr;
+	HIPCHK(hipMalloc((void **)&p, (n ? n : 1) * sizeof(T)));
+	return p;
+}
+
+template <typename T>
+T *up(hipStream_t s, const std::vector<T> &v)
+{
+	T *p = dal<T>(v.size());
+	if (!v.empty()) HIPCHK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+	return p;
+}
+
+int km_class_hm(int km)
+{
+	if ((km & ~pe::KM_WIRE) == 0) return pe::KM_WIRE;
+	if ((km & ~pe::KM_RIBBON) == 0) return pe::KM_RIBBON;
+	if ((km & ~(pe::KM_RIBBON | pe::KM_BAR)) == 0) return pe::KM_RIBBON | pe::KM_BAR;
+	if ((km & ~(pe::KM_WIRE | pe::KM_RIBBON | pe::KM_BAR)) == 0) return pe::KM_WIRE | pe::KM_RIBBON | pe::KM_BAR;
+	return 15;
+}
+
+template <int KM>
+void launch_aca(hipStream_t s, const pe::aca_args &a, int nfar)
+{
+	if (nfar > 0) hipLaunchKernelGGL(pe::k_aca<KM>, dim3(nfar), dim3(pe::AT), 0, s, a);
+}
+
+template <int KM>
+void launch_dense(hipStream_t s, const pe::hent &E, const int *perm, const pe::hdense *blk, int nb, double2 *D)
+{
+	if (nb > 0) hipLaunchKernelGGL(pe::k_hm_dense<KM>, dim3(nb), dim3(pe::AT), 0, s, E, perm, blk, D);
+}
+
+void aca_dispatch(int km, hipStream_t s, const pe::aca_args &a, int nfar)
+{
+	switch (km_class_hm(km)) {
+	case pe::KM_WIRE: launch_aca<pe::KM_WIRE>(s, a, nfar); break;
+	case pe::KM_RIBBON: launch_aca<pe::KM_RIBBON>(s, a, nfar); break;
+	case pe::KM_RIBBON | pe::KM_BAR: launch_aca<pe::KM_RIBBON | pe::KM_BAR>(s, a, nfar); break;
+	case pe::KM_WIRE | pe::KM_RIBBON | pe::KM_BAR: launch_aca<pe::KM_WIRE | pe::KM_RIBBON | pe::KM_BAR>(s, a, nfar); break;
+	default: launch_aca<15>(s, a, nfar); break;
+	}
+	HIPCHK(hipGetLastError());
+}
+
+void dense_dispatch(int km, hipStream_t s, const pe::hent &E, const int *perm, const pe::hdense *blk, int nb, double2 *D)
+{
+	switch (km_class_hm(km)) {
+	case pe::KM_WIRE: launch_dense<pe::KM_WIRE>(s, E, perm, blk, nb, D); break;
+	case pe::KM_RIBBON: launch_dense<pe::KM_RIBBON>(s, E, perm, blk, nb, D); break;
+	case pe::KM_RIBBON | pe::KM_BAR: launch_dense<pe::KM_RIBBON | pe::KM_BAR>(s, E, perm, blk, nb, D); break;
+	case pe::KM_WIRE | pe::KM_RIBBON | pe::KM_BAR: launch_dense<pe::KM_WIRE | pe::KM_RIBBON | pe::KM_BAR>(s, E, perm, blk, nb, D); break;
+	default: launch_dense<15>(s, E, perm, blk, nb, D); break;
+	}
+	HIPCHK(hipGetLastError());
+}
+
+}  // namespace
+
+// Build the compressed kernel matrix of a cell set (kind 0 Lp over the current cells, 1 P over
+// the capacitive cells) at wavenumber kw.  pts: representative points in original numbering.
+struct hmat_t *hm_build_dev(pgpu_t *g, int kind, int n, const double *pts, double kw, double tol, const char *label,
+                            FILE *fp_log)
+{
+	if (n <= 0) return nullptr;
+	hmat_t *h = new hmat_t();
+	h->n = n;
+	h->kind = kind;
+	h->kw = kw;
+	h->tol = tol;
+	h->g = g;
+	h->s = pgpu_hip_stream(g);
+	hipStream_t s = h->s;
+	if (hm_tree_build(&h->t, n, pts)) {
+		if (fp_log) fprintf(fp_log, "*** compression : leaf clusters do not tile [0, %d)\n", n);
+		delete h;
+		return nullptr;
+	}
+	const hm_tree_t &t = h->t;
+	int km = 0;
+	const int *csoff = nullptr, *csidx = nullptr;
+	pe::hent E;
+	E.f = pgpu_set_args(g, kind, kw, &km, &csoff, &csidx);
+	E.kind = kind;
+	E.csoff = csoff;
+	E.csidx = csidx;
+	h->km = km;
+	h->d_perm = up(s, std::vector<int>(t.perm, t.perm + n));
+	// far blocks and their ACA scratch
+	std::vector<pe::hfar> far(t.nfar);
+	std::vector<long> soff(t.nfar + 1, 0), moff(t.nfar + 1, 0);
+	for (int b = 0; b < t.nfar; b++) {
+		pe::hfar &B = far[b];
+		B.r0 = t.lo[t.fr[b]];
+		B.m = t.hi[t.fr[b]] - B.r0;
+		B.c0 = t.lo[t.fc[b]];
+		B.nn = t.hi[t.fc[b]] - B.c0;
+		B.rank = 0;
+		B.uo = B.vo = B.fo = 0;
+		const int kmax = std::min(B.m, B.nn), lim = std::min(kmax, HM_MAXIT);
+		soff[b + 1] = soff[b] + (long)(B.m + B.nn) * lim;
+		moff[b + 1] = moff[b] + B.m;
+	}
+	std::vector<int> rank(t.nfar, 0);
+	pe::hfar *d_far = up(s, far);
+	if (t.nfar > 0) {
+		long *d_soff = up(s, soff), *d_moff = up(s, moff);
+		double2 *scr = dal<double2>(soff[t.nfar]);
+		double *urep = dal<double>(moff[t.nfar]);
+		int *rused = dal<int>(moff[t.nfar]);
+		int *d_rank = dal<int>(t.nfar);
+		pe::aca_args a;
+		a.E = E;
+		a.perm = h->d_perm;
+		a.far = d_far;
+		a.soff = d_soff;
+		a.moff = d_moff;
+		a.scr = scr;
+		a.urep = urep;
+		a.rused = rused;
+		a.rank = d_rank;
+		a.tol = tol;
+		aca_dispatch(km, s, a, t.nfar);
+		HIPCHK(hipMemcpyAsync(rank.data(), d_rank, t.nfar * sizeof(int), hipMemcpyDeviceToHost, s));
+		HIPCHK(hipStreamSynchronize(s));
+		// compact pools, dense storage for blocks that did not compress
+		long nu = 0, nv = 0, nd = 0, nf = 0;
+		for (int b = 0; b < t.nfar; b++) {
+			pe::hfar &B = far[b];
+			B.rank = (rank[b] > 0) ? rank[b] : 0;
+			if (B.rank > 0) {
+				B.uo = nu;
+				B.vo = nv;
+				B.fo = nf;
+				nu += (long)B.rank * B.m;
+				nv += (long)B.rank * B.nn;
+				nf += B.rank;
+			}
+		}
+		h->nU = nu;
+		h->nV = nv;
+		h->ftlen = nf;
+		h->d_U = dal<double2>(nu);
+		h->d_V = dal<double2>(nv);
+		HIPCHK(hipMemcpyAsync(d_far, far.data(), far.size() * sizeof(pe::hfar), hipMemcpyHostToDevice, s));
+		hipLaunchKernelGGL(pe::k_aca_compact, dim3(t.nfar), dim3(256), 0, s, (const pe::hfar *)d_far, (const long *)d_soff,
+		                   (const double2 *)scr, h->d_U, h->d_V);
+		HIPCHK(hipStreamSynchronize(s));
+		HIPCHK(hipFree(scr));
+		HIPCHK(hipFree(urep));
+		HIPCHK(hipFree(rused));
+		HIPCHK(hipFree(d_rank));
+		HIPCHK(hipFree(d_soff));
+		HIPCHK(hipFree(d_moff));
+		(void)nd;
+	}
+	// dense storage: far blocks that did not compress, then the near blocks
+	std::vector<pe::hdense> dense;
+	long nd = 0;
+	for (int b = 0; b < t.nfar; b++) {
+		if (far[b].rank > 0) continue;
+		far[b].uo = nd;
+		pe::hdense D{far[b].r0, far[b].m, far[b].c0, far[b].nn, nd};
+		dense.push_back(D);
+		nd += (long)far[b].m * far[b].nn;
+	}
+	std::vector<pe::hdense> near(t.nnear);
+	for (int q = 0; q < t.nnear; q++) {
+		pe::hdense &B = near[q];
+		B.r0 = t.lo[t.nr[q]];
+		B.m = t.hi[t.nr[q]] - B.r0;
+		B.c0 = t.lo[t.nc2[q]];
+		B.nn = t.hi[t.nc2[q]] - B.c0;
+		B.off = nd;
+		nd += (long)B.m * B.nn;
+		dense.push_back(B);
+	}
+	h->nD = nd;
+	h->d_D = dal<double2>(nd);
+	pe::hdense *d_dense = up(s, dense);
+	dense_dispatch(km, s, E, h->d_perm, d_dense, (int)dense.size(), h->d_D);
+	HIPCHK(hipMemcpyAsync(d_far, far.data(), far.size() * sizeof(pe::hfar), hipMemcpyHostToDevice, s));
+	h->d_far = d_far;
+	h->d_near = up(s, near);
+	HIPCHK(hipStreamSynchronize(s));
+	HIPCHK(hipFree(d_dense));
+	h->rank = rank;
+	h->far = far;
+	h->near = near;
+	// leaf-row lists
+	std::vector<int> llo(t.nleaf), lhi(t.nleaf);
+	for (int r = 0; r < t.nleaf; r++) {
+		llo[r] = t.lo[t.leafid[r]];
+		lhi[r] = t.hi[t.leafid[r]];
+	}
+	h->L.lo = up(s, llo);
+	h->L.hi = up(s, lhi);
+	h->L.rfoff = up(s, std::vector<int>(t.rfoff, t.rfoff + t.nleaf + 1));
+	h->L.rfidx = up(s, std::vector<int>(t.rfidx, t.rfidx + std::max(1, t.rfoff[t.nleaf])));
+	h->L.rnoff = up(s, std::vector<int>(t.rnoff, t.rnoff + t.nleaf + 1));
+	h->L.rnidx = up(s, std::vector<int>(t.rnidx, t.rnidx + std::max(1, t.rnoff[t.nleaf])));
+	h->d_ft = dal<double2>(std::max(1L, h->ftlen));
+	h->d_xp = dal<double2>(n);
+	h->d_x = dal<double2>(n);
+	h->d_y = dal<double2>(n);
+	HIPCHK(hipStreamSynchronize(s));
+	if (fp_log) {
+		const double mb = hmat_memory_mb(h), dm = hmat_dense_mb(n);
+		fprintf(fp_log, "compression %s : %d cells, %d far blocks + %d near blocks, tol = %.1e, %.2f MB (dense %.2f MB, %.1fx)\n",
+		        label, n, t.nfar, t.nnear, tol, mb, dm, (mb > 0) ? (dm / mb) : 0);
+		fflush(fp_log);
+	}
+	return h;
+}
+
+// y = H x with x, y device vectors in original numbering, on the H-matrix's stream
+void hm_matvec_dev(const hmat_t *h, const double2 *x, double2 *y)
+{
+	hipStream_t s = h->s;
+	const int n = h->n;
+	hipLaunchKernelGGL(pe::k_hm_gather, dim3((n + 255) / 256), dim3(256), 0, s, (const int *)h->d_perm, x, h->d_xp, n);
+	if (h->t.nfar > 0)
+		hipLaunchKernelGGL(pe::k_hm_vt, dim3(h->t.nfar), dim3(pe::AT), 0, s, (const pe::hfar *)h->d_far, (const double2 *)h->d_V,
+		                   (const double2 *)h->d_xp, h->d_ft);
+	hipLaunchKernelGGL(pe::k_hm_leaf, dim3(h->t.nleaf), dim3(pe::AT), 0, s, h->L, (const pe::hfar *)h->d_far,
+	                   (const pe::hdense *)h->d_near, (const double2 *)h->d_U, (const double2 *)h->d_D, (const double2 *)h->d_ft,
+	                   (const double2 *)h->d_xp, (const int *)h->d_perm, y);
+}
+
+// ------------------------------------------------------------------------------------------
+// the reference's C interface (peec.h)
+
+extern "C" {
+
+struct hmat_t *hmat_build_gen(int n, const double *pts, hmat_entry_fn fn, void *ctx, double kw, double tol,
+                              const char *label, FILE *fp_log)
+{
+	(void)n;
+	(void)pts;
+	(void)fn;
+	(void)ctx;
+	(void)kw;
+	(void)tol;
+	if (fp_log) fprintf(fp_log, "*** compression %s : entries are evaluated on the device; only hmat_build / hmat_build_pot kernels are available\n",
+	                    label ? label : "");
+	return nullptr;
+}
+
+struct hmat_t *hmat_build(peec_t *p, double f, FILE *fp_log)
+{
+	const int n = p->nseg;
+	if (n <= 0) return nullptr;
+	std::vector<double> pts((size_t)3 * n);
+	for (int i = 0; i < n; i++)
+		for (int c = 0; c < 3; c++) pts[3 * i + c] = 0.5 * (p->seg[i].x1[c] + p->seg[i].x2[c]);
+	const double kw = p->retardation ? (2 * PI * f / C0) : 0;
+	return hm_build_dev(pgpu_get(p), 0, n, pts.data(), kw, p->ctol, "Lp", fp_log);
+}
+
+struct hmat_t *hmat_build_pot(peec_t *p, double f, FILE *fp_log)
+{
+	if (pot_cells(p)) return nullptr;
+	if (p->ncell <= 0) return nullptr;
+	const double kw = p->retardation ? (2 * PI * f / C0) : 0;
+	return hm_build_dev(pgpu_get(p), 1, p->ncell, p->cpt, kw, p->ctol, "P", fp_log);
+}
+
+void hmat_free(struct hmat_t *h)
+{
+	if (h == nullptr) return;
+	HIPCHK(hipStreamSynchronize(h->s));
+	const void *bufs[] = {h->d_perm, h->d_far, h->d_near, h->d_U, h->d_V, h->d_D, h->L.lo, h->L.hi, h->L.rfoff, h->L.rfidx,
+	                      h->L.rnoff, h->L.rnidx, h->d_ft, h->d_xp, h->d_x, h->d_y};
+	for (const void *b : bufs)
+		if (b) HIPCHK(hipFree(const_cast<void *>(b)));
+	hm_tree_free(&h->t);
+	delete h;
+}
+
+// y = H x for host vectors (one copy each way)
+void hmat_matvec(const struct hmat_t *h, const d_complex_t *x, d_complex_t *y)
+{
+	const int n = h->n;
+	HIPCHK(hipMemcpyAsync(h->d_x, x, (size_t)n * sizeof(double2), hipMemcpyHostToDevice, h->s));
+	hm_matvec_dev(h, h->d_x, h->d_y);
+	HIPCHK(hipMemcpyAsync(y, h->d_y, (size_t)n * sizeof(double2), hipMemcpyDeviceToHost, h->s));
+	HIPCHK(hipStreamSynchronize(h->s));
+}
+
+int hmat_nleaf(const struct hmat_t *h)
+{
+	return (h != nullptr) ? h->t.nleaf : 0;
+}
+
+// leaf r: its cells (original numbers) and its diagonal block (host copy, m x m row-major)
+const d_complex_t *hmat_leaf_block(const struct hmat_t *h, int r, const int **idx, int *m)
+{
+	hmat_t *hh = const_cast<hmat_t *>(h);
+	const hm_tree_t &t = h->t;
+	const int a = t.leafid[r];
+	*idx = &t.perm[t.lo[a]];
+	*m = t.hi[a] - t.lo[a];
+	if (hh->leafblk.empty()) {
+		hh->leafoff.assign(t.nleaf + 1, -1);
+		std::vector<int> which(t.nleaf, -1);
+		long tot = 0;
+		for (int rr = 0; rr < t.nleaf; rr++) {
+			const int aa = t.leafid[rr];
+			for (int e = t.rnoff[rr]; e < t.rnoff[rr + 1]; e++) {
+				const int q = t.rnidx[e];
+				if ((t.nr[q] == aa) && (t.nc2[q] == aa)) which[rr] = q;
+			}
+			hh->leafoff[rr] = tot;
+			const int mm = t.hi[aa] - t.lo[aa];
+			tot += (long)mm * mm;
+		}
+		hh->leafblk.resize(std::max(1L, tot));
+		for (int rr = 0; rr < t.nleaf; rr++) {
+			if (which[rr] < 0) continue;
+			const pe::hdense &B = h->near[which[rr]];
+			HIPCHK(hipMemcpyAsync(&hh->leafblk[hh->leafoff[rr]], h->d_D + B.off, (size_t)B.m * B.nn * sizeof(double2),
+			                      hipMemcpyDeviceToHost, h->s));
+		}
+		HIPCHK(hipStreamSynchronize(h->s));
+		hh->leafhas = which;
+	}
+	if (hh->leafhas[r] < 0) return nullptr;
+	return (const d_complex_t *)&hh->leafblk[hh->leafoff[r]];
+}
+
+double hmat_memory_mb(const struct hmat_t *h)
+{
+	if (h == nullptr) return 0;
+	return (double)(h->nU + h->nV + h->nD) * sizeof(d_complex_t) / (1024.0 * 1024.0);
+}
+
+double hmat_dense_mb(int n)
+{
+	return (double)n * n * sizeof(d_complex_t) / (1024.0 * 1024.0);
+}
+
+// block structure and ranks for tests: counts[0..3] = nfar, nnear, nleaf, n; perm, ranks
+int hmat_info(const struct hmat_t *h, int *counts, int *perm, int *ranks)
+{
+	if (h == nullptr) return 1;
+	counts[0] = h->t.nfar;
+	counts[1] = h->t.nnear;
+	counts[2] = h->t.nleaf;
+	counts[3] = h->n;
+	if (perm) memcpy(perm, h->t.perm, (size_t)h->n * sizeof(int));
+	if (ranks)
+		for (int b = 0; b < h->t.nfar; b++) ranks[b] = h->rank[b];
+	return 0;
+}
+
+}  // extern "C"

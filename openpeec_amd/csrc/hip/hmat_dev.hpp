@@ -1,0 +1,74 @@
+// hmat_dev.hpp — device representation of a compressed kernel matrix (hmat.hip) for the
+// modules that apply it (the compressed solver, hsolve.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <vector>
+#include "fill_kernels.hpp"
+
+namespace pe {
+
+// entry context: which kernel (0 Lp, 1 P aggregated over sub-cells) over which cells
+struct hent {
+	fill_args f;
+	int kind;
+	const int *csoff, *csidx;
+};
+
+// far block: row cluster [r0, r0+m), column cluster [c0, c0+nn) of the permutation; rank > 0:
+// U (rank x m) at uo, V (rank x nn) at vo, V^T x at fo; rank 0: dense m x nn at uo in the D pool
+struct hfar {
+	int r0, m, c0, nn;
+	int rank, pad;
+	long uo, vo, fo;
+};
+
+struct hdense {
+	int r0, m, c0, nn;
+	long off;
+};
+
+struct hleafs {
+	const int *lo, *hi;            // leaf row ranges (permuted positions)
+	const int *rfoff, *rfidx;      // far blocks covering each leaf, registered order
+	const int *rnoff, *rnidx;      // near blocks
+};
+
+struct aca_args {
+	hent E;
+	const int *perm;
+	const hfar *far;
+	const long *soff, *moff;       // scratch offsets: factors (double2) and row flags
+	double2 *scr;
+	double *urep;
+	int *rused;
+	int *rank;
+	double tol;
+};
+
+}  // namespace pe
+
+struct hmat_t {
+	int n = 0, kind = 0, km = 0;
+	double kw = 0, tol = 0;
+	pgpu_t *g = nullptr;
+	hipStream_t s = nullptr;
+	hm_tree_t t{};
+	int *d_perm = nullptr;
+	pe::hfar *d_far = nullptr;
+	pe::hdense *d_near = nullptr;
+	double2 *d_U = nullptr, *d_V = nullptr, *d_D = nullptr;
+	long nU = 0, nV = 0, nD = 0, ftlen = 0;
+	pe::hleafs L{};
+	double2 *d_ft = nullptr, *d_xp = nullptr, *d_x = nullptr, *d_y = nullptr;
+	std::vector<int> rank;
+	std::vector<pe::hfar> far;
+	std::vector<pe::hdense> near;
+	std::vector<double2> leafblk;   // host copies of the leaf diagonal blocks (hmat_leaf_block)
+	std::vector<long> leafoff;
+	std::vector<int> leafhas;
+};
+
+struct hmat_t *hm_build_dev(pgpu_t *g, int kind, int n, const double *pts, double kw, double tol, const char *label,
+                            FILE *fp_log);
+void hm_matvec_dev(const hmat_t *h, const double2 *x, double2 *y);

@@ -519,6 +519,139 @@ void launch_gemm(hipStream_t s, int M, int N, int K, const double2 *A, long lda,
 	hipLaunchKernelGGL(k_zgemm, grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc);
 }
 
+// ---------------------------------------------------------------- triangular inversion
+// Explicit inverses of the triangular factors, for preconditioners applied many times.
+constexpr int TI = 64;
+
+__device__ __forceinline__ double2 ti_mul(double2 a, double2 b)
+{
+	return make_double2((a.x * b.x) - (a.y * b.y), (a.x * b.y) + (a.y * b.x));
+}
+
+__device__ __forceinline__ double2 ti_inv(double2 z)
+{
+	const double n2 = (z.x * z.x) + (z.y * z.y);
+	if (n2 <= 0) return make_double2(0, 0);
+	return make_double2(z.x / n2, -z.y / n2);
+}
+
+__global__ void k_ti_zero(double2 *X, long cnt)
+{
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e < cnt) X[e] = make_double2(0, 0);
+}
+
+unsigned tiblocks(long n)
+{
+	return (unsigned)((n + 255) / 256);
+}
+
+// Inverses of the TI x TI diagonal blocks of the packed LU factors (unit lower when LOWER,
+// else upper), FP64.  One workgroup per block; thread c solves T y = e_c in LDS.  Rows past
+// the end of a short last block act as identity.
+template <bool LOWER>
+__global__ __launch_bounds__(64) void k_diaginv(const double2 *A, long lda, int n, double2 *dinv)
+{
+	__shared__ double2 T[TI * TI];
+	__shared__ double2 Y[TI * TI];
+	const int b = blockIdx.x, c = threadIdx.x;
+	const int r0 = b * TI, nb = min(TI, n - r0);
+	for (int r = 0; r < TI; r++) {
+		double2 v = make_double2((r == c) ? 1.0 : 0.0, 0);
+		if ((r < nb) && (c < nb)) v = A[(long)(r0 + r) * lda + r0 + c];
+		T[r * TI + c] = v;
+	}
+	__syncthreads();
+	if (LOWER) {
+		for (int r = 0; r < TI; r++) {
+			double2 acc = make_double2((r == c) ? 1.0 : 0.0, 0);
+			for (int q = 0; q < r; q++) {
+				const double2 p = ti_mul(T[r * TI + q], Y[q * TI + c]);
+				acc.x -= p.x;
+				acc.y -= p.y;
+			}
+			Y[r * TI + c] = acc;
+		}
+	}
+	else {
+		for (int r = TI - 1; r >= 0; r--) {
+			double2 acc = make_double2((r == c) ? 1.0 : 0.0, 0);
+			for (int q = r + 1; q < TI; q++) {
+				const double2 p = ti_mul(T[r * TI + q], Y[q * TI + c]);
+				acc.x -= p.x;
+				acc.y -= p.y;
+			}
+			Y[r * TI + c] = ti_mul(acc, ti_inv(T[r * TI + r]));
+		}
+	}
+	__syncthreads();
+	for (int r = 0; r < TI; r++) dinv[((long)b * TI + r) * TI + c] = Y[r * TI + c];
+}
+
+// X[r][c] *= s over a rows x cols block (s = 0 clears, s = -1 negates)
+__global__ void k_scale_block(double2 *X, long ld, int rows, int cols, double sc)
+{
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= (long)rows * cols) return;
+	const int r = (int)(e / cols), c = (int)(e - (long)r * cols);
+	double2 *x = &X[(long)r * ld + c];
+	*x = (sc == 0) ? make_double2(0, 0) : make_double2(sc * x->x, sc * x->y);
+}
+
+// Out[r][c] = D[r][c] for an n x n corner of a TI x TI block
+__global__ void k_put_block(const double2 *D, double2 *Out, long ld, int n)
+{
+	const int e = blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= n * n) return;
+	const int r = e / n, c = e - r * n;
+	Out[(long)r * ld + c] = D[r * TI + c];
+}
+
+void scale_block(hipStream_t s, double2 *X, long ld, int rows, int cols, double sc)
+{
+	if ((rows <= 0) || (cols <= 0)) return;
+	hipLaunchKernelGGL(k_scale_block, dim3(tiblocks((long)rows * cols)), dim3(256), 0, s, X, ld, rows, cols, sc);
+}
+
+// Out[i0.., i0..] (n x n, leading dimension ld, zero on entry) = inverse of the unit-lower (lower)
+// or upper part of the packed factor T over the same rows/columns.  Halves split on TI
+// boundaries; with T = [T11 0; T21 T22] the off-diagonal block of the inverse is
+// -inv(T22) T21 inv(T11) (lower) and -inv(T11) T12 inv(T22) (upper), two GEMMs on MFMA.
+// Leaves are the diagonal-block inverses.  scr holds at least ceil(n/2)^2 + TI n entries.
+void tri_inv(hipStream_t s, bool lower, const double2 *T, long ld, double2 *Out, int i0, int n, const double2 *dinv,
+             double2 *scr)
+{
+	if (n <= TI) {
+		hipLaunchKernelGGL(k_put_block, dim3(tiblocks((long)n * n)), dim3(256), 0, s, dinv + (long)(i0 / TI) * TI * TI,
+		                   Out + (long)i0 * ld + i0, ld, n);
+		return;
+	}
+	const int n1 = ((n / 2 + TI - 1) / TI) * TI, n2 = n - n1;
+	tri_inv(s, lower, T, ld, Out, i0, n1, dinv, scr);
+	tri_inv(s, lower, T, ld, Out, i0 + n1, n2, dinv, scr);
+	const double2 *O11 = Out + (long)i0 * ld + i0;
+	const double2 *O22 = Out + (long)(i0 + n1) * ld + (i0 + n1);
+	if (lower) {
+		// C = -T21 inv(T11) (n2 x n1); Out21 = -(inv(T22) C), then negated
+		scale_block(s, scr, n1, n2, n1, 0);
+		dlu_zgemm_sub(s, n2, n1, n1, (const d_complex_t *)(T + (long)(i0 + n1) * ld + i0), ld, (const d_complex_t *)O11, ld,
+		              (d_complex_t *)scr, n1);
+		double2 *O21 = Out + (long)(i0 + n1) * ld + i0;
+		dlu_zgemm_sub(s, n2, n1, n2, (const d_complex_t *)O22, ld, (const d_complex_t *)scr, n1, (d_complex_t *)O21, ld);
+		scale_block(s, O21, ld, n2, n1, -1);
+	}
+	else {
+		// C = -T12 inv(T22) (n1 x n2); Out12 = -(inv(T11) C), then negated
+		scale_block(s, scr, n2, n1, n2, 0);
+		dlu_zgemm_sub(s, n1, n2, n2, (const d_complex_t *)(T + (long)i0 * ld + (i0 + n1)), ld, (const d_complex_t *)O22, ld,
+		              (d_complex_t *)scr, n2);
+		double2 *O12 = Out + (long)i0 * ld + (i0 + n1);
+		dlu_zgemm_sub(s, n1, n2, n1, (const d_complex_t *)O11, ld, (const d_complex_t *)scr, n2, (d_complex_t *)O12, ld);
+		scale_block(s, O12, ld, n1, n2, -1);
+	}
+}
+
+
 }  // namespace
 
 extern "C" void dlu_zgemm_sub(void *stream, int M, int N, int K, const d_complex_t *A, long lda,
@@ -646,4 +779,20 @@ extern "C" void dlu_solve_perm(void *stream, int n, const d_complex_t *dA_, int 
 		else
 			launch_gemm(s, r0, nrhs, nb, A + r0, lda, B + (long)r0 * ldb, ldb, B, ldb);
 	}
+}
+
+// Out (n x n, leading dimension n) = inverse of the unit-lower (lower = 1) or upper part of the
+// packed LU factors in LU (leading dimension lda).  dinv receives the 64 x 64 diagonal-block
+// inverses (ceil(n/64) blocks), scr is GEMM scratch of at least (ceil(n/2) + 64)^2 entries.
+extern "C" void dlu_tri_inverse(void *stream, int n, const d_complex_t *LU, int lda, int lower, d_complex_t *Out,
+                                d_complex_t *scr, d_complex_t *dinv)
+{
+	hipStream_t s = (hipStream_t)stream;
+	const int nblk = (n + TI - 1) / TI;
+	if (lower)
+		hipLaunchKernelGGL(k_diaginv<true>, dim3(nblk), dim3(64), 0, s, (const double2 *)LU, (long)lda, n, (double2 *)dinv);
+	else
+		hipLaunchKernelGGL(k_diaginv<false>, dim3(nblk), dim3(64), 0, s, (const double2 *)LU, (long)lda, n, (double2 *)dinv);
+	hipLaunchKernelGGL(k_ti_zero, dim3(tiblocks((long)n * n)), dim3(256), 0, s, (double2 *)Out, (long)n * n);
+	tri_inv(s, lower != 0, (const double2 *)LU, lda, (double2 *)Out, 0, n, (const double2 *)dinv, (double2 *)scr);
 }

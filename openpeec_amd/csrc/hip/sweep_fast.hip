@@ -95,48 +95,6 @@ __global__ void k_demote(const double2 *src, float2 *dst, long count)
 	dst[e] = make_float2((float)v.x, (float)v.y);
 }
 
-// Inverses of the TB x TB diagonal blocks of the packed LU factors (unit lower when LOWER,
-// else upper), FP64.  One workgroup per block; thread c solves T y = e_c in LDS.  Rows past
-// the end of a short last block act as identity.
-template <bool LOWER>
-__global__ __launch_bounds__(64) void k_diaginv(const double2 *A, long lda, int n, double2 *dinv)
-{
-	__shared__ double2 T[TB * TB];
-	__shared__ double2 Y[TB * TB];
-	const int b = blockIdx.x, c = threadIdx.x;
-	const int r0 = b * TB, nb = min(TB, n - r0);
-	for (int r = 0; r < TB; r++) {
-		double2 v = make_double2((r == c) ? 1.0 : 0.0, 0);
-		if ((r < nb) && (c < nb)) v = A[(long)(r0 + r) * lda + r0 + c];
-		T[r * TB + c] = v;
-	}
-	__syncthreads();
-	if (LOWER) {
-		for (int r = 0; r < TB; r++) {
-			double2 acc = make_double2((r == c) ? 1.0 : 0.0, 0);
-			for (int q = 0; q < r; q++) {
-				const double2 p = zmul(T[r * TB + q], Y[q * TB + c]);
-				acc.x -= p.x;
-				acc.y -= p.y;
-			}
-			Y[r * TB + c] = acc;
-		}
-	}
-	else {
-		for (int r = TB - 1; r >= 0; r--) {
-			double2 acc = make_double2((r == c) ? 1.0 : 0.0, 0);
-			for (int q = r + 1; q < TB; q++) {
-				const double2 p = zmul(T[r * TB + q], Y[q * TB + c]);
-				acc.x -= p.x;
-				acc.y -= p.y;
-			}
-			Y[r * TB + c] = zmul(acc, zinv(T[r * TB + r]));
-		}
-	}
-	__syncthreads();
-	for (int r = 0; r < TB; r++) dinv[((long)b * TB + r) * TB + c] = Y[r * TB + c];
-}
-
 // ---------------------------------------------------------------- batched dense products
 
 // One system of the batch: CSR stamps and its dense kernel blocks.
@@ -404,25 +362,6 @@ __global__ void k_permute(const double2 *src, double2 *dst, const int *perm, int
 // have no sequential dependence, unlike substitution, whose 64-row steps each wait for the
 // previous one.  The inverses are built once per factorisation by recursive blocking (below).
 
-// X[r][c] *= s over a rows x cols block (s = 0 clears, s = -1 negates)
-__global__ void k_scale_block(double2 *X, long ld, int rows, int cols, double sc)
-{
-	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (e >= (long)rows * cols) return;
-	const int r = (int)(e / cols), c = (int)(e - (long)r * cols);
-	double2 *x = &X[(long)r * ld + c];
-	*x = (sc == 0) ? make_double2(0, 0) : make_double2(sc * x->x, sc * x->y);
-}
-
-// Out[r][c] = D[r][c] for an n x n corner of a TB x TB block
-__global__ void k_put_block(const double2 *D, double2 *Out, long ld, int n)
-{
-	const int e = blockIdx.x * blockDim.x + threadIdx.x;
-	if (e >= n * n) return;
-	const int r = e / n, c = e - r * n;
-	Out[(long)r * ld + c] = D[r * TB + c];
-}
-
 // Y[:, f] = Mt X[:, f] for a triangular FP32 matrix (lower when LOWER), X and Y n x F in FP64.
 // A workgroup owns 16 rows (4 per wave) and walks the 256-column chunks its rows touch; each
 // chunk of X is staged in LDS once and reused by the 16 rows; a lane takes 4 columns of the
@@ -534,60 +473,14 @@ struct fast {
 	double *scale = nullptr;
 };
 
-void scale_block(hipStream_t s, double2 *X, long ld, int rows, int cols, double sc)
-{
-	if ((rows <= 0) || (cols <= 0)) return;
-	hipLaunchKernelGGL(k_scale_block, dim3(blocks((long)rows * cols)), dim3(256), 0, s, X, ld, rows, cols, sc);
-}
-
-// Out[i0.., i0..] (n x n, leading dimension ld, zero on entry) = inverse of the unit-lower (lower)
-// or upper part of the packed factor T over the same rows/columns.  Halves split on TB
-// boundaries; with T = [T11 0; T21 T22] the off-diagonal block of the inverse is
-// -inv(T22) T21 inv(T11) (lower) and -inv(T11) T12 inv(T22) (upper), two GEMMs on MFMA.
-// Leaves are the diagonal-block inverses.  scr holds at least ceil(n/2)^2 + TB n entries.
-void tri_inv(hipStream_t s, bool lower, const double2 *T, long ld, double2 *Out, int i0, int n, const double2 *dinv,
-             double2 *scr)
-{
-	if (n <= TB) {
-		hipLaunchKernelGGL(k_put_block, dim3(blocks((long)n * n)), dim3(256), 0, s, dinv + (long)(i0 / TB) * TB * TB,
-		                   Out + (long)i0 * ld + i0, ld, n);
-		return;
-	}
-	const int n1 = ((n / 2 + TB - 1) / TB) * TB, n2 = n - n1;
-	tri_inv(s, lower, T, ld, Out, i0, n1, dinv, scr);
-	tri_inv(s, lower, T, ld, Out, i0 + n1, n2, dinv, scr);
-	const double2 *O11 = Out + (long)i0 * ld + i0;
-	const double2 *O22 = Out + (long)(i0 + n1) * ld + (i0 + n1);
-	if (lower) {
-		// C = -T21 inv(T11) (n2 x n1); Out21 = -(inv(T22) C), then negated
-		scale_block(s, scr, n1, n2, n1, 0);
-		dlu_zgemm_sub(s, n2, n1, n1, (const d_complex_t *)(T + (long)(i0 + n1) * ld + i0), ld, (const d_complex_t *)O11, ld,
-		              (d_complex_t *)scr, n1);
-		double2 *O21 = Out + (long)(i0 + n1) * ld + i0;
-		dlu_zgemm_sub(s, n2, n1, n2, (const d_complex_t *)O22, ld, (const d_complex_t *)scr, n1, (d_complex_t *)O21, ld);
-		scale_block(s, O21, ld, n2, n1, -1);
-	}
-	else {
-		// C = -T12 inv(T22) (n1 x n2); Out12 = -(inv(T11) C), then negated
-		scale_block(s, scr, n2, n1, n2, 0);
-		dlu_zgemm_sub(s, n1, n2, n2, (const d_complex_t *)(T + (long)i0 * ld + (i0 + n1)), ld, (const d_complex_t *)O22, ld,
-		              (d_complex_t *)scr, n2);
-		double2 *O12 = Out + (long)i0 * ld + (i0 + n1);
-		dlu_zgemm_sub(s, n1, n2, n1, (const d_complex_t *)O11, ld, (const d_complex_t *)scr, n2, (d_complex_t *)O12, ld);
-		scale_block(s, O12, ld, n1, n2, -1);
-	}
-}
-
 // both FP32 inverses from the FP64 factors in c.luw
 void build_inverses(fast &c, hipStream_t s)
 {
-	const int n = c.n, nblk = (n + TB - 1) / TB;
-	hipLaunchKernelGGL(k_diaginv<true>, dim3(nblk), dim3(64), 0, s, (const double2 *)c.luw, (long)n, n, c.dinvL);
-	hipLaunchKernelGGL(k_diaginv<false>, dim3(nblk), dim3(64), 0, s, (const double2 *)c.luw, (long)n, n, c.dinvU);
+	const int n = c.n;
 	for (int pass = 0; pass < 2; pass++) {
 		const bool lower = (pass == 0);
-		hipLaunchKernelGGL(k_zero, dim3(blocks((long)n * n)), dim3(256), 0, s, c.inv, (long)n * n);
-		tri_inv(s, lower, c.luw, n, c.inv, 0, n, lower ? c.dinvL : c.dinvU, c.scr);
+		dlu_tri_inverse(s, n, (const d_complex_t *)c.luw, n, lower ? 1 : 0, (d_complex_t *)c.inv, (d_complex_t *)c.scr,
+		                (d_complex_t *)(lower ? c.dinvL : c.dinvU));
 		hipLaunchKernelGGL(k_demote, dim3(blocks((long)n * n)), dim3(256), 0, s, (const double2 *)c.inv, lower ? c.ML : c.MU,
 		                   (long)n * n);
 	}

@@ -261,10 +261,6 @@ out:
 int solve(peec_t *p, FILE *fp_log)
 {
 	const int np = p->nport;
-	if (p->compress) {
-		printf("%s\n", "*** compression = 1 : the compressed (H-matrix) solve is not available in this build");
-		return 1;
-	}
 	if (p->pw || (p->nsrc > 0)) {
 		printf("%s\n", "*** planewave / independent sources are not available in this build");
 		return 1;
@@ -284,6 +280,20 @@ int solve(peec_t *p, FILE *fp_log)
 	pgpu_t *g = pgpu_get(p);
 	for (int ifreq = 0; ifreq < p->nfreq; ifreq++) {
 		const double f = freq_at(p, ifreq);
+		if (p->compress) {
+			/* solve.c:246-300: H matrices, leaf/Schur preconditioner, GMRES per port */
+			if (pgpu_sweep_compressed(g, p, ifreq, 1, &p->zmat[ZIDX(p, ifreq, 0, 0)], NULL)) {
+				if (fp_log) fprintf(fp_log, "*** compressed solve failed at f = %.5e Hz\n", f);
+				return 1;
+			}
+			for (int j = 0; j < np; j++) p->zin[(size_t)j * p->nfreq + ifreq] = p->zmat[ZIDX(p, ifreq, j, j)];
+			if (z_to_s(p, ifreq)) {
+				printf("*** singular (Z + Z0) at f = %.5e Hz\n", f);
+				if (fp_log) fprintf(fp_log, "*** singular (Z + Z0) at f = %.5e Hz\n", f);
+				return 1;
+			}
+			continue;
+		}
 		const int rc = pgpu_sweep(g, p, ifreq, 1, &p->zmat[ZIDX(p, ifreq, 0, 0)],
 			p->segi ? &p->segi[DIDX(p, ifreq, 0, 0)] : NULL,
 			p->cellq ? &p->cellq[QIDX(p, ifreq, 0, 0)] : NULL, NULL);
@@ -359,6 +369,7 @@ int peec_sweep_ex(peec_t *p, int ifreq0, int nfreq, int form, int batch, d_compl
 		st->form = form;
 		st->ising = -1;
 	}
+	if (p->compress) return pgpu_sweep_compressed(pgpu_get(p), p, ifreq0, nfreq, zmat, st);
 	if (form == PEEC_FORM_PLAIN) return pgpu_sweep(pgpu_get(p), p, ifreq0, nfreq, zmat, NULL, NULL, st);
 	if (form == PEEC_FORM_MIXED) return pgpu_sweep_fast(pgpu_get(p), p, ifreq0, nfreq, zmat, st, batch);
 	printf("%s\n", "*** peec_sweep: unknown solver form");

@@ -65,6 +65,28 @@ int cells_pack(const seg_t *src, int n, int mirror, double gpz,
 	dcell_t **cells, dedge_t **edges, dpnorm_t **pnorm, int *nedge);
 
 /* ---------------------------------------------------------------------------------------- */
+/* H-matrix geometry (hmat_host.c): cluster tree over a permutation of the cells, far/near    */
+/* block lists, leaf-row grouping.  Frequency independent.                                   */
+#define HM_LEAF  32
+#define HM_ETA   2.0
+#define HM_MAXIT 200
+#define HM_PROBE 4
+typedef struct {
+	int n;
+	int *perm;                       /* cluster order -> cell index */
+	int nc, cap;
+	int *lo, *hi, *left, *right;     /* cluster [lo, hi) of perm, children (-1: leaf) */
+	double *cen, *rad;               /* bounding-box centre [3 nc], half diagonal */
+	int nfar, nnear;
+	int *fr, *fc;                    /* far blocks (row cluster, column cluster) */
+	int *nr, *nc2;                   /* near blocks */
+	int nleaf;
+	int *leafid, *rfoff, *rfidx, *rnoff, *rnidx;
+} hm_tree_t;
+int  hm_tree_build(hm_tree_t *t, int n, const double *pts);   /* 0 ok */
+void hm_tree_free(hm_tree_t *t);
+
+/* ---------------------------------------------------------------------------------------- */
 /* Device mirror of one peec_t (gpu_runtime.hip).                                            */
 typedef struct pgpu pgpu_t;
 
@@ -111,6 +133,11 @@ int  pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq, d_comple
 	pgpu_stats_t *st, int batch);
 void pgpu_fast_reset(void);
 
+/* compressed (compression = 1) sweep (hsolve.hip): H matrices, leaf/Schur preconditioner,
+ * GMRES to 1e-10 per port.  Returns 0, 4 (GMRES), 5 (H matrix) or 6 (preconditioner). */
+int  pgpu_sweep_compressed(pgpu_t *g, peec_t *p, int ifreq0, int nfreq, d_complex_t *zmat,
+	pgpu_stats_t *st);
+
 /* ---------------------------------------------------------------------------------------- */
 /* Dense complex LU on device buffers (lu.hip).  Matrices are row-major, leading dim lda.   */
 /* factor in place; dpiv[k] as lu_decomp; dperm (may be NULL) receives the row permutation
@@ -119,6 +146,10 @@ int  dlu_factor(void *stream, int n, d_complex_t *dA, int lda, int *dpiv, int *d
 /* B <- A^-1 B (nrhs columns, row-major, leading dim ldb); dwork holds n * nrhs entries */
 void dlu_solve_perm(void *stream, int n, const d_complex_t *dA, int lda, const int *dperm,
 	d_complex_t *dB, int ldb, int nrhs, d_complex_t *dwork);
+/* explicit inverse of the unit-lower (lower = 1) or upper part of packed LU factors (n x n,
+ * leading dimension n out); dinv: ceil(n/64) 64 x 64 blocks, scr: (ceil(n/2) + 64)^2 entries */
+void dlu_tri_inverse(void *stream, int n, const d_complex_t *LU, int lda, int lower, d_complex_t *Out,
+	d_complex_t *scr, d_complex_t *dinv);
 /* C[M x N] -= A[M x K] B[K x N] (complex FP64 MFMA) */
 void dlu_zgemm_sub(void *stream, int M, int N, int K, const d_complex_t *A, long lda,
 	const d_complex_t *B, long ldb, d_complex_t *C, long ldc);

@@ -19,6 +19,7 @@ def declared_functions() -> set[str]:
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         text = re.sub(r"//.*", "", text)
         text = re.sub(r"static inline[^{]*\{.*?\n\}", "", text, flags=re.S)   # inline helpers (complex.h, freq_at)
+        text = re.sub(r"^typedef[^;]*;", "", text, flags=re.M)                  # function-pointer typedefs
         for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?[\s\*](\w+)\s*\(", text, flags=re.M):
             name = m.group(1)
             if name in ("if", "for", "while", "return", "sizeof"):
