@@ -16,7 +16,11 @@ extern "C" {
 
 /* solver forms of the device sweep */
 #define PEEC_FORM_PLAIN 0   /* FP64 fill, FP64 LU (MFMA FP64 trailing update)                  */
-#define PEEC_FORM_MIXED 1   /* FP64 fill, LU with FP32 MFMA trailing update + FP64 refinement  */
+#define PEEC_FORM_MIXED 1   /* k-factored fill, FP64 LU at a reference frequency applied as FP32
+                               inverse factors, FGMRES to a true residual of 1e-10            */
+#define PEEC_FORM_PRECISION 2 /* FP64 fill, LU of an FP32 copy (FP64 panels and pivoting, FP32
+                               MFMA trailing update), FP64 iterative refinement against the
+                               FP64 matrix to a normwise backward error of 1e-10              */
 
 /* per-stage timings of device sweeps (accumulated over points; milliseconds) */
 typedef struct {
@@ -27,8 +31,8 @@ typedef struct {
 	int    lu_gemm_launches;
 	double fill_flop;       /* nominal fill flops (reference op count)        */
 	int    ising;           /* singular row of the last failure, or -1        */
-	int    refine_its;      /* refinement sweeps (mixed form)                 */
-	double berr;            /* worst backward error of the solutions (mixed)  */
+	int    refine_its;      /* refinement sweeps (precision form)             */
+	double berr;            /* worst backward error of the solutions (precision) */
 	int    n_lu;            /* factorisations performed (mixed form)          */
 	long   gmres_its;       /* GMRES iterations summed over systems (mixed)   */
 	double kf_setup_ms;     /* k-factored fill preparation (mixed), in fill_ms */
@@ -36,6 +40,7 @@ typedef struct {
 	double mv_ms;           /* GMRES products A x: summed kernel time (mixed) */
 	double mv_bytes;        /* their algorithmic HBM bytes (matrix blocks read) */
 	long   mv_launches;
+	int    fp64_fallbacks;  /* points the precision form solved in FP64 instead */
 } peec_stats_t;
 
 /* Library identification: "libpeec_mi355x <version> gfx950". */
@@ -88,6 +93,12 @@ int hmat_info(const struct hmat_t *h, int *counts, int *perm, int *ranks);
  * blocks, leaves, n}; perm (n); far / near receive (r0, m, c0, nn) per block when non-NULL
  * (call first with NULL to size them).  Returns 0, or 1 when the leaves do not tile [0, n). */
 int hmat_tree_info(int n, const double *pts, int *counts, int *perm, int *far, int *near);
+
+/* lu_decomp (src/lu.c:16-62) with the factors stored in FP32: the matrix is rounded to FP32,
+ * factored with FP64 panel arithmetic and pivot search and FP32 MFMA trailing updates, and the
+ * factors are returned widened to FP64 in lu_decomp's layout.  The precision form's factorisation;
+ * no counterpart in the reference.  Returns -1, or the first zero pivot's row. */
+int lu_decomp32(int n, d_complex_t *a, int *piv);
 
 /* Sizes for bindings */
 size_t peec_sizeof_peec(void);

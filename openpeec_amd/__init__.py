@@ -28,6 +28,7 @@ _lib = None
 
 PEEC_FORM_PLAIN = 0
 PEEC_FORM_MIXED = 1
+PEEC_FORM_PRECISION = 2
 
 
 def load_library(build_if_missing: bool = True) -> C.CDLL:

@@ -131,7 +131,8 @@ class peec_stats_t(C.Structure):
                 ("fill_flop", C.c_double), ("ising", C.c_int), ("refine_its", C.c_int),
                 ("berr", C.c_double), ("n_lu", C.c_int), ("gmres_its", C.c_long),
                 ("kf_setup_ms", C.c_double), ("kf_direct_pairs", C.c_long),
-                ("mv_ms", C.c_double), ("mv_bytes", C.c_double), ("mv_launches", C.c_long)]
+                ("mv_ms", C.c_double), ("mv_bytes", C.c_double), ("mv_launches", C.c_long),
+                ("fp64_fallbacks", C.c_int)]
 
 
 def declare(lib: C.CDLL, reference: bool = False) -> None:
@@ -151,6 +152,7 @@ def declare(lib: C.CDLL, reference: bool = False) -> None:
         "mna_numbering": (C.c_int, [PP, C.c_void_p]),
         "solve": (C.c_int, [PP, C.c_void_p]),
         "lu_decomp": (C.c_int, [C.c_int, PZ, C.POINTER(C.c_int)]),
+        "lu_decomp32": (C.c_int, [C.c_int, PZ, C.POINTER(C.c_int)]),
         "lu_solve": (None, [C.c_int, PZ, C.POINTER(C.c_int), PZ]),
         "hmat_build": (C.c_void_p, [PP, C.c_double, C.c_void_p]),
         "hmat_build_pot": (C.c_void_p, [PP, C.c_double, C.c_void_p]),

@@ -222,6 +222,31 @@ int lu_decomp(int n, d_complex_t *a, int *piv)
 	return info;
 }
 
+// precision form's factorisation: the matrix is rounded to FP32 storage and factored there
+// (FP64 panel arithmetic and pivot search, FP32 MFMA trailing update); factors come back
+// widened to FP64 in the layout lu_decomp writes
+int lu_decomp32(int n, d_complex_t *a, int *piv)
+{
+	if (n <= 0) return -1;
+	int hip_dev_count = 0;
+	if ((hipGetDeviceCount(&hip_dev_count) != hipSuccess) || (hip_dev_count <= 0)) pgpu_fatal("no HIP device");
+	const long cnt = (long)n * n;
+	std::vector<float2> h((size_t)cnt);
+	for (long e = 0; e < cnt; e++) h[e] = make_float2((float)a[e].r, (float)a[e].i);
+	float2 *dA = nullptr;
+	int *dpiv = nullptr;
+	HIPCHK(hipMalloc((void **)&dA, (size_t)cnt * sizeof(float2)));
+	HIPCHK(hipMalloc((void **)&dpiv, (size_t)n * sizeof(int)));
+	HIPCHK(hipMemcpy(dA, h.data(), (size_t)cnt * sizeof(float2), hipMemcpyHostToDevice));
+	const int info = slu_factor(nullptr, n, dA, n, dpiv, nullptr, nullptr);
+	HIPCHK(hipMemcpy(h.data(), dA, (size_t)cnt * sizeof(float2), hipMemcpyDeviceToHost));
+	HIPCHK(hipMemcpy(piv, dpiv, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+	HIPCHK(hipFree(dA));
+	HIPCHK(hipFree(dpiv));
+	for (long e = 0; e < cnt; e++) a[e] = d_complex(h[e].x, h[e].y);
+	return info;
+}
+
 void lu_solve(int n, const d_complex_t *a, const int *piv, d_complex_t *b)
 {
 	if (n <= 0) return;

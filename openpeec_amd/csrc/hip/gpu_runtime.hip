@@ -46,6 +46,9 @@ struct pgpu {
 	double2 *A = nullptr;
 	int *apiv = nullptr, *aperm = nullptr;
 	double2 *B = nullptr, *Bw = nullptr;   // n x nport right-hand sides and workspace
+	float2 *A32 = nullptr;                 // precision form: FP32 copy that is factored
+	double2 *Brhs = nullptr, *R = nullptr; // precision form: right-hand sides, residuals
+	unsigned long long *cberr = nullptr;  // precision form: componentwise backward error per port
 	int *cellrow = nullptr;      // [n] capacitive cell of node row r, or -1
 	int *d_st_ri = nullptr, *d_st_ci = nullptr;
 	double2 *d_st_v = nullptr;
@@ -262,6 +265,60 @@ __global__ void k_cell_charge(const double2 *C, int ncell, const int *rowofcell,
 	q[e] = make_double2(qr, qi);
 }
 
+// ---- precision form: FP32-stored factors + FP64 iterative refinement
+
+__global__ void k_demote64(const double2 *src, float2 *dst, long cnt)
+{
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e < cnt) dst[e] = make_float2((float)src[e].x, (float)src[e].y);
+}
+
+// R[:, j] = B[:, j] - A X[:, j] in FP64 (one wave per row), and the componentwise (Oettli-Prager)
+// backward error of each column, max_i |r_i| / (|A| |x| + |b|)_i, folded into cb[j] with an
+// integer max (non-negative doubles order as their bit patterns)
+__global__ __launch_bounds__(256) void k_resid(const double2 *A, int n, const double2 *X, const double2 *B, double2 *R, int nrhs,
+                                               unsigned long long *cb)
+{
+	const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+	if (row >= n) return;
+	const double2 *a = A + (long)row * n;
+	for (int j = 0; j < nrhs; j++) {
+		double sr = 0, si = 0, sa = 0;
+		for (int c = lane; c < n; c += 64) {
+			const double2 u = a[c], x = X[(long)c * nrhs + j];
+			sr += (u.x * x.x) - (u.y * x.y);
+			si += (u.x * x.y) + (u.y * x.x);
+			sa += sqrt((u.x * u.x) + (u.y * u.y)) * sqrt((x.x * x.x) + (x.y * x.y));
+		}
+		for (int off = 32; off > 0; off >>= 1) {
+			sr += __shfl_xor(sr, off);
+			si += __shfl_xor(si, off);
+			sa += __shfl_xor(sa, off);
+		}
+		if (lane == 0) {
+			const double2 b = B[(long)row * nrhs + j];
+			const double rr = b.x - sr, ri = b.y - si;
+			R[(long)row * nrhs + j] = make_double2(rr, ri);
+			const double den = sa + sqrt((b.x * b.x) + (b.y * b.y));
+			const double num = sqrt((rr * rr) + (ri * ri));
+			const double be = (den > 0) ? (num / den) : ((num > 0) ? 1.0 : 0.0);
+			atomicMax(&cb[j], (unsigned long long)__double_as_longlong(be));
+		}
+	}
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int cnt)
+{
+	const int e = blockIdx.x * blockDim.x + threadIdx.x;
+	if (e < cnt) p[e] = 0ull;
+}
+
+__global__ void k_addto(double2 *X, const double2 *D, long cnt)
+{
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e < cnt) X[e] = make_double2(X[e].x + D[e].x, X[e].y + D[e].y);
+}
+
 void ensure_mna(pgpu *g, const peec_t *p)
 {
 	if (g->A != nullptr) return;
@@ -375,6 +432,10 @@ void peec_gpu_release(peec_t *p)
 		dfree(g->aperm);
 		dfree(g->B);
 		dfree(g->Bw);
+		dfree(g->A32);
+		dfree(g->Brhs);
+		dfree(g->R);
+		dfree(g->cberr);
 		dfree(g->cellrow);
 		dfree(g->d_st_ri);
 		dfree(g->d_st_ci);
@@ -618,7 +679,7 @@ d_complex_t pgpu_pot_entry(pgpu_t *g, int i, int j, double kw)
 
 // ------------------------------------------------------------------------------------------
 int pgpu_sweep(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq, d_complex_t *zmat, d_complex_t *segi,
-               d_complex_t *cellq, pgpu_stats_t *st)
+               d_complex_t *cellq, pgpu_stats_t *st, int precision)
 {
 	const int n = p->nunknown, np = p->nport;
 	ensure_cells(g);
@@ -704,8 +765,30 @@ int pgpu_sweep(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq, d_complex_t *z
 		HIPCHK(hipEventRecord(ev[4], g->s));
 		HIPCHK(hipStreamSynchronize(g->s));   // stamps host buffers are released below
 		mna_stamps_free(&sp);
-		// factor
-		const int info = dlu_factor(g->s, n, (d_complex_t *)g->A, n, g->apiv, g->aperm, st);
+		// factor: in place in FP64, or an FP32 copy (the FP64 matrix is kept for the refinement)
+		int info;
+		bool fp64 = !precision;
+		if (precision) {
+			if (g->A32 == nullptr) {
+				g->A32 = dalloc<float2>((size_t)n * n);
+				const int npa = (np > 0) ? np : 1;
+				g->Brhs = dalloc<double2>((size_t)n * npa);
+				g->R = dalloc<double2>((size_t)n * npa);
+				g->cberr = dalloc<unsigned long long>(npa);
+			}
+			hipLaunchKernelGGL(k_demote64, dim3((unsigned)(((long)n * n + 255) / 256)), dim3(256), 0, g->s, (const double2 *)g->A,
+			                   g->A32, (long)n * n);
+			info = slu_factor(g->s, n, g->A32, n, g->apiv, g->aperm, st);
+			if (info >= 0) {
+				// a pivot that rounds to zero in FP32 storage: this point is solved in FP64
+				if (st) st->fp64_fallbacks++;
+				info = dlu_factor(g->s, n, (d_complex_t *)g->A, n, g->apiv, g->aperm, st);
+				fp64 = true;
+			}
+		}
+		else {
+			info = dlu_factor(g->s, n, (d_complex_t *)g->A, n, g->apiv, g->aperm, st);
+		}
 		HIPCHK(hipEventRecord(ev[5], g->s));
 		if (info >= 0) {
 			printf("*** singular matrix at f = %.5e Hz (row %d : floating node or ideal voltage source loop?)\n", freq, info);
@@ -716,8 +799,53 @@ int pgpu_sweep(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq, d_complex_t *z
 		// all ports at once
 		const long tb = (long)n * np;
 		hipLaunchKernelGGL(k_port_rhs, dim3((unsigned)((tb + 255) / 256)), dim3(256), 0, g->s, g->B, n, np, d_i1, d_i2);
-		dlu_solve_perm(g->s, n, (const d_complex_t *)g->A, n, g->aperm, (d_complex_t *)g->B, np, np,
-		               (d_complex_t *)g->Bw);
+		if (fp64) {
+			dlu_solve_perm(g->s, n, (const d_complex_t *)g->A, n, g->aperm, (d_complex_t *)g->B, np, np,
+			               (d_complex_t *)g->Bw);
+		}
+		else {
+			// x = LU32^-1 b, then r = b - A x in FP64 and x += LU32^-1 r while the componentwise
+			// backward error max_i |r_i| / (|A| |x| + |b|)_i falls by half per sweep and is above
+			// 2 eps64 (the stopping rule of LAPACK's zgerfs, at most RF_MAX sweeps); a point whose
+			// error stays above 1e-10 is factored again in FP64
+			constexpr int RF_MAX = 10;
+			HIPCHK(hipMemcpyAsync(g->Brhs, g->B, (size_t)n * np * sizeof(double2), hipMemcpyDeviceToDevice, g->s));
+			slu_solve_perm(g->s, n, g->A32, n, g->aperm, (d_complex_t *)g->B, np, np, (d_complex_t *)g->Bw);
+			std::vector<double> h(np);
+			double berr = 1, last = 2;
+			int its = 0;
+			for (;;) {
+				hipLaunchKernelGGL(k_zero_u64, dim3((np + 63) / 64), dim3(64), 0, g->s, g->cberr, np);
+				hipLaunchKernelGGL(k_resid, dim3((n + 3) / 4), dim3(256), 0, g->s, (const double2 *)g->A, n, (const double2 *)g->B,
+				                   (const double2 *)g->Brhs, g->R, np, g->cberr);
+				HIPCHK(hipMemcpyAsync(h.data(), g->cberr, (size_t)np * sizeof(double), hipMemcpyDeviceToHost, g->s));
+				HIPCHK(hipStreamSynchronize(g->s));
+				berr = 0;
+				for (int j = 0; j < np; j++)
+					if (h[j] > berr) berr = h[j];
+				if ((berr <= 4.5e-16) || (berr > 0.5 * last) || (its == RF_MAX)) break;
+				last = berr;
+				slu_solve_perm(g->s, n, g->A32, n, g->aperm, (d_complex_t *)g->R, np, np, (d_complex_t *)g->Bw);
+				hipLaunchKernelGGL(k_addto, dim3((unsigned)(((long)n * np + 255) / 256)), dim3(256), 0, g->s, g->B,
+				                   (const double2 *)g->R, (long)n * np);
+				its++;
+			}
+			if (berr > 1e-10) {
+				if (st) st->fp64_fallbacks++;
+				HIPCHK(hipMemcpyAsync(g->B, g->Brhs, (size_t)n * np * sizeof(double2), hipMemcpyDeviceToDevice, g->s));
+				if (dlu_factor(g->s, n, (d_complex_t *)g->A, n, g->apiv, g->aperm, st) >= 0) {
+					rc = 2;
+					break;
+				}
+				dlu_solve_perm(g->s, n, (const d_complex_t *)g->A, n, g->aperm, (d_complex_t *)g->B, np, np,
+				               (d_complex_t *)g->Bw);
+				berr = 0;
+			}
+			if (st) {
+				st->refine_its += its;
+				if (berr > st->berr) st->berr = berr;
+			}
+		}
 		if (usecap && (cellq != nullptr)) {
 			const long tq = (long)p->ncell * np;
 			hipLaunchKernelGGL(k_cell_charge, dim3((unsigned)((tq + 255) / 256)), dim3(256), 0, g->s, g->C, p->ncell,

@@ -69,6 +69,13 @@ __device__ __forceinline__ void amax_merge(double &v, int &i, double ov, int oi)
 	}
 }
 
+// storage <-> arithmetic: factors may be stored in FP64 (double2) or FP32 (float2); every
+// operation below runs in FP64
+__device__ __forceinline__ double2 ld2(const double2 &v) { return v; }
+__device__ __forceinline__ double2 ld2(const float2 &v) { return make_double2(v.x, v.y); }
+__device__ __forceinline__ void st2(double2 &d, double2 v) { d = v; }
+__device__ __forceinline__ void st2(float2 &d, double2 v) { d = make_float2((float)v.x, (float)v.y); }
+
 struct panel_ws {
 	double  *pubv;      // [2][P]     local max |re|+|im|
 	int     *pubi;      // [2][P]     its global row
@@ -85,7 +92,8 @@ constexpr int SROW = NB + 1;    // padded LDS row stride (complex): spreads row-
 // wave 0 merges the P candidates and fetches the pivot row and the current diagonal row, then
 // every thread swaps/eliminates its own row.  Four workgroup barriers and two global round
 // trips per column.
-__global__ __launch_bounds__(PANEL_T) void k_panel(double2 *A, long lda, int n, int k0, int nb, int R, int *piv,
+template <typename TS>
+__global__ __launch_bounds__(PANEL_T) void k_panel(TS *A, long lda, int n, int k0, int nb, int R, int *piv,
                                                     panel_ws ws, unsigned bar_base)
 {
 	extern __shared__ double2 S[];   // [R][SROW]
@@ -101,7 +109,7 @@ __global__ __launch_bounds__(PANEL_T) void k_panel(double2 *A, long lda, int n, 
 
 	for (int e = tid; e < rc * nb; e += PANEL_T) {
 		const int r = e / nb, c = e - r * nb;
-		S[r * SROW + c] = A[(long)(r0 + r) * lda + k0 + c];
+		S[r * SROW + c] = ld2(A[(long)(r0 + r) * lda + k0 + c]);
 	}
 	__syncthreads();
 
@@ -194,12 +202,13 @@ __global__ __launch_bounds__(PANEL_T) void k_panel(double2 *A, long lda, int n, 
 	}
 	for (int e = tid; e < rc * nb; e += PANEL_T) {
 		const int r = e / nb, c = e - r * nb;
-		A[(long)(r0 + r) * lda + k0 + c] = S[r * SROW + c];
+		st2(A[(long)(r0 + r) * lda + k0 + c], S[r * SROW + c]);
 	}
 }
 
 // apply piv[k0 .. k0+nb) to every column outside the panel, and to the running permutation
-__global__ void k_laswp(double2 *A, long lda, int n, int k0, int nb, const int *piv, int *perm, const int *info)
+template <typename TS>
+__global__ void k_laswp(TS *A, long lda, int n, int k0, int nb, const int *piv, int *perm, const int *info)
 {
 	if (*info >= 0) return;
 	const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -209,7 +218,7 @@ __global__ void k_laswp(double2 *A, long lda, int n, int k0, int nb, const int *
 		for (int kk = k0; kk < k0 + nb; kk++) {
 			const int p = piv[kk];
 			if (p != kk) {
-				const double2 x = A[(long)kk * lda + j];
+				const TS x = A[(long)kk * lda + j];
 				A[(long)kk * lda + j] = A[(long)p * lda + j];
 				A[(long)p * lda + j] = x;
 			}
@@ -229,8 +238,8 @@ __global__ void k_laswp(double2 *A, long lda, int n, int k0, int nb, const int *
 
 // X <- T^-1 X for an nb x nb triangle T (unit lower if LOWER, else non-unit upper) at T[0][0]
 // and the nb x ncol block X; one thread per column, T and X staged in LDS.
-template <bool LOWER>
-__global__ __launch_bounds__(64) void k_trsm(const double2 *T, long ldt, double2 *X, long ldx, int nb, int ncol)
+template <bool LOWER, typename TS>
+__global__ __launch_bounds__(64) void k_trsm(const TS *T, long ldt, TS *X, long ldx, int nb, int ncol)
 {
 	__shared__ double2 St[NB * NB];
 	__shared__ double2 Sx[NB * 64];
@@ -238,10 +247,10 @@ __global__ __launch_bounds__(64) void k_trsm(const double2 *T, long ldt, double2
 	const int j = blockIdx.x * 64 + tid;
 	for (int e = tid; e < nb * nb; e += 64) {
 		const int r = e / nb, c = e - r * nb;
-		St[r * NB + c] = T[(long)r * ldt + c];
+		St[r * NB + c] = ld2(T[(long)r * ldt + c]);
 	}
 	if (j < ncol)
-		for (int r = 0; r < nb; r++) Sx[r * 64 + tid] = X[(long)r * ldx + j];
+		for (int r = 0; r < nb; r++) Sx[r * 64 + tid] = ld2(X[(long)r * ldx + j]);
 	__syncthreads();
 	if (j >= ncol) return;
 	if (LOWER) {
@@ -266,7 +275,7 @@ __global__ __launch_bounds__(64) void k_trsm(const double2 *T, long ldt, double2
 			Sx[r * 64 + tid] = cmul(sacc, cinv(St[r * NB + r]));
 		}
 	}
-	for (int r = 0; r < nb; r++) X[(long)r * ldx + j] = Sx[r * 64 + tid];
+	for (int r = 0; r < nb; r++) st2(X[(long)r * ldx + j], Sx[r * 64 + tid]);
 }
 
 // C[M x N] -= A[M x K] B[K x N] on FP64 MFMA.  Wave tile 32 x 64 (2 x 4 blocks of 16 x 16),
@@ -367,14 +376,14 @@ __global__ __launch_bounds__(256, 2) void k_zgemm(int M, int N, int K, const dou
 // Few right-hand sides (nrhs <= 4): triangular solve of one NB x NB diagonal block by a single
 // wavefront, lane r holding row r (its NB triangle entries in registers) and x[r]; the column
 // sweep broadcasts x[q] with a lane shuffle.  Unit lower when LOWER, else non-unit upper.
-template <bool LOWER>
-__global__ __launch_bounds__(64) void k_trsv(const double2 *__restrict__ T, long ldt, double2 *X, long ldx, int nb, int nrhs)
+template <bool LOWER, typename TM>
+__global__ __launch_bounds__(64) void k_trsv(const TM *__restrict__ T, long ldt, double2 *X, long ldx, int nb, int nrhs)
 {
 	const int r = threadIdx.x;
 	double2 t[NB];
 #pragma unroll
-	for (int q = 0; q < NB; q++) t[q] = ((r < nb) && (q < nb)) ? T[(long)r * ldt + q] : make_double2(0, 0);
-	const double2 dinv = (r < nb) ? cinv(T[(long)r * ldt + r]) : make_double2(0, 0);
+	for (int q = 0; q < NB; q++) t[q] = ((r < nb) && (q < nb)) ? ld2(T[(long)r * ldt + q]) : make_double2(0, 0);
+	const double2 dinv = (r < nb) ? cinv(ld2(T[(long)r * ldt + r])) : make_double2(0, 0);
 	for (int j = 0; j < nrhs; j++) {
 		double2 x = (r < nb) ? X[(long)r * ldx + j] : make_double2(0, 0);
 		if (LOWER) {
@@ -406,7 +415,8 @@ __global__ __launch_bounds__(64) void k_trsv(const double2 *__restrict__ T, long
 
 // Y[M x nrhs] -= A[M x K] X[K x nrhs] for K <= NB and nrhs <= 4: 16 lanes per row, 16 rows per
 // 256-thread workgroup, partial dot products reduced by shuffles.
-__global__ __launch_bounds__(256) void k_gemv(int M, int K, int nrhs, const double2 *__restrict__ A, long lda,
+template <typename TM>
+__global__ __launch_bounds__(256) void k_gemv(int M, int K, int nrhs, const TM *__restrict__ A, long lda,
                                                const double2 *__restrict__ Xv, long ldx, double2 *Y, long ldy)
 {
 	__shared__ double2 xs[NB * 4];
@@ -421,7 +431,7 @@ __global__ __launch_bounds__(256) void k_gemv(int M, int K, int nrhs, const doub
 	for (int j = 0; j < 4; j++) acc[j] = make_double2(0, 0);
 	if (row < M) {
 		for (int k = sub; k < K; k += 16) {
-			const double2 a = A[(long)row * lda + k];
+			const double2 a = ld2(A[(long)row * lda + k]);
 			for (int j = 0; j < nrhs; j++) {
 				const double2 p = cmul(a, xs[k * 4 + j]);
 				acc[j].x += p.x;
@@ -480,7 +490,9 @@ lu_work g_lw;
 void ensure_ws(int n)
 {
 	if (!g_lw.attr) {
-		HIPCHK(hipFuncSetAttribute((const void *)k_panel, hipFuncAttributeMaxDynamicSharedMemorySize,
+		HIPCHK(hipFuncSetAttribute((const void *)k_panel<double2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+		                           PANEL_RMAX * SROW * (int)sizeof(double2)));
+		HIPCHK(hipFuncSetAttribute((const void *)k_panel<float2>, hipFuncAttributeMaxDynamicSharedMemorySize,
 		                           PANEL_RMAX * SROW * (int)sizeof(double2)));
 		int dev = 0;
 		HIPCHK(hipGetDevice(&dev));
@@ -517,6 +529,85 @@ void launch_gemm(hipStream_t s, int M, int N, int K, const double2 *A, long lda,
 	if ((M <= 0) || (N <= 0) || (K <= 0)) return;
 	dim3 grid((N + 127) / 128, (M + 63) / 64);
 	hipLaunchKernelGGL(k_zgemm, grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc);
+}
+
+// C[M x N] -= A[M x K] B[K x N] with complex FP32 storage, FP32 MFMA products accumulated in
+// FP32 (v_mfma_f32_16x16x4f32: A row = lane%16, k = lane/16; B col = lane%16, k = lane/16;
+// D register v of lane l holds row 4 (l/16) + v, col l%16).  Wave tile 32 x 32 (2 x 2 blocks),
+// workgroup 2 x 2 waves = 64 x 64; complex = 4 real MFMAs into (re, im) accumulators.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_cgemm32(int M, int N, int K, const float2 *__restrict__ A, long lda,
+                                                 const float2 *__restrict__ B, long ldb, float2 *__restrict__ C, long ldc)
+{
+	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	const int row0 = blockIdx.y * 64 + (wave >> 1) * 32, col0 = blockIdx.x * 64 + (wave & 1) * 32;
+	const int li = lane & 15, lk = lane >> 4;
+	f32x4 acr[2][2], aci[2][2];
+#pragma unroll
+	for (int a = 0; a < 2; a++)
+#pragma unroll
+		for (int b = 0; b < 2; b++) {
+			acr[a][b] = f32x4{0, 0, 0, 0};
+			aci[a][b] = f32x4{0, 0, 0, 0};
+		}
+	const float2 *arow[2], *bcol[2];
+	bool aok[2], bok[2];
+#pragma unroll
+	for (int rb = 0; rb < 2; rb++) {
+		const int r = row0 + rb * 16 + li;
+		aok[rb] = r < M;
+		arow[rb] = A + (long)(aok[rb] ? r : 0) * lda;
+	}
+#pragma unroll
+	for (int cb = 0; cb < 2; cb++) {
+		const int c = col0 + cb * 16 + li;
+		bok[cb] = c < N;
+		bcol[cb] = B + (bok[cb] ? c : 0);
+	}
+	for (int k = 0; k < K; k += 4) {
+		const int kk = k + lk;
+		float2 fa[2], fb[2];
+#pragma unroll
+		for (int rb = 0; rb < 2; rb++) fa[rb] = (aok[rb] && (kk < K)) ? arow[rb][kk] : make_float2(0, 0);
+#pragma unroll
+		for (int cb = 0; cb < 2; cb++) fb[cb] = (bok[cb] && (kk < K)) ? bcol[cb][(long)kk * ldb] : make_float2(0, 0);
+#pragma unroll
+		for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+			for (int cb = 0; cb < 2; cb++) {
+				acr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[rb].x, fb[cb].x, acr[rb][cb], 0, 0, 0);
+				acr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(-fa[rb].y, fb[cb].y, acr[rb][cb], 0, 0, 0);
+				aci[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[rb].x, fb[cb].y, aci[rb][cb], 0, 0, 0);
+				aci[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[rb].y, fb[cb].x, aci[rb][cb], 0, 0, 0);
+			}
+	}
+#pragma unroll
+	for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+		for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+			for (int v = 0; v < 4; v++) {
+				const int r = row0 + rb * 16 + 4 * lk + v, c = col0 + cb * 16 + li;
+				if ((r < M) && (c < N)) {
+					float2 &x = C[(long)r * ldc + c];
+					x = make_float2(x.x - acr[rb][cb][v], x.y - aci[rb][cb][v]);
+				}
+			}
+}
+
+void launch_update(hipStream_t s, int M, int N, int K, const double2 *A, long lda, const double2 *B, long ldb, double2 *C,
+                   long ldc)
+{
+	launch_gemm(s, M, N, K, A, lda, B, ldb, C, ldc);
+}
+
+void launch_update(hipStream_t s, int M, int N, int K, const float2 *A, long lda, const float2 *B, long ldb, float2 *C,
+                   long ldc)
+{
+	if ((M <= 0) || (N <= 0) || (K <= 0)) return;
+	dim3 grid((N + 63) / 64, (M + 63) / 64);
+	hipLaunchKernelGGL(k_cgemm32, grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc);
 }
 
 // ---------------------------------------------------------------- triangular inversion
@@ -661,10 +752,10 @@ extern "C" void dlu_zgemm_sub(void *stream, int M, int N, int K, const d_complex
 }
 
 
-extern "C" int dlu_factor(void *stream, int n, d_complex_t *dA_, int lda, int *dpiv, int *dperm, pgpu_stats_t *st)
+namespace {
+template <typename TS>
+int lu_factor_t(hipStream_t s, int n, TS *A, int lda, int *dpiv, int *dperm, pgpu_stats_t *st)
 {
-	hipStream_t s = (hipStream_t)stream;
-	double2 *A = (double2 *)dA_;
 	if (n <= 0) return -1;
 	ensure_ws(n);
 	int *perm = (dperm != nullptr) ? dperm : g_lw.perm;
@@ -686,7 +777,7 @@ extern "C" int dlu_factor(void *stream, int n, d_complex_t *dA_, int lda, int *d
 		int P = (m + PANEL_RMAX - 1) / PANEL_RMAX;
 		if (P < 1) P = 1;
 		if (P > pmax) {
-			fprintf(stderr, "*** dlu_factor: n = %d exceeds the panel capacity (%d rows)\n", n, pmax * PANEL_RMAX);
+			fprintf(stderr, "*** LU factorisation: n = %d exceeds the panel capacity (%d rows)\n", n, pmax * PANEL_RMAX);
 			return 0;
 		}
 		int R = (m + P - 1) / P;
@@ -694,25 +785,25 @@ extern "C" int dlu_factor(void *stream, int n, d_complex_t *dA_, int lda, int *d
 		int nn = n, kk0 = k0, nbb = nb;
 		panel_ws ws = g_lw.ws;
 		unsigned bb = bar_base;
-		double2 *Ap = A;
+		TS *Ap = A;
 		long ldp = ldal;
 		void *kargs[] = {(void *)&Ap, (void *)&ldp, (void *)&nn, (void *)&kk0, (void *)&nbb,
 		                 (void *)&R, (void *)&pivp, (void *)&ws, (void *)&bb};
-		HIPCHK(hipLaunchCooperativeKernel((const void *)k_panel, dim3(P), dim3(PANEL_T), kargs,
+		HIPCHK(hipLaunchCooperativeKernel((const void *)k_panel<TS>, dim3(P), dim3(PANEL_T), kargs,
 		                                  (unsigned)(R * SROW * sizeof(double2)), s));
 		bar_base += (unsigned)(P * nb);
 		const int nout = n - nb;
-		hipLaunchKernelGGL(k_laswp, dim3((nout + 1 + 255) / 256), dim3(256), 0, s, A, ldal, n, k0, nb, dpiv, perm,
+		hipLaunchKernelGGL(k_laswp<TS>, dim3((nout + 1 + 255) / 256), dim3(256), 0, s, A, ldal, n, k0, nb, dpiv, perm,
 		                   (const int *)g_lw.ws.info);
 		const int nr = n - k0 - nb;
 		if (nr > 0) {
-			double2 *A11 = A + (long)k0 * lda + k0;
-			double2 *A12 = A + (long)k0 * lda + k0 + nb;
-			double2 *A21 = A + (long)(k0 + nb) * lda + k0;
-			double2 *A22 = A + (long)(k0 + nb) * lda + k0 + nb;
-			hipLaunchKernelGGL(k_trsm<true>, dim3((nr + 63) / 64), dim3(64), 0, s, A11, ldal, A12, ldal, nb, nr);
+			TS *A11 = A + (long)k0 * lda + k0;
+			TS *A12 = A + (long)k0 * lda + k0 + nb;
+			TS *A21 = A + (long)(k0 + nb) * lda + k0;
+			TS *A22 = A + (long)(k0 + nb) * lda + k0 + nb;
+			hipLaunchKernelGGL((k_trsm<true, TS>), dim3((nr + 63) / 64), dim3(64), 0, s, A11, ldal, A12, ldal, nb, nr);
 			if (st) HIPCHK(hipEventRecord(e0, s));
-			launch_gemm(s, nr, nr, nb, A21, ldal, A12, ldal, A22, ldal);
+			launch_update(s, nr, nr, nb, (const TS *)A21, ldal, (const TS *)A12, ldal, A22, ldal);
 			if (st) {
 				HIPCHK(hipEventRecord(e1, s));
 				HIPCHK(hipEventSynchronize(e1));
@@ -734,6 +825,60 @@ extern "C" int dlu_factor(void *stream, int n, d_complex_t *dA_, int lda, int *d
 	return info;
 }
 
+}  // namespace
+
+extern "C" int dlu_factor(void *stream, int n, d_complex_t *dA_, int lda, int *dpiv, int *dperm, pgpu_stats_t *st)
+{
+	return lu_factor_t<double2>((hipStream_t)stream, n, (double2 *)dA_, lda, dpiv, dperm, st);
+}
+
+// The same factorisation with the matrix stored in FP32: panels, pivot search and TRSM in FP64
+// arithmetic on FP32 storage, trailing updates as FP32 complex GEMMs on MFMA.
+extern "C" int slu_factor(void *stream, int n, void *dA, int lda, int *dpiv, int *dperm, pgpu_stats_t *st)
+{
+	return lu_factor_t<float2>((hipStream_t)stream, n, (float2 *)dA, lda, dpiv, dperm, st);
+}
+
+// B <- A^-1 B (FP64 right-hand sides) with FP32-stored factors from slu_factor, nrhs <= 4 per
+// pass; dwork holds n * nrhs entries.
+extern "C" void slu_solve_perm(void *stream, int n, const void *dA_, int lda, const int *dperm, d_complex_t *dB_, int ldb,
+                               int nrhs, d_complex_t *dwork)
+{
+	hipStream_t s = (hipStream_t)stream;
+	const float2 *A = (const float2 *)dA_;
+	double2 *B0 = (double2 *)dB_;
+	double2 *W = (double2 *)dwork;
+	for (int j0 = 0; j0 < nrhs; j0 += 4) {
+		const int nr = (nrhs - j0 < 4) ? (nrhs - j0) : 4;
+		double2 *B = B0 + j0;
+		const long tot = (long)n * nr;
+		hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, (const double2 *)B, (long)ldb, W,
+		                   (long)nr, dperm, n, nr);
+		hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, (const double2 *)W, (long)nr, B,
+		                   (long)ldb, n, nr);
+		for (int r0 = 0; r0 < n; r0 += NB) {
+			const int nb = (n - r0 < NB) ? (n - r0) : NB;
+			hipLaunchKernelGGL((k_trsv<true, float2>), dim3(1), dim3(64), 0, s, A + (long)r0 * lda + r0, (long)lda,
+			                   B + (long)r0 * ldb, (long)ldb, nb, nr);
+			const int rest = n - r0 - nb;
+			if (rest > 0)
+				hipLaunchKernelGGL(k_gemv<float2>, dim3((rest + 15) / 16), dim3(256), 0, s, rest, nb, nr,
+				                   A + (long)(r0 + nb) * lda + r0, (long)lda, (const double2 *)(B + (long)r0 * ldb), (long)ldb,
+				                   B + (long)(r0 + nb) * ldb, (long)ldb);
+		}
+		const int nblk = (n + NB - 1) / NB;
+		for (int b = nblk - 1; b >= 0; b--) {
+			const int r0 = b * NB;
+			const int nb = (n - r0 < NB) ? (n - r0) : NB;
+			hipLaunchKernelGGL((k_trsv<false, float2>), dim3(1), dim3(64), 0, s, A + (long)r0 * lda + r0, (long)lda,
+			                   B + (long)r0 * ldb, (long)ldb, nb, nr);
+			if (r0 > 0)
+				hipLaunchKernelGGL(k_gemv<float2>, dim3((r0 + 15) / 16), dim3(256), 0, s, r0, nb, nr, A + r0, (long)lda,
+				                   (const double2 *)(B + (long)r0 * ldb), (long)ldb, B, (long)ldb);
+		}
+	}
+}
+
 // B <- A^-1 B using the factors and the row permutation recorded by the last dlu_factor on
 // this matrix (perm must be the one returned with these factors: dperm).
 extern "C" void dlu_solve_perm(void *stream, int n, const d_complex_t *dA_, int lda, const int *dperm,
@@ -753,12 +898,12 @@ extern "C" void dlu_solve_perm(void *stream, int n, const d_complex_t *dA_, int 
 	for (int r0 = 0; r0 < n; r0 += NB) {
 		const int nb = (n - r0 < NB) ? (n - r0) : NB;
 		const double2 *T = A + (long)r0 * lda + r0;
-		if (few) hipLaunchKernelGGL(k_trsv<true>, dim3(1), dim3(64), 0, s, T, (long)lda, B + (long)r0 * ldb, (long)ldb, nb, nrhs);
-		else hipLaunchKernelGGL(k_trsm<true>, dim3((nrhs + 63) / 64), dim3(64), 0, s, T, (long)lda, B + (long)r0 * ldb, (long)ldb, nb, nrhs);
+		if (few) hipLaunchKernelGGL((k_trsv<true, double2>), dim3(1), dim3(64), 0, s, T, (long)lda, B + (long)r0 * ldb, (long)ldb, nb, nrhs);
+		else hipLaunchKernelGGL((k_trsm<true, double2>), dim3((nrhs + 63) / 64), dim3(64), 0, s, T, (long)lda, B + (long)r0 * ldb, (long)ldb, nb, nrhs);
 		const int rest = n - r0 - nb;
 		if (rest <= 0) continue;
 		if (few)
-			hipLaunchKernelGGL(k_gemv, dim3((rest + 15) / 16), dim3(256), 0, s, rest, nb, nrhs, A + (long)(r0 + nb) * lda + r0,
+			hipLaunchKernelGGL(k_gemv<double2>, dim3((rest + 15) / 16), dim3(256), 0, s, rest, nb, nrhs, A + (long)(r0 + nb) * lda + r0,
 			                   (long)lda, B + (long)r0 * ldb, (long)ldb, B + (long)(r0 + nb) * ldb, (long)ldb);
 		else
 			launch_gemm(s, rest, nrhs, nb, A + (long)(r0 + nb) * lda + r0, lda, B + (long)r0 * ldb, ldb,
@@ -770,11 +915,11 @@ extern "C" void dlu_solve_perm(void *stream, int n, const d_complex_t *dA_, int 
 		const int r0 = b * NB;
 		const int nb = (n - r0 < NB) ? (n - r0) : NB;
 		const double2 *T = A + (long)r0 * lda + r0;
-		if (few) hipLaunchKernelGGL(k_trsv<false>, dim3(1), dim3(64), 0, s, T, (long)lda, B + (long)r0 * ldb, (long)ldb, nb, nrhs);
-		else hipLaunchKernelGGL(k_trsm<false>, dim3((nrhs + 63) / 64), dim3(64), 0, s, T, (long)lda, B + (long)r0 * ldb, (long)ldb, nb, nrhs);
+		if (few) hipLaunchKernelGGL((k_trsv<false, double2>), dim3(1), dim3(64), 0, s, T, (long)lda, B + (long)r0 * ldb, (long)ldb, nb, nrhs);
+		else hipLaunchKernelGGL((k_trsm<false, double2>), dim3((nrhs + 63) / 64), dim3(64), 0, s, T, (long)lda, B + (long)r0 * ldb, (long)ldb, nb, nrhs);
 		if (r0 <= 0) continue;
 		if (few)
-			hipLaunchKernelGGL(k_gemv, dim3((r0 + 15) / 16), dim3(256), 0, s, r0, nb, nrhs, A + r0, (long)lda,
+			hipLaunchKernelGGL(k_gemv<double2>, dim3((r0 + 15) / 16), dim3(256), 0, s, r0, nb, nrhs, A + r0, (long)lda,
 			                   B + (long)r0 * ldb, (long)ldb, B, (long)ldb);
 		else
 			launch_gemm(s, r0, nrhs, nb, A + r0, lda, B + (long)r0 * ldb, ldb, B, ldb);

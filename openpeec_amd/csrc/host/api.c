@@ -296,7 +296,7 @@ int solve(peec_t *p, FILE *fp_log)
 		}
 		const int rc = pgpu_sweep(g, p, ifreq, 1, &p->zmat[ZIDX(p, ifreq, 0, 0)],
 			p->segi ? &p->segi[DIDX(p, ifreq, 0, 0)] : NULL,
-			p->cellq ? &p->cellq[QIDX(p, ifreq, 0, 0)] : NULL, NULL);
+			p->cellq ? &p->cellq[QIDX(p, ifreq, 0, 0)] : NULL, NULL, 0);
 		if (rc == 1) {
 			if (fp_log) fprintf(fp_log, "*** singular potential coefficient matrix at f = %.5e Hz\n", f);
 			return 1;
@@ -370,7 +370,8 @@ int peec_sweep_ex(peec_t *p, int ifreq0, int nfreq, int form, int batch, d_compl
 		st->ising = -1;
 	}
 	if (p->compress) return pgpu_sweep_compressed(pgpu_get(p), p, ifreq0, nfreq, zmat, st);
-	if (form == PEEC_FORM_PLAIN) return pgpu_sweep(pgpu_get(p), p, ifreq0, nfreq, zmat, NULL, NULL, st);
+	if (form == PEEC_FORM_PLAIN) return pgpu_sweep(pgpu_get(p), p, ifreq0, nfreq, zmat, NULL, NULL, st, 0);
+	if (form == PEEC_FORM_PRECISION) return pgpu_sweep(pgpu_get(p), p, ifreq0, nfreq, zmat, NULL, NULL, st, 1);
 	if (form == PEEC_FORM_MIXED) return pgpu_sweep_fast(pgpu_get(p), p, ifreq0, nfreq, zmat, st, batch);
 	printf("%s\n", "*** peec_sweep: unknown solver form");
 	return 4;

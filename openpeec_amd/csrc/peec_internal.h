@@ -121,11 +121,12 @@ d_complex_t pgpu_pot_entry(pgpu_t *g, int i, int j, double kw);
 
 /* sweep of frequency points, device resident: fills, assembles, factors, solves every port.
  * Writes zmat (nport x nport per point), and optionally segi/cellq rows.  Returns 0 or an error
- * code (1 singular P, 2 singular MNA). */
+ * code (1 singular P, 2 singular MNA, 3 refinement stalled); precision = 1 factors an FP32 copy and
+ * refines in FP64 (PEEC_FORM_PRECISION). */
 typedef peec_stats_t pgpu_stats_t;
 
 int pgpu_sweep(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq, d_complex_t *zmat,
-	d_complex_t *segi, d_complex_t *cellq, pgpu_stats_t *st);
+	d_complex_t *segi, d_complex_t *cellq, pgpu_stats_t *st, int precision);
 
 /* accelerated sweep (sweep_fast.hip): charge-unknown system, stale FP32-stored LU as GMRES
  * preconditioner, frequencies in lock-step batches of `batch` systems */
@@ -150,6 +151,11 @@ void dlu_solve_perm(void *stream, int n, const d_complex_t *dA, int lda, const i
  * leading dimension n out); dinv: ceil(n/64) 64 x 64 blocks, scr: (ceil(n/2) + 64)^2 entries */
 void dlu_tri_inverse(void *stream, int n, const d_complex_t *LU, int lda, int lower, d_complex_t *Out,
 	d_complex_t *scr, d_complex_t *dinv);
+/* the same with FP32 storage (float2 pairs; FP64 panels, FP32-MFMA trailing updates), and the
+ * solve of FP64 right-hand sides with those factors */
+int  slu_factor(void *stream, int n, void *dA, int lda, int *dpiv, int *dperm, pgpu_stats_t *st);
+void slu_solve_perm(void *stream, int n, const void *dA, int lda, const int *dperm,
+	d_complex_t *dB, int ldb, int nrhs, d_complex_t *dwork);
 /* C[M x N] -= A[M x K] B[K x N] (complex FP64 MFMA) */
 void dlu_zgemm_sub(void *stream, int M, int N, int K, const d_complex_t *A, long lda,
 	const d_complex_t *B, long ldb, d_complex_t *C, long ldc);
