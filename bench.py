@@ -19,7 +19,9 @@ Zin of the mixed form is compared with the plain FP64 form on three of the timed
 metric is quoted at Zin rel err <= 1e-6); the plain form's own rate on those points is reported
 beside it.  The roofline block is the LU trailing-update GEMM (k_zgemm, FP64 MFMA) timed with HIP
 events on the library's stream around every launch; the GMRES matvec is reported in GB/s the
-same way.  cpu_baseline times the oracle (the reference's arithmetic restated in C, OpenMP on
+same way.  The precision form (FP32-stored LU, FP32 MFMA trailing updates, FP64 refinement)
+runs on the same three points and reports its rate and its Zin error against the plain form per
+point.  cpu_baseline times the oracle (the reference's arithmetic restated in C, OpenMP on
 all host cores) on a bounded sample and extrapolates to one frequency point.
 """
 from __future__ import annotations
@@ -40,6 +42,7 @@ sys.path.insert(0, str(ROOT))
 METRIC = "freq points/s (Lp/P fill+solve) at Zin rel err ≤1e-6 vs FP64; fill GFLOP/s"
 PEAK_FP64_MFMA = 78.6     # TFLOP/s, MI355X dense FP64 matrix
 PEAK_FP64_VECTOR = 78.6   # TFLOP/s, MI355X FP64 vector
+PEAK_FP32_MFMA = 157.3    # TFLOP/s, MI355X dense FP32 matrix
 
 
 def parse():
@@ -144,6 +147,39 @@ def plain_points(m, idx):
     return np.array(zs), time.perf_counter() - t0, st
 
 
+def precision_points(m, idx, zp):
+    """Precision form (FP32-stored LU with FP32 MFMA trailing updates, FP64 refinement) at the
+    given points: rate, and the error of each Zin against the plain FP64 form."""
+    from openpeec_amd import PEEC_FORM_PRECISION
+    m.sweep(idx[0], 1, PEEC_FORM_PRECISION)   # first use allocates the FP32 copy
+    zs, sts = [], []
+    t0 = time.perf_counter()
+    for i in idx:
+        z, st = m.sweep(i, 1, PEEC_FORM_PRECISION)
+        zs.append(z[0, 0, 0])
+        sts.append(st)
+    elapsed = time.perf_counter() - t0
+    zs = np.array(zs)
+    errs = np.abs(zs - zp) / np.abs(zp)
+    gemm_ms = sum(s.lu_gemm_ms for s in sts)
+    gemm_flop = sum(s.lu_gemm_flop for s in sts)
+    tf = gemm_flop / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else None
+    last = sts[-1]
+    return {
+        "points_per_s": len(idx) / elapsed,
+        "zin_rel_err_vs_plain_fp64": [float(e) for e in errs],
+        "max_backward_error": max(s.berr for s in sts),
+        "backward_error_kind": "componentwise, max_i |b - A x|_i / (|A| |x| + |b|)_i, FP64 residual",
+        "refinement_sweeps": [int(s.refine_its) for s in sts],
+        "fp64_fallbacks": int(sum(s.fp64_fallbacks for s in sts)),
+        "stages_ms_last_point": {"lp_fill": last.fill_ms, "charge_fill": last.pot_ms, "p_lu_and_inverse": last.cinv_ms,
+                                 "mna_assembly": last.asm_ms, "mna_lu_fp32": last.lu_ms,
+                                 "solve_and_refine": last.solve_ms, "total": last.total_ms},
+        "lu_fp32_gemm": {"TFLOP_per_s": tf, "frac_of_fp32_mfma_peak": (tf / PEAK_FP32_MFMA) if tf else None,
+                         "launches": int(sum(s.lu_gemm_launches for s in sts))},
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -176,6 +212,7 @@ def main():
         idx = sorted({0, steps // 2, steps - 1})
         zm = np.array([z[i, 0, 0] for i in idx])
         zp, t_plain, stp = plain_points(m, idx)
+        prec = precision_points(m, idx, zp)
         fill_flop_top = lib.peec_fill_flops(C.byref(m.p), m.freq(idx[-1]))
         plain_fill_ms = stp.fill_ms + stp.pot_ms
         err = float(np.max(np.abs(zm - zp) / np.abs(zp)))
@@ -200,6 +237,7 @@ def main():
                           "bar": 1e-6, "ok": err <= 1e-6},
             "plain_fp64_points_per_s": len(idx) / t_plain,
             "speedup_vs_plain_fp64": (steps / elapsed) / (len(idx) / t_plain),
+            "precision_form": prec,
             "stages_ms": {"fill_incl_kf_setup": st.fill_ms, "kf_setup": st.kf_setup_ms,
                           "lu_and_inverse": st.lu_ms, "gmres": st.solve_ms, "total": st.total_ms},
             "n_lu": st.n_lu, "gmres_its_per_point": st.gmres_its / steps, "kf_direct_pairs": st.kf_direct_pairs,
