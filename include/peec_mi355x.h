@@ -100,6 +100,39 @@ int hmat_tree_info(int n, const double *pts, int *counts, int *perm, int *far, i
  * no counterpart in the reference.  Returns -1, or the first zero pivot's row. */
 int lu_decomp32(int n, d_complex_t *a, int *piv);
 
+/* Batched sweep of many structures (many decks, each of the reference's solve() loops at
+ * once).  Every (structure, frequency) pair is one dense charge-unknown system; frequencies
+ * ifreq0 .. ifreq0 + nfreq - 1 of each structure's own sweep.  Fills from k-factored records,
+ * factorisation and port solves in persistent batched kernels replayed as hipGraphs; the LU
+ * panel width, persistent grid and job order come from a timing autotuner (peec_batch_tune, or
+ * the first sweep).  No counterpart in the reference (its callers loop over decks); each
+ * structure's Z equals what peec_sweep gives for it.  Structures must be set up
+ * (peec_setup), dense (compression 0), with 1..8 ports and at most 4096 charge-form unknowns. */
+typedef struct peec_batch peec_batch_t;
+typedef struct {
+	double total_ms, tune_ms;
+	double fill_ms;         /* fill graphs (and element-by-element fallbacks)  */
+	double lu_ms;           /* stamp + factor + solve graphs                   */
+	double stamp_ms;        /* host stamps and uploads                         */
+	double lu_flop;         /* 8/3 N^3 + 8 N^2 per port, summed over jobs      */
+	long   jobs;
+	int    chunks;
+	int    nb, wg_per_cu, order, tune_trials;
+	int    direct_structures; /* filled element by element (no usable records) */
+} peec_batch_stats_t;
+peec_batch_t *peec_batch_create(peec_t **models, int nmodel, int ifreq0, int nfreq);
+int  peec_batch_tune(peec_batch_t *b, peec_batch_stats_t *st);
+/* zmat[s] receives nfreq x nport x nport of structure s; returns 0, 2 (a singular system) or 4 */
+int  peec_batch_sweep(peec_batch_t *b, d_complex_t **zmat, peec_batch_stats_t *st);
+/* the autotuner's candidates and their LU times (ms); returns their count */
+int  peec_batch_trials(const peec_batch_t *b, int *nb, int *wg_per_cu, int *order, double *ms, int cap);
+void peec_batch_free(peec_batch_t *b);
+/* The batched factorisation alone (tests): nmat row-major matrices of orders n[j],
+ * concatenated in a, factored in place as lu_decomp (src/lu.c:16-66) does, one workgroup each
+ * (panel width nb = 16 or 32); piv receives each matrix's interchanges, concatenated.  Returns
+ * -1, the first matrix with a pivot below 1e-30, or -2 for bad arguments. */
+int  peec_batch_lu_decomp(int nmat, const int *n, d_complex_t *a, int *piv, int nb);
+
 /* Sizes for bindings */
 size_t peec_sizeof_peec(void);
 size_t peec_sizeof_seg(void);

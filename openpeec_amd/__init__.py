@@ -178,3 +178,55 @@ class Peec:
     def smat(self) -> np.ndarray:
         np_ = self.p.nport
         return cview(self.p.smat, (self.p.nfreq, np_, np_)).copy()
+
+
+class Batch:
+    """Many set-up structures swept together (peec_batch_*): every (structure, frequency) pair is
+    one system of the persistent batched kernels.  Frequencies ifreq0 .. ifreq0 + nfreq - 1 of
+    each structure's own sweep."""
+
+    def __init__(self, models: "list[Peec]", ifreq0: int, nfreq: int):
+        if not models:
+            raise ValueError("no structures")
+        self.lib = models[0].lib
+        self.models = models
+        self.nfreq = nfreq
+        arr = (C.POINTER(abi.peec_t) * len(models))(*[C.pointer(m.p) for m in models])
+        self._arr = arr
+        self.h = self.lib.peec_batch_create(arr, len(models), ifreq0, nfreq)
+        if not self.h:
+            raise RuntimeError("peec_batch_create rejected the structures")
+
+    def tune(self) -> abi.peec_batch_stats_t:
+        st = abi.peec_batch_stats_t()
+        if self.lib.peec_batch_tune(self.h, C.byref(st)) != 0:
+            raise RuntimeError("peec_batch_tune failed")
+        return st
+
+    def trials(self) -> "list[tuple[int, int, int, float]]":
+        cap = 64
+        nb, wpc, order = (C.c_int * cap)(), (C.c_int * cap)(), (C.c_int * cap)()
+        ms = (C.c_double * cap)()
+        n = self.lib.peec_batch_trials(self.h, nb, wpc, order, ms, cap)
+        return [(nb[k], wpc[k], order[k], ms[k]) for k in range(min(n, cap))]
+
+    def sweep(self):
+        """[Z per structure (nfreq x nport x nport)], stats."""
+        zs = [np.zeros((self.nfreq, m.p.nport, m.p.nport), dtype=np.complex128) for m in self.models]
+        ptrs = (C.POINTER(abi.d_complex_t) * len(zs))(*[z.ctypes.data_as(C.POINTER(abi.d_complex_t)) for z in zs])
+        st = abi.peec_batch_stats_t()
+        rc = self.lib.peec_batch_sweep(self.h, ptrs, C.byref(st))
+        if rc != 0:
+            raise RuntimeError(f"peec_batch_sweep failed (code {rc})")
+        return zs, st
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.peec_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,1085 @@
+// batch.hip — many structures x many frequencies in persistent batched kernels.
+//
+// A caller with many small structures (the reference solves each deck with its own solve(),
+// solve.c:171-430, one frequency after the other) hands them over at once.  Every (structure,
+// frequency) pair is a job: one dense charge-unknown MNA system (unknowns [nodes | L | V |
+// branch currents | cell charges], the reference's compressed formulation mna.c:86-107 with the
+// kernel blocks dense: -jw Lp on the branch block, P on the charge block, no P^-1), factored
+// and solved for every port.  Per chunk of frequencies (sized to an HBM budget) the device work
+// is two hipGraphs:
+//
+//   fill graph    k_b_zero   clears the chunk's system arena (a kernel, so the graph holds no
+//                            memset node)
+//                 k_b_lp     -jw Lp(f) of every job from the structure's k-factored records
+//                            (one sincos + 15-term Horner per pair), written into the system
+//                 k_b_p      P(f) = sum of sub-cell integrals / (4 pi eps0 A1 A2) from the
+//                            charge records, in the reference's accumulation order
+//   solve graph   k_b_stamp  the sparse stamps (lumped elements, incidence, internal impedance,
+//                            charge couplings), computed on the host per job
+//                 k_b_reset  work-queue counter
+//                 k_b_lu     persistent: every workgroup takes the next job from the queue
+//                            (largest first or interleaved, as tuned), factors it in place
+//                            (right-looking blocked LU, |re|+|im| first-maximum row pivoting as
+//                            lu.c:16-66, FP64 panel, TRSM, trailing update on FP64 MFMA), solves
+//                            every port right-hand side and writes Z
+//
+// A structure whose k-factored records list pairs for direct evaluation (quadrature order
+// changing inside the band) is filled element by element per job between the two graphs.
+//
+// The LU panel width, the workgroups per CU of the persistent grid and the job order are chosen
+// by a timing autotuner on a sample of the first chunk's jobs (peec_batch_tune); the choice is
+// kept per batch shape.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <thread>
+#include <tuple>
+#include <vector>
+#include "hip_check.hpp"
+#include "fill_kernels.hpp"
+#include "kf_eval.hpp"
+#include "../peec_internal.h"
+#include "peec_mi355x.h"
+
+namespace {
+
+constexpr int BMAXN = 4096;    // largest system order of a batched job (LDS holds b and pivots)
+constexpr int BMAXPORT = 8;
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+struct bdesc {                 // one structure
+	int N;                     // system order
+	int offS, nseg;            // branch block
+	int offQ, ncell;           // charge block (ncell 0: none)
+	int T;                     // record terms
+	int ntS, ntH;              // tile grid sizes of the Lp and sub-cell triangles
+	long nslS, nslH;           // record slots
+	const double *rec_lp, *rec_ig;
+	const int *csoff, *csidx;
+	const double *carea;
+	int np;
+	int i1[BMAXPORT], i2[BMAXPORT];   // port node rows (-1: reference node)
+	int direct;                // filled element by element (records unusable)
+};
+
+struct bjob {                  // one structure at one frequency
+	int s;
+	int N;
+	double f, omega, kw;
+	long aoff;                 // system offset in the arena (entries)
+	long zoff;                 // Z offset in the chunk's output (entries)
+	long tS, tP;               // first Lp tile, first P tile of the job in the flattened grids
+};
+
+// ---------------------------------------------------------------------------------- fills
+
+__global__ void k_b_zero(double2 *A, long cnt)
+{
+	const long stride = (long)gridDim.x * blockDim.x;
+	for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < cnt; e += stride) A[e] = make_double2(0, 0);
+}
+
+// job of a flattened tile index: last job whose first tile is <= L (tiles ascend with jobs)
+__device__ __forceinline__ int job_of(const bjob *J, int nj, long L, bool lp)
+{
+	int lo = 0, hi = nj - 1;
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		const long t = lp ? J[mid].tS : J[mid].tP;
+		if (t <= L) lo = mid;
+		else hi = mid - 1;
+	}
+	return lo;
+}
+
+__device__ __forceinline__ void tri_of(long L, int nt, int &ti, int &tj)
+{
+	const double b = 2.0 * nt + 1.0;
+	int r = (int)floor((b - sqrt(b * b - 8.0 * (double)L)) * 0.5);
+	if (r < 0) r = 0;
+	if (r > nt - 1) r = nt - 1;
+	auto start = [nt](long rr) { return rr * (long)nt - (rr * (rr - 1)) / 2; };
+	while ((r > 0) && (start(r) > L)) r--;
+	while ((r < nt - 1) && (start(r + 1) <= L)) r++;
+	ti = r;
+	tj = r + (int)(L - start(r));
+}
+
+// -jw Lp of every job: one 16 x 16 tile of a job's upper triangle per workgroup
+__global__ __launch_bounds__(256) void k_b_lp(const bdesc *D, const bjob *J, int nj, double2 *arena)
+{
+	const long L = blockIdx.x;
+	const int jb = job_of(J, nj, L, true);
+	const bjob job = J[jb];
+	const bdesc &d = D[job.s];
+	if (d.direct) return;
+	int ti, tj;
+	const long t = L - job.tS;
+	tri_of(t, d.ntS, ti, tj);
+	const int i = ti * 16 + (threadIdx.x >> 4), j = tj * 16 + (threadIdx.x & 15);
+	if ((i >= d.nseg) || (j >= d.nseg) || (j < i)) return;
+	double vr, vi;
+	pe::kf_value(d.rec_lp, d.nslS, t * 256 + threadIdx.x, d.T, job.kw, vr, vi);
+	// d_mul((0, -w), v)
+	const double w = job.omega;
+	const double2 x = make_double2((0 * vr) - ((-w) * vi), (0 * vi) + ((-w) * vr));
+	double2 *A = arena + job.aoff;
+	const long N = job.N;
+	A[(long)(d.offS + i) * N + d.offS + j] = x;
+	if (i != j) A[(long)(d.offS + j) * N + d.offS + i] = x;
+}
+
+// P of every job: one 16 x 16 tile of a job's (full) cell x cell block per workgroup
+__global__ __launch_bounds__(256) void k_b_p(const bdesc *D, const bjob *J, int nj, double k4pe, double2 *arena)
+{
+	const long L = blockIdx.x;
+	const int jb = job_of(J, nj, L, false);
+	const bjob job = J[jb];
+	const bdesc &d = D[job.s];
+	if (d.direct || (d.ncell == 0)) return;
+	const int ntc = (d.ncell + 15) / 16;
+	const long t = L - job.tP;
+	const int ti = (int)(t / ntc), tj = (int)(t - (long)ti * ntc);
+	const int c1 = ti * 16 + (threadIdx.x >> 4), c2 = tj * 16 + (threadIdx.x & 15);
+	if ((c1 >= d.ncell) || (c2 >= d.ncell)) return;
+	double sr = 0, si = 0;
+	for (int p = d.csoff[c1]; p < d.csoff[c1 + 1]; p++) {
+		const int h1 = d.csidx[p];
+		for (int q = d.csoff[c2]; q < d.csoff[c2 + 1]; q++) {
+			const int h2 = d.csidx[q];
+			const int lo = (h1 <= h2) ? h1 : h2, hi = (h1 <= h2) ? h2 : h1;
+			double gr, gi;
+			pe::kf_value(d.rec_ig, d.nslH, pe::kf_slot(lo, hi, d.ntH), d.T, job.kw, gr, gi);
+			sr = sr + gr;
+			si = si + gi;
+		}
+	}
+	const double s = 1 / (k4pe * d.carea[c1] * d.carea[c2]);
+	arena[job.aoff + (long)(d.offQ + c1) * job.N + d.offQ + c2] = make_double2(s * sr, s * si);
+}
+
+// element-by-element blocks of one job (structures with directly evaluated pairs)
+__global__ void k_b_scatter(double2 *A, int N, int offS, int nseg, const double2 *lp, double omega, int offQ, int nc,
+                            const double2 *P)
+{
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	const long ns2 = (long)nseg * nseg, nc2 = (long)nc * nc;
+	if (e < ns2) {
+		const int r = (int)(e / nseg), c = (int)(e - (long)r * nseg);
+		const double2 x = lp[e];
+		A[(long)(offS + r) * N + offS + c] = make_double2((0 * x.x) - ((-omega) * x.y), (0 * x.y) + ((-omega) * x.x));
+	}
+	else if ((P != nullptr) && (e < ns2 + nc2)) {
+		const long q = e - ns2;
+		const int r = (int)(q / nc), c = (int)(q - (long)r * nc);
+		A[(long)(offQ + r) * N + offQ + c] = P[q];
+	}
+}
+
+__global__ void k_b_stamp(const bjob *J, const int *sj, const int *ri, const int *ci, const double2 *val, long nst,
+                          double2 *arena)
+{
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= nst) return;
+	const bjob &job = J[sj[e]];
+	double2 *a = &arena[job.aoff + (long)ri[e] * job.N + ci[e]];
+	const double2 x = *a, v = val[e];
+	*a = make_double2(v.x + x.x, v.y + x.y);
+}
+
+__global__ void k_b_reset(unsigned *counter)
+{
+	if ((blockIdx.x == 0) && (threadIdx.x == 0)) *counter = 0u;
+}
+
+// ---------------------------------------------------------------------------------- LU
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+	return make_double2((a.x * b.x) - (a.y * b.y), (a.x * b.y) + (a.y * b.x));
+}
+
+__device__ __forceinline__ double2 crecip(double2 a)
+{
+	const double d = (a.x * a.x) + (a.y * a.y);
+	return make_double2(a.x / d, -a.y / d);
+}
+
+// (value, index) maximum over the workgroup; ties keep the smaller index
+__device__ __forceinline__ void wg_argmax(double &v, int &ix, double *rv, int *ri)
+{
+	for (int off = 32; off > 0; off >>= 1) {
+		const double ov = __shfl_xor(v, off);
+		const int oi = __shfl_xor(ix, off);
+		if ((ov > v) || ((ov == v) && (oi < ix))) {
+			v = ov;
+			ix = oi;
+		}
+	}
+	const int wave = threadIdx.x >> 6;
+	if ((threadIdx.x & 63) == 0) {
+		rv[wave] = v;
+		ri[wave] = ix;
+	}
+	__syncthreads();
+	v = rv[0];
+	ix = ri[0];
+	for (int w = 1; w < 4; w++)
+		if ((rv[w] > v) || ((rv[w] == v) && (ri[w] < ix))) {
+			v = rv[w];
+			ix = ri[w];
+		}
+	__syncthreads();
+}
+
+// C[M x Nc] -= A[M x K] B[K x Nc] (row-major, leading dimension ld, K <= NB): each wave takes
+// 32 x 32 tiles, complex as four real FP64 MFMA 16x16x4 products
+template <int NB>
+__device__ void trailing_update(double2 *A, long ld, int k0, int nb, int m)
+{
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int li = lane & 15, lk = lane >> 4;
+	const int tn = (m + 31) / 32;
+	const double2 *L21 = A + (long)(k0 + nb) * ld + k0;        // m x nb
+	const double2 *U12 = A + (long)k0 * ld + k0 + nb;          // nb x m
+	double2 *C = A + (long)(k0 + nb) * ld + k0 + nb;
+	for (int t = wave; t < tn * tn; t += 4) {
+		const int r0 = (t / tn) * 32, c0 = (t % tn) * 32;
+		d4 ar[2][2], ai[2][2];
+#pragma unroll
+		for (int a = 0; a < 2; a++)
+#pragma unroll
+			for (int b = 0; b < 2; b++) {
+				ar[a][b] = d4{0, 0, 0, 0};
+				ai[a][b] = d4{0, 0, 0, 0};
+			}
+#pragma unroll 1
+		for (int k = 0; k < nb; k += 4) {
+			const int kk = k + lk;
+			double2 fa[2], fb[2];
+#pragma unroll
+			for (int rb = 0; rb < 2; rb++) {
+				const int r = r0 + rb * 16 + li;
+				fa[rb] = ((r < m) && (kk < nb)) ? L21[(long)r * ld + kk] : make_double2(0, 0);
+			}
+#pragma unroll
+			for (int cb = 0; cb < 2; cb++) {
+				const int c = c0 + cb * 16 + li;
+				fb[cb] = ((c < m) && (kk < nb)) ? U12[(long)kk * ld + c] : make_double2(0, 0);
+			}
+#pragma unroll
+			for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+				for (int cb = 0; cb < 2; cb++) ar[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[rb].x, fb[cb].x, ar[rb][cb], 0, 0, 0);
+#pragma unroll
+			for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+				for (int cb = 0; cb < 2; cb++) ai[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[rb].x, fb[cb].y, ai[rb][cb], 0, 0, 0);
+#pragma unroll
+			for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+				for (int cb = 0; cb < 2; cb++) ar[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(-fa[rb].y, fb[cb].y, ar[rb][cb], 0, 0, 0);
+#pragma unroll
+			for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+				for (int cb = 0; cb < 2; cb++) ai[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[rb].y, fb[cb].x, ai[rb][cb], 0, 0, 0);
+		}
+#pragma unroll
+		for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+			for (int cb = 0; cb < 2; cb++) {
+				const int c = c0 + cb * 16 + li;
+				if (c >= m) continue;
+#pragma unroll
+				for (int v = 0; v < 4; v++) {
+					const int r = r0 + rb * 16 + lk + 4 * v;
+					if (r >= m) continue;
+					double2 x = C[(long)r * ld + c];
+					x.x -= ar[rb][cb][v];
+					x.y -= ai[rb][cb][v];
+					C[(long)r * ld + c] = x;
+				}
+			}
+	}
+}
+
+// persistent: factor and solve jobs from the queue until it is empty.  Dynamic LDS holds the
+// right-hand side (N complex) and the pivot rows (N ints) of the current job.
+// in-place LU of the N x N row-major system A: right-looking, panel width NB, row pivoting by
+// the first maximum of |re| + |im| (lu.c:19-30), full-row interchanges recorded in piv (any
+// address space); returns -1, or the column whose pivot is below 1e-30.  All 256 threads call.
+template <int NB>
+__device__ int blu_factor(double2 *A, int N, int *piv, double2 *s_row, double2 *s_L, double *s_rv, int *s_ri)
+{
+	const int tid = threadIdx.x;
+	int info = -1;
+	for (int k0 = 0; (k0 < N) && (info < 0); k0 += NB) {
+		const int nb = (N - k0 < NB) ? (N - k0) : NB;
+		const int kend = k0 + nb;
+		for (int c = k0; c < kend; c++) {
+			double best = -1;
+			int bi = N;
+			for (int r = c + tid; r < N; r += 256) {
+				const double2 v = A[(long)r * N + c];
+				const double a = fabs(v.x) + fabs(v.y);
+				if (a > best) {
+					best = a;
+					bi = r;
+				}
+			}
+			wg_argmax(best, bi, s_rv, s_ri);
+			if (best < 1e-30) {
+				info = c;
+				break;
+			}
+			if (tid == 0) piv[c] = bi;
+			if (bi != c)
+				for (int col = tid; col < N; col += 256) {
+					const double2 t = A[(long)c * N + col];
+					A[(long)c * N + col] = A[(long)bi * N + col];
+					A[(long)bi * N + col] = t;
+				}
+			__syncthreads();
+			const double2 inv = crecip(A[(long)c * N + c]);
+			for (int r = c + 1 + tid; r < N; r += 256) A[(long)r * N + c] = cmul(A[(long)r * N + c], inv);
+			const int w = kend - c - 1;
+			if (tid < w) s_row[tid] = A[(long)c * N + c + 1 + tid];
+			__syncthreads();
+			if (w > 0) {
+				// rank-1 update of the panel's remaining columns, every row below c
+				for (int r = c + 1 + tid; r < N; r += 256) {
+					double2 *ar = A + (long)r * N;
+					const double2 l = ar[c];
+					for (int t = 0; t < w; t++) {
+						const double2 u = s_row[t];
+						double2 x = ar[c + 1 + t];
+						x.x -= (l.x * u.x) - (l.y * u.y);
+						x.y -= (l.x * u.y) + (l.y * u.x);
+						ar[c + 1 + t] = x;
+					}
+				}
+			}
+			__syncthreads();
+		}
+		if (info >= 0) break;
+		const int m = N - kend;
+		if (m <= 0) break;
+		// U12 = L11^-1 A12 once the panel's interchanges are all applied: every thread owns
+		// columns, eliminating down the panel rows (coalesced across threads, row by row)
+		for (int e = tid; e < nb * nb; e += 256) {
+			const int r = e / nb, cc = e - r * nb;
+			s_L[r * NB + cc] = A[(long)(k0 + r) * N + k0 + cc];
+		}
+		__syncthreads();
+		for (int col = kend + tid; col < N; col += 256)
+			for (int c = 0; c < nb - 1; c++) {
+				const double2 u = A[(long)(k0 + c) * N + col];
+				for (int r = c + 1; r < nb; r++) {
+					const double2 l = s_L[r * NB + c];
+					double2 x = A[(long)(k0 + r) * N + col];
+					x.x -= (l.x * u.x) - (l.y * u.y);
+					x.y -= (l.x * u.y) + (l.y * u.x);
+					A[(long)(k0 + r) * N + col] = x;
+				}
+			}
+		__syncthreads();
+		trailing_update<NB>(A, N, k0, nb, m);
+		__syncthreads();
+	}
+	return info;
+}
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_b_lu(const bdesc *D, const bjob *J, const int *order, int nj, unsigned *counter,
+                                              double2 *arena, double2 *zout, int *info_out)
+{
+	extern __shared__ double2 s_dyn[];
+	__shared__ double2 s_row[NB];
+	__shared__ double2 s_L[NB * NB];
+	__shared__ double s_rv[4];
+	__shared__ int s_ri[4];
+	__shared__ int s_job;
+	const int tid = threadIdx.x;
+	for (;;) {
+		if (tid == 0) s_job = (int)atomicAdd(counter, 1u);
+		__syncthreads();
+		const int q = s_job;
+		__syncthreads();
+		if (q >= nj) break;
+		const int jid = order[q];
+		const bjob job = J[jid];
+		const bdesc &d = D[job.s];
+		const int N = job.N;
+		double2 *A = arena + job.aoff;
+		double2 *b = s_dyn;
+		int *piv = (int *)(s_dyn + N);
+		const int info = blu_factor<NB>(A, N, piv, s_row, s_L, s_rv, s_ri);
+		if (info >= 0) {
+			if (tid == 0) info_out[jid] = info;
+			__syncthreads();
+			continue;
+		}
+		// every port: b = e_i1 - e_i2, row interchanges, unit-lower then upper substitution
+		for (int pj = 0; pj < d.np; pj++) {
+			for (int r = tid; r < N; r += 256) b[r] = make_double2(0, 0);
+			__syncthreads();
+			if (tid == 0) {
+				if (d.i1[pj] >= 0) b[d.i1[pj]].x += 1;
+				if (d.i2[pj] >= 0) b[d.i2[pj]].x -= 1;
+				for (int c = 0; c < N; c++) {
+					const int pr = piv[c];
+					if (pr != c) {
+						const double2 t = b[c];
+						b[c] = b[pr];
+						b[pr] = t;
+					}
+				}
+			}
+			__syncthreads();
+			for (int c = 0; c < N - 1; c++) {
+				const double2 xc = b[c];
+				if ((xc.x != 0) || (xc.y != 0))
+					for (int r = c + 1 + tid; r < N; r += 256) {
+						const double2 l = A[(long)r * N + c];
+						b[r].x -= (l.x * xc.x) - (l.y * xc.y);
+						b[r].y -= (l.x * xc.y) + (l.y * xc.x);
+					}
+				__syncthreads();
+			}
+			for (int c = N - 1; c >= 0; c--) {
+				if (tid == 0) b[c] = cmul(b[c], crecip(A[(long)c * N + c]));
+				__syncthreads();
+				const double2 xc = b[c];
+				for (int r = tid; r < c; r += 256) {
+					const double2 u = A[(long)r * N + c];
+					b[r].x -= (u.x * xc.x) - (u.y * xc.y);
+					b[r].y -= (u.x * xc.y) + (u.y * xc.x);
+				}
+				__syncthreads();
+			}
+			if (tid < d.np) {
+				const double2 v1 = (d.i1[tid] >= 0) ? b[d.i1[tid]] : make_double2(0, 0);
+				const double2 v2 = (d.i2[tid] >= 0) ? b[d.i2[tid]] : make_double2(0, 0);
+				zout[job.zoff + (long)tid * d.np + pj] = make_double2(v1.x - v2.x, v1.y - v2.y);
+			}
+			__syncthreads();
+		}
+		if (tid == 0) info_out[jid] = -1;
+		__syncthreads();
+	}
+}
+
+// one matrix per workgroup through blu_factor (the batched factorisation on its own)
+template <int NB>
+__global__ __launch_bounds__(256) void k_b_lu_only(double2 *A, const long *aoff, const int *n, int *piv, const long *poff,
+                                                   int *info)
+{
+	__shared__ double2 s_row[NB];
+	__shared__ double2 s_L[NB * NB];
+	__shared__ double s_rv[4];
+	__shared__ int s_ri[4];
+	const int j = blockIdx.x;
+	const int r = blu_factor<NB>(A + aoff[j], n[j], piv + poff[j], s_row, s_L, s_rv, s_ri);
+	if (threadIdx.x == 0) info[j] = r;
+}
+
+// ---------------------------------------------------------------------------------- host
+
+template <typename T>
+struct dbuf {                  // device buffer that grows; true when it moved
+	T *p = nullptr;
+	long cap = 0;
+	bool ensure(long n)
+	{
+		if (n <= cap) return false;
+		if (p) HIPCHK(hipFree(p));
+		cap = n + n / 4 + 16;
+		HIPCHK(hipMalloc((void **)&p, (size_t)cap * sizeof(T)));
+		return true;
+	}
+	~dbuf()
+	{
+		if (p) HIPCHK(hipFree(p));
+	}
+};
+
+template <typename T>
+struct hbuf {                  // pinned host buffer that grows
+	T *p = nullptr;
+	long cap = 0;
+	void ensure(long n)
+	{
+		if (n <= cap) return;
+		if (p) HIPCHK(hipHostFree(p));
+		cap = n + n / 4 + 16;
+		HIPCHK(hipHostMalloc((void **)&p, (size_t)cap * sizeof(T), hipHostMallocDefault));
+	}
+	~hbuf()
+	{
+		if (p) HIPCHK(hipHostFree(p));
+	}
+};
+
+struct tune_t {
+	int nb = 32;        // LU panel width
+	int wpc = 2;        // persistent workgroups per CU
+	int order = 0;      // 0 largest first, 1 interleaved (structure order)
+	double ms = 0;      // LU time of the sample with this choice
+};
+
+double freq_of(const peec_t *p, int i)
+{
+	if (p->nfreq <= 1) return p->f0;
+	if (p->flog) return p->f0 * pow(p->f1 / p->f0, (double)i / (p->nfreq - 1));
+	return p->f0 + (p->f1 - p->f0) * i / (p->nfreq - 1);
+}
+
+struct chunk {                 // jobs of one pass through the arena
+	std::vector<bjob> J;
+	std::vector<int> ord;
+	long arena_n = 0, tilesS = 0, tilesP = 0, zn = 0, nst = 0;
+	int Nmax = 0;
+};
+
+struct gkey {
+	int nj;
+	long nst, arena_n, tilesS, tilesP;
+	int nb, wpc;
+	bool operator<(const gkey &o) const
+	{
+		return std::tie(nj, nst, arena_n, tilesS, tilesP, nb, wpc) <
+		       std::tie(o.nj, o.nst, o.arena_n, o.tilesS, o.tilesP, o.nb, o.wpc);
+	}
+};
+
+struct batch {
+	int nm = 0;
+	std::vector<const peec_t *> P;
+	std::vector<pgpu_t *> G;
+	std::vector<bdesc> hd;
+	std::vector<pgpu_bview> views;
+	dbuf<bdesc> dd;
+	hipStream_t s = nullptr;
+	int ncu = 0;
+	dbuf<double2> arena, zdev;
+	dbuf<bjob> dj;
+	dbuf<int> dorder, dinfo, dsj, dri, dci;
+	dbuf<unsigned> dcnt;
+	dbuf<double2> dval;
+	hbuf<int> hsj, hri, hci, hinfo;
+	hbuf<double2> hval, hz;
+	std::map<gkey, std::pair<hipGraphExec_t, hipGraphExec_t>> graphs;
+	bool tuned = false;
+	tune_t tune;
+	std::vector<tune_t> trials;
+
+	~batch()
+	{
+		if (s) HIPCHK(hipStreamSynchronize(s));
+		for (auto &kv : graphs) {
+			hipGraphExecDestroy(kv.second.first);
+			hipGraphExecDestroy(kv.second.second);
+		}
+		if (s) HIPCHK(hipStreamDestroy(s));
+	}
+};
+
+size_t lu_lds(int Nmax)
+{
+	return (size_t)Nmax * sizeof(double2) + (size_t)Nmax * sizeof(int) + 16;
+}
+
+void launch_lu(hipStream_t s, batch &B, int nb, int wpc, int nj, size_t lds)
+{
+	const int grid = std::max(1, std::min(nj, B.ncu * wpc));
+	if (nb == 16)
+		hipLaunchKernelGGL((k_b_lu<16>), dim3(grid), dim3(256), lds, s, B.dd.p, B.dj.p, B.dorder.p, nj, B.dcnt.p,
+		                   B.arena.p, B.zdev.p, B.dinfo.p);
+	else
+		hipLaunchKernelGGL((k_b_lu<32>), dim3(grid), dim3(256), lds, s, B.dd.p, B.dj.p, B.dorder.p, nj, B.dcnt.p,
+		                   B.arena.p, B.zdev.p, B.dinfo.p);
+}
+
+void record_fill(hipStream_t s, batch &B, const chunk &c)
+{
+	const int nj = (int)c.J.size();
+	hipLaunchKernelGGL(k_b_zero, dim3(4096), dim3(256), 0, s, B.arena.p, c.arena_n);
+	if (c.tilesS > 0) hipLaunchKernelGGL(k_b_lp, dim3((unsigned)c.tilesS), dim3(256), 0, s, B.dd.p, B.dj.p, nj, B.arena.p);
+	if (c.tilesP > 0)
+		hipLaunchKernelGGL(k_b_p, dim3((unsigned)c.tilesP), dim3(256), 0, s, B.dd.p, B.dj.p, nj, 4 * PI * EPS0, B.arena.p);
+}
+
+void record_solve(hipStream_t s, batch &B, const chunk &c, int nb, int wpc)
+{
+	const int nj = (int)c.J.size();
+	if (c.nst > 0)
+		hipLaunchKernelGGL(k_b_stamp, dim3((unsigned)((c.nst + 255) / 256)), dim3(256), 0, s, B.dj.p, B.dsj.p, B.dri.p,
+		                   B.dci.p, B.dval.p, c.nst, B.arena.p);
+	hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, s, B.dcnt.p);
+	launch_lu(s, B, nb, wpc, nj, lu_lds(c.Nmax));
+}
+
+template <typename F>
+hipGraphExec_t capture(hipStream_t s, F body)
+{
+	hipGraph_t g = nullptr;
+	HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+	body(s);
+	HIPCHK(hipStreamEndCapture(s, &g));
+	hipGraphExec_t ex = nullptr;
+	HIPCHK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+	HIPCHK(hipGraphDestroy(g));
+	return ex;
+}
+
+// job table of a list of (structure, frequency index) items
+void build_chunk(const batch &B, const std::vector<std::pair<int, int>> &items, int order, chunk &c)
+{
+	c.J.resize(items.size());
+	c.arena_n = c.tilesS = c.tilesP = c.zn = c.nst = 0;
+	c.Nmax = 0;
+	for (size_t k = 0; k < items.size(); k++) {
+		const int s = items[k].first, fi = items[k].second;
+		const bdesc &d = B.hd[s];
+		const double f = freq_of(B.P[s], fi);
+		bjob &j = c.J[k];
+		j.s = s;
+		j.N = d.N;
+		j.f = f;
+		j.omega = 2 * PI * f;
+		j.kw = B.P[s]->retardation ? 2 * PI * f / C0 : 0.0;
+		j.aoff = c.arena_n;
+		j.zoff = c.zn;
+		j.tS = c.tilesS;
+		j.tP = c.tilesP;
+		c.arena_n += (long)d.N * d.N;
+		c.zn += (long)d.np * d.np;
+		c.tilesS += (long)d.ntS * (d.ntS + 1) / 2;
+		const long ntc = (d.ncell + 15) / 16;
+		c.tilesP += ntc * ntc;
+		c.Nmax = std::max(c.Nmax, d.N);
+	}
+	c.ord.resize(items.size());
+	for (size_t k = 0; k < items.size(); k++) c.ord[k] = (int)k;
+	if (order == 0) std::stable_sort(c.ord.begin(), c.ord.end(), [&](int a, int b) { return c.J[a].N > c.J[b].N; });
+}
+
+// stamps of every job on host threads into the pinned staging arrays (job order), then the
+// uploads of the job table, the order and the stamps on the batch stream
+void stage_chunk(batch &B, chunk &c)
+{
+	const int nj = (int)c.J.size();
+	std::vector<stamps_t> st(nj);
+	unsigned nth = std::thread::hardware_concurrency();
+	nth = std::max(1u, std::min(nth, 16u));
+	std::vector<std::thread> th;
+	for (unsigned t = 0; t < nth; t++)
+		th.emplace_back([&, t]() {
+			for (int k = (int)t; k < nj; k += (int)nth) mna_stamps_charge(B.P[c.J[k].s], c.J[k].f, &st[k]);
+		});
+	for (auto &x : th) x.join();
+	long tot = 0;
+	for (int k = 0; k < nj; k++) tot += st[k].n;
+	c.nst = tot;
+	// the previous chunk's uploads must have been consumed before the staging is overwritten
+	HIPCHK(hipStreamSynchronize(B.s));
+	B.hsj.ensure(tot);
+	B.hri.ensure(tot);
+	B.hci.ensure(tot);
+	B.hval.ensure(tot);
+	long e = 0;
+	for (int k = 0; k < nj; k++) {
+		for (int q = 0; q < st[k].n; q++) {
+			B.hsj.p[e] = k;
+			B.hri.p[e] = st[k].ri[q];
+			B.hci.p[e] = st[k].ci[q];
+			B.hval.p[e] = make_double2(st[k].val[q].r, st[k].val[q].i);
+			e++;
+		}
+		mna_stamps_free(&st[k]);
+	}
+	bool moved = false;
+	moved |= B.dj.ensure(nj);
+	moved |= B.dorder.ensure(nj);
+	moved |= B.dinfo.ensure(nj);
+	moved |= B.dsj.ensure(tot);
+	moved |= B.dri.ensure(tot);
+	moved |= B.dci.ensure(tot);
+	moved |= B.dval.ensure(tot);
+	moved |= B.arena.ensure(c.arena_n);
+	moved |= B.zdev.ensure(c.zn);
+	if (moved) {
+		// captured graphs hold the old addresses
+		for (auto &kv : B.graphs) {
+			HIPCHK(hipGraphExecDestroy(kv.second.first));
+			HIPCHK(hipGraphExecDestroy(kv.second.second));
+		}
+		B.graphs.clear();
+	}
+	HIPCHK(hipMemcpyAsync(B.dj.p, c.J.data(), (size_t)nj * sizeof(bjob), hipMemcpyHostToDevice, B.s));
+	HIPCHK(hipMemcpyAsync(B.dorder.p, c.ord.data(), (size_t)nj * sizeof(int), hipMemcpyHostToDevice, B.s));
+	if (tot > 0) {
+		HIPCHK(hipMemcpyAsync(B.dsj.p, B.hsj.p, (size_t)tot * sizeof(int), hipMemcpyHostToDevice, B.s));
+		HIPCHK(hipMemcpyAsync(B.dri.p, B.hri.p, (size_t)tot * sizeof(int), hipMemcpyHostToDevice, B.s));
+		HIPCHK(hipMemcpyAsync(B.dci.p, B.hci.p, (size_t)tot * sizeof(int), hipMemcpyHostToDevice, B.s));
+		HIPCHK(hipMemcpyAsync(B.dval.p, B.hval.p, (size_t)tot * sizeof(double2), hipMemcpyHostToDevice, B.s));
+	}
+}
+
+// structures filled element by element: their blocks of every job, between the two graphs
+void direct_blocks(batch &B, const chunk &c)
+{
+	bool any = false;
+	for (const bdesc &d : B.hd) any = any || d.direct;
+	if (!any) return;
+	HIPCHK(hipStreamSynchronize(B.s));
+	for (const bjob &j : c.J) {
+		const bdesc &d = B.hd[j.s];
+		if (!d.direct) continue;
+		const void *lp = nullptr, *Pm = nullptr;
+		pgpu_fill_direct_dev(B.G[j.s], j.kw, &lp, &Pm);
+		const long tot = (long)d.nseg * d.nseg + (Pm ? (long)d.ncell * d.ncell : 0);
+		hipLaunchKernelGGL(k_b_scatter, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, B.s, B.arena.p + j.aoff, d.N,
+		                   d.offS, d.nseg, (const double2 *)lp, j.omega, d.offQ, Pm ? d.ncell : 0, (const double2 *)Pm);
+		HIPCHK(hipStreamSynchronize(B.s));   // the mirror's buffers are reused by the next job
+	}
+}
+
+std::pair<hipGraphExec_t, hipGraphExec_t> graphs_for(batch &B, const chunk &c, int nb, int wpc)
+{
+	const gkey k{(int)c.J.size(), c.nst, c.arena_n, c.tilesS, c.tilesP, nb, wpc};
+	auto it = B.graphs.find(k);
+	if (it != B.graphs.end()) return it->second;
+	hipGraphExec_t gf = capture(B.s, [&](hipStream_t s) { record_fill(s, B, c); });
+	hipGraphExec_t gs = capture(B.s, [&](hipStream_t s) { record_solve(s, B, c, nb, wpc); });
+	B.graphs[k] = {gf, gs};
+	return {gf, gs};
+}
+
+// runs one staged chunk; returns the LU graph's time (ms) when timed
+double run_chunk(batch &B, const chunk &c, int nb, int wpc, double *fill_ms)
+{
+	auto gg = graphs_for(B, c, nb, wpc);
+	hipEvent_t e0, e1, e2;
+	HIPCHK(hipEventCreate(&e0));
+	HIPCHK(hipEventCreate(&e1));
+	HIPCHK(hipEventCreate(&e2));
+	HIPCHK(hipEventRecord(e0, B.s));
+	HIPCHK(hipGraphLaunch(gg.first, B.s));
+	direct_blocks(B, c);
+	HIPCHK(hipEventRecord(e1, B.s));
+	HIPCHK(hipGraphLaunch(gg.second, B.s));
+	HIPCHK(hipEventRecord(e2, B.s));
+	HIPCHK(hipEventSynchronize(e2));
+	float a = 0, b = 0;
+	HIPCHK(hipEventElapsedTime(&a, e0, e1));
+	HIPCHK(hipEventElapsedTime(&b, e1, e2));
+	HIPCHK(hipEventDestroy(e0));
+	HIPCHK(hipEventDestroy(e1));
+	HIPCHK(hipEventDestroy(e2));
+	if (fill_ms) *fill_ms = a;
+	return b;
+}
+
+int prepare(batch &B, peec_t **models, int nm, int ifreq0, int nfreq)
+{
+	int dev = 0;
+	HIPCHK(hipGetDevice(&dev));
+	HIPCHK(hipDeviceGetAttribute(&B.ncu, hipDeviceAttributeMultiprocessorCount, dev));
+	HIPCHK(hipStreamCreateWithFlags(&B.s, hipStreamNonBlocking));
+	HIPCHK(hipFuncSetAttribute((const void *)k_b_lu<16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lu_lds(BMAXN)));
+	HIPCHK(hipFuncSetAttribute((const void *)k_b_lu<32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lu_lds(BMAXN)));
+	B.nm = nm;
+	B.P.resize(nm);
+	B.G.resize(nm);
+	B.hd.resize(nm);
+	B.views.resize(nm);
+	for (int s = 0; s < nm; s++) {
+		const peec_t *p = models[s];
+		B.P[s] = p;
+		if ((p->compress != 0) || (p->nport <= 0) || (p->nport > BMAXPORT) || (ifreq0 < 0) || (ifreq0 + nfreq > p->nfreq)) {
+			printf("*** batched sweep: structure %d is not a dense deck with 1..%d ports and frequencies %d..%d\n", s,
+			       BMAXPORT, ifreq0, ifreq0 + nfreq - 1);
+			return 4;
+		}
+		const int nc = (p->capacitance && (p->ncell > 0)) ? p->ncell : 0;
+		const int N = p->nunknown + nc;
+		if (N > BMAXN) {
+			printf("*** batched sweep: structure %d has %d unknowns (at most %d per batched system)\n", s, N, BMAXN);
+			return 4;
+		}
+		B.G[s] = pgpu_get(p);
+	}
+	// k-factored records of every structure over its band, setups running concurrently
+	std::vector<int> ok(nm, 0);
+	for (int s = 0; s < nm; s++) {
+		const peec_t *p = B.P[s];
+		const double kr = p->retardation ? 2 * PI / C0 : 0.0;
+		const double klo = kr * freq_of(p, ifreq0), khi = kr * freq_of(p, ifreq0 + nfreq - 1);
+		ok[s] = (pgpu_kf_setup_launch(B.G[s], std::min(klo, khi), std::max(klo, khi)) == 0);
+	}
+	for (int s = 0; s < nm; s++)
+		if (ok[s]) pgpu_kf_setup_finish(B.G[s]);
+	for (int s = 0; s < nm; s++) {
+		const peec_t *p = B.P[s];
+		pgpu_bview &v = B.views[s];
+		pgpu_batch_view(B.G[s], &v);
+		bdesc &d = B.hd[s];
+		memset(&d, 0, sizeof(d));
+		d.ncell = (p->capacitance && (p->ncell > 0)) ? p->ncell : 0;
+		d.N = p->nunknown + d.ncell;
+		d.offS = p->offS;
+		d.nseg = p->nseg;
+		d.offQ = p->nunknown;
+		d.T = v.T;
+		d.ntS = (p->nseg + 15) / 16;
+		d.ntH = (v.nh + 15) / 16;
+		d.nslS = kf_slots(p->nseg);
+		d.nslH = (v.nh > 0) ? kf_slots(v.nh) : 0;
+		d.rec_lp = v.rec_lp;
+		d.rec_ig = v.rec_ig;
+		d.csoff = v.csoff;
+		d.csidx = v.csidx;
+		d.carea = v.carea;
+		d.np = p->nport;
+		for (int k = 0; k < p->nport; k++) {
+			d.i1[k] = p->nodemap[p->port[k].n1];
+			d.i2[k] = p->nodemap[p->port[k].n2];
+		}
+		d.direct = (!ok[s]) || (v.nd_lp > 0) || (d.ncell > 0 && v.nd_ig > 0) || (v.rec_lp == nullptr && p->nseg > 0) ||
+		           (d.ncell > 0 && v.rec_ig == nullptr);
+	}
+	B.dd.ensure(nm);
+	HIPCHK(hipMemcpy(B.dd.p, B.hd.data(), (size_t)nm * sizeof(bdesc), hipMemcpyHostToDevice));
+	B.dcnt.ensure(1);
+	return 0;
+}
+
+// HBM budget of one chunk's systems (bytes)
+double arena_budget()
+{
+	const char *e = getenv("PEEC_BATCH_GB");
+	const double gb = e ? atof(e) : 24.0;
+	return ((gb > 0.01) ? gb : 24.0) * 1e9;
+}
+
+// chunks of whole frequency indices (all structures at each), within the HBM budget
+std::vector<std::vector<std::pair<int, int>>> make_chunks(const batch &B, int ifreq0, int nfreq)
+{
+	double per_f = 0;
+	for (const bdesc &d : B.hd) per_f += 16.0 * d.N * (double)d.N;
+	int fc = (int)std::max(1.0, std::floor(arena_budget() / std::max(per_f, 1.0)));
+	fc = std::min(fc, nfreq);
+	// even chunks so every chunk but possibly none differ in shape
+	const int nch = (nfreq + fc - 1) / fc;
+	fc = (nfreq + nch - 1) / nch;
+	std::vector<std::vector<std::pair<int, int>>> out;
+	for (int f0 = 0; f0 < nfreq; f0 += fc) {
+		std::vector<std::pair<int, int>> items;
+		for (int f = f0; f < std::min(nfreq, f0 + fc); f++)
+			for (int s = 0; s < B.nm; s++) items.push_back({s, ifreq0 + f});
+		out.push_back(items);
+	}
+	return out;
+}
+
+// timing autotuner on a sample of the first chunk: every candidate (panel width, workgroups per
+// CU, job order) re-assembles the sample and factors it; the fastest LU wins
+void autotune(batch &B, const std::vector<std::pair<int, int>> &first)
+{
+	std::vector<std::pair<int, int>> sample;
+	const size_t stride = std::max<size_t>(1, first.size() / 512);
+	for (size_t k = 0; k < first.size(); k += stride) sample.push_back(first[k]);
+	const int nbs[2] = {16, 32};
+	const int wpcs[3] = {1, 2, 4};
+	B.trials.clear();
+	tune_t best;
+	best.ms = 1e300;
+	for (int order = 0; order < 2; order++) {
+		chunk c;
+		build_chunk(B, sample, order, c);
+		stage_chunk(B, c);
+		for (int nb : nbs)
+			for (int wpc : wpcs) {
+				const double ms = run_chunk(B, c, nb, wpc, nullptr);
+				tune_t t;
+				t.nb = nb;
+				t.wpc = wpc;
+				t.order = order;
+				t.ms = ms;
+				B.trials.push_back(t);
+				if (ms < best.ms) best = t;
+			}
+	}
+	B.tune = best;
+	B.tuned = true;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+extern "C" {
+
+struct peec_batch {
+	batch b;
+	int ifreq0 = 0, nfreq = 0;
+};
+
+peec_batch_t *peec_batch_create(peec_t **models, int nmodel, int ifreq0, int nfreq)
+{
+	if ((models == nullptr) || (nmodel <= 0) || (nfreq <= 0)) return nullptr;
+	peec_batch_t *h = new peec_batch();
+	h->ifreq0 = ifreq0;
+	h->nfreq = nfreq;
+	const auto t0 = std::chrono::steady_clock::now();
+	if (prepare(h->b, models, nmodel, ifreq0, nfreq) != 0) {
+		delete h;
+		return nullptr;
+	}
+	(void)t0;
+	return h;
+}
+
+void peec_batch_free(peec_batch_t *h)
+{
+	delete h;
+}
+
+int peec_batch_tune(peec_batch_t *h, peec_batch_stats_t *st)
+{
+	if (h == nullptr) return 4;
+	const auto t0 = std::chrono::steady_clock::now();
+	auto chunks = make_chunks(h->b, h->ifreq0, h->nfreq);
+	autotune(h->b, chunks[0]);
+	if (st) {
+		st->tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+		st->nb = h->b.tune.nb;
+		st->wg_per_cu = h->b.tune.wpc;
+		st->order = h->b.tune.order;
+		st->tune_trials = (int)h->b.trials.size();
+	}
+	return 0;
+}
+
+int peec_batch_sweep(peec_batch_t *h, d_complex_t **zmat, peec_batch_stats_t *st)
+{
+	if (h == nullptr) return 4;
+	batch &B = h->b;
+	if (st) memset(st, 0, sizeof(*st));
+	const auto t0 = std::chrono::steady_clock::now();
+	auto chunks = make_chunks(B, h->ifreq0, h->nfreq);
+	if (!B.tuned) autotune(B, chunks[0]);
+	const tune_t tn = B.tune;
+	int rc = 0;
+	for (size_t ci = 0; ci < chunks.size() && rc == 0; ci++) {
+		chunk c;
+		build_chunk(B, chunks[ci], tn.order, c);
+		const auto th = std::chrono::steady_clock::now();
+		stage_chunk(B, c);
+		if (st) st->stamp_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th).count();
+		double fms = 0;
+		const double lms = run_chunk(B, c, tn.nb, tn.wpc, &fms);
+		const int nj = (int)c.J.size();
+		B.hz.ensure(c.zn);
+		B.hinfo.ensure(nj);
+		HIPCHK(hipMemcpyAsync(B.hz.p, B.zdev.p, (size_t)c.zn * sizeof(double2), hipMemcpyDeviceToHost, B.s));
+		HIPCHK(hipMemcpyAsync(B.hinfo.p, B.dinfo.p, (size_t)nj * sizeof(int), hipMemcpyDeviceToHost, B.s));
+		HIPCHK(hipStreamSynchronize(B.s));
+		for (int k = 0; k < nj; k++) {
+			const bjob &j = c.J[k];
+			const int fi = chunks[ci][k].second - h->ifreq0;
+			if (B.hinfo.p[k] >= 0) {
+				printf("*** batched sweep: singular system (structure %d, f = %.5e Hz, row %d)\n", j.s, j.f,
+				       B.hinfo.p[k]);
+				rc = 2;
+				continue;
+			}
+			const int np = B.hd[j.s].np;
+			d_complex_t *z = zmat[j.s] + (size_t)fi * np * np;
+			for (int e = 0; e < np * np; e++) z[e] = d_complex(B.hz.p[j.zoff + e].x, B.hz.p[j.zoff + e].y);
+		}
+		if (st) {
+			st->fill_ms += fms;
+			st->lu_ms += lms;
+			st->chunks++;
+			st->jobs += nj;
+			for (const bjob &j : c.J) {
+				const double N = j.N;
+				st->lu_flop += (8.0 / 3.0) * N * N * N + 8.0 * N * N * B.hd[j.s].np;
+			}
+		}
+	}
+	if (st) {
+		st->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+		st->nb = tn.nb;
+		st->wg_per_cu = tn.wpc;
+		st->order = tn.order;
+		st->tune_trials = (int)B.trials.size();
+		for (const bdesc &d : B.hd) st->direct_structures += d.direct;
+	}
+	return rc;
+}
+
+int peec_batch_lu_decomp(int nmat, const int *n, d_complex_t *a, int *piv, int nb)
+{
+	if ((nmat <= 0) || ((nb != 16) && (nb != 32))) return -2;
+	std::vector<long> aoff(nmat), poff(nmat);
+	long na = 0, np = 0;
+	for (int j = 0; j < nmat; j++) {
+		if ((n[j] <= 0) || (n[j] > BMAXN)) return -2;
+		aoff[j] = na;
+		poff[j] = np;
+		na += (long)n[j] * n[j];
+		np += n[j];
+	}
+	double2 *dA = nullptr;
+	long *daoff = nullptr, *dpoff = nullptr;
+	int *dn = nullptr, *dpiv = nullptr, *dinfo = nullptr;
+	HIPCHK(hipMalloc((void **)&dA, (size_t)na * sizeof(double2)));
+	HIPCHK(hipMalloc((void **)&daoff, (size_t)nmat * sizeof(long)));
+	HIPCHK(hipMalloc((void **)&dpoff, (size_t)nmat * sizeof(long)));
+	HIPCHK(hipMalloc((void **)&dn, (size_t)nmat * sizeof(int)));
+	HIPCHK(hipMalloc((void **)&dpiv, (size_t)np * sizeof(int)));
+	HIPCHK(hipMalloc((void **)&dinfo, (size_t)nmat * sizeof(int)));
+	HIPCHK(hipMemcpy(dA, a, (size_t)na * sizeof(double2), hipMemcpyHostToDevice));
+	HIPCHK(hipMemcpy(daoff, aoff.data(), (size_t)nmat * sizeof(long), hipMemcpyHostToDevice));
+	HIPCHK(hipMemcpy(dpoff, poff.data(), (size_t)nmat * sizeof(long), hipMemcpyHostToDevice));
+	HIPCHK(hipMemcpy(dn, n, (size_t)nmat * sizeof(int), hipMemcpyHostToDevice));
+	if (nb == 16) hipLaunchKernelGGL((k_b_lu_only<16>), dim3(nmat), dim3(256), 0, 0, dA, daoff, dn, dpiv, dpoff, dinfo);
+	else hipLaunchKernelGGL((k_b_lu_only<32>), dim3(nmat), dim3(256), 0, 0, dA, daoff, dn, dpiv, dpoff, dinfo);
+	HIPCHK(hipGetLastError());
+	std::vector<int> info(nmat);
+	HIPCHK(hipMemcpy(a, dA, (size_t)na * sizeof(double2), hipMemcpyDeviceToHost));
+	HIPCHK(hipMemcpy(piv, dpiv, (size_t)np * sizeof(int), hipMemcpyDeviceToHost));
+	HIPCHK(hipMemcpy(info.data(), dinfo, (size_t)nmat * sizeof(int), hipMemcpyDeviceToHost));
+	HIPCHK(hipFree(dA));
+	HIPCHK(hipFree(daoff));
+	HIPCHK(hipFree(dpoff));
+	HIPCHK(hipFree(dn));
+	HIPCHK(hipFree(dpiv));
+	HIPCHK(hipFree(dinfo));
+	for (int j = 0; j < nmat; j++)
+		if (info[j] >= 0) return j;
+	return -1;
+}
+
+int peec_batch_trials(const peec_batch_t *h, int *nb, int *wpc, int *order, double *ms, int cap)
+{
+	if (h == nullptr) return 0;
+	const int n = (int)h->b.trials.size();
+	for (int k = 0; k < n && k < cap; k++) {
+		nb[k] = h->b.trials[k].nb;
+		wpc[k] = h->b.trials[k].wpc;
+		order[k] = h->b.trials[k].order;
+		ms[k] = h->b.trials[k].ms;
+	}
+	return n;
+}
+
+}  // extern "C"

@@ -48,3 +48,24 @@ void kf_eval_launch(hipStream_t s, int which, const pe::fill_args &f, const doub
 // charge cells (carea set, sub-cell CSR returned)
 pe::fill_args pgpu_set_args(pgpu_t *g, int which, double kw, int *km, const int **csoff, const int **csidx);
 hipStream_t pgpu_hip_stream(pgpu_t *g);
+
+// one structure's device mirror as the batched sweep sees it (batch.hip)
+struct pgpu_bview {
+	hipStream_t s;
+	pe::fill_args seg, chg;     // cell sets (kw, out unset)
+	int km_seg, km_chg;
+	int nseg, nh, ncell;        // current cells, charge sub-cells, capacitive cells (0: none)
+	bool ident;                 // one sub-cell per capacitive cell
+	const double *rec_lp, *rec_ig;
+	int T;                      // record terms per pair
+	int nd_lp, nd_ig;           // pairs evaluated directly at every frequency
+	const int2 *list_lp, *list_ig;
+	const int *csoff, *csidx;
+	const double *carea;
+};
+extern "C" {
+int pgpu_kf_setup_launch(pgpu_t *g, double klo, double khi);
+void pgpu_kf_setup_finish(pgpu_t *g);
+void pgpu_batch_view(pgpu_t *g, pgpu_bview *v);
+void pgpu_fill_direct_dev(pgpu_t *g, double kw, const void **lp, const void **P);
+}

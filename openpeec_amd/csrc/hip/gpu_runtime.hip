@@ -55,7 +55,7 @@ struct pgpu {
 	int st_cap = 0;
 	double2 *qbuf = nullptr;     // ncell x nport cell charges
 	// k-factored fill over a wavenumber band (kfill.hip)
-	bool kf_ready = false;
+	bool kf_ready = false, kf_pending = false;
 	double kf_lo = 0, kf_hi = 0;
 	double *kf_rec_lp = nullptr, *kf_rec_ig = nullptr;
 	int kf_T = 0;
@@ -482,7 +482,9 @@ int pgpu_fill_pot(pgpu_t *g, double kw)
 // Prepare the k-factored fill for wavenumbers in [klo, khi] (reused while a later band lies
 // inside it).  Returns 0 when ready, 1 when the records would not fit the memory budget (the
 // caller then fills directly).
-int pgpu_kf_setup(pgpu_t *g, double klo, double khi)
+// launches the record setup on the mirror's stream; pgpu_kf_setup_finish reads the direct-pair
+// counts (so that many structures' setups can run concurrently)
+int pgpu_kf_setup_launch(pgpu_t *g, double klo, double khi)
 {
 	const peec_t *p = g->p;
 	if (g->kf_ready && (klo >= g->kf_lo) && (khi <= g->kf_hi)) return 0;
@@ -517,15 +519,51 @@ int pgpu_kf_setup(pgpu_t *g, double klo, double khi)
 		pe::fill_args a = base_args(g->chg, nh, khi, p->gp, nullptr, nh);
 		kf_setup_launch(g->s, 1, g->chg.km, a, klo, khi, g->kf_rho_chg, g->kf_rec_ig, T, g->kf_list_ig, g->kf_cnt + 1);
 	}
+	g->kf_lo = klo;
+	g->kf_hi = khi;
+	g->kf_pending = true;
+	return 0;
+}
+
+void pgpu_kf_setup_finish(pgpu_t *g)
+{
+	if (!g->kf_pending) return;
 	unsigned cnt[2] = {0, 0};
 	HIPCHK(hipMemcpyAsync(cnt, g->kf_cnt, sizeof(cnt), hipMemcpyDeviceToHost, g->s));
 	HIPCHK(hipStreamSynchronize(g->s));
 	g->kf_nd_lp = (int)cnt[0];
 	g->kf_nd_ig = (int)cnt[1];
-	g->kf_lo = klo;
-	g->kf_hi = khi;
+	g->kf_pending = false;
 	g->kf_ready = true;
+}
+
+int pgpu_kf_setup(pgpu_t *g, double klo, double khi)
+{
+	if (pgpu_kf_setup_launch(g, klo, khi)) return 1;
+	pgpu_kf_setup_finish(g);
 	return 0;
+}
+
+void pgpu_batch_view(pgpu_t *g, pgpu_bview *v)
+{
+	const peec_t *p = g->p;
+	v->s = g->s;
+	v->seg = pgpu_set_args(g, 0, 0, &v->km_seg, nullptr, nullptr);
+	v->nseg = g->seg.n;
+	v->nh = g->chg.n;
+	v->ncell = (p->capacitance && (g->chg.n > 0)) ? p->ncell : 0;
+	if (v->ncell > 0) v->chg = pgpu_set_args(g, 1, 0, &v->km_chg, nullptr, nullptr);
+	v->ident = g->cells_identity;
+	v->rec_lp = g->kf_rec_lp;
+	v->rec_ig = g->kf_rec_ig;
+	v->T = g->kf_T;
+	v->nd_lp = g->kf_nd_lp;
+	v->nd_ig = g->kf_nd_ig;
+	v->list_lp = g->kf_list_lp;
+	v->list_ig = g->kf_list_ig;
+	v->csoff = g->csoff;
+	v->csidx = g->csidx;
+	v->carea = g->carea;
 }
 
 // Lp (nseg x nseg) and P (ncell x ncell) at kw from the records into caller buffers
@@ -584,6 +622,22 @@ void pgpu_fill_to_host(pgpu_t *g, double kw, int mode, d_complex_t *lp, d_comple
 	if (ns > 0) HIPCHK(hipMemcpyAsync(lp, g->lp, (size_t)ns * ns * sizeof(double2), hipMemcpyDeviceToHost, g->s));
 	if (cap && P) HIPCHK(hipMemcpyAsync(P, g->P, (size_t)nc * nc * sizeof(double2), hipMemcpyDeviceToHost, g->s));
 	HIPCHK(hipStreamSynchronize(g->s));
+}
+
+// element-by-element Lp and P at kw into the mirror's own device buffers (synchronised);
+// *lp / *P receive them (P null without capacitive cells)
+void pgpu_fill_direct_dev(pgpu_t *g, double kw, const void **lp, const void **P)
+{
+	const peec_t *p = g->p;
+	const int ns = g->seg.n, nc = p->ncell;
+	const bool cap = p->capacitance && (g->chg.n > 0) && (nc > 0);
+	if (ns > 0 && g->lp == nullptr) g->lp = dalloc<double2>((size_t)ns * ns);
+	if (cap && g->P == nullptr) g->P = dalloc<double2>((size_t)nc * nc);
+	pgpu_fill_lp(g, kw);
+	if (cap) pgpu_fill_pot(g, kw);
+	HIPCHK(hipStreamSynchronize(g->s));
+	*lp = g->lp;
+	*P = cap ? g->P : nullptr;
 }
 
 int pgpu_cmat(pgpu_t *g, int *ising)

@@ -28,11 +28,10 @@
 #include "elements.hpp"
 #include "hip_check.hpp"
 #include "fill_kernels.hpp"
+#include "kf_eval.hpp"
 
 namespace pe {
 
-constexpr int KF_M = 14;
-constexpr int KF_Q = KF_M + 3;
 constexpr double KF_X0 = 0.5;
 
 __device__ __forceinline__ void tri_tile_kf(long L, int nt, int &ti, int &tj)
@@ -359,30 +358,8 @@ __global__ __launch_bounds__(256) void k_kf_eval(kf_args a, int which)
 	const int j = tj * 16 + (threadIdx.x & 15);
 	if ((i >= f.n) || (j >= f.n) || (j < i)) return;
 	const long slot = tile * 256 + threadIdx.x;
-	const double kw = f.kw;
-	double vr = 0, vi = 0;
-	for (int t = 0; t < a.T; t++) {
-		const double *r = a.rec + (long)t * KF_Q * a.nslots + slot;
-		// z = sum_m N'_m (-jk)^m / m!, Horner from the top
-		double zr = r[(long)(2 + KF_M) * a.nslots], zi = 0;
-#pragma unroll
-		for (int m = KF_M - 1; m >= 0; m--) {
-			const double s = kw / (m + 1);
-			const double nr = zi * s, ni = -zr * s;   // z * (-j s)
-			zr = r[(long)(2 + m) * a.nslots] + nr;
-			zi = ni;
-		}
-		const double Rc = r[a.nslots];
-		double er = zr, ei = zi;
-		if (Rc != 0) {
-			double sn, cs;
-			sincos(kw * Rc, &sn, &cs);
-			er = (cs * zr) + (sn * zi);
-			ei = (cs * zi) - (sn * zr);
-		}
-		vr += r[0] + er;
-		vi += ei;
-	}
+	double vr, vi;
+	kf_value(a.rec, a.nslots, slot, a.T, f.kw, vr, vi);
 	double2 *out = (double2 *)f.out;
 	if ((which == 1) && (f.carea != nullptr)) {
 		const double ai = f.carea[i], aj = f.carea[j];

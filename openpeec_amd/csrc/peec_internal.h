@@ -167,6 +167,9 @@ typedef struct {
 	d_complex_t *val;
 } stamps_t;
 void mna_stamps(const peec_t *p, double f, stamps_t *st);
+/* the dense numbering extended by one charge unknown per capacitive cell after the last
+ * branch current (the system of the accelerated and batched sweeps) */
+void mna_stamps_charge(const peec_t *p, double f, stamps_t *st);
 void mna_stamps_free(stamps_t *st);
 
 #ifdef __cplusplus
