@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--grid", type=int, default=64, help="plate grid (64 = the precision workload)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--no-check", action="store_true", help="skip the small-deck oracle check")
+    ap.add_argument("--no-batch", action="store_true", help="skip the many-structure batching workload")
+    ap.add_argument("--batch-structs", type=int, default=256, help="structures of the batching workload")
     return ap.parse_args()
 
 
@@ -180,6 +182,66 @@ def precision_points(m, idx, zp):
     }
 
 
+def bench_batch(lib, nstruct: int = 256, seed0: int = 0x5EED0003, plain_sample: int = 2) -> dict:
+    """The batching workload: nstruct random wire structures (20-400 segments), 64 frequencies
+    each, swept together by the batched kernels; the plain FP64 form of each structure on
+    plain_sample of its frequencies for the rate and Zin comparison."""
+    from openpeec_amd import PEEC_FORM_PLAIN, Batch, Peec, decks
+    t0 = time.perf_counter()
+    ms = []
+    for s in range(nstruct):
+        m = Peec.from_deck_text(decks.random_wire_deck(seed0 + s), lib)
+        m.setup()
+        ms.append(m)
+    t_setup = time.perf_counter() - t0
+    nf = ms[0].p.nfreq
+    t0 = time.perf_counter()
+    b = Batch(ms, 0, nf)
+    t_create = time.perf_counter() - t0
+    st_t = b.tune()
+    trials = b.trials()
+    b.sweep()                      # warm: graphs captured, buffers sized
+    t0 = time.perf_counter()
+    zs, st = b.sweep()
+    elapsed = time.perf_counter() - t0
+    jobs = st.jobs
+    # plain FP64 form: every structure at plain_sample frequencies spread over its sweep
+    idx = [int(round(x)) for x in np.linspace(0, nf - 1, plain_sample)]
+    t0 = time.perf_counter()
+    err = 0.0
+    for m, z in zip(ms, zs):
+        for i in idx:
+            zp, _ = m.sweep(i, 1, PEEC_FORM_PLAIN)
+            err = max(err, float(np.max(np.abs(z[i] - zp[0]) / np.abs(zp[0]))))
+    t_plain = time.perf_counter() - t0
+    plain_rate = nstruct * len(idx) / t_plain
+    nseg = [m.p.nseg for m in ms]
+    order = [m.p.nunknown + (m.p.ncell if m.p.capacitance else 0) for m in ms]
+    b.close()
+    for m in ms:
+        m.close()
+    lu_tf = st.lu_flop / (st.lu_ms * 1e-3) / 1e12 if st.lu_ms > 0 else None
+    return {
+        "workload": f"{nstruct} random wire structures (seed {seed0:#x}+s), 20-400 segments of 5-20 mm in a 0.3 m cube, "
+                    f"radius 0.5-2 mm, 1 mm feed gap, retarded + capacitive, {nf} frequencies 50 MHz-2 GHz each",
+        "points": jobs, "points_per_s": jobs / elapsed, "ms": 1e3 * elapsed,
+        "plain_fp64_points_per_s": plain_rate, "speedup_vs_plain_fp64": (jobs / elapsed) / plain_rate,
+        "zin_max_rel_err_vs_plain_fp64": err, "zin_checked_points": nstruct * len(idx),
+        "segments": {"min": min(nseg), "max": max(nseg), "mean": float(np.mean(nseg))},
+        "system_order": {"min": min(order), "max": max(order)},
+        "stages_ms": {"fill_graphs": st.fill_ms, "stamp_host": st.stamp_ms, "lu_solve_graphs": st.lu_ms,
+                      "total": st.total_ms},
+        "lu": {"TFLOP_per_s": lu_tf, "frac_of_fp64_mfma_peak": (lu_tf / PEAK_FP64_MFMA) if lu_tf else None,
+               "flop": st.lu_flop},
+        "chunks": st.chunks, "direct_pair_evaluations": st.direct_pairs,
+        "tuned": {"panel_width": st.nb, "tile_cols": st.tile_cols, "wg_per_cu": st.wg_per_cu,
+                  "order": "largest first" if st.order == 0 else "interleaved",
+                  "trials": [{"nb": t[0], "tile_cols": t[1], "wpc": t[2], "order": t[3], "lu_ms": t[4]} for t in trials],
+                  "tune_ms": st_t.tune_ms},
+        "setup_s": {"decks_and_geometry": t_setup, "records_and_tables": t_create},
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -297,6 +359,8 @@ def main():
         })
     if not args.no_check:
         result["small_deck_check"] = small_check(lib)
+    if not args.no_batch:
+        result["batching"] = bench_batch(lib, args.batch_structs)
     if not args.no_cpu:
         sys.path.insert(0, str(ROOT / "tests"))
         from oracle_py import oracle_peec

@@ -118,14 +118,17 @@ typedef struct {
 	long   jobs;
 	int    chunks;
 	int    nb, wg_per_cu, order, tune_trials;
+	int    tile_cols;       /* trailing-update tile width (32 rows x tile_cols) */
 	int    direct_structures; /* filled element by element (no usable records) */
+	long   direct_pairs;    /* pair evaluations by the element code (no convergent expansion) */
 } peec_batch_stats_t;
 peec_batch_t *peec_batch_create(peec_t **models, int nmodel, int ifreq0, int nfreq);
 int  peec_batch_tune(peec_batch_t *b, peec_batch_stats_t *st);
 /* zmat[s] receives nfreq x nport x nport of structure s; returns 0, 2 (a singular system) or 4 */
 int  peec_batch_sweep(peec_batch_t *b, d_complex_t **zmat, peec_batch_stats_t *st);
-/* the autotuner's candidates and their LU times (ms); returns their count */
-int  peec_batch_trials(const peec_batch_t *b, int *nb, int *wg_per_cu, int *order, double *ms, int cap);
+/* the autotuner's candidates (panel width, tile width, workgroups per CU, order) and their LU
+ * times (ms); returns their count */
+int  peec_batch_trials(const peec_batch_t *b, int *nb, int *tile_cols, int *wg_per_cu, int *order, double *ms, int cap);
 void peec_batch_free(peec_batch_t *b);
 /* The batched factorisation alone (tests): nmat row-major matrices of orders n[j],
  * concatenated in a, factored in place as lu_decomp (src/lu.c:16-66) does, one workgroup each

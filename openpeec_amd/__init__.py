@@ -203,12 +203,13 @@ class Batch:
             raise RuntimeError("peec_batch_tune failed")
         return st
 
-    def trials(self) -> "list[tuple[int, int, int, float]]":
+    def trials(self) -> "list[tuple[int, int, int, int, float]]":
+        """(panel width, tile columns, workgroups per CU, order, LU ms) of every candidate."""
         cap = 64
-        nb, wpc, order = (C.c_int * cap)(), (C.c_int * cap)(), (C.c_int * cap)()
+        nb, tc, wpc, order = (C.c_int * cap)(), (C.c_int * cap)(), (C.c_int * cap)(), (C.c_int * cap)()
         ms = (C.c_double * cap)()
-        n = self.lib.peec_batch_trials(self.h, nb, wpc, order, ms, cap)
-        return [(nb[k], wpc[k], order[k], ms[k]) for k in range(min(n, cap))]
+        n = self.lib.peec_batch_trials(self.h, nb, tc, wpc, order, ms, cap)
+        return [(nb[k], tc[k], wpc[k], order[k], ms[k]) for k in range(min(n, cap))]
 
     def sweep(self):
         """[Z per structure (nfreq x nport x nport)], stats."""

@@ -139,7 +139,8 @@ class peec_batch_stats_t(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("tune_ms", C.c_double), ("fill_ms", C.c_double),
                 ("lu_ms", C.c_double), ("stamp_ms", C.c_double), ("lu_flop", C.c_double),
                 ("jobs", C.c_long), ("chunks", C.c_int), ("nb", C.c_int), ("wg_per_cu", C.c_int),
-                ("order", C.c_int), ("tune_trials", C.c_int), ("direct_structures", C.c_int)]
+                ("order", C.c_int), ("tune_trials", C.c_int), ("tile_cols", C.c_int),
+                ("direct_structures", C.c_int), ("direct_pairs", C.c_long)]
 
 
 def declare(lib: C.CDLL, reference: bool = False) -> None:
@@ -217,7 +218,7 @@ def declare(lib: C.CDLL, reference: bool = False) -> None:
             "peec_batch_tune": (C.c_int, [C.c_void_p, C.POINTER(peec_batch_stats_t)]),
             "peec_batch_sweep": (C.c_int, [C.c_void_p, C.POINTER(PZ), C.POINTER(peec_batch_stats_t)]),
             "peec_batch_trials": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
-                                            C.POINTER(C.c_double), C.c_int]),
+                                            C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int]),
             "peec_batch_free": (None, [C.c_void_p]),
             "peec_batch_lu_decomp": (C.c_int, [C.c_int, C.POINTER(C.c_int), PZ, C.POINTER(C.c_int), C.c_int]),
         })

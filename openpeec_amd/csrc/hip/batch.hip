@@ -12,8 +12,10 @@
 //                            memset node)
 //                 k_b_lp     -jw Lp(f) of every job from the structure's k-factored records
 //                            (one sincos + 15-term Horner per pair), written into the system
-//                 k_b_p      P(f) = sum of sub-cell integrals / (4 pi eps0 A1 A2) from the
-//                            charge records, in the reference's accumulation order
+//                 k_b_g      sub-cell integrals G(f) from the charge records into a scratch
+//                            (tile order, coalesced record reads)
+//                 k_b_p      P(f) = sum of sub-cell integrals / (4 pi eps0 A1 A2), in the
+//                            reference's accumulation order
 //   solve graph   k_b_stamp  the sparse stamps (lumped elements, incidence, internal impedance,
 //                            charge couplings), computed on the host per job
 //                 k_b_reset  work-queue counter
@@ -43,6 +45,7 @@
 #include "hip_check.hpp"
 #include "fill_kernels.hpp"
 #include "kf_eval.hpp"
+#include "fill_entries.hpp"
 #include "../peec_internal.h"
 #include "peec_mi355x.h"
 
@@ -59,6 +62,7 @@ struct bdesc {                 // one structure
 	int offQ, ncell;           // charge block (ncell 0: none)
 	int T;                     // record terms
 	int ntS, ntH;              // tile grid sizes of the Lp and sub-cell triangles
+	int nh;                    // charge sub-cells
 	long nslS, nslH;           // record slots
 	const double *rec_lp, *rec_ig;
 	const int *csoff, *csidx;
@@ -66,6 +70,9 @@ struct bdesc {                 // one structure
 	int np;
 	int i1[BMAXPORT], i2[BMAXPORT];   // port node rows (-1: reference node)
 	int direct;                // filled element by element (records unusable)
+	int nd_lp, nd_ig;          // pairs without a convergent expansion, evaluated directly per job
+	const int2 *list_lp, *list_ig;
+	pe::fill_args fs, fc;      // current / charge cell sets for the element code
 };
 
 struct bjob {                  // one structure at one frequency
@@ -74,7 +81,9 @@ struct bjob {                  // one structure at one frequency
 	double f, omega, kw;
 	long aoff;                 // system offset in the arena (entries)
 	long zoff;                 // Z offset in the chunk's output (entries)
-	long tS, tP;               // first Lp tile, first P tile of the job in the flattened grids
+	long tS, tP, tG;           // first Lp tile, P tile, sub-cell tile of the job in the flattened grids
+	long tL;                   // first directly evaluated pair of the job
+	long goff;                 // sub-cell integral block offset in the G scratch (entries)
 };
 
 // ---------------------------------------------------------------------------------- fills
@@ -86,12 +95,12 @@ __global__ void k_b_zero(double2 *A, long cnt)
 }
 
 // job of a flattened tile index: last job whose first tile is <= L (tiles ascend with jobs)
-__device__ __forceinline__ int job_of(const bjob *J, int nj, long L, bool lp)
+__device__ __forceinline__ int job_of(const bjob *J, int nj, long L, int which)
 {
 	int lo = 0, hi = nj - 1;
 	while (lo < hi) {
 		const int mid = (lo + hi + 1) >> 1;
-		const long t = lp ? J[mid].tS : J[mid].tP;
+		const long t = (which == 0) ? J[mid].tS : (which == 1) ? J[mid].tP : (which == 2) ? J[mid].tG : J[mid].tL;
 		if (t <= L) lo = mid;
 		else hi = mid - 1;
 	}
@@ -115,7 +124,7 @@ __device__ __forceinline__ void tri_of(long L, int nt, int &ti, int &tj)
 __global__ __launch_bounds__(256) void k_b_lp(const bdesc *D, const bjob *J, int nj, double2 *arena)
 {
 	const long L = blockIdx.x;
-	const int jb = job_of(J, nj, L, true);
+	const int jb = job_of(J, nj, L, 0);
 	const bjob job = J[jb];
 	const bdesc &d = D[job.s];
 	if (d.direct) return;
@@ -135,11 +144,68 @@ __global__ __launch_bounds__(256) void k_b_lp(const bdesc *D, const bjob *J, int
 	if (i != j) A[(long)(d.offS + j) * N + d.offS + i] = x;
 }
 
-// P of every job: one 16 x 16 tile of a job's (full) cell x cell block per workgroup
-__global__ __launch_bounds__(256) void k_b_p(const bdesc *D, const bjob *J, int nj, double k4pe, double2 *arena)
+// sub-cell integrals G(h1, h2) of every job from the charge records: one 16 x 16 tile of a job's
+// upper triangle per workgroup (slot order, so the record reads coalesce), mirrored
+__global__ __launch_bounds__(256) void k_b_g(const bdesc *D, const bjob *J, int nj, double2 *G)
 {
 	const long L = blockIdx.x;
-	const int jb = job_of(J, nj, L, false);
+	const int jb = job_of(J, nj, L, 2);
+	const bjob job = J[jb];
+	const bdesc &d = D[job.s];
+	if (d.direct || (d.ncell == 0)) return;
+	int ti, tj;
+	const long t = L - job.tG;
+	tri_of(t, d.ntH, ti, tj);
+	const int nh = d.nh;
+	const int i = ti * 16 + (threadIdx.x >> 4), j = tj * 16 + (threadIdx.x & 15);
+	if ((i >= nh) || (j >= nh) || (j < i)) return;
+	double vr, vi;
+	pe::kf_value(d.rec_ig, d.nslH, t * 256 + threadIdx.x, d.T, job.kw, vr, vi);
+	double2 *g = G + job.goff;
+	g[(long)i * nh + j] = make_double2(vr, vi);
+	if (i != j) g[(long)j * nh + i] = make_double2(vr, vi);
+}
+
+// pairs the records list for direct evaluation: the element code at the job's wavenumber,
+// overwriting the zero the record evaluation left (-jw Lp into the system, G into the scratch)
+template <int KM>
+__global__ __launch_bounds__(256) void k_b_list(const bdesc *D, const bjob *J, int nj, long total, double2 *arena,
+                                                double2 *G)
+{
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= total) return;
+	const int jb = job_of(J, nj, e, 3);
+	const bjob job = J[jb];
+	const bdesc &d = D[job.s];
+	const int q = (int)(e - job.tL);
+	if (q < d.nd_lp) {
+		const int2 ij = d.list_lp[q];
+		const pe::fill_args &a = d.fs;
+		const pe::cplx v = pe::lp_entry<KM>(a.S, a.pS, a.SM, a.pSM, ij.x, ij.y, job.kw, a.coef, a.gp);
+		const double w = job.omega;
+		const double2 x = make_double2((0 * v.r) - ((-w) * v.i), (0 * v.i) + ((-w) * v.r));
+		double2 *A = arena + job.aoff;
+		const long N = job.N;
+		A[(long)(d.offS + ij.x) * N + d.offS + ij.y] = x;
+		if (ij.x != ij.y) A[(long)(d.offS + ij.y) * N + d.offS + ij.x] = x;
+		return;
+	}
+	const int2 ij = d.list_ig[q - d.nd_lp];
+	const pe::fill_args &a = d.fc;
+	const pe::cplx v = pe::ig_entry<KM>(a.S, a.pS, a.SM, a.pSM, ij.x, ij.y, job.kw, a.gp);
+	double2 *g = G + job.goff;
+	g[(long)ij.x * d.nh + ij.y] = make_double2(v.r, v.i);
+	if (ij.x != ij.y) g[(long)ij.y * d.nh + ij.x] = make_double2(v.r, v.i);
+}
+
+// P of every job: P(c1, c2) = sum over sub-cell pairs of G / (4 pi eps0 A1 A2) in the
+// reference's accumulation order (potential.c:245-256); one 16 x 16 tile of a job's cell block
+// per workgroup
+__global__ __launch_bounds__(256) void k_b_p(const bdesc *D, const bjob *J, int nj, double k4pe, const double2 *G,
+                                             double2 *arena)
+{
+	const long L = blockIdx.x;
+	const int jb = job_of(J, nj, L, 1);
 	const bjob job = J[jb];
 	const bdesc &d = D[job.s];
 	if (d.direct || (d.ncell == 0)) return;
@@ -148,16 +214,15 @@ __global__ __launch_bounds__(256) void k_b_p(const bdesc *D, const bjob *J, int 
 	const int ti = (int)(t / ntc), tj = (int)(t - (long)ti * ntc);
 	const int c1 = ti * 16 + (threadIdx.x >> 4), c2 = tj * 16 + (threadIdx.x & 15);
 	if ((c1 >= d.ncell) || (c2 >= d.ncell)) return;
+	const double2 *g = G + job.goff;
+	const int nh = d.nh;
 	double sr = 0, si = 0;
 	for (int p = d.csoff[c1]; p < d.csoff[c1 + 1]; p++) {
-		const int h1 = d.csidx[p];
+		const long row = (long)d.csidx[p] * nh;
 		for (int q = d.csoff[c2]; q < d.csoff[c2 + 1]; q++) {
-			const int h2 = d.csidx[q];
-			const int lo = (h1 <= h2) ? h1 : h2, hi = (h1 <= h2) ? h2 : h1;
-			double gr, gi;
-			pe::kf_value(d.rec_ig, d.nslH, pe::kf_slot(lo, hi, d.ntH), d.T, job.kw, gr, gi);
-			sr = sr + gr;
-			si = si + gi;
+			const double2 v = g[row + d.csidx[q]];
+			sr = sr + v.x;
+			si = si + v.y;
 		}
 	}
 	const double s = 1 / (k4pe * d.carea[c1] * d.carea[c2]);
@@ -238,62 +303,82 @@ __device__ __forceinline__ void wg_argmax(double &v, int &ix, double *rv, int *r
 	__syncthreads();
 }
 
-// C[M x Nc] -= A[M x K] B[K x Nc] (row-major, leading dimension ld, K <= NB): each wave takes
-// 32 x 32 tiles, complex as four real FP64 MFMA 16x16x4 products
-template <int NB>
-__device__ void trailing_update(double2 *A, long ld, int k0, int nb, int m)
+// A22 -= L21 U12 (row-major, leading dimension ld, K = nb <= NB): each wave takes 32 x 16 CB tiles
+// (2 x CB blocks of 16 x 16), complex as four real FP64 MFMA 16x16x4 products per block, with the
+// next k-step's fragments loaded before the current step's products
+template <int NB, int CB>
+__device__ __forceinline__ void trailing_update(double2 *A, long ld, int k0, int nb, int m)
 {
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int li = lane & 15, lk = lane >> 4;
-	const int tn = (m + 31) / 32;
+	const int tr = (m + 31) / 32, tc = (m + 16 * CB - 1) / (16 * CB);
 	const double2 *L21 = A + (long)(k0 + nb) * ld + k0;        // m x nb
 	const double2 *U12 = A + (long)k0 * ld + k0 + nb;          // nb x m
 	double2 *C = A + (long)(k0 + nb) * ld + k0 + nb;
-	for (int t = wave; t < tn * tn; t += 4) {
-		const int r0 = (t / tn) * 32, c0 = (t % tn) * 32;
-		d4 ar[2][2], ai[2][2];
+#pragma unroll 1
+	for (int t = wave; t < tr * tc; t += 4) {
+		const int r0 = (t / tc) * 32, c0 = (t % tc) * (16 * CB);
+		d4 ar[2][CB], ai[2][CB];
 #pragma unroll
 		for (int a = 0; a < 2; a++)
 #pragma unroll
-			for (int b = 0; b < 2; b++) {
+			for (int b = 0; b < CB; b++) {
 				ar[a][b] = d4{0, 0, 0, 0};
 				ai[a][b] = d4{0, 0, 0, 0};
 			}
+		const double2 *arow[2];
+		const double2 *bcol[CB];
+		bool aok[2], bok[CB];
+#pragma unroll
+		for (int rb = 0; rb < 2; rb++) {
+			const int r = r0 + rb * 16 + li;
+			aok[rb] = r < m;
+			arow[rb] = L21 + (long)(aok[rb] ? r : 0) * ld;
+		}
+#pragma unroll
+		for (int cb = 0; cb < CB; cb++) {
+			const int c = c0 + cb * 16 + li;
+			bok[cb] = c < m;
+			bcol[cb] = U12 + (bok[cb] ? c : 0);
+		}
+		double2 fa[2], fb[CB];
+#pragma unroll
+		for (int rb = 0; rb < 2; rb++) fa[rb] = (aok[rb] && (lk < nb)) ? arow[rb][lk] : make_double2(0, 0);
+#pragma unroll
+		for (int cb = 0; cb < CB; cb++) fb[cb] = (bok[cb] && (lk < nb)) ? bcol[cb][(long)lk * ld] : make_double2(0, 0);
 #pragma unroll 1
 		for (int k = 0; k < nb; k += 4) {
-			const int kk = k + lk;
-			double2 fa[2], fb[2];
+			const int kk = k + 4 + lk;
+			double2 na[2], nb4[CB];
 #pragma unroll
-			for (int rb = 0; rb < 2; rb++) {
-				const int r = r0 + rb * 16 + li;
-				fa[rb] = ((r < m) && (kk < nb)) ? L21[(long)r * ld + kk] : make_double2(0, 0);
-			}
+			for (int rb = 0; rb < 2; rb++) na[rb] = (aok[rb] && (kk < nb)) ? arow[rb][kk] : make_double2(0, 0);
 #pragma unroll
-			for (int cb = 0; cb < 2; cb++) {
-				const int c = c0 + cb * 16 + li;
-				fb[cb] = ((c < m) && (kk < nb)) ? U12[(long)kk * ld + c] : make_double2(0, 0);
-			}
+			for (int cb = 0; cb < CB; cb++) nb4[cb] = (bok[cb] && (kk < nb)) ? bcol[cb][(long)kk * ld] : make_double2(0, 0);
 #pragma unroll
 			for (int rb = 0; rb < 2; rb++)
 #pragma unroll
-				for (int cb = 0; cb < 2; cb++) ar[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[rb].x, fb[cb].x, ar[rb][cb], 0, 0, 0);
+				for (int cb = 0; cb < CB; cb++) ar[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[rb].x, fb[cb].x, ar[rb][cb], 0, 0, 0);
 #pragma unroll
 			for (int rb = 0; rb < 2; rb++)
 #pragma unroll
-				for (int cb = 0; cb < 2; cb++) ai[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[rb].x, fb[cb].y, ai[rb][cb], 0, 0, 0);
+				for (int cb = 0; cb < CB; cb++) ai[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[rb].x, fb[cb].y, ai[rb][cb], 0, 0, 0);
 #pragma unroll
 			for (int rb = 0; rb < 2; rb++)
 #pragma unroll
-				for (int cb = 0; cb < 2; cb++) ar[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(-fa[rb].y, fb[cb].y, ar[rb][cb], 0, 0, 0);
+				for (int cb = 0; cb < CB; cb++) ar[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(-fa[rb].y, fb[cb].y, ar[rb][cb], 0, 0, 0);
 #pragma unroll
 			for (int rb = 0; rb < 2; rb++)
 #pragma unroll
-				for (int cb = 0; cb < 2; cb++) ai[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[rb].y, fb[cb].x, ai[rb][cb], 0, 0, 0);
+				for (int cb = 0; cb < CB; cb++) ai[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[rb].y, fb[cb].x, ai[rb][cb], 0, 0, 0);
+#pragma unroll
+			for (int q = 0; q < 2; q++) fa[q] = na[q];
+#pragma unroll
+			for (int q = 0; q < CB; q++) fb[q] = nb4[q];
 		}
 #pragma unroll
 		for (int rb = 0; rb < 2; rb++)
 #pragma unroll
-			for (int cb = 0; cb < 2; cb++) {
+			for (int cb = 0; cb < CB; cb++) {
 				const int c = c0 + cb * 16 + li;
 				if (c >= m) continue;
 #pragma unroll
@@ -309,13 +394,11 @@ __device__ void trailing_update(double2 *A, long ld, int k0, int nb, int m)
 	}
 }
 
-// persistent: factor and solve jobs from the queue until it is empty.  Dynamic LDS holds the
-// right-hand side (N complex) and the pivot rows (N ints) of the current job.
 // in-place LU of the N x N row-major system A: right-looking, panel width NB, row pivoting by
 // the first maximum of |re| + |im| (lu.c:19-30), full-row interchanges recorded in piv (any
 // address space); returns -1, or the column whose pivot is below 1e-30.  All 256 threads call.
-template <int NB>
-__device__ int blu_factor(double2 *A, int N, int *piv, double2 *s_row, double2 *s_L, double *s_rv, int *s_ri)
+template <int NB, int CB>
+__device__ __forceinline__ int blu_factor(double2 *A, int N, int *piv, double2 *s_row, double2 *s_L, double *s_rv, int *s_ri)
 {
 	const int tid = threadIdx.x;
 	int info = -1;
@@ -356,6 +439,7 @@ __device__ int blu_factor(double2 *A, int N, int *piv, double2 *s_row, double2 *
 				for (int r = c + 1 + tid; r < N; r += 256) {
 					double2 *ar = A + (long)r * N;
 					const double2 l = ar[c];
+#pragma unroll 4
 					for (int t = 0; t < w; t++) {
 						const double2 u = s_row[t];
 						double2 x = ar[c + 1 + t];
@@ -378,8 +462,10 @@ __device__ int blu_factor(double2 *A, int N, int *piv, double2 *s_row, double2 *
 		}
 		__syncthreads();
 		for (int col = kend + tid; col < N; col += 256)
+#pragma unroll 1
 			for (int c = 0; c < nb - 1; c++) {
 				const double2 u = A[(long)(k0 + c) * N + col];
+#pragma unroll 4
 				for (int r = c + 1; r < nb; r++) {
 					const double2 l = s_L[r * NB + c];
 					double2 x = A[(long)(k0 + r) * N + col];
@@ -389,13 +475,13 @@ __device__ int blu_factor(double2 *A, int N, int *piv, double2 *s_row, double2 *
 				}
 			}
 		__syncthreads();
-		trailing_update<NB>(A, N, k0, nb, m);
+		trailing_update<NB, CB>(A, N, k0, nb, m);
 		__syncthreads();
 	}
 	return info;
 }
 
-template <int NB>
+template <int NB, int CB>
 __global__ __launch_bounds__(256) void k_b_lu(const bdesc *D, const bjob *J, const int *order, int nj, unsigned *counter,
                                               double2 *arena, double2 *zout, int *info_out)
 {
@@ -419,7 +505,7 @@ __global__ __launch_bounds__(256) void k_b_lu(const bdesc *D, const bjob *J, con
 		double2 *A = arena + job.aoff;
 		double2 *b = s_dyn;
 		int *piv = (int *)(s_dyn + N);
-		const int info = blu_factor<NB>(A, N, piv, s_row, s_L, s_rv, s_ri);
+		const int info = blu_factor<NB, CB>(A, N, piv, s_row, s_L, s_rv, s_ri);
 		if (info >= 0) {
 			if (tid == 0) info_out[jid] = info;
 			__syncthreads();
@@ -485,7 +571,7 @@ __global__ __launch_bounds__(256) void k_b_lu_only(double2 *A, const long *aoff,
 	__shared__ double s_rv[4];
 	__shared__ int s_ri[4];
 	const int j = blockIdx.x;
-	const int r = blu_factor<NB>(A + aoff[j], n[j], piv + poff[j], s_row, s_L, s_rv, s_ri);
+	const int r = blu_factor<NB, 2>(A + aoff[j], n[j], piv + poff[j], s_row, s_L, s_rv, s_ri);
 	if (threadIdx.x == 0) info[j] = r;
 }
 
@@ -528,6 +614,7 @@ struct hbuf {                  // pinned host buffer that grows
 
 struct tune_t {
 	int nb = 32;        // LU panel width
+	int cb = 2;         // trailing-update tile width in 16-column blocks (2 or 4)
 	int wpc = 2;        // persistent workgroups per CU
 	int order = 0;      // 0 largest first, 1 interleaved (structure order)
 	double ms = 0;      // LU time of the sample with this choice
@@ -543,18 +630,18 @@ double freq_of(const peec_t *p, int i)
 struct chunk {                 // jobs of one pass through the arena
 	std::vector<bjob> J;
 	std::vector<int> ord;
-	long arena_n = 0, tilesS = 0, tilesP = 0, zn = 0, nst = 0;
+	long arena_n = 0, tilesS = 0, tilesP = 0, tilesG = 0, nlist = 0, gn = 0, zn = 0, nst = 0;
 	int Nmax = 0;
 };
 
 struct gkey {
 	int nj;
-	long nst, arena_n, tilesS, tilesP;
-	int nb, wpc;
+	long nst, arena_n, tilesS, tilesP, tilesG, nlist;
+	int nb, cb, wpc;
 	bool operator<(const gkey &o) const
 	{
-		return std::tie(nj, nst, arena_n, tilesS, tilesP, nb, wpc) <
-		       std::tie(o.nj, o.nst, o.arena_n, o.tilesS, o.tilesP, o.nb, o.wpc);
+		return std::tie(nj, nst, arena_n, tilesS, tilesP, tilesG, nlist, nb, cb, wpc) <
+		       std::tie(o.nj, o.nst, o.arena_n, o.tilesS, o.tilesP, o.tilesG, o.nlist, o.nb, o.cb, o.wpc);
 	}
 };
 
@@ -567,7 +654,7 @@ struct batch {
 	dbuf<bdesc> dd;
 	hipStream_t s = nullptr;
 	int ncu = 0;
-	dbuf<double2> arena, zdev;
+	dbuf<double2> arena, zdev, gscr;
 	dbuf<bjob> dj;
 	dbuf<int> dorder, dinfo, dsj, dri, dci;
 	dbuf<unsigned> dcnt;
@@ -575,6 +662,7 @@ struct batch {
 	hbuf<int> hsj, hri, hci, hinfo;
 	hbuf<double2> hval, hz;
 	std::map<gkey, std::pair<hipGraphExec_t, hipGraphExec_t>> graphs;
+	int km_class = 15;          // element kinds present (compiled specialisation of the list kernel)
 	bool tuned = false;
 	tune_t tune;
 	std::vector<tune_t> trials;
@@ -595,15 +683,20 @@ size_t lu_lds(int Nmax)
 	return (size_t)Nmax * sizeof(double2) + (size_t)Nmax * sizeof(int) + 16;
 }
 
-void launch_lu(hipStream_t s, batch &B, int nb, int wpc, int nj, size_t lds)
+template <int NB, int CB>
+void launch_lu_t(hipStream_t s, batch &B, int grid, int nj, size_t lds)
 {
-	const int grid = std::max(1, std::min(nj, B.ncu * wpc));
-	if (nb == 16)
-		hipLaunchKernelGGL((k_b_lu<16>), dim3(grid), dim3(256), lds, s, B.dd.p, B.dj.p, B.dorder.p, nj, B.dcnt.p,
-		                   B.arena.p, B.zdev.p, B.dinfo.p);
-	else
-		hipLaunchKernelGGL((k_b_lu<32>), dim3(grid), dim3(256), lds, s, B.dd.p, B.dj.p, B.dorder.p, nj, B.dcnt.p,
-		                   B.arena.p, B.zdev.p, B.dinfo.p);
+	hipLaunchKernelGGL((k_b_lu<NB, CB>), dim3(grid), dim3(256), lds, s, B.dd.p, B.dj.p, B.dorder.p, nj, B.dcnt.p,
+	                   B.arena.p, B.zdev.p, B.dinfo.p);
+}
+
+void launch_lu(hipStream_t s, batch &B, const tune_t &t, int nj, size_t lds)
+{
+	const int grid = std::max(1, std::min(nj, B.ncu * t.wpc));
+	if ((t.nb == 16) && (t.cb == 2)) launch_lu_t<16, 2>(s, B, grid, nj, lds);
+	else if ((t.nb == 16) && (t.cb == 4)) launch_lu_t<16, 4>(s, B, grid, nj, lds);
+	else if (t.cb == 4) launch_lu_t<32, 4>(s, B, grid, nj, lds);
+	else launch_lu_t<32, 2>(s, B, grid, nj, lds);
 }
 
 void record_fill(hipStream_t s, batch &B, const chunk &c)
@@ -611,18 +704,30 @@ void record_fill(hipStream_t s, batch &B, const chunk &c)
 	const int nj = (int)c.J.size();
 	hipLaunchKernelGGL(k_b_zero, dim3(4096), dim3(256), 0, s, B.arena.p, c.arena_n);
 	if (c.tilesS > 0) hipLaunchKernelGGL(k_b_lp, dim3((unsigned)c.tilesS), dim3(256), 0, s, B.dd.p, B.dj.p, nj, B.arena.p);
+	if (c.tilesG > 0) hipLaunchKernelGGL(k_b_g, dim3((unsigned)c.tilesG), dim3(256), 0, s, B.dd.p, B.dj.p, nj, B.gscr.p);
+	if (c.nlist > 0) {
+		const dim3 g((unsigned)((c.nlist + 255) / 256)), b(256);
+		switch (B.km_class) {
+		case pe::KM_WIRE: hipLaunchKernelGGL(k_b_list<pe::KM_WIRE>, g, b, 0, s, B.dd.p, B.dj.p, nj, c.nlist, B.arena.p, B.gscr.p); break;
+		case pe::KM_RIBBON:
+			hipLaunchKernelGGL(k_b_list<pe::KM_RIBBON>, g, b, 0, s, B.dd.p, B.dj.p, nj, c.nlist, B.arena.p, B.gscr.p);
+			break;
+		default: hipLaunchKernelGGL(k_b_list<15>, g, b, 0, s, B.dd.p, B.dj.p, nj, c.nlist, B.arena.p, B.gscr.p); break;
+		}
+	}
 	if (c.tilesP > 0)
-		hipLaunchKernelGGL(k_b_p, dim3((unsigned)c.tilesP), dim3(256), 0, s, B.dd.p, B.dj.p, nj, 4 * PI * EPS0, B.arena.p);
+		hipLaunchKernelGGL(k_b_p, dim3((unsigned)c.tilesP), dim3(256), 0, s, B.dd.p, B.dj.p, nj, 4 * PI * EPS0,
+		                   (const double2 *)B.gscr.p, B.arena.p);
 }
 
-void record_solve(hipStream_t s, batch &B, const chunk &c, int nb, int wpc)
+void record_solve(hipStream_t s, batch &B, const chunk &c, const tune_t &t)
 {
 	const int nj = (int)c.J.size();
 	if (c.nst > 0)
 		hipLaunchKernelGGL(k_b_stamp, dim3((unsigned)((c.nst + 255) / 256)), dim3(256), 0, s, B.dj.p, B.dsj.p, B.dri.p,
 		                   B.dci.p, B.dval.p, c.nst, B.arena.p);
 	hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, s, B.dcnt.p);
-	launch_lu(s, B, nb, wpc, nj, lu_lds(c.Nmax));
+	launch_lu(s, B, t, nj, lu_lds(c.Nmax));
 }
 
 template <typename F>
@@ -642,7 +747,7 @@ hipGraphExec_t capture(hipStream_t s, F body)
 void build_chunk(const batch &B, const std::vector<std::pair<int, int>> &items, int order, chunk &c)
 {
 	c.J.resize(items.size());
-	c.arena_n = c.tilesS = c.tilesP = c.zn = c.nst = 0;
+	c.arena_n = c.tilesS = c.tilesP = c.tilesG = c.nlist = c.gn = c.zn = c.nst = 0;
 	c.Nmax = 0;
 	for (size_t k = 0; k < items.size(); k++) {
 		const int s = items[k].first, fi = items[k].second;
@@ -658,11 +763,19 @@ void build_chunk(const batch &B, const std::vector<std::pair<int, int>> &items, 
 		j.zoff = c.zn;
 		j.tS = c.tilesS;
 		j.tP = c.tilesP;
+		j.tG = c.tilesG;
+		j.tL = c.nlist;
+		j.goff = c.gn;
+		if (!d.direct) c.nlist += d.nd_lp + ((d.ncell > 0) ? d.nd_ig : 0);
 		c.arena_n += (long)d.N * d.N;
 		c.zn += (long)d.np * d.np;
 		c.tilesS += (long)d.ntS * (d.ntS + 1) / 2;
 		const long ntc = (d.ncell + 15) / 16;
 		c.tilesP += ntc * ntc;
+		if (d.ncell > 0) {
+			c.tilesG += (long)d.ntH * (d.ntH + 1) / 2;
+			c.gn += (long)d.nh * d.nh;
+		}
 		c.Nmax = std::max(c.Nmax, d.N);
 	}
 	c.ord.resize(items.size());
@@ -714,6 +827,7 @@ void stage_chunk(batch &B, chunk &c)
 	moved |= B.dval.ensure(tot);
 	moved |= B.arena.ensure(c.arena_n);
 	moved |= B.zdev.ensure(c.zn);
+	moved |= B.gscr.ensure(c.gn);
 	if (moved) {
 		// captured graphs hold the old addresses
 		for (auto &kv : B.graphs) {
@@ -751,21 +865,21 @@ void direct_blocks(batch &B, const chunk &c)
 	}
 }
 
-std::pair<hipGraphExec_t, hipGraphExec_t> graphs_for(batch &B, const chunk &c, int nb, int wpc)
+std::pair<hipGraphExec_t, hipGraphExec_t> graphs_for(batch &B, const chunk &c, const tune_t &t)
 {
-	const gkey k{(int)c.J.size(), c.nst, c.arena_n, c.tilesS, c.tilesP, nb, wpc};
+	const gkey k{(int)c.J.size(), c.nst, c.arena_n, c.tilesS, c.tilesP, c.tilesG, c.nlist, t.nb, t.cb, t.wpc};
 	auto it = B.graphs.find(k);
 	if (it != B.graphs.end()) return it->second;
 	hipGraphExec_t gf = capture(B.s, [&](hipStream_t s) { record_fill(s, B, c); });
-	hipGraphExec_t gs = capture(B.s, [&](hipStream_t s) { record_solve(s, B, c, nb, wpc); });
+	hipGraphExec_t gs = capture(B.s, [&](hipStream_t s) { record_solve(s, B, c, t); });
 	B.graphs[k] = {gf, gs};
 	return {gf, gs};
 }
 
 // runs one staged chunk; returns the LU graph's time (ms) when timed
-double run_chunk(batch &B, const chunk &c, int nb, int wpc, double *fill_ms)
+double run_chunk(batch &B, const chunk &c, const tune_t &t, double *fill_ms)
 {
-	auto gg = graphs_for(B, c, nb, wpc);
+	auto gg = graphs_for(B, c, t);
 	hipEvent_t e0, e1, e2;
 	HIPCHK(hipEventCreate(&e0));
 	HIPCHK(hipEventCreate(&e1));
@@ -793,8 +907,10 @@ int prepare(batch &B, peec_t **models, int nm, int ifreq0, int nfreq)
 	HIPCHK(hipGetDevice(&dev));
 	HIPCHK(hipDeviceGetAttribute(&B.ncu, hipDeviceAttributeMultiprocessorCount, dev));
 	HIPCHK(hipStreamCreateWithFlags(&B.s, hipStreamNonBlocking));
-	HIPCHK(hipFuncSetAttribute((const void *)k_b_lu<16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lu_lds(BMAXN)));
-	HIPCHK(hipFuncSetAttribute((const void *)k_b_lu<32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lu_lds(BMAXN)));
+	HIPCHK(hipFuncSetAttribute((const void *)k_b_lu<16, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lu_lds(BMAXN)));
+	HIPCHK(hipFuncSetAttribute((const void *)k_b_lu<16, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lu_lds(BMAXN)));
+	HIPCHK(hipFuncSetAttribute((const void *)k_b_lu<32, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lu_lds(BMAXN)));
+	HIPCHK(hipFuncSetAttribute((const void *)k_b_lu<32, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lu_lds(BMAXN)));
 	B.nm = nm;
 	B.P.resize(nm);
 	B.G.resize(nm);
@@ -826,6 +942,11 @@ int prepare(batch &B, peec_t **models, int nm, int ifreq0, int nfreq)
 	}
 	for (int s = 0; s < nm; s++)
 		if (ok[s]) pgpu_kf_setup_finish(B.G[s]);
+	// PEEC_BATCH_DIRECT=1 fills every structure element by element (the path of structures
+	// whose records cannot be built)
+	const char *fd = getenv("PEEC_BATCH_DIRECT");
+	const bool force_direct = (fd != nullptr) && (atoi(fd) != 0);
+	int kmu = 0;
 	for (int s = 0; s < nm; s++) {
 		const peec_t *p = B.P[s];
 		pgpu_bview &v = B.views[s];
@@ -839,6 +960,7 @@ int prepare(batch &B, peec_t **models, int nm, int ifreq0, int nfreq)
 		d.offQ = p->nunknown;
 		d.T = v.T;
 		d.ntS = (p->nseg + 15) / 16;
+		d.nh = v.nh;
 		d.ntH = (v.nh + 15) / 16;
 		d.nslS = kf_slots(p->nseg);
 		d.nslH = (v.nh > 0) ? kf_slots(v.nh) : 0;
@@ -852,9 +974,16 @@ int prepare(batch &B, peec_t **models, int nm, int ifreq0, int nfreq)
 			d.i1[k] = p->nodemap[p->port[k].n1];
 			d.i2[k] = p->nodemap[p->port[k].n2];
 		}
-		d.direct = (!ok[s]) || (v.nd_lp > 0) || (d.ncell > 0 && v.nd_ig > 0) || (v.rec_lp == nullptr && p->nseg > 0) ||
-		           (d.ncell > 0 && v.rec_ig == nullptr);
+		d.direct = force_direct || (!ok[s]) || (v.rec_lp == nullptr && p->nseg > 0) || (d.ncell > 0 && v.rec_ig == nullptr);
+		d.nd_lp = v.nd_lp;
+		d.nd_ig = v.nd_ig;
+		d.list_lp = v.list_lp;
+		d.list_ig = v.list_ig;
+		d.fs = v.seg;
+		d.fc = v.chg;
+		kmu |= v.km_seg | ((d.ncell > 0) ? v.km_chg : 0);
 	}
+	B.km_class = ((kmu & ~pe::KM_WIRE) == 0) ? pe::KM_WIRE : ((kmu & ~pe::KM_RIBBON) == 0) ? pe::KM_RIBBON : 15;
 	B.dd.ensure(nm);
 	HIPCHK(hipMemcpy(B.dd.p, B.hd.data(), (size_t)nm * sizeof(bdesc), hipMemcpyHostToDevice));
 	B.dcnt.ensure(1);
@@ -873,7 +1002,7 @@ double arena_budget()
 std::vector<std::vector<std::pair<int, int>>> make_chunks(const batch &B, int ifreq0, int nfreq)
 {
 	double per_f = 0;
-	for (const bdesc &d : B.hd) per_f += 16.0 * d.N * (double)d.N;
+	for (const bdesc &d : B.hd) per_f += 16.0 * d.N * (double)d.N + ((d.ncell > 0) ? 16.0 * d.nh * (double)d.nh : 0.0);
 	int fc = (int)std::max(1.0, std::floor(arena_budget() / std::max(per_f, 1.0)));
 	fc = std::min(fc, nfreq);
 	// even chunks so every chunk but possibly none differ in shape
@@ -897,7 +1026,8 @@ void autotune(batch &B, const std::vector<std::pair<int, int>> &first)
 	const size_t stride = std::max<size_t>(1, first.size() / 512);
 	for (size_t k = 0; k < first.size(); k += stride) sample.push_back(first[k]);
 	const int nbs[2] = {16, 32};
-	const int wpcs[3] = {1, 2, 4};
+	const int cbs[2] = {2, 4};
+	const int wpcs[2] = {1, 2};
 	B.trials.clear();
 	tune_t best;
 	best.ms = 1e300;
@@ -906,16 +1036,17 @@ void autotune(batch &B, const std::vector<std::pair<int, int>> &first)
 		build_chunk(B, sample, order, c);
 		stage_chunk(B, c);
 		for (int nb : nbs)
-			for (int wpc : wpcs) {
-				const double ms = run_chunk(B, c, nb, wpc, nullptr);
-				tune_t t;
-				t.nb = nb;
-				t.wpc = wpc;
-				t.order = order;
-				t.ms = ms;
-				B.trials.push_back(t);
-				if (ms < best.ms) best = t;
-			}
+			for (int cb : cbs)
+				for (int wpc : wpcs) {
+					tune_t t;
+					t.nb = nb;
+					t.cb = cb;
+					t.wpc = wpc;
+					t.order = order;
+					t.ms = run_chunk(B, c, t, nullptr);
+					B.trials.push_back(t);
+					if (t.ms < best.ms) best = t;
+				}
 	}
 	B.tune = best;
 	B.tuned = true;
@@ -960,6 +1091,7 @@ int peec_batch_tune(peec_batch_t *h, peec_batch_stats_t *st)
 	if (st) {
 		st->tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 		st->nb = h->b.tune.nb;
+		st->tile_cols = 16 * h->b.tune.cb;
 		st->wg_per_cu = h->b.tune.wpc;
 		st->order = h->b.tune.order;
 		st->tune_trials = (int)h->b.trials.size();
@@ -984,7 +1116,7 @@ int peec_batch_sweep(peec_batch_t *h, d_complex_t **zmat, peec_batch_stats_t *st
 		stage_chunk(B, c);
 		if (st) st->stamp_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th).count();
 		double fms = 0;
-		const double lms = run_chunk(B, c, tn.nb, tn.wpc, &fms);
+		const double lms = run_chunk(B, c, tn, &fms);
 		const int nj = (int)c.J.size();
 		B.hz.ensure(c.zn);
 		B.hinfo.ensure(nj);
@@ -1009,6 +1141,7 @@ int peec_batch_sweep(peec_batch_t *h, d_complex_t **zmat, peec_batch_stats_t *st
 			st->lu_ms += lms;
 			st->chunks++;
 			st->jobs += nj;
+			st->direct_pairs += c.nlist;
 			for (const bjob &j : c.J) {
 				const double N = j.N;
 				st->lu_flop += (8.0 / 3.0) * N * N * N + 8.0 * N * N * B.hd[j.s].np;
@@ -1018,6 +1151,7 @@ int peec_batch_sweep(peec_batch_t *h, d_complex_t **zmat, peec_batch_stats_t *st
 	if (st) {
 		st->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 		st->nb = tn.nb;
+		st->tile_cols = 16 * tn.cb;
 		st->wg_per_cu = tn.wpc;
 		st->order = tn.order;
 		st->tune_trials = (int)B.trials.size();
@@ -1069,12 +1203,13 @@ int peec_batch_lu_decomp(int nmat, const int *n, d_complex_t *a, int *piv, int n
 	return -1;
 }
 
-int peec_batch_trials(const peec_batch_t *h, int *nb, int *wpc, int *order, double *ms, int cap)
+int peec_batch_trials(const peec_batch_t *h, int *nb, int *tile_cols, int *wpc, int *order, double *ms, int cap)
 {
 	if (h == nullptr) return 0;
 	const int n = (int)h->b.trials.size();
 	for (int k = 0; k < n && k < cap; k++) {
 		nb[k] = h->b.trials[k].nb;
+		tile_cols[k] = 16 * h->b.trials[k].cb;
 		wpc[k] = h->b.trials[k].wpc;
 		order[k] = h->b.trials[k].order;
 		ms[k] = h->b.trials[k].ms;

@@ -60,3 +60,26 @@ def test_layout_identical_to_reference(tmp_path):
         subprocess.run(["gcc", "-std=c11", f"-I{inc}", str(probe), "-o", str(exe), "-lm"], check=True)
         outs.append(subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout)
     assert outs[0] == outs[1]
+
+
+def _c_struct_fields(header: str, name: str) -> list[str]:
+    """Member names, in order, of `typedef struct { ... } name;` in an include/ header."""
+    text = (ROOT / "include" / header).read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    m = re.search(r"typedef struct\s*\{((?:(?!typedef struct).)*?)\}\s*" + name + r"\s*;", text, flags=re.S)
+    assert m, name
+    out = []
+    for decl in m.group(1).split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        parts = decl.split(None, 1)
+        out += [v.strip().lstrip("*") for v in parts[1].split(",")]
+    return out
+
+
+@pytest.mark.parametrize("header,name,cls", [("peec_mi355x.h", "peec_stats_t", abi.peec_stats_t),
+                                             ("peec_mi355x.h", "peec_batch_stats_t", abi.peec_batch_stats_t)])
+def test_stats_struct_fields_follow_header(header, name, cls):
+    """The ctypes mirrors of the statistics structs list the header's members in its order."""
+    assert [f[0] for f in cls._fields_] == _c_struct_fields(header, name)

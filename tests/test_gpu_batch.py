@@ -6,8 +6,8 @@ and solved by the persistent batched kernels, replayed as hipGraphs.
     (k-factored fill <= 1e-10 per entry, a different elimination order; the metric's bar is 1e-6);
   * mixed decks in one batch -- ground-plane images (two record terms), two ports, ribbons with
     several sub-cells per capacitive cell, a lumped RLC network, and a wide-band dipole whose
-    quadrature order changes inside the band (filled element by element) -- against the plain
-    FP64 sweep of each structure;
+    quadrature order changes inside the band (those pairs evaluated by the element code in the
+    batched list kernel) -- against the plain FP64 sweep of each structure;
   * a second sweep replays the captured graphs and reproduces the first bit for bit; the
     autotuner timed every candidate and kept the fastest.
 """
@@ -109,6 +109,25 @@ def test_batch_random_wires_match_oracle(lib):
         m.close()
 
 
+def test_batch_element_fill_path_matches_oracle(lib, monkeypatch):
+    """Structures whose records cannot be built are filled element by element between the two
+    graphs (forced here for every structure)."""
+    monkeypatch.setenv("PEEC_BATCH_DIRECT", "1")
+    texts = [decks.random_wire_deck(3000 + s, nmin=20, nmax=40) for s in range(3)]
+    ms = _models(lib, texts)
+    print(f"[batch] element-fill path n = {[m.p.nseg for m in ms]}", flush=True)
+    b = Batch(ms, 5, 3)
+    zs, st = b.sweep()
+    assert st.direct_structures == 3
+    for t, z in zip(texts, zs):
+        r = _rel(z, oracle_zmat(t, 5, 3))
+        print(f"  rel {r:.2e}", flush=True)
+        assert r <= 1e-8, r
+    b.close()
+    for m in ms:
+        m.close()
+
+
 def test_batch_mixed_decks_match_plain(lib):
     texts = [
         _sweep(deck_text("monopole_gp.peec"), 1e8, 1e9, 4),
@@ -123,7 +142,7 @@ def test_batch_mixed_decks_match_plain(lib):
     print(f"[batch] mixed decks nunknown = {[m.p.nunknown for m in ms]}, {nf} frequencies", flush=True)
     b = Batch(ms, 0, nf)
     zs, st = b.sweep()
-    assert st.direct_structures >= 1   # the wide-band dipole
+    assert st.direct_pairs > 0   # the wide-band dipole: quadrature order changes inside the band
     for m, z in zip(ms, zs):
         zp, _ = m.sweep(0, nf, PEEC_FORM_PLAIN)
         r = _rel(z, zp)
@@ -141,9 +160,9 @@ def test_batch_replay_and_autotune(lib):
     b = Batch(ms, 0, 8)
     st0 = b.tune()
     trials = b.trials()
-    assert len(trials) == st0.tune_trials == 12
-    best = min(trials, key=lambda t: t[3])
-    assert (st0.nb, st0.wg_per_cu, st0.order) == best[:3]
+    assert len(trials) == st0.tune_trials == 16
+    best = min(trials, key=lambda t: t[4])
+    assert (st0.nb, st0.tile_cols, st0.wg_per_cu, st0.order) == best[:4]
     z1, st1 = b.sweep()
     z2, _ = b.sweep()
     for a, c in zip(z1, z2):
