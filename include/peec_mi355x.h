@@ -20,7 +20,10 @@ extern "C" {
                                inverse factors, FGMRES to a true residual of 1e-10            */
 #define PEEC_FORM_PRECISION 2 /* FP64 fill, LU of an FP32 copy (FP64 panels and pivoting, FP32
                                MFMA trailing update), FP64 iterative refinement against the
-                               FP64 matrix to a normwise backward error of 1e-10              */
+                               FP64 matrix to a componentwise backward error of 1e-10         */
+#define PEEC_FORM_LEARNED 3 /* compressed decks: precond.c's preconditioner kept across
+                               frequencies with the learned near-field correction; GMRES true
+                               residual 1e-8 (see peec_lpc_*)                                  */
 
 /* per-stage timings of device sweeps (accumulated over points; milliseconds) */
 typedef struct {
@@ -41,6 +44,9 @@ typedef struct {
 	double mv_bytes;        /* their algorithmic HBM bytes (matrix blocks read) */
 	long   mv_launches;
 	int    fp64_fallbacks;  /* points the precision form solved in FP64 instead */
+	int    pc_builds;       /* preconditioner builds (learned form)           */
+	int    lpc_fallbacks;   /* learned solves redone with a fresh M, theta 0  */
+	double theta_abs_sum;   /* sum over points of |theta| (learned form)      */
 } peec_stats_t;
 
 /* Library identification: "libpeec_mi355x <version> gfx950". */
@@ -135,6 +141,26 @@ void peec_batch_free(peec_batch_t *b);
  * (panel width nb = 16 or 32); piv receives each matrix's interchanges, concatenated.  Returns
  * -1, the first matrix with a pivot below 1e-30, or -2 for bad arguments. */
 int  peec_batch_lu_decomp(int nmat, const int *n, d_complex_t *a, int *piv, int nb);
+
+/* Learned near-field correction of the compressed solver's preconditioner (no counterpart in
+ * the reference; precond.c's method is the fallback): P^-1 = M^-1 - theta M^-1 N M^-1 with N
+ * the near-field blocks off the block diagonal and theta from a small network (8 features ->
+ * 16 -> 16 -> 2).  The parameters are set per process; without them theta = 0. */
+int  peec_lpc_nparams(void);
+int  peec_lpc_nfeatures(void);
+void peec_lpc_set_model(const double *W);      /* NULL clears */
+int  peec_lpc_get_model(double *W);            /* 1 when set  */
+/* training samples generated by the solver on a compressed deck: for each (fref, f) pair the
+ * features X (8) and the label of the iteration-optimal theta among 0, 0.1, .., 0.6 measured by
+ * GMRES, in the network's quadratic loss form coef (4: Re p, Im p, c, d); its (may be NULL)
+ * receives the 7 iteration counts of each pair */
+int  peec_lpc_samples(peec_t *p, const double *fref, const double *f, int npairs, double *X, double *coef, int *its);
+/* the network on the device: forward (theta[2 s], theta[2 s + 1]), mean loss and gradient, and
+ * full-batch Adam from a seeded start (loss_hist: steps + 1 values) */
+int  peec_lpc_mlp_forward(const double *W, const double *X, int S, double *theta);
+int  peec_lpc_mlp_grad(const double *W, const double *X, const double *coef, int S, double *grad, double *loss);
+int  peec_lpc_fit(const double *X, const double *coef, int S, int steps, double lr, unsigned long long seed, double *W,
+                  double *loss_hist);
 
 /* Sizes for bindings */
 size_t peec_sizeof_peec(void);

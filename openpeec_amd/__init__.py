@@ -29,6 +29,7 @@ _lib = None
 PEEC_FORM_PLAIN = 0
 PEEC_FORM_MIXED = 1
 PEEC_FORM_PRECISION = 2
+PEEC_FORM_LEARNED = 3
 
 
 def load_library(build_if_missing: bool = True) -> C.CDLL:

@@ -132,7 +132,8 @@ class peec_stats_t(C.Structure):
                 ("berr", C.c_double), ("n_lu", C.c_int), ("gmres_its", C.c_long),
                 ("kf_setup_ms", C.c_double), ("kf_direct_pairs", C.c_long),
                 ("mv_ms", C.c_double), ("mv_bytes", C.c_double), ("mv_launches", C.c_long),
-                ("fp64_fallbacks", C.c_int)]
+                ("fp64_fallbacks", C.c_int), ("pc_builds", C.c_int), ("lpc_fallbacks", C.c_int),
+                ("theta_abs_sum", C.c_double)]
 
 
 class peec_batch_stats_t(C.Structure):
@@ -220,6 +221,17 @@ def declare(lib: C.CDLL, reference: bool = False) -> None:
             "peec_batch_trials": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                             C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int]),
             "peec_batch_free": (None, [C.c_void_p]),
+            "peec_lpc_nparams": (C.c_int, []),
+            "peec_lpc_nfeatures": (C.c_int, []),
+            "peec_lpc_set_model": (None, [C.POINTER(C.c_double)]),
+            "peec_lpc_get_model": (C.c_int, [C.POINTER(C.c_double)]),
+            "peec_lpc_samples": (C.c_int, [PP, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double),
+                                           C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+            "peec_lpc_mlp_forward": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double)]),
+            "peec_lpc_mlp_grad": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
+                                            C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+            "peec_lpc_fit": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int, C.c_int, C.c_double,
+                                       C.c_ulonglong, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
             "peec_batch_lu_decomp": (C.c_int, [C.c_int, C.POINTER(C.c_int), PZ, C.POINTER(C.c_int), C.c_int]),
         })
     for name, (res, args) in sig.items():

@@ -410,6 +410,59 @@ __global__ __launch_bounds__(AT) void k_hm_leaf(hleafs L, const hfar *far, const
 	}
 }
 
+// y = N x with N the near-field blocks off the block diagonal (every near block except the leaf
+// self-blocks r0 == c0, m == nn): one workgroup per leaf row range, output in cell order
+__global__ __launch_bounds__(AT) void k_hm_near_off(hleafs L, const hdense *near, const double2 *Dpool, const double2 *xp,
+                                                    const int *perm, double2 *y)
+{
+	const int r = blockIdx.x;
+	const int i0 = L.lo[r], i1 = L.hi[r];
+	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	for (int i = i0 + wave; i < i1; i += AT / 64) {
+		double sr = 0, si = 0;
+		for (int e = L.rnoff[r]; e < L.rnoff[r + 1]; e++) {
+			const hdense B = near[L.rnidx[e]];
+			if ((B.r0 == B.c0) && (B.m == B.nn)) continue;
+			const double2 *row = Dpool + B.off + (long)(i - B.r0) * B.nn;
+			for (int j = lane; j < B.nn; j += 64) {
+				const double2 a = row[j], x = xp[B.c0 + j];
+				sr += (a.x * x.x) - (a.y * x.y);
+				si += (a.x * x.y) + (a.y * x.x);
+			}
+		}
+		for (int off = 32; off > 0; off >>= 1) {
+			sr += __shfl_xor(sr, off);
+			si += __shfl_xor(si, off);
+		}
+		if (lane == 0) y[perm[i]] = make_double2(sr, si);
+	}
+}
+
+// sum of |entry|^2 of each near block (out[2 q] diagonal self-blocks, out[2 q + 1] the others)
+__global__ __launch_bounds__(256) void k_hm_near_sq(const hdense *near, int nnear, const double2 *Dpool, double *out)
+{
+	__shared__ double red[256];
+	const int q = blockIdx.x;
+	const hdense B = near[q];
+	const long cnt = (long)B.m * B.nn;
+	double acc = 0;
+	for (long e = threadIdx.x; e < cnt; e += 256) {
+		const double2 a = Dpool[B.off + e];
+		acc += (a.x * a.x) + (a.y * a.y);
+	}
+	red[threadIdx.x] = acc;
+	__syncthreads();
+	for (int w = 128; w > 0; w >>= 1) {
+		if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		const bool diag = (B.r0 == B.c0) && (B.m == B.nn);
+		out[2 * q] = diag ? red[0] : 0.0;
+		out[2 * q + 1] = diag ? 0.0 : red[0];
+	}
+}
+
 }  // namespace pe
 
 // ------------------------------------------------------------------------------------------
@@ -648,6 +701,39 @@ void hm_matvec_dev(const hmat_t *h, const double2 *x, double2 *y)
 	hipLaunchKernelGGL(pe::k_hm_leaf, dim3(h->t.nleaf), dim3(pe::AT), 0, s, h->L, (const pe::hfar *)h->d_far,
 	                   (const pe::hdense *)h->d_near, (const double2 *)h->d_U, (const double2 *)h->d_D, (const double2 *)h->d_ft,
 	                   (const double2 *)h->d_xp, (const int *)h->d_perm, y);
+}
+
+// y = N x: near-field blocks off the block diagonal (x, y in cell order, device)
+void hm_near_off_dev(const hmat_t *h, const double2 *x, double2 *y)
+{
+	hipStream_t s = h->s;
+	const int n = h->n;
+	hipLaunchKernelGGL(pe::k_hm_gather, dim3((n + 255) / 256), dim3(256), 0, s, (const int *)h->d_perm, x, h->d_xp, n);
+	hipLaunchKernelGGL(pe::k_hm_near_off, dim3(h->t.nleaf), dim3(pe::AT), 0, s, h->L, (const pe::hdense *)h->d_near,
+	                   (const double2 *)h->d_D, (const double2 *)h->d_xp, (const int *)h->d_perm, y);
+}
+
+// Frobenius norms of the near field: off the block diagonal and of the leaf self-blocks
+void hm_near_norms(const hmat_t *h, double *off, double *diag)
+{
+	const int nn = (int)h->near.size();
+	*off = *diag = 0;
+	if (nn == 0) return;
+	double *d = nullptr;
+	HIPCHK(hipMalloc((void **)&d, (size_t)2 * nn * sizeof(double)));
+	hipLaunchKernelGGL(pe::k_hm_near_sq, dim3(nn), dim3(256), 0, h->s, (const pe::hdense *)h->d_near, nn,
+	                   (const double2 *)h->d_D, d);
+	std::vector<double> hsq((size_t)2 * nn);
+	HIPCHK(hipMemcpyAsync(hsq.data(), d, hsq.size() * sizeof(double), hipMemcpyDeviceToHost, h->s));
+	HIPCHK(hipStreamSynchronize(h->s));
+	HIPCHK(hipFree(d));
+	double so = 0, sd = 0;
+	for (int q = 0; q < nn; q++) {
+		sd += hsq[2 * q];
+		so += hsq[2 * q + 1];
+	}
+	*off = std::sqrt(so);
+	*diag = std::sqrt(sd);
 }
 
 // ------------------------------------------------------------------------------------------

@@ -72,3 +72,5 @@ struct hmat_t {
 struct hmat_t *hm_build_dev(pgpu_t *g, int kind, int n, const double *pts, double kw, double tol, const char *label,
                             FILE *fp_log);
 void hm_matvec_dev(const hmat_t *h, const double2 *x, double2 *y);
+void hm_near_off_dev(const hmat_t *h, const double2 *x, double2 *y);
+void hm_near_norms(const hmat_t *h, double *off, double *diag);

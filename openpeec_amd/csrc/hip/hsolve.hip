@@ -21,12 +21,14 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <vector>
 #include "../peec_internal.h"
 #include "../../../include/peec_mi355x.h"
 #include "fill_kernels.hpp"
 #include "hip_check.hpp"
 #include "hmat_dev.hpp"
+#include "mlp.hpp"
 
 namespace {
 
@@ -851,6 +853,234 @@ precond_t *pc_build_dev(const peec_t *p, const mna_sparse_t *sp, const hblock_t 
 // ------------------------------------------------------------------------------------------
 // the reference's C interface
 
+// ------------------------------------------------------------------------------------------
+// learned near-field correction of the preconditioner
+//
+// P^-1 = M^-1 - theta M^-1 N M^-1, with M the preconditioner of precond.c (leaf blocks and the
+// circuit Schur complement, possibly built at another frequency of the sweep), N the near-field
+// blocks of the compressed kernel matrices off their block diagonal (scaled as in the system:
+// -jw for Lp, 1 for P) at the current frequency, and theta a complex weight predicted by the
+// network (mlp.hip) from features of the system.  theta = 0 is precond.c's method.  GMRES's
+// true residual |b - A x| <= tol |b| decides convergence, so theta changes iteration counts
+// but not the answer; a solve that does not converge is redone with a fresh M and theta = 0.
+
+namespace {
+
+// the network's parameters set by the caller (empty: theta = 0)
+std::vector<double> g_lpc_model;
+
+// y = N x over the kernel blocks (rows outside them 0)
+void near_apply(hop &o, const double2 *x, double2 *y)
+{
+	hipLaunchKernelGGL(k_zero, dim3(nb(o.n)), dim3(256), 0, o.s, y, (long)o.n);
+	for (int b = 0; b < o.nblk; b++) {
+		if ((o.h[b] == nullptr) || (o.bn[b] <= 0)) continue;
+		hm_near_off_dev(o.h[b], x + o.off[b], o.work);
+		hipLaunchKernelGGL(k_addscaled, dim3(nb(o.bn[b])), dim3(256), 0, o.s, y + o.off[b], (const double2 *)o.work, o.bn[b],
+		                   o.scale[b]);
+	}
+}
+
+struct lpc_ws {
+	double2 *t = nullptr, *w = nullptr;
+	int n = 0;
+	void ensure(int nn)
+	{
+		if (nn <= n) return;
+		if (t) HIPCHK(hipFree(t));
+		if (w) HIPCHK(hipFree(w));
+		t = dal<double2>(nn);
+		w = dal<double2>(nn);
+		n = nn;
+	}
+	~lpc_ws()
+	{
+		if (t) HIPCHK(hipFree(t));
+		if (w) HIPCHK(hipFree(w));
+	}
+};
+
+// z = (M^-1 - theta M^-1 N M^-1) v
+void lpc_apply(hop &o, precond_t *pc, std::complex<double> theta, lpc_ws &ws, const double2 *v, double2 *z)
+{
+	pc_apply_dev(pc, v, z);
+	if (theta == std::complex<double>(0, 0)) return;
+	near_apply(o, z, ws.t);
+	pc_apply_dev(pc, ws.t, ws.w);
+	hipLaunchKernelGGL(k_axpy, dim3(nb(o.n)), dim3(256), 0, o.s, z, (const double2 *)ws.w,
+	                   make_double2(-theta.real(), -theta.imag()), o.n);
+}
+
+// Right-preconditioned restarted GMRES(GM_M) with a fixed preconditioner, ended by the true
+// residual |b - A x| <= tol |b| checked at every restart (at most maxcyc cycles).  hist (may be
+// null) receives copies of the first cycle's Arnoldi vectors (capacity GM_M), *nhist their
+// number.  Returns the iterations, or -1 without convergence.
+int gmres_fn(hop &o, const std::function<void(const double2 *, double2 *)> &prec, const double2 *b, double2 *x, double tol,
+             int maxcyc, double2 *hist, int *nhist)
+{
+	typedef std::complex<double> zc;
+	const int n = o.n;
+	hipStream_t s = o.s;
+	double2 *V = dal<double2>((size_t)(GM_M + 1) * n);
+	double2 *Z = dal<double2>((size_t)GM_M * n);
+	double2 *w = dal<double2>(n);
+	double2 *r = dal<double2>(n);
+	const double bnorm = std::sqrt(hdot(o, b, b, n).real());
+	hipLaunchKernelGGL(k_zero, dim3(nb(n)), dim3(256), 0, s, x, (long)n);
+	if (nhist) *nhist = 0;
+	int total = 0, done = (bnorm <= 0);
+	const double tolabs = tol * bnorm;
+	for (int cyc = 0; (cyc <= maxcyc) && !done; cyc++) {
+		hop_apply(o, x, r);
+		hipLaunchKernelGGL(k_sub, dim3(nb(n)), dim3(256), 0, s, b, r, n);
+		const double beta = std::sqrt(hdot(o, r, r, n).real());
+		if (beta <= tolabs) {
+			done = 1;
+			break;
+		}
+		if (cyc == maxcyc) break;
+		hipLaunchKernelGGL(k_scal, dim3(nb(n)), dim3(256), 0, s, (const double2 *)r, V, 1.0 / beta, n);
+		std::vector<zc> H((size_t)(GM_M + 1) * GM_M), cs(GM_M), sn(GM_M), g(GM_M + 1);
+		g[0] = beta;
+		int j, k = 0;
+		for (j = 0; j < GM_M; j++) {
+			const double2 *vj = V + (long)j * n;
+			double2 *zj = Z + (long)j * n;
+			prec(vj, zj);
+			hop_apply(o, zj, w);
+			zc *Hc = &H[(size_t)j * (GM_M + 1)];
+			for (int pass = 0; pass < 2; pass++)
+				for (int i = 0; i <= j; i++) {
+					const zc hij = hdot(o, V + (long)i * n, w, n);
+					Hc[i] += hij;
+					hipLaunchKernelGGL(k_axpy, dim3(nb(n)), dim3(256), 0, s, w, V + (long)i * n, make_double2(-hij.real(), -hij.imag()), n);
+				}
+			const double hn = std::sqrt(hdot(o, w, w, n).real());
+			Hc[j + 1] = hn;
+			if (hn > 0) hipLaunchKernelGGL(k_scal, dim3(nb(n)), dim3(256), 0, s, (const double2 *)w, V + (long)(j + 1) * n, 1.0 / hn, n);
+			for (int i = 0; i < j; i++) {
+				const zc a = Hc[i], c = Hc[i + 1];
+				Hc[i] = std::conj(cs[i]) * a + std::conj(sn[i]) * c;
+				Hc[i + 1] = cs[i] * c - sn[i] * a;
+			}
+			const double rho = std::sqrt(std::norm(Hc[j]) + std::norm(Hc[j + 1]));
+			if (rho <= 0) break;
+			cs[j] = Hc[j] / rho;
+			sn[j] = Hc[j + 1] / rho;
+			Hc[j] = rho;
+			Hc[j + 1] = 0;
+			g[j + 1] = -sn[j] * g[j];
+			g[j] = std::conj(cs[j]) * g[j];
+			k = j + 1;
+			total++;
+			if ((std::abs(g[j + 1]) <= tolabs) || (hn <= 0)) break;
+		}
+		if ((cyc == 0) && hist && nhist) {
+			HIPCHK(hipMemcpyAsync(hist, V, (size_t)k * n * sizeof(double2), hipMemcpyDeviceToDevice, s));
+			*nhist = k;
+		}
+		std::vector<zc> y(k);
+		for (int i = k - 1; i >= 0; i--) {
+			zc acc = g[i];
+			for (int l = i + 1; l < k; l++) acc -= H[(size_t)l * (GM_M + 1) + i] * y[l];
+			y[i] = acc / H[(size_t)i * (GM_M + 1) + i];
+		}
+		for (int l = 0; l < k; l++)
+			hipLaunchKernelGGL(k_axpy, dim3(nb(n)), dim3(256), 0, s, x, Z + (long)l * n, make_double2(y[l].real(), y[l].imag()), n);
+	}
+	HIPCHK(hipStreamSynchronize(s));
+	HIPCHK(hipFree(V));
+	HIPCHK(hipFree(Z));
+	HIPCHK(hipFree(w));
+	HIPCHK(hipFree(r));
+	return done ? std::max(1, total) : -1;
+}
+
+// the system's compressed blocks, sparse part and operator at one frequency
+struct hsys {
+	double f = 0;
+	hmat_t *hl = nullptr, *hp = nullptr;
+	hblock_t blk[HBLK_MAX];
+	int nblk = 0;
+	mna_sparse_t sp{};
+	hop o;
+	bool ok = false;
+	void build(peec_t *p, double freq, hipStream_t s)
+	{
+		f = freq;
+		nblk = 0;
+		hl = hmat_build(p, f, nullptr);
+		if (hl == nullptr) return;
+		blk[nblk++] = hblock_t{p->offS, p->nseg, hl, d_complex(0, -2 * PI * f)};
+		if (p->capacitance && (p->ncell > 0)) {
+			hp = hmat_build_pot(p, f, nullptr);
+			if (hp == nullptr) return;
+			blk[nblk++] = hblock_t{p->offQ, p->ncell, hp, d_complex(1, 0)};
+		}
+		mna_sparse(p, f, &sp);
+		hop_init(o, p, &sp, blk, nblk, s);
+		ok = true;
+	}
+	~hsys()
+	{
+		if (ok) hop_free(o);
+		if (sp.ri) mna_sparse_free(&sp);
+		hmat_free(hl);
+		hmat_free(hp);
+	}
+};
+
+// features of the system at f with the preconditioner built at fref
+void lpc_features(const peec_t *p, const hsys &S, double fref, double *x)
+{
+	double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+	for (int i = 0; i < p->nseg; i++)
+		for (int c = 0; c < 3; c++) {
+			const double v = 0.5 * (p->seg[i].x1[c] + p->seg[i].x2[c]);
+			lo[c] = std::min(lo[c], v);
+			hi[c] = std::max(hi[c], v);
+		}
+	double D = 0;
+	for (int c = 0; c < 3; c++) D += (hi[c] - lo[c]) * (hi[c] - lo[c]);
+	D = std::sqrt(D);
+	const double kr = p->retardation ? 2 * PI / C0 : 0.0;
+	const double k = kr * S.f, k0 = kr * fref;
+	double off = 0, dg = 1, offp = 0, dgp = 1;
+	if (S.hl) hm_near_norms(S.hl, &off, &dg);
+	if (S.hp) hm_near_norms(S.hp, &offp, &dgp);
+	x[0] = k * D;
+	x[1] = k0 * D;
+	x[2] = (k - k0) * D;
+	x[3] = k * D / std::sqrt((double)std::max(1, p->nseg));
+	x[4] = (dg > 0) ? off / dg : 0;
+	x[5] = (S.hp && (dgp > 0)) ? offp / dgp : 0;
+	x[6] = std::log10((double)std::max(1, p->nunknown)) / 5;
+	x[7] = (double)(p->capacitance ? p->ncell : 0) / std::max(1, p->nunknown);
+}
+
+std::complex<double> lpc_theta(hipStream_t s, const double *x)
+{
+	// PEEC_LPC_THETA="re,im" fixes theta (comparisons against the network's choice)
+	const char *fix = getenv("PEEC_LPC_THETA");
+	if (fix != nullptr) {
+		double re = 0, im = 0;
+		if (sscanf(fix, "%lf,%lf", &re, &im) >= 1) return {re, im};
+	}
+	if (g_lpc_model.size() != (size_t)MLP_NP) return {0, 0};
+	double *dW = dal<double>(MLP_NP), *dX = dal<double>(MLP_NF), *dt = dal<double>(2);
+	HIPCHK(hipMemcpyAsync(dW, g_lpc_model.data(), MLP_NP * sizeof(double), hipMemcpyHostToDevice, s));
+	HIPCHK(hipMemcpyAsync(dX, x, MLP_NF * sizeof(double), hipMemcpyHostToDevice, s));
+	mlp_forward_dev(s, dW, dX, 1, dt);
+	double t[2];
+	HIPCHK(hipMemcpyAsync(t, dt, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+	HIPCHK(hipStreamSynchronize(s));
+	for (double *q : {dW, dX, dt}) HIPCHK(hipFree(q));
+	// the network was trained on labels in [LPC_GRID[0], LPC_GRID[last]] (real): no extrapolation
+	return {std::min(std::max(t[0], 0.0), 0.6), std::min(std::max(t[1], -0.1), 0.1)};
+}
+
+}  // namespace
+
 extern "C" {
 
 void mna_sparse(const peec_t *p, double f, mna_sparse_t *sp)
@@ -1017,6 +1247,191 @@ int pgpu_sweep_compressed(pgpu_t *g, peec_t *p, int ifreq0, int nfreq, d_complex
 		hmat_free(hl);
 		hmat_free(hp);
 	}
+	HIPCHK(hipFree(db));
+	HIPCHK(hipFree(dx));
+	return rc;
+}
+
+
+void peec_lpc_set_model(const double *W)
+{
+	if (W == nullptr) g_lpc_model.clear();
+	else g_lpc_model.assign(W, W + MLP_NP);
+}
+
+int peec_lpc_get_model(double *W)
+{
+	if (g_lpc_model.size() != (size_t)MLP_NP) return 0;
+	memcpy(W, g_lpc_model.data(), MLP_NP * sizeof(double));
+	return 1;
+}
+
+// Training samples from the solver: for each pair (fref[i], f[i]) the preconditioner of
+// precond.c is built at fref and the system at f is solved for port 1 by GMRES (true residual
+// 1e-8) with the correction at every theta of LPC_GRID; the label theta* is the one with the
+// fewest iterations (the smallest on ties).  X[8 i ..] are the features of the system at f with
+// fref; coef[4 i ..] = (-theta*, 0, 1, 1 + theta*^2) puts the sample's loss
+// (d + 2 Re(theta p) + |theta|^2 c) / d = (1 + |theta - theta*|^2) / (1 + theta*^2) in the
+// network's quadratic form; its (may be NULL) receives the iteration counts, LPC_NGRID per pair.
+// Returns 0, or 1 when a build fails.
+constexpr int LPC_NGRID = 7;
+constexpr double LPC_GRID[LPC_NGRID] = {0.0, 0.1, 0.2, 0.3, 0.4, 0.5, 0.6};
+
+int peec_lpc_samples(peec_t *p, const double *fref, const double *f, int npairs, double *X, double *coef, int *its)
+{
+	if (!p->compress) return 1;
+	pgpu_t *g = pgpu_get(p);
+	hipStream_t s = pgpu_hip_stream(g);
+	const int n = p->nunknown;
+	std::vector<d_complex_t> hb(n);
+	double2 *db = dal<double2>(n), *dx = dal<double2>(n);
+	lpc_ws ws;
+	ws.ensure(n);
+	int rc = 0;
+	for (int k = 0; (k < npairs) && !rc; k++) {
+		precond_t *pc = nullptr;
+		{
+			hsys R;
+			R.build(p, fref[k], s);
+			if (!R.ok) {
+				printf("*** learned preconditioner samples: compressed build failed at f = %.5e Hz\n", fref[k]);
+				rc = 1;
+				break;
+			}
+			pc = pc_build_dev(p, &R.sp, R.blk, R.nblk, s);
+		}
+		if (pc == nullptr) {
+			printf("*** learned preconditioner samples: preconditioner failed at f = %.5e Hz\n", fref[k]);
+			rc = 1;
+			break;
+		}
+		hsys S;
+		S.build(p, f[k], s);
+		if (!S.ok) {
+			pc_free(pc);
+			rc = 1;
+			break;
+		}
+		mna_rhs_port(p, 0, hb.data());
+		HIPCHK(hipMemcpyAsync(db, hb.data(), (size_t)n * sizeof(double2), hipMemcpyHostToDevice, s));
+		int best = -1, bestits = 0;
+		for (int q = 0; q < LPC_NGRID; q++) {
+			const std::complex<double> th(LPC_GRID[q], 0);
+			int it = gmres_fn(S.o, [&](const double2 *v, double2 *z) { lpc_apply(S.o, pc, th, ws, v, z); }, db, dx, 1e-8, GM_CYC,
+			                  nullptr, nullptr);
+			if (it < 0) it = (GM_CYC + 1) * GM_M;
+			if (its) its[(long)k * LPC_NGRID + q] = it;
+			if ((best < 0) || (it < bestits)) {
+				best = q;
+				bestits = it;
+			}
+		}
+		const double t = LPC_GRID[best];
+		lpc_features(p, S, fref[k], X + (long)MLP_NF * k);
+		coef[4 * k] = -t;
+		coef[4 * k + 1] = 0;
+		coef[4 * k + 2] = 1;
+		coef[4 * k + 3] = 1 + t * t;
+		pc_free(pc);
+	}
+	HIPCHK(hipFree(db));
+	HIPCHK(hipFree(dx));
+	return rc;
+}
+
+// Compressed sweep with the learned correction: the preconditioner is built at a reference
+// frequency and kept while GMRES needs at most max(1.5 x, +10) the iterations of the first solve
+// after the build; theta from the network (0 without a model: precond.c's method under the same
+// reuse rule); tolerance tol on the true residual; a solve that fails is redone with a fresh
+// preconditioner and theta = 0.
+int pgpu_sweep_learned(pgpu_t *g, peec_t *p, int ifreq0, int nfreq, d_complex_t *zmat, pgpu_stats_t *st, double tol)
+{
+	hipStream_t s = pgpu_hip_stream(g);
+	const int np = p->nport, n = p->nunknown;
+	std::vector<d_complex_t> hb(n);
+	std::vector<double2> hx(n);
+	double2 *db = dal<double2>(n), *dx = dal<double2>(n);
+	lpc_ws ws;
+	ws.ensure(n);
+	precond_t *pc = nullptr;
+	double fref = 0;
+	int its_fresh = -1;
+	bool rebuild = true;
+	int rc = 0;
+	for (int k = 0; (k < nfreq) && !rc; k++) {
+		const double f = freq_at(p, ifreq0 + k);
+		const auto t0 = std::chrono::steady_clock::now();
+		hsys S;
+		S.build(p, f, s);
+		if (!S.ok) {
+			rc = 5;
+			break;
+		}
+		const auto t1 = std::chrono::steady_clock::now();
+		if (rebuild || (pc == nullptr)) {
+			if (pc) pc_free(pc);
+			pc = pc_build_dev(p, &S.sp, S.blk, S.nblk, s);
+			if (pc == nullptr) {
+				printf("*** compression : preconditioner failed at f = %.5e Hz (floating node?)\n", f);
+				rc = 6;
+				break;
+			}
+			fref = f;
+			its_fresh = -1;
+			rebuild = false;
+			if (st) st->pc_builds++;
+		}
+		double x[MLP_NF];
+		lpc_features(p, S, fref, x);
+		const std::complex<double> theta = lpc_theta(s, x);
+		if (st) st->theta_abs_sum += std::abs(theta);
+		const auto t2 = std::chrono::steady_clock::now();
+		for (int j = 0; (j < np) && !rc; j++) {
+			mna_rhs_port(p, j, hb.data());
+			HIPCHK(hipMemcpyAsync(db, hb.data(), (size_t)n * sizeof(double2), hipMemcpyHostToDevice, s));
+			int it = gmres_fn(S.o, [&](const double2 *v, double2 *z) { lpc_apply(S.o, pc, theta, ws, v, z); }, db, dx, tol,
+			                  GM_CYC, nullptr, nullptr);
+			if (it < 0) {
+				// fallback: precond.c's preconditioner at this frequency, theta = 0
+				if (st) st->lpc_fallbacks++;
+				pc_free(pc);
+				pc = pc_build_dev(p, &S.sp, S.blk, S.nblk, s);
+				if (pc == nullptr) {
+					rc = 6;
+					break;
+				}
+				if (st) st->pc_builds++;
+				fref = f;
+				its_fresh = -1;
+				it = gmres_fn(S.o, [&](const double2 *v, double2 *z) { pc_apply_dev(pc, v, z); }, db, dx, tol, GM_CYC, nullptr,
+				              nullptr);
+				if (it < 0) {
+					printf("*** GMRES did not converge at f = %.5e Hz\n", f);
+					rc = 4;
+					break;
+				}
+			}
+			if (st) st->gmres_its += it;
+			if (its_fresh < 0) its_fresh = it;
+			else if (it > std::max(3 * its_fresh / 2, its_fresh + 10)) rebuild = true;
+			HIPCHK(hipMemcpyAsync(hx.data(), dx, (size_t)n * sizeof(double2), hipMemcpyDeviceToHost, s));
+			HIPCHK(hipStreamSynchronize(s));
+			for (int i = 0; i < np; i++) {
+				const int i1 = p->nodemap[p->port[i].n1], i2 = p->nodemap[p->port[i].n2];
+				const double2 v1 = (i1 < 0) ? make_double2(0, 0) : hx[i1];
+				const double2 v2 = (i2 < 0) ? make_double2(0, 0) : hx[i2];
+				zmat[((size_t)k * np + i) * np + j] = d_complex(v1.x - v2.x, v1.y - v2.y);
+			}
+		}
+		const auto t3 = std::chrono::steady_clock::now();
+		if (st) {
+			st->fill_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+			st->lu_ms += std::chrono::duration<double, std::milli>(t2 - t1).count();
+			st->solve_ms += std::chrono::duration<double, std::milli>(t3 - t2).count();
+			st->total_ms += std::chrono::duration<double, std::milli>(t3 - t0).count();
+		}
+	}
+	if (pc) pc_free(pc);
 	HIPCHK(hipFree(db));
 	HIPCHK(hipFree(dx));
 	return rc;

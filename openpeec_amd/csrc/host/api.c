@@ -369,6 +369,7 @@ int peec_sweep_ex(peec_t *p, int ifreq0, int nfreq, int form, int batch, d_compl
 		st->form = form;
 		st->ising = -1;
 	}
+	if (p->compress && (form == PEEC_FORM_LEARNED)) return pgpu_sweep_learned(pgpu_get(p), p, ifreq0, nfreq, zmat, st, 1e-8);
 	if (p->compress) return pgpu_sweep_compressed(pgpu_get(p), p, ifreq0, nfreq, zmat, st);
 	if (form == PEEC_FORM_PLAIN) return pgpu_sweep(pgpu_get(p), p, ifreq0, nfreq, zmat, NULL, NULL, st, 0);
 	if (form == PEEC_FORM_PRECISION) return pgpu_sweep(pgpu_get(p), p, ifreq0, nfreq, zmat, NULL, NULL, st, 1);

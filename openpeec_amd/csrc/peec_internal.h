@@ -138,6 +138,8 @@ void pgpu_fast_reset(void);
  * GMRES to 1e-10 per port.  Returns 0, 4 (GMRES), 5 (H matrix) or 6 (preconditioner). */
 int  pgpu_sweep_compressed(pgpu_t *g, peec_t *p, int ifreq0, int nfreq, d_complex_t *zmat,
 	pgpu_stats_t *st);
+/* compressed sweep with the learned preconditioner correction (hsolve.hip) */
+int  pgpu_sweep_learned(pgpu_t *g, peec_t *p, int ifreq0, int nfreq, d_complex_t *zmat, pgpu_stats_t *st, double tol);
 
 /* ---------------------------------------------------------------------------------------- */
 /* Dense complex LU on device buffers (lu.hip).  Matrices are row-major, leading dim lda.   */

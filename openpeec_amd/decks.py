@@ -7,6 +7,10 @@ Each function returns .peec text in the reference's deck grammar (OpenPEEC input
   plate_deck       copper plate 100 x 100 mm, 35 um, n x n grid, corner-to-corner port,
                    10 MHz - 3 GHz in 201 points (n = 64: the precision workload)
   random_wire_decks  seeded random polylines (batching workload), 20-400 segments each
+  disk_quad_deck   disk panel under a quad panel, compressed (learned-preconditioner workload:
+                   nr = 48 rings, 32 x 16 quad, 100 MHz - 4 GHz in 101 points; nsec = 32 sectors,
+                   the most a disk takes: its centre cell is a 2 nsec-gon and a cell record
+                   holds at most 64 polygon vertices, peec.h seg_t.pv)
 """
 from __future__ import annotations
 
@@ -102,3 +106,23 @@ def random_wire_deck(seed: int, nmin: int = 20, nmax: int = 400, box: float = 0.
         lines.append(f"wire = {fmt(pts[k])} {fmt(pts[k + 1])} {a!r} 5.8e7 1")
     lines += ["port = 1 2 50", f"frequency = {f0!r} {f1!r} {ndiv}", "end", ""]
     return "\n".join(lines)
+
+
+def disk_quad_deck(nr: int = 48, nsec: int = 32, qx: int = 32, qy: int = 16, radius: float = 0.05,
+                   qw: float = 0.08, qh: float = 0.04, gap: float = 0.005, f0: float = 1e8, f1: float = 4e9,
+                   ndiv: int = 100, ctol: float = 1e-6, thick: float = 3.5e-5) -> str:
+    """Disk panel of radius `radius` (nr rings x nsec sectors) at z = 0 and a qw x qh quad panel
+    (qx x qy) centred above it at z = gap, compressed (ACA tolerance ctol), retarded and
+    capacitive; the port runs from the disk centre to the quad's centre node (qx and qy even, so
+    that the centre is a grid node)."""
+    if (qx % 2) or (qy % 2):
+        raise ValueError("qx and qy must be even: the port node is the quad's centre")
+    x0, y0 = -qw / 2, -qh / 2
+    return "\n".join([
+        "OpenPEEC 1 0", f"title = disk {nr}x{nsec} under quad {qx}x{qy}",
+        "capacitance = 1", "retardation = 1", f"compression = 1 {ctol!r}",
+        "node = 1 0 0 0", f"node = 2 0 0 {gap!r}",
+        f"disk = 0 0 0 0 0 1 {radius!r} {thick!r} 5.8e7 {nr} {nsec}",
+        f"quad = {x0!r} {y0!r} {gap!r}  {-x0!r} {y0!r} {gap!r}  {-x0!r} {-y0!r} {gap!r}  {x0!r} {-y0!r} {gap!r}  "
+        f"{thick!r} 5.8e7 {qx} {qy}",
+        "port = 1 2 50", f"frequency = {f0!r} {f1!r} {ndiv}", "end", ""])

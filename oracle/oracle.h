@@ -54,4 +54,10 @@ int  orc_sweep(const peec_t *p, int ifreq0, int nfreq, d_complex_t *zmat);
  * integral matrix at wavenumber kw (no output kept) */
 void orc_fill_rows(const peec_t *p, double kw, int r0, int r1, int pot);
 
+/* the learned preconditioner's network (mlp.c; no reference counterpart): parameter count,
+ * forward (theta[2 s], theta[2 s + 1]) and mean loss with its gradient (grad may be NULL) */
+int    orc_mlp_nparams(void);
+void   orc_mlp_forward(const double *W, const double *X, int S, double *theta);
+double orc_mlp_loss_grad(const double *W, const double *X, const double *coef, int S, double *grad);
+
 #endif
