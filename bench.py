@@ -57,6 +57,9 @@ def parse():
     ap.add_argument("--no-check", action="store_true", help="skip the small-deck oracle check")
     ap.add_argument("--no-batch", action="store_true", help="skip the many-structure batching workload")
     ap.add_argument("--batch-structs", type=int, default=256, help="structures of the batching workload")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner workload")
+    ap.add_argument("--learned-points", type=int, default=101, help="frequency points of the learned workload")
+    ap.add_argument("--only", choices=["batch", "learned"], default=None, help="run one side workload alone")
     return ap.parse_args()
 
 
@@ -182,6 +185,11 @@ def precision_points(m, idx, zp):
     }
 
 
+def log(msg: str) -> None:
+    """Progress to stderr (the JSON line alone goes to stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def bench_batch(lib, nstruct: int = 256, seed0: int = 0x5EED0003, plain_sample: int = 2) -> dict:
     """The batching workload: nstruct random wire structures (20-400 segments), 64 frequencies
     each, swept together by the batched kernels; the plain FP64 form of each structure on
@@ -198,12 +206,14 @@ def bench_batch(lib, nstruct: int = 256, seed0: int = 0x5EED0003, plain_sample: 
     t0 = time.perf_counter()
     b = Batch(ms, 0, nf)
     t_create = time.perf_counter() - t0
+    log(f"batching: {nstruct} structures set up")
     st_t = b.tune()
     trials = b.trials()
     b.sweep()                      # warm: graphs captured, buffers sized
     t0 = time.perf_counter()
     zs, st = b.sweep()
     elapsed = time.perf_counter() - t0
+    log(f"batching: sweep {elapsed:.2f} s")
     jobs = st.jobs
     # plain FP64 form: every structure at plain_sample frequencies spread over its sweep
     idx = [int(round(x)) for x in np.linspace(0, nf - 1, plain_sample)]
@@ -242,6 +252,83 @@ def bench_batch(lib, nstruct: int = 256, seed0: int = 0x5EED0003, plain_sample: 
     }
 
 
+LPC_TRAIN_GEOMS = [dict(nr=8, nsec=16, qx=8, qy=4), dict(nr=10, nsec=20, qx=10, qy=6, gap=0.004),
+                   dict(nr=12, nsec=24, qx=12, qy=6, gap=0.007), dict(nr=24, nsec=32, qx=20, qy=10, gap=0.006)]
+LPC_TRAIN_PAIRS = [(fr, fr * s) for fr in (4e8, 1e9, 2e9) for s in (1.0, 1.3, 1.8, 2.5)]
+
+
+def bench_learned(lib, points: int = 101) -> dict:
+    """The learned-preconditioner workload: disk 48 x 32 under a 32 x 16 quad, compressed, swept
+    over `points` of 100 MHz - 4 GHz three ways: the reference's rule (precond.c's preconditioner
+    rebuilt at every frequency, reference GMRES termination), the same preconditioner reused
+    across frequencies without the network (theta = 0), and with the network trained on the
+    device on small geometries (training time reported separately)."""
+    import ctypes as C
+    from openpeec_amd import PEEC_FORM_LEARNED, PEEC_FORM_PLAIN, Peec, decks
+    PD = C.POINTER(C.c_double)
+    t0 = time.perf_counter()
+    Xs, Cs = [], []
+    for g in LPC_TRAIN_GEOMS:
+        m = Peec.from_deck_text(decks.disk_quad_deck(**g, f0=1e8, f1=4e9, ndiv=4), lib)
+        m.setup()
+        fr = np.array([a for a, _ in LPC_TRAIN_PAIRS])
+        fv = np.array([b for _, b in LPC_TRAIN_PAIRS])
+        X = np.zeros(len(fr) * 8)
+        coef = np.zeros(len(fr) * 4)
+        lib.peec_lpc_samples(C.byref(m.p), fr.ctypes.data_as(PD), fv.ctypes.data_as(PD), len(fr),
+                             X.ctypes.data_as(PD), coef.ctypes.data_as(PD), None)
+        Xs.append(X)
+        Cs.append(coef)
+        m.close()
+        log(f"learned: samples of {g} done")
+    t_samples = time.perf_counter() - t0
+    X = np.ascontiguousarray(np.concatenate(Xs))
+    coef = np.ascontiguousarray(np.concatenate(Cs))
+    S = len(coef) // 4
+    W = np.zeros(lib.peec_lpc_nparams())
+    hist = np.zeros(1501)
+    t0 = time.perf_counter()
+    lib.peec_lpc_fit(X.ctypes.data_as(PD), coef.ctypes.data_as(PD), S, 1500, 3e-3, 7, W.ctypes.data_as(PD),
+                     hist.ctypes.data_as(PD))
+    t_fit = time.perf_counter() - t0
+    text = decks.disk_quad_deck(ndiv=points - 1)
+    m = Peec.from_deck_text(text, lib)
+    m.setup()
+    out = {"workload": f"disk r = 50 mm, 48 rings x 32 sectors, under a 80 x 40 mm quad (32 x 16) 5 mm above, "
+                       f"compressed (ACA 1e-6), retarded + capacitive, {points} points 100 MHz-4 GHz",
+           "unknowns": m.p.nunknown, "branches": m.p.nseg, "charge_cells": m.p.ncell,
+           "training": {"samples": S, "geometries": len(LPC_TRAIN_GEOMS), "sample_s": t_samples, "fit_s": t_fit,
+                        "loss_first": float(hist[0]), "loss_last": float(hist[-1])}}
+    runs = {}
+    lib.peec_lpc_set_model(W.ctypes.data_as(PD))
+    t0 = time.perf_counter()
+    z_l, st = m.sweep(0, points, PEEC_FORM_LEARNED)
+    runs["learned"] = (time.perf_counter() - t0, st)
+    log(f"learned: sweep with the network {runs['learned'][0]:.1f} s, {st.gmres_its} iterations")
+    lib.peec_lpc_set_model(None)
+    t0 = time.perf_counter()
+    z_0, st = m.sweep(0, points, PEEC_FORM_LEARNED)
+    runs["reuse_without_network"] = (time.perf_counter() - t0, st)
+    log(f"learned: sweep without the network {runs['reuse_without_network'][0]:.1f} s, {st.gmres_its} iterations")
+    t0 = time.perf_counter()
+    z_r, st = m.sweep(0, points, PEEC_FORM_PLAIN)
+    runs["reference_rule"] = (time.perf_counter() - t0, st)
+    log(f"learned: reference-rule sweep {runs['reference_rule'][0]:.1f} s, {st.gmres_its} iterations")
+    for k, (t, st) in runs.items():
+        out[k] = {"s": t, "points_per_s": points / t, "gmres_its": int(st.gmres_its),
+                  "its_per_point": st.gmres_its / points, "preconditioner_builds": int(st.pc_builds) if k != "reference_rule" else points,
+                  "fallbacks": int(st.lpc_fallbacks), "stages_ms": {"compressed_build": st.fill_ms, "preconditioner": st.lu_ms,
+                                                                   "gmres": st.solve_ms}}
+    out["learned"]["mean_abs_theta"] = runs["learned"][1].theta_abs_sum / points
+    out["zin_max_rel_err"] = {"learned_vs_reference_rule": float(np.max(np.abs(z_l - z_r) / np.abs(z_r))),
+                              "without_vs_reference_rule": float(np.max(np.abs(z_0 - z_r) / np.abs(z_r)))}
+    out["gain_vs_without_network"] = {"iterations": runs["reuse_without_network"][1].gmres_its / max(1, runs["learned"][1].gmres_its),
+                                      "time": runs["reuse_without_network"][0] / runs["learned"][0]}
+    out["speedup_vs_reference_rule"] = runs["reference_rule"][0] / runs["learned"][0]
+    m.close()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -253,6 +340,12 @@ def main():
     lib = load_library(build_if_missing=True)
     if not lib.peec_device_available():
         raise SystemExit("no HIP device: the library has no CPU path")
+    if args.only == "batch":
+        print(json.dumps({"batching": bench_batch(lib, args.batch_structs)}))
+        return
+    if args.only == "learned":
+        print(json.dumps({"learned_preconditioner": bench_learned(lib, args.learned_points)}))
+        return
     text = decks.plate_deck(n=args.grid)
     m = Peec.from_deck_text(text, lib)
     m.setup()
@@ -361,6 +454,8 @@ def main():
         result["small_deck_check"] = small_check(lib)
     if not args.no_batch:
         result["batching"] = bench_batch(lib, args.batch_structs)
+    if not args.no_learned:
+        result["learned_preconditioner"] = bench_learned(lib, args.learned_points)
     if not args.no_cpu:
         sys.path.insert(0, str(ROOT / "tests"))
         from oracle_py import oracle_peec

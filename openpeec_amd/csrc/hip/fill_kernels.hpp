@@ -68,4 +68,5 @@ int pgpu_kf_setup_launch(pgpu_t *g, double klo, double khi);
 void pgpu_kf_setup_finish(pgpu_t *g);
 void pgpu_batch_view(pgpu_t *g, pgpu_bview *v);
 void pgpu_fill_direct_dev(pgpu_t *g, double kw, const void **lp, const void **P);
+const double *pgpu_kf_records(pgpu_t *g, int which, double kw, long *nslots, int *T);
 }

@@ -544,6 +544,17 @@ int pgpu_kf_setup(pgpu_t *g, double klo, double khi)
 	return 0;
 }
 
+// records of one cell set (0 current cells, 1 charge sub-cells) when a band covering kw is set up
+const double *pgpu_kf_records(pgpu_t *g, int which, double kw, long *nslots, int *T)
+{
+	if (!g->kf_ready || (kw < g->kf_lo * (1 - 1e-12)) || (kw > g->kf_hi * (1 + 1e-12))) return nullptr;
+	const double *r = (which == 0) ? g->kf_rec_lp : g->kf_rec_ig;
+	if (r == nullptr) return nullptr;
+	*nslots = kf_slots((which == 0) ? g->seg.n : g->chg.n);
+	*T = g->kf_T;
+	return r;
+}
+
 void pgpu_batch_view(pgpu_t *g, pgpu_bview *v)
 {
 	const peec_t *p = g->p;

@@ -24,12 +24,15 @@
 #include "fill_entries.hpp"
 #include "fill_kernels.hpp"
 #include "hip_check.hpp"
+#include "kf_eval.hpp"
 #include "hmat_dev.hpp"
 
 namespace pe {
 
 // one entry of the kernel matrix in original numbering: Lp (kind 0) or P (kind 1, summed over
 // the sub-cells of both capacitive cells, pot_entry potential.c:124-150)
+// An entry from the k-factored records of the sweep band when they are given and the pair has
+// one (one Horner per pair), else from the element code.
 template <int KM>
 __device__ cplx hm_entry(const hent &E, int i, int j)
 {
@@ -39,6 +42,14 @@ __device__ cplx hm_entry(const hent &E, int i, int j)
 			i = j;
 			j = t;
 		}
+		if (E.rec != nullptr) {
+			const long sl = kf_slot(i, j, E.f.nt);
+			if (kf_has(E.rec, E.nslots, sl)) {
+				double vr, vi;
+				kf_value(E.rec, E.nslots, sl, E.T, E.f.kw, vr, vi);
+				return mk(vr, vi);
+			}
+		}
 		return lp_entry<KM>(E.f.S, E.f.pS, E.f.SM, E.f.pSM, i, j, E.f.kw, E.f.coef, E.f.gp);
 	}
 	double sr = 0, si = 0;
@@ -47,7 +58,16 @@ __device__ cplx hm_entry(const hent &E, int i, int j)
 		for (int q = E.csoff[j]; q < E.csoff[j + 1]; q++) {
 			const int h2 = E.csidx[q];
 			const int lo = (h1 <= h2) ? h1 : h2, hi = (h1 <= h2) ? h2 : h1;
-			const cplx v = ig_entry<KM>(E.f.S, E.f.pS, E.f.SM, E.f.pSM, lo, hi, E.f.kw, E.f.gp);
+			cplx v;
+			const long sl = (E.rec != nullptr) ? kf_slot(lo, hi, E.f.nt) : 0;
+			if ((E.rec != nullptr) && kf_has(E.rec, E.nslots, sl)) {
+				double vr, vi;
+				kf_value(E.rec, E.nslots, sl, E.T, E.f.kw, vr, vi);
+				v = mk(vr, vi);
+			}
+			else {
+				v = ig_entry<KM>(E.f.S, E.f.pS, E.f.SM, E.f.pSM, lo, hi, E.f.kw, E.f.gp);
+			}
 			sr = sr + v.r;
 			si = si + v.i;
 		}
@@ -560,6 +580,7 @@ struct hmat_t *hm_build_dev(pgpu_t *g, int kind, int n, const double *pts, doubl
 	E.kind = kind;
 	E.csoff = csoff;
 	E.csidx = csidx;
+	E.rec = pgpu_kf_records(g, kind, kw, &E.nslots, &E.T);
 	h->km = km;
 	h->d_perm = up(s, std::vector<int>(t.perm, t.perm + n));
 	// far blocks and their ACA scratch

@@ -13,6 +13,9 @@ struct hent {
 	fill_args f;
 	int kind;
 	const int *csoff, *csidx;
+	const double *rec;        // k-factored records of the cell set covering f.kw, or null
+	long nslots;
+	int T;
 };
 
 // far block: row cluster [r0, r0+m), column cluster [c0, c0+nn) of the permutation; rank > 0:

@@ -866,6 +866,20 @@ precond_t *pc_build_dev(const peec_t *p, const mna_sparse_t *sp, const hblock_t 
 
 namespace {
 
+}  // namespace
+
+// k-factored records of the band [f0, f1] for the compressed builds (entries by one Horner each);
+// PEEC_KF=0 keeps the element code
+extern "C" void pgpu_kf_band(pgpu_t *g, const peec_t *p, double f0, double f1)
+{
+	const char *e = getenv("PEEC_KF");
+	if ((e && e[0] == '0') || !p->retardation) return;
+	const double kr = 2 * PI / C0;
+	pgpu_kf_setup(g, kr * std::min(f0, f1), kr * std::max(f0, f1));
+}
+
+namespace {
+
 // the network's parameters set by the caller (empty: theta = 0)
 std::vector<double> g_lpc_model;
 
@@ -1178,6 +1192,15 @@ int gmres_hmat(const peec_t *p, const mna_sparse_t *sp, const hblock_t *blk, int
 int pgpu_sweep_compressed(pgpu_t *g, peec_t *p, int ifreq0, int nfreq, d_complex_t *zmat, pgpu_stats_t *st)
 {
 	hipStream_t s = pgpu_hip_stream(g);
+	{
+		const auto t0 = std::chrono::steady_clock::now();
+		pgpu_kf_band(g, p, freq_at(p, ifreq0), freq_at(p, ifreq0 + nfreq - 1));
+		if (st) {
+			st->kf_setup_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+			st->fill_ms += st->kf_setup_ms;
+			st->total_ms += st->kf_setup_ms;
+		}
+	}
 	const int np = p->nport, n = p->nunknown;
 	std::vector<d_complex_t> hb(n);
 	double2 *db = dal<double2>(n), *dx = dal<double2>(n);
@@ -1283,6 +1306,14 @@ int peec_lpc_samples(peec_t *p, const double *fref, const double *f, int npairs,
 	pgpu_t *g = pgpu_get(p);
 	hipStream_t s = pgpu_hip_stream(g);
 	const int n = p->nunknown;
+	{
+		double lo = 1e300, hi = 0;
+		for (int k = 0; k < npairs; k++) {
+			lo = std::min(lo, std::min(fref[k], f[k]));
+			hi = std::max(hi, std::max(fref[k], f[k]));
+		}
+		if (npairs > 0) pgpu_kf_band(g, p, lo, hi);
+	}
 	std::vector<d_complex_t> hb(n);
 	double2 *db = dal<double2>(n), *dx = dal<double2>(n);
 	lpc_ws ws;
@@ -1341,12 +1372,22 @@ int peec_lpc_samples(peec_t *p, const double *fref, const double *f, int npairs,
 
 // Compressed sweep with the learned correction: the preconditioner is built at a reference
 // frequency and kept while GMRES needs at most max(1.5 x, +10) the iterations of the first solve
-// after the build; theta from the network (0 without a model: precond.c's method under the same
-// reuse rule); tolerance tol on the true residual; a solve that fails is redone with a fresh
+// after the build, unless a build costs less than half the last point's solves (then it is
+// rebuilt at every point); theta from the network (0 without a model: precond.c's method under
+// the same rules); tolerance tol on the true residual; a solve that fails is redone with a fresh
 // preconditioner and theta = 0.
 int pgpu_sweep_learned(pgpu_t *g, peec_t *p, int ifreq0, int nfreq, d_complex_t *zmat, pgpu_stats_t *st, double tol)
 {
 	hipStream_t s = pgpu_hip_stream(g);
+	{
+		const auto t0 = std::chrono::steady_clock::now();
+		pgpu_kf_band(g, p, freq_at(p, ifreq0), freq_at(p, ifreq0 + nfreq - 1));
+		if (st) {
+			st->kf_setup_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+			st->fill_ms += st->kf_setup_ms;
+			st->total_ms += st->kf_setup_ms;
+		}
+	}
 	const int np = p->nport, n = p->nunknown;
 	std::vector<d_complex_t> hb(n);
 	std::vector<double2> hx(n);
@@ -1357,6 +1398,7 @@ int pgpu_sweep_learned(pgpu_t *g, peec_t *p, int ifreq0, int nfreq, d_complex_t 
 	double fref = 0;
 	int its_fresh = -1;
 	bool rebuild = true;
+	double t_build = 0, t_solve = 0;   // ms of the last build and of the last point's solves
 	int rc = 0;
 	for (int k = 0; (k < nfreq) && !rc; k++) {
 		const double f = freq_at(p, ifreq0 + k);
@@ -1368,9 +1410,11 @@ int pgpu_sweep_learned(pgpu_t *g, peec_t *p, int ifreq0, int nfreq, d_complex_t 
 			break;
 		}
 		const auto t1 = std::chrono::steady_clock::now();
-		if (rebuild || (pc == nullptr)) {
+		if (rebuild || (pc == nullptr) || (t_build < 0.5 * t_solve)) {
 			if (pc) pc_free(pc);
+			const auto tb = std::chrono::steady_clock::now();
 			pc = pc_build_dev(p, &S.sp, S.blk, S.nblk, s);
+			t_build = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tb).count();
 			if (pc == nullptr) {
 				printf("*** compression : preconditioner failed at f = %.5e Hz (floating node?)\n", f);
 				rc = 6;
@@ -1424,10 +1468,11 @@ int pgpu_sweep_learned(pgpu_t *g, peec_t *p, int ifreq0, int nfreq, d_complex_t 
 			}
 		}
 		const auto t3 = std::chrono::steady_clock::now();
+		t_solve = std::chrono::duration<double, std::milli>(t3 - t2).count();
 		if (st) {
 			st->fill_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
 			st->lu_ms += std::chrono::duration<double, std::milli>(t2 - t1).count();
-			st->solve_ms += std::chrono::duration<double, std::milli>(t3 - t2).count();
+			st->solve_ms += t_solve;
 			st->total_ms += std::chrono::duration<double, std::milli>(t3 - t0).count();
 		}
 	}

@@ -20,6 +20,12 @@ __device__ __forceinline__ long kf_slot(int i, int j, int nt)
 	return tile * 256 + ((i & 15) << 4) + (j & 15);
 }
 
+// false for a pair the records list for direct evaluation (Rc of its first term is -1)
+__device__ __forceinline__ bool kf_has(const double *rec, long nslots, long slot)
+{
+	return rec[nslots + slot] >= 0;
+}
+
 // sum over the T terms of  base + exp(-jk Rc) sum_m (-jk)^m / m! N'_m  (Horner from the top)
 __device__ __forceinline__ void kf_value(const double *rec, long nslots, long slot, int T, double kw, double &vr,
                                          double &vi)

@@ -255,9 +255,12 @@ __device__ __forceinline__ void zero_term(const kf_args &a, long slot, int t)
 	for (int q = 0; q < KF_Q; q++) r[(long)q * a.nslots] = 0;
 }
 
+// a listed pair's records are zero (its evaluation from them gives 0, overwritten by the direct
+// value) with Rc = -1 in the first term marking it for other readers
 __device__ __forceinline__ void to_list(const kf_args &a, long slot, int i, int j)
 {
 	for (int t = 0; t < a.T; t++) zero_term(a, slot, t);
+	a.rec[slot + a.nslots] = -1.0;
 	const unsigned k = atomicAdd(a.nlist, 1u);
 	a.list[k] = make_int2(i, j);
 }

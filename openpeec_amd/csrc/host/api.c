@@ -278,6 +278,7 @@ int solve(peec_t *p, FILE *fp_log)
 		return 1;
 	}
 	pgpu_t *g = pgpu_get(p);
+	if (p->compress && (p->nfreq > 0)) pgpu_kf_band(g, p, freq_at(p, 0), freq_at(p, p->nfreq - 1));
 	for (int ifreq = 0; ifreq < p->nfreq; ifreq++) {
 		const double f = freq_at(p, ifreq);
 		if (p->compress) {

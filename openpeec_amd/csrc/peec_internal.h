@@ -109,6 +109,8 @@ void pgpu_cmat_from_host(pgpu_t *g, const d_complex_t *cmat);
  * of the band straight into device buffers.  setup returns 1 when the records exceed the
  * memory budget. */
 int  pgpu_kf_setup(pgpu_t *g, double klo, double khi);
+/* records of the frequency band [f0, f1] for the compressed builds (no-op when static or PEEC_KF=0) */
+void pgpu_kf_band(pgpu_t *g, const peec_t *p, double f0, double f1);
 void pgpu_kf_fill(pgpu_t *g, double kw, d_complex_t *dlp, d_complex_t *dP);
 int  pgpu_kf_direct_pairs(pgpu_t *g);
 void pgpu_kf_reset_all(void);     /* every device mirror prepares its records afresh */
