@@ -58,8 +58,11 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the many-structure batching workload")
     ap.add_argument("--batch-structs", type=int, default=256, help="structures of the batching workload")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner workload")
-    ap.add_argument("--learned-points", type=int, default=101, help="frequency points of the learned workload")
+    ap.add_argument("--learned-points", type=int, default=21,
+                    help="frequency points of the learned workload (21: every fifth of its 101)")
     ap.add_argument("--only", choices=["batch", "learned"], default=None, help="run one side workload alone")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (calibrates the HBM byte counters)")
     return ap.parse_args()
 
 
@@ -329,8 +332,25 @@ def bench_learned(lib, points: int = 101) -> dict:
     return out
 
 
+def copy_probe(mib: int) -> None:
+    """A device-to-device copy of known size (torch's elementwise copy kernel): its FETCH_SIZE /
+    WRITE_SIZE per dispatch against mib MiB read and written scales the counters of the path's
+    kernels to bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_mib": mib, "bytes_read": 4 * n, "bytes_written": 4 * n, "dispatches": 3}))
+
+
 def main():
     args = parse()
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU frequency sharding is "
                                    "described in DESIGN.md but not part of this build", "n_gpus": args.gpus}))
