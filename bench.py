@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--no-check", action="store_true", help="skip the small-deck oracle check")
     ap.add_argument("--no-batch", action="store_true", help="skip the many-structure batching workload")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="accelerated form only, without the plain / precision comparison points (profiling runs)")
     ap.add_argument("--batch-structs", type=int, default=256, help="structures of the batching workload")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner workload")
     ap.add_argument("--learned-points", type=int, default=21,
@@ -333,15 +335,15 @@ def bench_learned(lib, points: int = 101) -> dict:
 
 
 def copy_probe(mib: int) -> None:
-    """A device-to-device copy of known size (torch's elementwise copy kernel): its FETCH_SIZE /
+    """A copy of known size through one elementwise kernel (y = 1.0 * x): its FETCH_SIZE /
     WRITE_SIZE per dispatch against mib MiB read and written scales the counters of the path's
     kernels to bytes."""
     import torch
     n = mib * (1 << 20) // 4
-    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    x = torch.empty(n, dtype=torch.float32, device="cuda")
     y = torch.empty_like(x)
     for _ in range(3):
-        y.copy_(x)
+        torch.mul(x, 1.0, out=y)   # one elementwise kernel: reads and writes mib MiB
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_mib": mib, "bytes_read": 4 * n, "bytes_written": 4 * n, "dispatches": 3}))
 
@@ -386,6 +388,13 @@ def main():
         elapsed = time.perf_counter() - t0
         idx = sorted({0, steps // 2, steps - 1})
         zm = np.array([z[i, 0, 0] for i in idx])
+        if args.no_compare:
+            print(json.dumps({"steps": steps, "ms_per_step": 1e3 * elapsed / steps,
+                              "k_zgemm_avg_launch_ms": st.lu_gemm_ms / max(1, st.lu_gemm_launches),
+                              "k_zgemm_avg_launch_flop": st.lu_gemm_flop / max(1, st.lu_gemm_launches),
+                              "k_zgemm_launches": st.lu_gemm_launches}))
+            m.close()
+            return
         zp, t_plain, stp = plain_points(m, idx)
         prec = precision_points(m, idx, zp)
         fill_flop_top = lib.peec_fill_flops(C.byref(m.p), m.freq(idx[-1]))

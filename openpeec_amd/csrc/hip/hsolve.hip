@@ -297,6 +297,43 @@ __global__ __launch_bounds__(RD) void k_dotp(const double2 *a, const double2 *c,
 	if (t == 0) partial[blockIdx.x] = red[0];
 }
 
+// conj(V_i) . w for the nv vectors V_i = V + i n: block (b, i) sums rows b, b + MD_BLK, ... of
+// vector i into partial[i MD_BLK + b] (fixed order)
+constexpr int MD_BLK = 64;
+__global__ __launch_bounds__(RD) void k_mdot(const double2 *V, long n, int nv, const double2 *w, double2 *partial)
+{
+	__shared__ double2 red[RD];
+	const int t = threadIdx.x, i = blockIdx.y;
+	const double2 *a = V + (long)i * n;
+	double sr = 0, si = 0;
+	for (long r = (long)blockIdx.x * RD + t; r < n; r += (long)MD_BLK * RD) {
+		const double2 u = a[r], v = w[r];
+		sr += (u.x * v.x) + (u.y * v.y);
+		si += (u.x * v.y) - (u.y * v.x);
+	}
+	red[t] = make_double2(sr, si);
+	__syncthreads();
+	for (int s2 = RD / 2; s2 > 0; s2 >>= 1) {
+		if (t < s2) red[t] = make_double2(red[t].x + red[t + s2].x, red[t].y + red[t + s2].y);
+		__syncthreads();
+	}
+	if (t == 0) partial[(long)i * MD_BLK + blockIdx.x] = red[0];
+}
+
+// w -= sum_i h_i V_i
+__global__ void k_maxpy(double2 *w, const double2 *V, long n, int nv, const double2 *h)
+{
+	const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= n) return;
+	double2 acc = w[r];
+	for (int i = 0; i < nv; i++) {
+		const double2 v = V[(long)i * n + r], c = h[i];
+		acc.x -= (c.x * v.x) - (c.y * v.y);
+		acc.y -= (c.x * v.y) + (c.y * v.x);
+	}
+	w[r] = acc;
+}
+
 // w -= h v  and  x += y v  style updates
 __global__ void k_axpy(double2 *w, const double2 *v, double2 h, int n)
 {
@@ -340,6 +377,8 @@ struct hop {
 	double2 *work = nullptr;
 	double2 *partial = nullptr;
 	std::vector<double2> hp;
+	double2 *mpart = nullptr, *mh = nullptr;   // multi-vector dots: partials, coefficients
+	std::vector<double2> hmp, hmh;
 };
 
 struct pcblk {
@@ -422,11 +461,15 @@ void hop_init(hop &o, const peec_t *p, const mna_sparse_t *sp, const hblock_t *b
 	o.work = dal<double2>(maxn);
 	o.partial = dal<double2>(DOT_BLK);
 	o.hp.resize(DOT_BLK);
+	o.mpart = dal<double2>((size_t)(GM_M + 1) * MD_BLK);
+	o.mh = dal<double2>(GM_M + 1);
+	o.hmp.resize((size_t)(GM_M + 1) * MD_BLK);
+	o.hmh.resize(GM_M + 1);
 }
 
 void hop_free(hop &o)
 {
-	void *b[] = {o.rp, o.col, o.val, o.work, o.partial};
+	void *b[] = {o.rp, o.col, o.val, o.work, o.partial, o.mpart, o.mh};
 	for (void *x : b)
 		if (x) HIPCHK(hipFree(x));
 	o = hop();
@@ -444,6 +487,30 @@ std::complex<double> hdot(hop &o, const double2 *a, const double2 *c, int n)
 		si += o.hp[b].y;
 	}
 	return {sr, si};
+}
+
+// classical Gram-Schmidt with one reorthogonalisation of w against V_0 .. V_j: per pass one
+// multi-vector dot kernel, one copy each way and one multi-vector update; Hc[i] += the
+// coefficients of both passes
+void orth_cgs2(hop &o, const double2 *V, int j, double2 *w, std::complex<double> *Hc)
+{
+	const int n = o.n, nv = j + 1;
+	for (int pass = 0; pass < 2; pass++) {
+		hipLaunchKernelGGL(k_mdot, dim3(MD_BLK, nv), dim3(RD), 0, o.s, V, (long)n, nv, (const double2 *)w, o.mpart);
+		HIPCHK(hipMemcpyAsync(o.hmp.data(), o.mpart, (size_t)nv * MD_BLK * sizeof(double2), hipMemcpyDeviceToHost, o.s));
+		HIPCHK(hipStreamSynchronize(o.s));
+		for (int i = 0; i < nv; i++) {
+			double sr = 0, si = 0;
+			for (int b = 0; b < MD_BLK; b++) {
+				sr += o.hmp[(size_t)i * MD_BLK + b].x;
+				si += o.hmp[(size_t)i * MD_BLK + b].y;
+			}
+			o.hmh[i] = make_double2(sr, si);
+			Hc[i] += std::complex<double>(sr, si);
+		}
+		HIPCHK(hipMemcpyAsync(o.mh, o.hmh.data(), (size_t)nv * sizeof(double2), hipMemcpyHostToDevice, o.s));
+		hipLaunchKernelGGL(k_maxpy, dim3(nb(n)), dim3(256), 0, o.s, w, V, (long)n, nv, (const double2 *)o.mh);
+	}
 }
 
 // z = M^-1 v on the device (v, z length n; v is not modified)
@@ -518,12 +585,7 @@ int gmres_dev(hop &o, precond_t *pc, const double2 *b, double2 *x, double tol)
 			else hipLaunchKernelGGL(k_copy, dim3(nb(n)), dim3(256), 0, s, vj, zj, n);
 			hop_apply(o, zj, w);
 			zc *Hc = &H[(size_t)j * (GM_M + 1)];
-			for (int pass = 0; pass < 2; pass++)
-				for (int i = 0; i <= j; i++) {
-					const zc hij = hdot(o, V + (long)i * n, w, n);
-					Hc[i] += hij;
-					hipLaunchKernelGGL(k_axpy, dim3(nb(n)), dim3(256), 0, s, w, V + (long)i * n, make_double2(-hij.real(), -hij.imag()), n);
-				}
+			orth_cgs2(o, V, j, w, Hc);
 			const double hn = std::sqrt(hdot(o, w, w, n).real());
 			Hc[j + 1] = hn;
 			if (hn > 0) hipLaunchKernelGGL(k_scal, dim3(nb(n)), dim3(256), 0, s, (const double2 *)w, V + (long)(j + 1) * n, 1.0 / hn, n);
@@ -963,12 +1025,7 @@ int gmres_fn(hop &o, const std::function<void(const double2 *, double2 *)> &prec
 			prec(vj, zj);
 			hop_apply(o, zj, w);
 			zc *Hc = &H[(size_t)j * (GM_M + 1)];
-			for (int pass = 0; pass < 2; pass++)
-				for (int i = 0; i <= j; i++) {
-					const zc hij = hdot(o, V + (long)i * n, w, n);
-					Hc[i] += hij;
-					hipLaunchKernelGGL(k_axpy, dim3(nb(n)), dim3(256), 0, s, w, V + (long)i * n, make_double2(-hij.real(), -hij.imag()), n);
-				}
+			orth_cgs2(o, V, j, w, Hc);
 			const double hn = std::sqrt(hdot(o, w, w, n).real());
 			Hc[j + 1] = hn;
 			if (hn > 0) hipLaunchKernelGGL(k_scal, dim3(nb(n)), dim3(256), 0, s, (const double2 *)w, V + (long)(j + 1) * n, 1.0 / hn, n);
