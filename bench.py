@@ -190,6 +190,18 @@ def precision_points(m, idx, zp):
     }
 
 
+def zgemm_traffic() -> float | None:
+    """HBM bytes per LU trailing-update launch, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+    (calibrated on a copy of known size) and recorded in profiles/k_zgemm_plate64_mixed_sweep.md."""
+    f = ROOT / "profiles" / "k_zgemm_plate64_mixed_sweep.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def log(msg: str) -> None:
     """Progress to stderr (the JSON line alone goes to stdout)."""
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -436,7 +448,9 @@ def main():
             "roofline": {
                 "kernel": "k_zgemm (LU trailing update of A(f_ref), FP64 MFMA)",
                 "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_MFMA, "traffic": None,
+                "frac": achieved / PEAK_FP64_MFMA, "traffic": zgemm_traffic(),
+                "traffic_note": "HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a copy of known size, "
+                                "profiles/k_zgemm_plate64_mixed_sweep.md); algorithmic 16 (2 M N + (M + N) K) B = 3.0e9",
                 "launches": st.lu_gemm_launches, "avg_launch_ms": gemm_ms, "avg_launch_flop": gemm_flop,
             },
         })
