@@ -190,10 +190,25 @@ def precision_points(m, idx, zp):
     }
 
 
-def zgemm_traffic() -> float | None:
+def lu_roofline(achieved: float) -> dict:
+    """Roofline of the accelerated sweep's dominant kernel, the trailing update of the LU of
+    A(f_ref): FP32 MFMA (k_cgemm32) by default, FP64 MFMA (k_zgemm) with PEEC_MIXED_LU64=1."""
+    if os.environ.get("PEEC_MIXED_LU64", "0") == "1":
+        name, peak, prof = "k_zgemm (LU trailing update of A(f_ref), FP64 MFMA)", PEAK_FP64_MFMA, "k_zgemm_plate64_mixed_sweep.md"
+        alg = "16 (2 M N + (M + N) K) B"
+    else:
+        name, peak, prof = "k_cgemm32 (LU trailing update of A(f_ref), FP32 MFMA)", PEAK_FP32_MFMA, "k_cgemm32_plate64_mixed_sweep.md"
+        alg = "8 (2 M N + (M + N) K) B"
+    return {"kernel": name, "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": lu_traffic(prof),
+            "traffic_note": f"HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a copy of known size, "
+                            f"profiles/{prof}); algorithmic {alg}"}
+
+
+def lu_traffic(name: str) -> float | None:
     """HBM bytes per LU trailing-update launch, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
-    (calibrated on a copy of known size) and recorded in profiles/k_zgemm_plate64_mixed_sweep.md."""
-    f = ROOT / "profiles" / "k_zgemm_plate64_mixed_sweep.md"
+    (calibrated on a copy of known size) and recorded in the named profile."""
+    f = ROOT / "profiles" / name
     if not f.exists():
         return None
     for line in f.read_text().splitlines():
@@ -445,14 +460,9 @@ def main():
                            "frac_of_fp64_vector_peak": fill_flop_top / (plain_fill_ms * 1e-3) / 1e12 / PEAK_FP64_VECTOR,
                            "note": "element-by-element FP64 fill of the plain form at the last checked point"},
             "kfactored_fill_ms_per_point": (st.fill_ms - st.kf_setup_ms) / steps,
-            "roofline": {
-                "kernel": "k_zgemm (LU trailing update of A(f_ref), FP64 MFMA)",
-                "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_MFMA, "traffic": zgemm_traffic(),
-                "traffic_note": "HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a copy of known size, "
-                                "profiles/k_zgemm_plate64_mixed_sweep.md); algorithmic 16 (2 M N + (M + N) K) B = 3.0e9",
+            "roofline": (lu_roofline(achieved) | {
                 "launches": st.lu_gemm_launches, "avg_launch_ms": gemm_ms, "avg_launch_flop": gemm_flop,
-            },
+            }),
         })
     else:
         form = PEEC_FORM_PLAIN

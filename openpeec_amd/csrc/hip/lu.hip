@@ -284,13 +284,35 @@ __global__ __launch_bounds__(64) void k_trsm(const TS *T, long ldt, TS *X, long 
 // lane l holds row lane/16 + 4 v, col lane%16.  Complex = 4 real MFMAs into (re, im)
 // accumulators.  The fragments of step k+4 are loaded before the MFMAs of step k, so each
 // wave keeps its loads in flight behind 64 MFMAs.
+// Tile of this workgroup, XCD-aware.  Workgroups are dispatched round-robin over the 8 XCDs
+// (linear id w runs on XCD w mod 8), each with its own L2.  The linear id is remapped so that
+// every XCD takes a contiguous run of tiles, walked in groups of GT tile rows (down the rows of a
+// group first), so the tiles in flight on one XCD share the rows of A and the columns of B they
+// read.  A bijection of [0, gx gy).
+__device__ __forceinline__ void xcd_tile(int &tx, int &ty)
+{
+	constexpr int NX = 8, GT = 8;
+	const int gx = gridDim.x, gy = gridDim.y, total = gx * gy;
+	const int pid = blockIdx.y * gx + blockIdx.x;
+	const int per = (total + NX - 1) / NX, tall = total - (per - 1) * NX;
+	const int x = pid % NX, l = pid / NX;
+	const int q = (x < tall) ? (x * per + l) : (tall * per + (x - tall) * (per - 1) + l);
+	const int gsz = GT * gx;
+	const int g = q / gsz, fy = g * GT, h = (gy - fy < GT) ? (gy - fy) : GT;
+	const int r = q - g * gsz;
+	ty = fy + r % h;
+	tx = r / h;
+}
+
 __global__ __launch_bounds__(256, 2) void k_zgemm(int M, int N, int K, const double2 *__restrict__ A, long lda,
                                                 const double2 *__restrict__ B, long ldb, double2 *__restrict__ C,
                                                 long ldc)
 {
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-	const int row0 = blockIdx.y * 64 + (wave >> 1) * 32;
-	const int col0 = blockIdx.x * 128 + (wave & 1) * 64;
+	int tx, ty;
+	xcd_tile(tx, ty);
+	const int row0 = ty * 64 + (wave >> 1) * 32;
+	const int col0 = tx * 128 + (wave & 1) * 64;
 	const int li = lane & 15, lk = lane >> 4;
 	const double2 *arow[2];
 	const double2 *bcol[4];
@@ -541,7 +563,9 @@ __global__ __launch_bounds__(256) void k_cgemm32(int M, int N, int K, const floa
                                                  const float2 *__restrict__ B, long ldb, float2 *__restrict__ C, long ldc)
 {
 	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-	const int row0 = blockIdx.y * 64 + (wave >> 1) * 32, col0 = blockIdx.x * 64 + (wave & 1) * 32;
+	int tx, ty;
+	xcd_tile(tx, ty);
+	const int row0 = ty * 64 + (wave >> 1) * 32, col0 = tx * 64 + (wave & 1) * 32;
 	const int li = lane & 15, lk = lane >> 4;
 	f32x4 acr[2][2], aci[2][2];
 #pragma unroll
@@ -626,10 +650,11 @@ __device__ __forceinline__ double2 ti_inv(double2 z)
 	return make_double2(z.x / n2, -z.y / n2);
 }
 
-__global__ void k_ti_zero(double2 *X, long cnt)
+template <typename TS>
+__global__ void k_ti_zero(TS *X, long cnt)
 {
 	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (e < cnt) X[e] = make_double2(0, 0);
+	if (e < cnt) st2(X[e], make_double2(0, 0));
 }
 
 unsigned tiblocks(long n)
@@ -638,10 +663,10 @@ unsigned tiblocks(long n)
 }
 
 // Inverses of the TI x TI diagonal blocks of the packed LU factors (unit lower when LOWER,
-// else upper), FP64.  One workgroup per block; thread c solves T y = e_c in LDS.  Rows past
-// the end of a short last block act as identity.
-template <bool LOWER>
-__global__ __launch_bounds__(64) void k_diaginv(const double2 *A, long lda, int n, double2 *dinv)
+// else upper), FP64 arithmetic on either storage.  One workgroup per block; thread c solves
+// T y = e_c in LDS.  Rows past the end of a short last block act as identity.
+template <bool LOWER, typename TS>
+__global__ __launch_bounds__(64) void k_diaginv(const TS *A, long lda, int n, TS *dinv)
 {
 	__shared__ double2 T[TI * TI];
 	__shared__ double2 Y[TI * TI];
@@ -649,7 +674,7 @@ __global__ __launch_bounds__(64) void k_diaginv(const double2 *A, long lda, int 
 	const int r0 = b * TI, nb = min(TI, n - r0);
 	for (int r = 0; r < TI; r++) {
 		double2 v = make_double2((r == c) ? 1.0 : 0.0, 0);
-		if ((r < nb) && (c < nb)) v = A[(long)(r0 + r) * lda + r0 + c];
+		if ((r < nb) && (c < nb)) v = ld2(A[(long)(r0 + r) * lda + r0 + c]);
 		T[r * TI + c] = v;
 	}
 	__syncthreads();
@@ -676,21 +701,24 @@ __global__ __launch_bounds__(64) void k_diaginv(const double2 *A, long lda, int 
 		}
 	}
 	__syncthreads();
-	for (int r = 0; r < TI; r++) dinv[((long)b * TI + r) * TI + c] = Y[r * TI + c];
+	for (int r = 0; r < TI; r++) st2(dinv[((long)b * TI + r) * TI + c], Y[r * TI + c]);
 }
 
 // X[r][c] *= s over a rows x cols block (s = 0 clears, s = -1 negates)
-__global__ void k_scale_block(double2 *X, long ld, int rows, int cols, double sc)
+template <typename TS>
+__global__ void k_scale_block(TS *X, long ld, int rows, int cols, double sc)
 {
 	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (e >= (long)rows * cols) return;
 	const int r = (int)(e / cols), c = (int)(e - (long)r * cols);
-	double2 *x = &X[(long)r * ld + c];
-	*x = (sc == 0) ? make_double2(0, 0) : make_double2(sc * x->x, sc * x->y);
+	TS &x = X[(long)r * ld + c];
+	const double2 v = ld2(x);
+	st2(x, (sc == 0) ? make_double2(0, 0) : make_double2(sc * v.x, sc * v.y));
 }
 
 // Out[r][c] = D[r][c] for an n x n corner of a TI x TI block
-__global__ void k_put_block(const double2 *D, double2 *Out, long ld, int n)
+template <typename TS>
+__global__ void k_put_block(const TS *D, TS *Out, long ld, int n)
 {
 	const int e = blockIdx.x * blockDim.x + threadIdx.x;
 	if (e >= n * n) return;
@@ -698,50 +726,59 @@ __global__ void k_put_block(const double2 *D, double2 *Out, long ld, int n)
 	Out[(long)r * ld + c] = D[r * TI + c];
 }
 
-void scale_block(hipStream_t s, double2 *X, long ld, int rows, int cols, double sc)
+template <typename TS>
+void scale_block(hipStream_t s, TS *X, long ld, int rows, int cols, double sc)
 {
 	if ((rows <= 0) || (cols <= 0)) return;
-	hipLaunchKernelGGL(k_scale_block, dim3(tiblocks((long)rows * cols)), dim3(256), 0, s, X, ld, rows, cols, sc);
+	hipLaunchKernelGGL((k_scale_block<TS>), dim3(tiblocks((long)rows * cols)), dim3(256), 0, s, X, ld, rows, cols, sc);
 }
 
 // Out[i0.., i0..] (n x n, leading dimension ld, zero on entry) = inverse of the unit-lower (lower)
 // or upper part of the packed factor T over the same rows/columns.  Halves split on TI
 // boundaries; with T = [T11 0; T21 T22] the off-diagonal block of the inverse is
-// -inv(T22) T21 inv(T11) (lower) and -inv(T11) T12 inv(T22) (upper), two GEMMs on MFMA.
-// Leaves are the diagonal-block inverses.  scr holds at least ceil(n/2)^2 + TI n entries.
-void tri_inv(hipStream_t s, bool lower, const double2 *T, long ld, double2 *Out, int i0, int n, const double2 *dinv,
-             double2 *scr)
+// -inv(T22) T21 inv(T11) (lower) and -inv(T11) T12 inv(T22) (upper), two GEMMs on MFMA (FP64
+// for double2 storage, FP32 for float2).  Leaves are the diagonal-block inverses.  scr holds at
+// least ceil(n/2)^2 + TI n entries.
+template <typename TS>
+void tri_inv(hipStream_t s, bool lower, const TS *T, long ld, TS *Out, int i0, int n, const TS *dinv, TS *scr)
 {
 	if (n <= TI) {
-		hipLaunchKernelGGL(k_put_block, dim3(tiblocks((long)n * n)), dim3(256), 0, s, dinv + (long)(i0 / TI) * TI * TI,
+		hipLaunchKernelGGL((k_put_block<TS>), dim3(tiblocks((long)n * n)), dim3(256), 0, s, dinv + (long)(i0 / TI) * TI * TI,
 		                   Out + (long)i0 * ld + i0, ld, n);
 		return;
 	}
 	const int n1 = ((n / 2 + TI - 1) / TI) * TI, n2 = n - n1;
 	tri_inv(s, lower, T, ld, Out, i0, n1, dinv, scr);
 	tri_inv(s, lower, T, ld, Out, i0 + n1, n2, dinv, scr);
-	const double2 *O11 = Out + (long)i0 * ld + i0;
-	const double2 *O22 = Out + (long)(i0 + n1) * ld + (i0 + n1);
+	const TS *O11 = Out + (long)i0 * ld + i0;
+	const TS *O22 = Out + (long)(i0 + n1) * ld + (i0 + n1);
 	if (lower) {
 		// C = -T21 inv(T11) (n2 x n1); Out21 = -(inv(T22) C), then negated
 		scale_block(s, scr, n1, n2, n1, 0);
-		dlu_zgemm_sub(s, n2, n1, n1, (const d_complex_t *)(T + (long)(i0 + n1) * ld + i0), ld, (const d_complex_t *)O11, ld,
-		              (d_complex_t *)scr, n1);
-		double2 *O21 = Out + (long)(i0 + n1) * ld + i0;
-		dlu_zgemm_sub(s, n2, n1, n2, (const d_complex_t *)O22, ld, (const d_complex_t *)scr, n1, (d_complex_t *)O21, ld);
+		launch_update(s, n2, n1, n1, T + (long)(i0 + n1) * ld + i0, ld, O11, ld, scr, n1);
+		TS *O21 = Out + (long)(i0 + n1) * ld + i0;
+		launch_update(s, n2, n1, n2, O22, ld, (const TS *)scr, n1, O21, ld);
 		scale_block(s, O21, ld, n2, n1, -1);
 	}
 	else {
 		// C = -T12 inv(T22) (n1 x n2); Out12 = -(inv(T11) C), then negated
 		scale_block(s, scr, n2, n1, n2, 0);
-		dlu_zgemm_sub(s, n1, n2, n2, (const d_complex_t *)(T + (long)i0 * ld + (i0 + n1)), ld, (const d_complex_t *)O22, ld,
-		              (d_complex_t *)scr, n2);
-		double2 *O12 = Out + (long)i0 * ld + (i0 + n1);
-		dlu_zgemm_sub(s, n1, n2, n1, (const d_complex_t *)O11, ld, (const d_complex_t *)scr, n2, (d_complex_t *)O12, ld);
+		launch_update(s, n1, n2, n2, T + (long)i0 * ld + (i0 + n1), ld, O22, ld, scr, n2);
+		TS *O12 = Out + (long)i0 * ld + (i0 + n1);
+		launch_update(s, n1, n2, n1, O11, ld, (const TS *)scr, n2, O12, ld);
 		scale_block(s, O12, ld, n1, n2, -1);
 	}
 }
 
+template <typename TS>
+void tri_inverse_t(hipStream_t s, int n, const TS *LU, int lda, int lower, TS *Out, TS *scr, TS *dinv)
+{
+	const int nblk = (n + TI - 1) / TI;
+	if (lower) hipLaunchKernelGGL((k_diaginv<true, TS>), dim3(nblk), dim3(64), 0, s, LU, (long)lda, n, dinv);
+	else hipLaunchKernelGGL((k_diaginv<false, TS>), dim3(nblk), dim3(64), 0, s, LU, (long)lda, n, dinv);
+	hipLaunchKernelGGL((k_ti_zero<TS>), dim3(tiblocks((long)n * n)), dim3(256), 0, s, Out, (long)n * n);
+	tri_inv(s, lower != 0, LU, (long)lda, Out, 0, n, (const TS *)dinv, scr);
+}
 
 }  // namespace
 
@@ -932,12 +969,14 @@ extern "C" void dlu_solve_perm(void *stream, int n, const d_complex_t *dA_, int 
 extern "C" void dlu_tri_inverse(void *stream, int n, const d_complex_t *LU, int lda, int lower, d_complex_t *Out,
                                 d_complex_t *scr, d_complex_t *dinv)
 {
-	hipStream_t s = (hipStream_t)stream;
-	const int nblk = (n + TI - 1) / TI;
-	if (lower)
-		hipLaunchKernelGGL(k_diaginv<true>, dim3(nblk), dim3(64), 0, s, (const double2 *)LU, (long)lda, n, (double2 *)dinv);
-	else
-		hipLaunchKernelGGL(k_diaginv<false>, dim3(nblk), dim3(64), 0, s, (const double2 *)LU, (long)lda, n, (double2 *)dinv);
-	hipLaunchKernelGGL(k_ti_zero, dim3(tiblocks((long)n * n)), dim3(256), 0, s, (double2 *)Out, (long)n * n);
-	tri_inv(s, lower != 0, (const double2 *)LU, lda, (double2 *)Out, 0, n, (const double2 *)dinv, (double2 *)scr);
+	tri_inverse_t<double2>((hipStream_t)stream, n, (const double2 *)LU, lda, lower, (double2 *)Out, (double2 *)scr,
+	                       (double2 *)dinv);
+}
+
+// the same on FP32 storage (factors from slu_factor): diagonal blocks inverted in FP64 arithmetic,
+// off-diagonal blocks by FP32 MFMA GEMMs; Out, scr and dinv are float2 arrays
+extern "C" void slu_tri_inverse(void *stream, int n, const void *LU, int lda, int lower, void *Out, void *scr, void *dinv)
+{
+	tri_inverse_t<float2>((hipStream_t)stream, n, (const float2 *)LU, lda, lower, (float2 *)Out, (float2 *)scr,
+	                      (float2 *)dinv);
 }

@@ -746,6 +746,14 @@ void mna_stamps_charge(const peec_t *p, double f, stamps_t *st);
 // PEEC_TRACE=1 prints the iteration counts of every batch to stderr
 static const bool g_trace = getenv("PEEC_TRACE") && getenv("PEEC_TRACE")[0] == '1';
 
+// PEEC_MIXED_LU64=1 factors and inverts the preconditioner in FP64 (default: FP32 storage, FP64
+// panels, FP32 MFMA updates and inversion GEMMs)
+static bool lu64()
+{
+	const char *e = getenv("PEEC_MIXED_LU64");
+	return e && (e[0] == '1');
+}
+
 // PEEC_KF=0 in the environment switches the k-factored fill off (element-by-element fills)
 static bool pgpu_kf_enabled()
 {
@@ -862,9 +870,29 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 			if (sb.nnz > 0)
 				hipLaunchKernelGGL(k_stamp, dim3(blocks(sb.nnz)), dim3(256), 0, s, c.luw, (long)n, sb.ri, sb.ci,
 				                   (const double2 *)sb.val, sb.nnz);
-			const int info = dlu_factor(s, n, (d_complex_t *)c.luw, n, c.piv, c.perm, st);
-			if (info >= 0) return 2;
-			build_inverses(c, s);
+			if (lu64()) {
+				const int info = dlu_factor(s, n, (d_complex_t *)c.luw, n, c.piv, c.perm, st);
+				if (info >= 0) return 2;
+				build_inverses(c, s);
+			}
+			else {
+				// FP32 factors (FP64 panels, FP32 MFMA updates) inverted on FP32 MFMA: the
+				// preconditioner is stored in FP32 either way; the FP64 inverse workspace
+				// holds the FP32 copy
+				float2 *a32 = (float2 *)c.inv;
+				hipLaunchKernelGGL(k_demote, dim3(blocks((long)n * n)), dim3(256), 0, s, (const double2 *)c.luw, a32, (long)n * n);
+				int info = slu_factor(s, n, a32, n, c.piv, c.perm, st);
+				if (info >= 0) {
+					// a pivot that rounds to zero in FP32: the FP64 factorisation
+					const int info64 = dlu_factor(s, n, (d_complex_t *)c.luw, n, c.piv, c.perm, st);
+					if (info64 >= 0) return 2;
+					build_inverses(c, s);
+				}
+				else {
+					slu_tri_inverse(s, n, a32, n, 1, c.ML, c.scr, c.dinvL);
+					slu_tri_inverse(s, n, a32, n, 0, c.MU, c.scr, c.dinvU);
+				}
+			}
 			HIPCHK(hipStreamSynchronize(s));
 			c.fref = freq_at(p, ifreq0 + f0 + b);
 			if (st) {

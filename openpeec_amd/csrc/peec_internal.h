@@ -155,6 +155,8 @@ void dlu_solve_perm(void *stream, int n, const d_complex_t *dA, int lda, const i
  * leading dimension n out); dinv: ceil(n/64) 64 x 64 blocks, scr: (ceil(n/2) + 64)^2 entries */
 void dlu_tri_inverse(void *stream, int n, const d_complex_t *LU, int lda, int lower, d_complex_t *Out,
 	d_complex_t *scr, d_complex_t *dinv);
+/* the same on FP32 storage (slu_factor's factors); Out, scr, dinv float2 */
+void slu_tri_inverse(void *stream, int n, const void *LU, int lda, int lower, void *Out, void *scr, void *dinv);
 /* the same with FP32 storage (float2 pairs; FP64 panels, FP32-MFMA trailing updates), and the
  * solve of FP64 right-hand sides with those factors */
 int  slu_factor(void *stream, int n, void *dA, int lda, int *dpiv, int *dperm, pgpu_stats_t *st);

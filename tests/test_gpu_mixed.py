@@ -58,3 +58,21 @@ def test_mixed_short_first_batch(lib):
     zo = oracle_zmat(text, 7, 1)
     assert _rel(z, zo) <= 1e-6
     m.close()
+
+
+def test_mixed_fp64_preconditioner_build(lib, monkeypatch):
+    """The preconditioner factored and inverted in FP64 (PEEC_MIXED_LU64=1) instead of the
+    default FP32 storage with FP32 MFMA updates: the same Z to the form's bar, with no more
+    GMRES iterations than the FP32 build needs plus a margin."""
+    text = decks.plate_deck(n=24, ndiv=40)
+    m = Peec.from_deck_text(text, lib)
+    m.setup()
+    lib.peec_sweep_reset()
+    z32, st32 = m.sweep(0, 12, PEEC_FORM_MIXED, 4)
+    monkeypatch.setenv("PEEC_MIXED_LU64", "1")
+    lib.peec_sweep_reset()
+    z64, st64 = m.sweep(0, 12, PEEC_FORM_MIXED, 4)
+    print(f"[mixed] GMRES iterations: FP32 build {st32.gmres_its}, FP64 build {st64.gmres_its}", flush=True)
+    assert _rel(z32, z64) <= 1e-6
+    assert st32.gmres_its <= 1.25 * st64.gmres_its + 8
+    m.close()
