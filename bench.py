@@ -416,7 +416,9 @@ def main():
         idx = sorted({0, steps // 2, steps - 1})
         zm = np.array([z[i, 0, 0] for i in idx])
         if args.no_compare:
-            print(json.dumps({"steps": steps, "ms_per_step": 1e3 * elapsed / steps,
+            print(json.dumps({"steps": steps, "ms_per_step": 1e3 * elapsed / steps, "value": steps / elapsed,
+                              "stages_ms": {"fill": st.fill_ms, "lu_and_inverse": st.lu_ms, "gmres": st.solve_ms},
+                              "gmres_its": st.gmres_its, "n_lu": st.n_lu,
                               "k_zgemm_avg_launch_ms": st.lu_gemm_ms / max(1, st.lu_gemm_launches),
                               "k_zgemm_avg_launch_flop": st.lu_gemm_flop / max(1, st.lu_gemm_launches),
                               "k_zgemm_launches": st.lu_gemm_launches}))

@@ -367,60 +367,76 @@ __global__ void k_permute(const double2 *src, double2 *dst, const int *perm, int
 // chunk of X is staged in LDS once and reused by the 16 rows; a lane takes 4 columns of the
 // chunk, so 16 matrix loads per lane are in flight between two barriers.
 constexpr int TG_CH = 256;
-template <bool LOWER>
+// Y = T X for a triangular FP32-stored T (lower: columns <= the row's block; upper: from the
+// row's 256-column block on; entries outside the triangle are stored zeros) and F FP64 columns
+// X (n x F, row-major).  A workgroup of 4 waves takes 4 RW consecutive rows; the X chunk of 256
+// rows is staged in LDS by column (xs[f][c], conflict-free), each lane covers columns lane +
+// 64 u (u < 4) of RW rows, and the next chunk's matrix entries are loaded before the current
+// chunk's products.  FP64 accumulation.
+template <bool LOWER, int F, int RW>
 __global__ __launch_bounds__(256) void k_tgemv(const float2 *__restrict__ Mt, long ld, int n, const double2 *__restrict__ X,
-                                               double2 *__restrict__ Y, int F)
+                                               double2 *__restrict__ Y)
 {
-	__shared__ double2 xs[TG_CH * FMAX];
-	const int r0 = blockIdx.x * 16;
+	__shared__ double2 xs[F * TG_CH];
 	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-	const int cbeg = LOWER ? 0 : ((r0 / TG_CH) * TG_CH);
-	const int cend = LOWER ? min(n, r0 + 16) : n;
-	double ar[4][FMAX], ai[4][FMAX];
+	const int r0 = blockIdx.x * (4 * RW) + wave * RW;
+	const int wr0 = blockIdx.x * (4 * RW);
+	const int cbeg = LOWER ? 0 : ((wr0 / TG_CH) * TG_CH);
+	const int cend = LOWER ? min(n, wr0 + 4 * RW) : n;
+	double ar[RW][F], ai[RW][F];
 #pragma unroll
-	for (int q = 0; q < 4; q++)
+	for (int q = 0; q < RW; q++)
 #pragma unroll
-		for (int f = 0; f < FMAX; f++) ar[q][f] = ai[q][f] = 0;
-	for (int c0 = cbeg; c0 < cend; c0 += TG_CH) {
+		for (int f = 0; f < F; f++) ar[q][f] = ai[q][f] = 0;
+	auto load = [&](int c0, float2 (&m)[RW][4]) {
 		const int nc = min(TG_CH, n - c0);
-		__syncthreads();
-		for (int e = threadIdx.x; e < nc * F; e += 256) xs[e] = X[(long)c0 * F + e];
-		__syncthreads();
-		float2 m[4][4];
 #pragma unroll
-		for (int q = 0; q < 4; q++) {
-			const int row = r0 + wave * 4 + q;
+		for (int q = 0; q < RW; q++) {
+			const int row = r0 + q;
 #pragma unroll
 			for (int u = 0; u < 4; u++) {
 				const int c = lane + 64 * u;
 				m[q][u] = ((row < n) && (c < nc)) ? Mt[(long)row * ld + c0 + c] : make_float2(0, 0);
 			}
 		}
+	};
+	float2 m[RW][4];
+	if (cbeg < cend) load(cbeg, m);
+	for (int c0 = cbeg; c0 < cend; c0 += TG_CH) {
+		const int nc = min(TG_CH, n - c0);
+		__syncthreads();
+		for (int e = threadIdx.x; e < nc * F; e += 256) {
+			const int c = e / F, f = e - c * F;
+			xs[f * TG_CH + c] = X[(long)c0 * F + e];
+		}
+		__syncthreads();
+		float2 mn[RW][4];
+		if (c0 + TG_CH < cend) load(c0 + TG_CH, mn);
 #pragma unroll
 		for (int u = 0; u < 4; u++) {
 			const int c = lane + 64 * u;
 			if (c >= nc) continue;
-			const double2 *xc = &xs[c * F];
 #pragma unroll
-			for (int f = 0; f < FMAX; f++) {
-				if (f < F) {
-					const double2 x = xc[f];
+			for (int f = 0; f < F; f++) {
+				const double2 x = xs[f * TG_CH + c];
 #pragma unroll
-					for (int q = 0; q < 4; q++) {
-						const double mr = m[q][u].x, mi = m[q][u].y;
-						ar[q][f] += (mr * x.x) - (mi * x.y);
-						ai[q][f] += (mr * x.y) + (mi * x.x);
-					}
+				for (int q = 0; q < RW; q++) {
+					const double mr = m[q][u].x, mi = m[q][u].y;
+					ar[q][f] += (mr * x.x) - (mi * x.y);
+					ai[q][f] += (mr * x.y) + (mi * x.x);
 				}
 			}
 		}
+#pragma unroll
+		for (int q = 0; q < RW; q++)
+#pragma unroll
+			for (int u = 0; u < 4; u++) m[q][u] = mn[q][u];
 	}
 #pragma unroll
-	for (int q = 0; q < 4; q++) {
-		const int row = r0 + wave * 4 + q;
+	for (int q = 0; q < RW; q++) {
+		const int row = r0 + q;
 #pragma unroll
-		for (int f = 0; f < FMAX; f++) {
-			if (f >= F) continue;
+		for (int f = 0; f < F; f++) {
 			double sr = ar[q][f], si = ai[q][f];
 			for (int off = 32; off > 0; off >>= 1) {
 				sr += __shfl_xor(sr, off);
@@ -428,6 +444,39 @@ __global__ __launch_bounds__(256) void k_tgemv(const float2 *__restrict__ Mt, lo
 			}
 			if ((lane == 0) && (row < n)) Y[(long)row * F + f] = make_double2(sr, si);
 		}
+	}
+}
+
+// rows per wave of the triangular products (PEEC_TG_RW = 4 or 8; 4 by default)
+int tg_rw()
+{
+	static int rw = -1;
+	if (rw < 0) {
+		const char *e = getenv("PEEC_TG_RW");
+		rw = (e && (atoi(e) == 8)) ? 8 : 4;
+	}
+	return rw;
+}
+
+template <bool LOWER, int F>
+void tgemv_f(hipStream_t s, const float2 *Mt, int n, const double2 *X, double2 *Y)
+{
+	if (tg_rw() == 8) hipLaunchKernelGGL((k_tgemv<LOWER, F, 8>), dim3((n + 31) / 32), dim3(256), 0, s, Mt, (long)n, n, X, Y);
+	else hipLaunchKernelGGL((k_tgemv<LOWER, F, 4>), dim3((n + 15) / 16), dim3(256), 0, s, Mt, (long)n, n, X, Y);
+}
+
+template <bool LOWER>
+void tgemv(hipStream_t s, const float2 *Mt, int n, const double2 *X, double2 *Y, int F)
+{
+	switch (F) {
+	case 1: tgemv_f<LOWER, 1>(s, Mt, n, X, Y); break;
+	case 2: tgemv_f<LOWER, 2>(s, Mt, n, X, Y); break;
+	case 3: tgemv_f<LOWER, 3>(s, Mt, n, X, Y); break;
+	case 4: tgemv_f<LOWER, 4>(s, Mt, n, X, Y); break;
+	case 5: tgemv_f<LOWER, 5>(s, Mt, n, X, Y); break;
+	case 6: tgemv_f<LOWER, 6>(s, Mt, n, X, Y); break;
+	case 7: tgemv_f<LOWER, 7>(s, Mt, n, X, Y); break;
+	default: tgemv_f<LOWER, 8>(s, Mt, n, X, Y); break;
 	}
 }
 
@@ -490,9 +539,8 @@ void build_inverses(fast &c, hipStream_t s)
 void precond(fast &c, hipStream_t s, const double2 *v, double2 *z, int F)
 {
 	hipLaunchKernelGGL(k_permute, dim3(blocks((long)c.n * F)), dim3(256), 0, s, v, z, c.perm, c.n, F);
-	const unsigned g = (unsigned)((c.n + 15) / 16);
-	hipLaunchKernelGGL(k_tgemv<true>, dim3(g), dim3(256), 0, s, (const float2 *)c.ML, (long)c.n, c.n, (const double2 *)z, c.T1, F);
-	hipLaunchKernelGGL(k_tgemv<false>, dim3(g), dim3(256), 0, s, (const float2 *)c.MU, (long)c.n, c.n, (const double2 *)c.T1, z, F);
+	tgemv<true>(s, (const float2 *)c.ML, c.n, (const double2 *)z, c.T1, F);
+	tgemv<false>(s, (const float2 *)c.MU, c.n, (const double2 *)c.T1, z, F);
 }
 
 int g_offS = 0, g_offQ = 0;
