@@ -247,6 +247,16 @@ def bench_batch(lib, nstruct: int = 256, seed0: int = 0x5EED0003, plain_sample: 
     elapsed = time.perf_counter() - t0
     log(f"batching: sweep {elapsed:.2f} s")
     jobs = st.jobs
+    # the precision form of the batched factorisation on the same jobs (tuned for it first)
+    from openpeec_amd import PEEC_FORM_PRECISION
+    b.set_form(PEEC_FORM_PRECISION)
+    b.tune()
+    b.sweep()
+    t0 = time.perf_counter()
+    zq, stq = b.sweep()
+    t_prec = time.perf_counter() - t0
+    err_q = max(float(np.max(np.abs(a - c) / np.abs(c))) for a, c in zip(zq, zs))
+    log(f"batching: precision form {t_prec:.2f} s")
     # plain FP64 form: every structure at plain_sample frequencies spread over its sweep
     idx = [int(round(x)) for x in np.linspace(0, nf - 1, plain_sample)]
     t0 = time.perf_counter()
@@ -276,6 +286,11 @@ def bench_batch(lib, nstruct: int = 256, seed0: int = 0x5EED0003, plain_sample: 
         "lu": {"TFLOP_per_s": lu_tf, "frac_of_fp64_mfma_peak": (lu_tf / PEAK_FP64_MFMA) if lu_tf else None,
                "flop": st.lu_flop},
         "chunks": st.chunks, "direct_pair_evaluations": st.direct_pairs,
+        "precision_form": {"points_per_s": jobs / t_prec, "ms": 1e3 * t_prec, "lu_solve_graphs_ms": stq.lu_ms,
+                           "zin_max_rel_err_vs_fp64_batched": err_q, "max_backward_error_accepted": stq.berr_max,
+                           "refinement_sweeps_per_port": stq.refine_sweeps / max(1, jobs),
+                           "fp64_redos": stq.fp64_fallbacks, "tuned_panel_width": stq.nb,
+                           "tuned_tile_cols": stq.tile_cols, "tuned_wg_per_cu": stq.wg_per_cu},
         "tuned": {"panel_width": st.nb, "tile_cols": st.tile_cols, "wg_per_cu": st.wg_per_cu,
                   "order": "largest first" if st.order == 0 else "interleaved",
                   "trials": [{"nb": t[0], "tile_cols": t[1], "wpc": t[2], "order": t[3], "lu_ms": t[4]} for t in trials],

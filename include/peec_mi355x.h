@@ -127,9 +127,18 @@ typedef struct {
 	int    tile_cols;       /* trailing-update tile width (32 rows x tile_cols) */
 	int    direct_structures; /* filled element by element (no usable records) */
 	long   direct_pairs;    /* pair evaluations by the element code (no convergent expansion) */
+	long   refine_sweeps;   /* precision form: refinement sweeps over all jobs and ports */
+	double berr_max;        /* precision form: worst componentwise backward error of the
+	                           port solutions accepted from the FP32 factors            */
+	int    fp64_fallbacks;  /* precision form: jobs factored again in FP64             */
 } peec_batch_stats_t;
 peec_batch_t *peec_batch_create(peec_t **models, int nmodel, int ifreq0, int nfreq);
 int  peec_batch_tune(peec_batch_t *b, peec_batch_stats_t *st);
+/* PEEC_FORM_PLAIN (default): FP64 factorisation; PEEC_FORM_PRECISION: FP32-stored factorisation
+ * (FP64 panels, FP32 MFMA updates) with every port solution refined in FP64 against the FP64
+ * system to a componentwise backward error of 1e-10 (FP64 factorisation for a job that does not
+ * get there).  Returns 0, or 4 for another form. */
+int  peec_batch_set_form(peec_batch_t *b, int form);
 /* zmat[s] receives nfreq x nport x nport of structure s; returns 0, 2 (a singular system) or 4 */
 int  peec_batch_sweep(peec_batch_t *b, d_complex_t **zmat, peec_batch_stats_t *st);
 /* the autotuner's candidates (panel width, tile width, workgroups per CU, order) and their LU

@@ -198,6 +198,11 @@ class Batch:
         if not self.h:
             raise RuntimeError("peec_batch_create rejected the structures")
 
+    def set_form(self, form: int) -> None:
+        """PEEC_FORM_PLAIN (FP64 factorisation) or PEEC_FORM_PRECISION (FP32 + FP64 refinement)."""
+        if self.lib.peec_batch_set_form(self.h, form) != 0:
+            raise ValueError("batched sweep: form must be plain or precision")
+
     def tune(self) -> abi.peec_batch_stats_t:
         st = abi.peec_batch_stats_t()
         if self.lib.peec_batch_tune(self.h, C.byref(st)) != 0:

@@ -141,7 +141,8 @@ class peec_batch_stats_t(C.Structure):
                 ("lu_ms", C.c_double), ("stamp_ms", C.c_double), ("lu_flop", C.c_double),
                 ("jobs", C.c_long), ("chunks", C.c_int), ("nb", C.c_int), ("wg_per_cu", C.c_int),
                 ("order", C.c_int), ("tune_trials", C.c_int), ("tile_cols", C.c_int),
-                ("direct_structures", C.c_int), ("direct_pairs", C.c_long)]
+                ("direct_structures", C.c_int), ("direct_pairs", C.c_long), ("refine_sweeps", C.c_long),
+                ("berr_max", C.c_double), ("fp64_fallbacks", C.c_int)]
 
 
 def declare(lib: C.CDLL, reference: bool = False) -> None:
@@ -221,6 +222,7 @@ def declare(lib: C.CDLL, reference: bool = False) -> None:
             "peec_batch_trials": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                             C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int]),
             "peec_batch_free": (None, [C.c_void_p]),
+            "peec_batch_set_form": (C.c_int, [C.c_void_p, C.c_int]),
             "peec_lpc_nparams": (C.c_int, []),
             "peec_lpc_nfeatures": (C.c_int, []),
             "peec_lpc_set_model": (None, [C.POINTER(C.c_double)]),

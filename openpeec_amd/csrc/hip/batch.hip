@@ -394,11 +394,88 @@ __device__ __forceinline__ void trailing_update(double2 *A, long ld, int k0, int
 	}
 }
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ double2 ld2(const double2 &v) { return v; }
+__device__ __forceinline__ double2 ld2(const float2 &v) { return make_double2(v.x, v.y); }
+__device__ __forceinline__ void st2(double2 &d, double2 v) { d = v; }
+__device__ __forceinline__ void st2(float2 &d, double2 v) { d = make_float2((float)v.x, (float)v.y); }
+
+// The same update on FP32 storage: FP32 MFMA 16x16x4 products accumulated in FP32 (the
+// accumulator holds rows 4 (lane / 16) + v of the block)
+template <int NB, int CB>
+__device__ __forceinline__ void trailing_update(float2 *A, long ld, int k0, int nb, int m)
+{
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int li = lane & 15, lk = lane >> 4;
+	const int tr = (m + 31) / 32, tc = (m + 16 * CB - 1) / (16 * CB);
+	const float2 *L21 = A + (long)(k0 + nb) * ld + k0;
+	const float2 *U12 = A + (long)k0 * ld + k0 + nb;
+	float2 *C = A + (long)(k0 + nb) * ld + k0 + nb;
+#pragma unroll 1
+	for (int t = wave; t < tr * tc; t += 4) {
+		const int r0 = (t / tc) * 32, c0 = (t % tc) * (16 * CB);
+		f32x4 ar[2][CB], ai[2][CB];
+#pragma unroll
+		for (int a = 0; a < 2; a++)
+#pragma unroll
+			for (int b = 0; b < CB; b++) {
+				ar[a][b] = f32x4{0, 0, 0, 0};
+				ai[a][b] = f32x4{0, 0, 0, 0};
+			}
+		const float2 *arow[2];
+		const float2 *bcol[CB];
+		bool aok[2], bok[CB];
+#pragma unroll
+		for (int rb = 0; rb < 2; rb++) {
+			const int r = r0 + rb * 16 + li;
+			aok[rb] = r < m;
+			arow[rb] = L21 + (long)(aok[rb] ? r : 0) * ld;
+		}
+#pragma unroll
+		for (int cb = 0; cb < CB; cb++) {
+			const int c = c0 + cb * 16 + li;
+			bok[cb] = c < m;
+			bcol[cb] = U12 + (bok[cb] ? c : 0);
+		}
+#pragma unroll 1
+		for (int k = 0; k < nb; k += 4) {
+			const int kk = k + lk;
+			float2 fa[2], fb[CB];
+#pragma unroll
+			for (int rb = 0; rb < 2; rb++) fa[rb] = (aok[rb] && (kk < nb)) ? arow[rb][kk] : make_float2(0, 0);
+#pragma unroll
+			for (int cb = 0; cb < CB; cb++) fb[cb] = (bok[cb] && (kk < nb)) ? bcol[cb][(long)kk * ld] : make_float2(0, 0);
+#pragma unroll
+			for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+				for (int cb = 0; cb < CB; cb++) {
+					ar[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[rb].x, fb[cb].x, ar[rb][cb], 0, 0, 0);
+					ar[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(-fa[rb].y, fb[cb].y, ar[rb][cb], 0, 0, 0);
+					ai[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[rb].x, fb[cb].y, ai[rb][cb], 0, 0, 0);
+					ai[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[rb].y, fb[cb].x, ai[rb][cb], 0, 0, 0);
+				}
+		}
+#pragma unroll
+		for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+			for (int cb = 0; cb < CB; cb++)
+#pragma unroll
+				for (int v = 0; v < 4; v++) {
+					const int r = r0 + rb * 16 + 4 * lk + v, c = c0 + cb * 16 + li;
+					if ((r < m) && (c < m)) {
+						float2 &x = C[(long)r * ld + c];
+						x = make_float2(x.x - ar[rb][cb][v], x.y - ai[rb][cb][v]);
+					}
+				}
+	}
+}
+
 // in-place LU of the N x N row-major system A: right-looking, panel width NB, row pivoting by
 // the first maximum of |re| + |im| (lu.c:19-30), full-row interchanges recorded in piv (any
 // address space); returns -1, or the column whose pivot is below 1e-30.  All 256 threads call.
-template <int NB, int CB>
-__device__ __forceinline__ int blu_factor(double2 *A, int N, int *piv, double2 *s_row, double2 *s_L, double *s_rv, int *s_ri)
+template <int NB, int CB, typename TS>
+__device__ __forceinline__ int blu_factor(TS *A, int N, int *piv, double2 *s_row, double2 *s_L, double *s_rv, int *s_ri)
 {
 	const int tid = threadIdx.x;
 	int info = -1;
@@ -409,7 +486,7 @@ __device__ __forceinline__ int blu_factor(double2 *A, int N, int *piv, double2 *
 			double best = -1;
 			int bi = N;
 			for (int r = c + tid; r < N; r += 256) {
-				const double2 v = A[(long)r * N + c];
+				const double2 v = ld2(A[(long)r * N + c]);
 				const double a = fabs(v.x) + fabs(v.y);
 				if (a > best) {
 					best = a;
@@ -424,28 +501,28 @@ __device__ __forceinline__ int blu_factor(double2 *A, int N, int *piv, double2 *
 			if (tid == 0) piv[c] = bi;
 			if (bi != c)
 				for (int col = tid; col < N; col += 256) {
-					const double2 t = A[(long)c * N + col];
+					const TS t = A[(long)c * N + col];
 					A[(long)c * N + col] = A[(long)bi * N + col];
 					A[(long)bi * N + col] = t;
 				}
 			__syncthreads();
-			const double2 inv = crecip(A[(long)c * N + c]);
-			for (int r = c + 1 + tid; r < N; r += 256) A[(long)r * N + c] = cmul(A[(long)r * N + c], inv);
+			const double2 inv = crecip(ld2(A[(long)c * N + c]));
+			for (int r = c + 1 + tid; r < N; r += 256) st2(A[(long)r * N + c], cmul(ld2(A[(long)r * N + c]), inv));
 			const int w = kend - c - 1;
-			if (tid < w) s_row[tid] = A[(long)c * N + c + 1 + tid];
+			if (tid < w) s_row[tid] = ld2(A[(long)c * N + c + 1 + tid]);
 			__syncthreads();
 			if (w > 0) {
 				// rank-1 update of the panel's remaining columns, every row below c
 				for (int r = c + 1 + tid; r < N; r += 256) {
-					double2 *ar = A + (long)r * N;
-					const double2 l = ar[c];
+					TS *ar = A + (long)r * N;
+					const double2 l = ld2(ar[c]);
 #pragma unroll 4
 					for (int t = 0; t < w; t++) {
 						const double2 u = s_row[t];
-						double2 x = ar[c + 1 + t];
+						double2 x = ld2(ar[c + 1 + t]);
 						x.x -= (l.x * u.x) - (l.y * u.y);
 						x.y -= (l.x * u.y) + (l.y * u.x);
-						ar[c + 1 + t] = x;
+						st2(ar[c + 1 + t], x);
 					}
 				}
 			}
@@ -458,20 +535,20 @@ __device__ __forceinline__ int blu_factor(double2 *A, int N, int *piv, double2 *
 		// columns, eliminating down the panel rows (coalesced across threads, row by row)
 		for (int e = tid; e < nb * nb; e += 256) {
 			const int r = e / nb, cc = e - r * nb;
-			s_L[r * NB + cc] = A[(long)(k0 + r) * N + k0 + cc];
+			s_L[r * NB + cc] = ld2(A[(long)(k0 + r) * N + k0 + cc]);
 		}
 		__syncthreads();
 		for (int col = kend + tid; col < N; col += 256)
 #pragma unroll 1
 			for (int c = 0; c < nb - 1; c++) {
-				const double2 u = A[(long)(k0 + c) * N + col];
+				const double2 u = ld2(A[(long)(k0 + c) * N + col]);
 #pragma unroll 4
 				for (int r = c + 1; r < nb; r++) {
 					const double2 l = s_L[r * NB + c];
-					double2 x = A[(long)(k0 + r) * N + col];
+					double2 x = ld2(A[(long)(k0 + r) * N + col]);
 					x.x -= (l.x * u.x) - (l.y * u.y);
 					x.y -= (l.x * u.y) + (l.y * u.x);
-					A[(long)(k0 + r) * N + col] = x;
+					st2(A[(long)(k0 + r) * N + col], x);
 				}
 			}
 		__syncthreads();
@@ -481,9 +558,57 @@ __device__ __forceinline__ int blu_factor(double2 *A, int N, int *piv, double2 *
 	return info;
 }
 
-template <int NB, int CB>
+// v <- U^-1 L^-1 P v for the packed factors of one job (v in LDS or global memory), all threads
+template <typename TS>
+__device__ void lu_subst(const TS *A, int N, const int *piv, double2 *v)
+{
+	const int tid = threadIdx.x;
+	if (tid == 0)
+		for (int c = 0; c < N; c++) {
+			const int pr = piv[c];
+			if (pr != c) {
+				const double2 t = v[c];
+				v[c] = v[pr];
+				v[pr] = t;
+			}
+		}
+	__syncthreads();
+	for (int c = 0; c < N - 1; c++) {
+		const double2 xc = v[c];
+		if ((xc.x != 0) || (xc.y != 0))
+			for (int r = c + 1 + tid; r < N; r += 256) {
+				const double2 l = ld2(A[(long)r * N + c]);
+				v[r].x -= (l.x * xc.x) - (l.y * xc.y);
+				v[r].y -= (l.x * xc.y) + (l.y * xc.x);
+			}
+		__syncthreads();
+	}
+	for (int c = N - 1; c >= 0; c--) {
+		if (tid == 0) v[c] = cmul(v[c], crecip(ld2(A[(long)c * N + c])));
+		__syncthreads();
+		const double2 xc = v[c];
+		for (int r = tid; r < c; r += 256) {
+			const double2 u = ld2(A[(long)r * N + c]);
+			v[r].x -= (u.x * xc.x) - (u.y * xc.y);
+			v[r].y -= (u.x * xc.y) + (u.y * xc.x);
+		}
+		__syncthreads();
+	}
+}
+
+constexpr int BRF_MAX = 10;   // refinement sweeps of the FP32 batched factorisation
+
+// persistent: factor and solve jobs from the queue until it is empty.  Dynamic LDS holds the
+// right-hand side (N complex) and the pivot rows (N ints) of the current job.  FP64 storage
+// (arena, TS = double2) or FP32 storage (TS = float2) with every port solution refined against
+// the FP64 system A64 (r = b - A64 x in FP64, x += LU^-1 r) while the componentwise backward
+// error max_i |r_i| / (|A64| |x| + |b|)_i falls by half and is above 2 eps64; a job whose error
+// stays above 1e-10 is marked (info -2) for an FP64 factorisation.  rscr holds one residual
+// vector per workgroup.
+template <int NB, int CB, typename TS>
 __global__ __launch_bounds__(256) void k_b_lu(const bdesc *D, const bjob *J, const int *order, int nj, unsigned *counter,
-                                              double2 *arena, double2 *zout, int *info_out)
+                                              TS *arena, const double2 *arena64, double2 *rscr, long rstride,
+                                              double2 *zout, int *info_out, double *berr_out, int *sweeps_out)
 {
 	extern __shared__ double2 s_dyn[];
 	__shared__ double2 s_row[NB];
@@ -491,7 +616,7 @@ __global__ __launch_bounds__(256) void k_b_lu(const bdesc *D, const bjob *J, con
 	__shared__ double s_rv[4];
 	__shared__ int s_ri[4];
 	__shared__ int s_job;
-	const int tid = threadIdx.x;
+	const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
 	for (;;) {
 		if (tid == 0) s_job = (int)atomicAdd(counter, 1u);
 		__syncthreads();
@@ -502,52 +627,67 @@ __global__ __launch_bounds__(256) void k_b_lu(const bdesc *D, const bjob *J, con
 		const bjob job = J[jid];
 		const bdesc &d = D[job.s];
 		const int N = job.N;
-		double2 *A = arena + job.aoff;
+		TS *A = arena + job.aoff;
 		double2 *b = s_dyn;
 		int *piv = (int *)(s_dyn + N);
-		const int info = blu_factor<NB, CB>(A, N, piv, s_row, s_L, s_rv, s_ri);
+		const int info = blu_factor<NB, CB, TS>(A, N, piv, s_row, s_L, s_rv, s_ri);
 		if (info >= 0) {
 			if (tid == 0) info_out[jid] = info;
 			__syncthreads();
 			continue;
 		}
-		// every port: b = e_i1 - e_i2, row interchanges, unit-lower then upper substitution
+		double worst = 0;
+		int sweeps = 0;
 		for (int pj = 0; pj < d.np; pj++) {
+			const int i1 = d.i1[pj], i2 = d.i2[pj];
 			for (int r = tid; r < N; r += 256) b[r] = make_double2(0, 0);
 			__syncthreads();
 			if (tid == 0) {
-				if (d.i1[pj] >= 0) b[d.i1[pj]].x += 1;
-				if (d.i2[pj] >= 0) b[d.i2[pj]].x -= 1;
-				for (int c = 0; c < N; c++) {
-					const int pr = piv[c];
-					if (pr != c) {
-						const double2 t = b[c];
-						b[c] = b[pr];
-						b[pr] = t;
-					}
-				}
+				if (i1 >= 0) b[i1].x += 1;
+				if (i2 >= 0) b[i2].x -= 1;
 			}
 			__syncthreads();
-			for (int c = 0; c < N - 1; c++) {
-				const double2 xc = b[c];
-				if ((xc.x != 0) || (xc.y != 0))
-					for (int r = c + 1 + tid; r < N; r += 256) {
-						const double2 l = A[(long)r * N + c];
-						b[r].x -= (l.x * xc.x) - (l.y * xc.y);
-						b[r].y -= (l.x * xc.y) + (l.y * xc.x);
+			lu_subst<TS>(A, N, piv, b);
+			if constexpr (sizeof(TS) == sizeof(float2)) {
+				const double2 *A64 = arena64 + job.aoff;
+				double2 *r = rscr + (long)blockIdx.x * rstride;
+				double be = 1, last = 2;
+				for (int it = 0;; it++) {
+					double wmax = 0;
+					for (int row = wave; row < N; row += 4) {
+						const double2 *a = A64 + (long)row * N;
+						double sr = 0, si = 0, sa = 0;
+						for (int c = lane; c < N; c += 64) {
+							const double2 u = a[c], x = b[c];
+							sr += (u.x * x.x) - (u.y * x.y);
+							si += (u.x * x.y) + (u.y * x.x);
+							sa += sqrt((u.x * u.x) + (u.y * u.y)) * sqrt((x.x * x.x) + (x.y * x.y));
+						}
+						for (int off = 32; off > 0; off >>= 1) {
+							sr += __shfl_xor(sr, off);
+							si += __shfl_xor(si, off);
+							sa += __shfl_xor(sa, off);
+						}
+						const double rhs = ((row == i1) ? 1.0 : 0.0) - ((row == i2) ? 1.0 : 0.0);
+						const double rr = rhs - sr, ri = -si;
+						if (lane == 0) r[row] = make_double2(rr, ri);
+						const double den = sa + fabs(rhs), num = sqrt((rr * rr) + (ri * ri));
+						const double e = (den > 0) ? (num / den) : ((num > 0) ? 1.0 : 0.0);
+						wmax = (e > wmax) ? e : wmax;
 					}
-				__syncthreads();
-			}
-			for (int c = N - 1; c >= 0; c--) {
-				if (tid == 0) b[c] = cmul(b[c], crecip(A[(long)c * N + c]));
-				__syncthreads();
-				const double2 xc = b[c];
-				for (int r = tid; r < c; r += 256) {
-					const double2 u = A[(long)r * N + c];
-					b[r].x -= (u.x * xc.x) - (u.y * xc.y);
-					b[r].y -= (u.x * xc.y) + (u.y * xc.x);
+					if (lane == 0) s_rv[wave] = wmax;
+					__syncthreads();
+					be = s_rv[0];
+					for (int w = 1; w < 4; w++) be = (s_rv[w] > be) ? s_rv[w] : be;
+					__syncthreads();
+					if ((be <= 4.5e-16) || (be > 0.5 * last) || (it == BRF_MAX)) break;
+					last = be;
+					lu_subst<TS>(A, N, piv, r);
+					for (int i = tid; i < N; i += 256) b[i] = make_double2(b[i].x + r[i].x, b[i].y + r[i].y);
+					__syncthreads();
+					sweeps++;
 				}
-				__syncthreads();
+				worst = (be > worst) ? be : worst;
 			}
 			if (tid < d.np) {
 				const double2 v1 = (d.i1[tid] >= 0) ? b[d.i1[tid]] : make_double2(0, 0);
@@ -556,9 +696,21 @@ __global__ __launch_bounds__(256) void k_b_lu(const bdesc *D, const bjob *J, con
 			}
 			__syncthreads();
 		}
-		if (tid == 0) info_out[jid] = -1;
+		if (tid == 0) {
+			info_out[jid] = (worst > 1e-10) ? -2 : -1;
+			if (berr_out) berr_out[jid] = worst;
+			if (sweeps_out) sweeps_out[jid] = sweeps;
+		}
 		__syncthreads();
 	}
+}
+
+// the FP32 copy of every job's system
+__global__ void k_b_demote(const double2 *src, float2 *dst, long cnt)
+{
+	const long stride = (long)gridDim.x * blockDim.x;
+	for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < cnt; e += stride)
+		dst[e] = make_float2((float)src[e].x, (float)src[e].y);
 }
 
 // one matrix per workgroup through blu_factor (the batched factorisation on its own)
@@ -571,7 +723,7 @@ __global__ __launch_bounds__(256) void k_b_lu_only(double2 *A, const long *aoff,
 	__shared__ double s_rv[4];
 	__shared__ int s_ri[4];
 	const int j = blockIdx.x;
-	const int r = blu_factor<NB, 2>(A + aoff[j], n[j], piv + poff[j], s_row, s_L, s_rv, s_ri);
+	const int r = blu_factor<NB, 2, double2>(A + aoff[j], n[j], piv + poff[j], s_row, s_L, s_rv, s_ri);
 	if (threadIdx.x == 0) info[j] = r;
 }
 
@@ -637,11 +789,11 @@ struct chunk {                 // jobs of one pass through the arena
 struct gkey {
 	int nj;
 	long nst, arena_n, tilesS, tilesP, tilesG, nlist;
-	int nb, cb, wpc;
+	int nb, cb, wpc, fp32;
 	bool operator<(const gkey &o) const
 	{
-		return std::tie(nj, nst, arena_n, tilesS, tilesP, tilesG, nlist, nb, cb, wpc) <
-		       std::tie(o.nj, o.nst, o.arena_n, o.tilesS, o.tilesP, o.tilesG, o.nlist, o.nb, o.cb, o.wpc);
+		return std::tie(nj, nst, arena_n, tilesS, tilesP, tilesG, nlist, nb, cb, wpc, fp32) <
+		       std::tie(o.nj, o.nst, o.arena_n, o.tilesS, o.tilesP, o.tilesG, o.nlist, o.nb, o.cb, o.wpc, o.fp32);
 	}
 };
 
@@ -655,6 +807,11 @@ struct batch {
 	hipStream_t s = nullptr;
 	int ncu = 0;
 	dbuf<double2> arena, zdev, gscr;
+	dbuf<float2> arena32;          // FP32 copies of the systems (precision form)
+	dbuf<double2> rscr;            // one residual vector per persistent workgroup (precision form)
+	dbuf<double> dberr;
+	dbuf<int> dsweeps;
+	int fp32 = 0;                  // 1: FP32-stored factorisation with FP64 refinement
 	dbuf<bjob> dj;
 	dbuf<int> dorder, dinfo, dsj, dri, dci;
 	dbuf<unsigned> dcnt;
@@ -683,20 +840,31 @@ size_t lu_lds(int Nmax)
 	return (size_t)Nmax * sizeof(double2) + (size_t)Nmax * sizeof(int) + 16;
 }
 
-template <int NB, int CB>
-void launch_lu_t(hipStream_t s, batch &B, int grid, int nj, size_t lds)
+template <int NB, int CB, typename TS>
+void launch_lu_t(hipStream_t s, batch &B, int grid, int nj, size_t lds, TS *arena, const int *order)
 {
-	hipLaunchKernelGGL((k_b_lu<NB, CB>), dim3(grid), dim3(256), lds, s, B.dd.p, B.dj.p, B.dorder.p, nj, B.dcnt.p,
-	                   B.arena.p, B.zdev.p, B.dinfo.p);
+	hipLaunchKernelGGL((k_b_lu<NB, CB, TS>), dim3(grid), dim3(256), lds, s, B.dd.p, B.dj.p, order, nj, B.dcnt.p, arena,
+	                   (const double2 *)B.arena.p, B.rscr.p, (long)BMAXN, B.zdev.p, B.dinfo.p, B.dberr.p, B.dsweeps.p);
 }
 
-void launch_lu(hipStream_t s, batch &B, const tune_t &t, int nj, size_t lds)
+// fp32: the FP32 arena with refinement against the FP64 one; else the FP64 arena in place
+void launch_lu(hipStream_t s, batch &B, const tune_t &t, int nj, size_t lds, bool fp32, const int *order)
 {
 	const int grid = std::max(1, std::min(nj, B.ncu * t.wpc));
-	if ((t.nb == 16) && (t.cb == 2)) launch_lu_t<16, 2>(s, B, grid, nj, lds);
-	else if ((t.nb == 16) && (t.cb == 4)) launch_lu_t<16, 4>(s, B, grid, nj, lds);
-	else if (t.cb == 4) launch_lu_t<32, 4>(s, B, grid, nj, lds);
-	else launch_lu_t<32, 2>(s, B, grid, nj, lds);
+	if (fp32) {
+		float2 *a = B.arena32.p;
+		if ((t.nb == 16) && (t.cb == 2)) launch_lu_t<16, 2>(s, B, grid, nj, lds, a, order);
+		else if ((t.nb == 16) && (t.cb == 4)) launch_lu_t<16, 4>(s, B, grid, nj, lds, a, order);
+		else if (t.cb == 4) launch_lu_t<32, 4>(s, B, grid, nj, lds, a, order);
+		else launch_lu_t<32, 2>(s, B, grid, nj, lds, a, order);
+	}
+	else {
+		double2 *a = B.arena.p;
+		if ((t.nb == 16) && (t.cb == 2)) launch_lu_t<16, 2>(s, B, grid, nj, lds, a, order);
+		else if ((t.nb == 16) && (t.cb == 4)) launch_lu_t<16, 4>(s, B, grid, nj, lds, a, order);
+		else if (t.cb == 4) launch_lu_t<32, 4>(s, B, grid, nj, lds, a, order);
+		else launch_lu_t<32, 2>(s, B, grid, nj, lds, a, order);
+	}
 }
 
 void record_fill(hipStream_t s, batch &B, const chunk &c)
@@ -726,8 +894,9 @@ void record_solve(hipStream_t s, batch &B, const chunk &c, const tune_t &t)
 	if (c.nst > 0)
 		hipLaunchKernelGGL(k_b_stamp, dim3((unsigned)((c.nst + 255) / 256)), dim3(256), 0, s, B.dj.p, B.dsj.p, B.dri.p,
 		                   B.dci.p, B.dval.p, c.nst, B.arena.p);
+	if (B.fp32) hipLaunchKernelGGL(k_b_demote, dim3(4096), dim3(256), 0, s, (const double2 *)B.arena.p, B.arena32.p, c.arena_n);
 	hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, s, B.dcnt.p);
-	launch_lu(s, B, t, nj, lu_lds(c.Nmax));
+	launch_lu(s, B, t, nj, lu_lds(c.Nmax), B.fp32 != 0, B.dorder.p);
 }
 
 template <typename F>
@@ -828,6 +997,12 @@ void stage_chunk(batch &B, chunk &c)
 	moved |= B.arena.ensure(c.arena_n);
 	moved |= B.zdev.ensure(c.zn);
 	moved |= B.gscr.ensure(c.gn);
+	moved |= B.dberr.ensure(nj);
+	moved |= B.dsweeps.ensure(nj);
+	if (B.fp32) {
+		moved |= B.arena32.ensure(c.arena_n);
+		moved |= B.rscr.ensure((long)BMAXN * 2 * B.ncu);
+	}
 	if (moved) {
 		// captured graphs hold the old addresses
 		for (auto &kv : B.graphs) {
@@ -867,7 +1042,7 @@ void direct_blocks(batch &B, const chunk &c)
 
 std::pair<hipGraphExec_t, hipGraphExec_t> graphs_for(batch &B, const chunk &c, const tune_t &t)
 {
-	const gkey k{(int)c.J.size(), c.nst, c.arena_n, c.tilesS, c.tilesP, c.tilesG, c.nlist, t.nb, t.cb, t.wpc};
+	const gkey k{(int)c.J.size(), c.nst, c.arena_n, c.tilesS, c.tilesP, c.tilesG, c.nlist, t.nb, t.cb, t.wpc, B.fp32};
 	auto it = B.graphs.find(k);
 	if (it != B.graphs.end()) return it->second;
 	hipGraphExec_t gf = capture(B.s, [&](hipStream_t s) { record_fill(s, B, c); });
@@ -907,10 +1082,14 @@ int prepare(batch &B, peec_t **models, int nm, int ifreq0, int nfreq)
 	HIPCHK(hipGetDevice(&dev));
 	HIPCHK(hipDeviceGetAttribute(&B.ncu, hipDeviceAttributeMultiprocessorCount, dev));
 	HIPCHK(hipStreamCreateWithFlags(&B.s, hipStreamNonBlocking));
-	HIPCHK(hipFuncSetAttribute((const void *)k_b_lu<16, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lu_lds(BMAXN)));
-	HIPCHK(hipFuncSetAttribute((const void *)k_b_lu<16, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lu_lds(BMAXN)));
-	HIPCHK(hipFuncSetAttribute((const void *)k_b_lu<32, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lu_lds(BMAXN)));
-	HIPCHK(hipFuncSetAttribute((const void *)k_b_lu<32, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lu_lds(BMAXN)));
+	{
+		const int lds = (int)lu_lds(BMAXN);
+		const void *fns[] = {(const void *)k_b_lu<16, 2, double2>, (const void *)k_b_lu<16, 4, double2>,
+		                     (const void *)k_b_lu<32, 2, double2>, (const void *)k_b_lu<32, 4, double2>,
+		                     (const void *)k_b_lu<16, 2, float2>,  (const void *)k_b_lu<16, 4, float2>,
+		                     (const void *)k_b_lu<32, 2, float2>,  (const void *)k_b_lu<32, 4, float2>};
+		for (const void *fn : fns) HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+	}
 	B.nm = nm;
 	B.P.resize(nm);
 	B.G.resize(nm);
@@ -1120,8 +1299,38 @@ int peec_batch_sweep(peec_batch_t *h, d_complex_t **zmat, peec_batch_stats_t *st
 		const int nj = (int)c.J.size();
 		B.hz.ensure(c.zn);
 		B.hinfo.ensure(nj);
-		HIPCHK(hipMemcpyAsync(B.hz.p, B.zdev.p, (size_t)c.zn * sizeof(double2), hipMemcpyDeviceToHost, B.s));
 		HIPCHK(hipMemcpyAsync(B.hinfo.p, B.dinfo.p, (size_t)nj * sizeof(int), hipMemcpyDeviceToHost, B.s));
+		HIPCHK(hipStreamSynchronize(B.s));
+		if (B.fp32) {
+			// refinement statistics, and jobs whose refinement did not reach 1e-10: their FP64
+			// systems (untouched by the FP32 factorisation) factored and solved in FP64
+			std::vector<double> be(nj);
+			std::vector<int> sw(nj);
+			HIPCHK(hipMemcpyAsync(be.data(), B.dberr.p, (size_t)nj * sizeof(double), hipMemcpyDeviceToHost, B.s));
+			HIPCHK(hipMemcpyAsync(sw.data(), B.dsweeps.p, (size_t)nj * sizeof(int), hipMemcpyDeviceToHost, B.s));
+			HIPCHK(hipStreamSynchronize(B.s));
+			std::vector<int> redo;
+			for (int k = 0; k < nj; k++) {
+				if (B.hinfo.p[k] == -2) redo.push_back(k);
+				if (st && (B.hinfo.p[k] < 0)) {
+					st->refine_sweeps += sw[k];
+					if (B.hinfo.p[k] == -1 && be[k] > st->berr_max) st->berr_max = be[k];
+				}
+			}
+			if (!redo.empty()) {
+				const auto tr = std::chrono::steady_clock::now();
+				HIPCHK(hipMemcpyAsync(B.dorder.p, redo.data(), redo.size() * sizeof(int), hipMemcpyHostToDevice, B.s));
+				hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, B.s, B.dcnt.p);
+				launch_lu(B.s, B, tn, (int)redo.size(), lu_lds(c.Nmax), false, B.dorder.p);
+				HIPCHK(hipMemcpyAsync(B.hinfo.p, B.dinfo.p, (size_t)nj * sizeof(int), hipMemcpyDeviceToHost, B.s));
+				HIPCHK(hipStreamSynchronize(B.s));
+				if (st) {
+					st->fp64_fallbacks += (int)redo.size();
+					st->lu_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tr).count();
+				}
+			}
+		}
+		HIPCHK(hipMemcpyAsync(B.hz.p, B.zdev.p, (size_t)c.zn * sizeof(double2), hipMemcpyDeviceToHost, B.s));
 		HIPCHK(hipStreamSynchronize(B.s));
 		for (int k = 0; k < nj; k++) {
 			const bjob &j = c.J[k];
@@ -1201,6 +1410,18 @@ int peec_batch_lu_decomp(int nmat, const int *n, d_complex_t *a, int *piv, int n
 	for (int j = 0; j < nmat; j++)
 		if (info[j] >= 0) return j;
 	return -1;
+}
+
+int peec_batch_set_form(peec_batch_t *h, int form)
+{
+	if (h == nullptr) return 4;
+	if ((form != PEEC_FORM_PLAIN) && (form != PEEC_FORM_PRECISION)) return 4;
+	const int f32 = (form == PEEC_FORM_PRECISION) ? 1 : 0;
+	if (f32 != h->b.fp32) {
+		h->b.fp32 = f32;
+		h->b.tuned = false;
+	}
+	return 0;
 }
 
 int peec_batch_trials(const peec_batch_t *h, int *nb, int *tile_cols, int *wpc, int *order, double *ms, int cap)

@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 from conftest import deck_text
-from openpeec_amd import PEEC_FORM_PLAIN, Batch, Peec, abi, decks
+from openpeec_amd import PEEC_FORM_PLAIN, PEEC_FORM_PRECISION, Batch, Peec, abi, decks
 from oracle_py import oracle_zmat
 
 pytestmark = pytest.mark.gpu
@@ -173,6 +173,31 @@ def test_batch_replay_and_autotune(lib):
         r = _rel(z1[k], zp)
         print(f"  structure {k}: rel {r:.2e}", flush=True)
         assert r <= 1e-8, r
+    b.close()
+    for m in ms:
+        m.close()
+
+
+def test_batch_precision_form(lib):
+    """FP32-stored batched factorisation with FP64 refinement inside each workgroup: every port
+    solution refined to a componentwise backward error <= 1e-10 without an FP64 redo, Z within
+    1e-10 of the FP64 batched sweep and 1e-8 of the oracle."""
+    texts = [decks.random_wire_deck(4000 + s, nmin=20, nmax=400) for s in range(8)]
+    ms = _models(lib, texts)
+    print(f"[batch-precision] n = {[m.p.nseg for m in ms]}", flush=True)
+    b = Batch(ms, 0, 6)
+    z64, _ = b.sweep()
+    b.set_form(PEEC_FORM_PRECISION)
+    z32, st = b.sweep()
+    print(f"  refine sweeps {st.refine_sweeps}, worst backward error {st.berr_max:.2e}, "
+          f"FP64 redos {st.fp64_fallbacks}", flush=True)
+    assert st.berr_max <= 1e-10 and st.fp64_fallbacks == 0 and st.refine_sweeps > 0
+    for a, c in zip(z32, z64):
+        assert _rel(a, c) <= 1e-10, _rel(a, c)
+    for k in (0, 3):
+        r = _rel(z32[k], oracle_zmat(texts[k], 0, 6))
+        print(f"  structure {k} vs oracle: {r:.2e}", flush=True)
+        assert r <= 1e-8
     b.close()
     for m in ms:
         m.close()
