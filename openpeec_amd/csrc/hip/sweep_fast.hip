@@ -95,6 +95,31 @@ __global__ void k_demote(const double2 *src, float2 *dst, long count)
 	dst[e] = make_float2((float)v.x, (float)v.y);
 }
 
+// dst = src^T in FP32 (both n x n, row-major), through 32 x 32 tiles in LDS so both the read
+// and the write are coalesced
+__device__ __forceinline__ double2 as_d2(const double2 &v) { return v; }
+__device__ __forceinline__ double2 as_d2(const float2 &v) { return make_double2(v.x, v.y); }
+
+template <typename TS>
+__global__ __launch_bounds__(256) void k_transpose32(const TS *__restrict__ src, float2 *__restrict__ dst, int n)
+{
+	__shared__ double2 t[32][33];
+	const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+	const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+	for (int k = 0; k < 32; k += 8) {
+		const int r = r0 + ty + k, c = c0 + tx;
+		if ((r < n) && (c < n)) t[ty + k][tx] = as_d2(src[(long)r * n + c]);
+	}
+	__syncthreads();
+	for (int k = 0; k < 32; k += 8) {
+		const int c = c0 + ty + k, r = r0 + tx;
+		if ((r < n) && (c < n)) {
+			const double2 v = t[tx][ty + k];
+			dst[(long)c * n + r] = make_float2((float)v.x, (float)v.y);
+		}
+	}
+}
+
 // ---------------------------------------------------------------- batched dense products
 
 // One system of the batch: CSR stamps and its dense kernel blocks.
@@ -362,121 +387,149 @@ __global__ void k_permute(const double2 *src, double2 *dst, const int *perm, int
 // have no sequential dependence, unlike substitution, whose 64-row steps each wait for the
 // previous one.  The inverses are built once per factorisation by recursive blocking (below).
 
-// Y[:, f] = Mt X[:, f] for a triangular FP32 matrix (lower when LOWER), X and Y n x F in FP64.
-// A workgroup owns 16 rows (4 per wave) and walks the 256-column chunks its rows touch; each
-// chunk of X is staged in LDS once and reused by the 16 rows; a lane takes 4 columns of the
-// chunk, so 16 matrix loads per lane are in flight between two barriers.
-constexpr int TG_CH = 256;
-// Y = T X for a triangular FP32-stored T (lower: columns <= the row's block; upper: from the
-// row's 256-column block on; entries outside the triangle are stored zeros) and F FP64 columns
-// X (n x F, row-major).  A workgroup of 4 waves takes 4 RW consecutive rows; the X chunk of 256
-// rows is staged in LDS by column (xs[f][c], conflict-free), each lane covers columns lane +
-// 64 u (u < 4) of RW rows, and the next chunk's matrix entries are loaded before the current
-// chunk's products.  FP64 accumulation.
-template <bool LOWER, int F, int RW>
-__global__ __launch_bounds__(256) void k_tgemv(const float2 *__restrict__ Mt, long ld, int n, const double2 *__restrict__ X,
-                                               double2 *__restrict__ Y)
+// Y = T X for a triangular FP32 T stored by columns (Tc[c * n + r] = T[r][c]; entries outside
+// the triangle are stored zeros) and F FP64 columns X (n x F, row-major).  The triangle is cut
+// into items (64-row block, column segment s of cs columns) so every wave streams about the same
+// share of T whatever its rows; a wave takes one item and lane l owns row 64 rb + l, so one
+// load instruction reads 64 consecutive entries of a column (512 B).  A step covers 16 columns
+// (the next step's entries loaded before the current step's products); the step's X entries are
+// the same for every lane (scalar loads); FP64 fused multiply-adds.  No cross-lane reduction:
+// each item writes its row sums to part[s][row][f] and k_tgsum adds a row's segments in
+// increasing s (deterministic).
+constexpr int TG_RB = 64;
+
+template <bool LOWER, int F>
+__global__ __launch_bounds__(256) void k_tgemv(const float2 *__restrict__ Tc, int n, int cs,
+                                               const double2 *__restrict__ X, double2 *__restrict__ part)
 {
-	__shared__ double2 xs[F * TG_CH];
-	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-	const int r0 = blockIdx.x * (4 * RW) + wave * RW;
-	const int wr0 = blockIdx.x * (4 * RW);
-	const int cbeg = LOWER ? 0 : ((wr0 / TG_CH) * TG_CH);
-	const int cend = LOWER ? min(n, wr0 + 4 * RW) : n;
-	double ar[RW][F], ai[RW][F];
+	// the wave index through readfirstlane: item bounds, and so the X addresses, are then known
+	// to be uniform and X is read with scalar loads
+	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+	const int rlo = (blockIdx.x * 4 + wave) * TG_RB, s = blockIdx.y;
+	if (rlo >= n) return;
+	const int rhi = min(n, rlo + TG_RB);
+	int cb, ce;
+	if (LOWER) {
+		cb = s * cs;
+		ce = min(rhi, cb + cs);
+	}
+	else {
+		cb = max(s * cs, rlo);
+		ce = min(n, (s + 1) * cs);
+	}
+	if (cb >= ce) return;   // item outside the triangle
+	const int row = rlo + lane;
+	const bool act = row < n;
+	const float2 *tr = Tc + (act ? row : rlo);
+	double ar[F], ai[F];
 #pragma unroll
-	for (int q = 0; q < RW; q++)
+	for (int f = 0; f < F; f++) ar[f] = ai[f] = 0;
+	auto load = [&](int c, float2 (&d)[16]) {
 #pragma unroll
-		for (int f = 0; f < F; f++) ar[q][f] = ai[q][f] = 0;
-	auto load = [&](int c0, float2 (&m)[RW][4]) {
-		const int nc = min(TG_CH, n - c0);
-#pragma unroll
-		for (int q = 0; q < RW; q++) {
-			const int row = r0 + q;
-#pragma unroll
-			for (int u = 0; u < 4; u++) {
-				const int c = lane + 64 * u;
-				m[q][u] = ((row < n) && (c < nc)) ? Mt[(long)row * ld + c0 + c] : make_float2(0, 0);
-			}
-		}
+		for (int u = 0; u < 16; u++) d[u] = act ? tr[(long)(c + u) * n] : make_float2(0, 0);
 	};
-	float2 m[RW][4];
-	if (cbeg < cend) load(cbeg, m);
-	for (int c0 = cbeg; c0 < cend; c0 += TG_CH) {
-		const int nc = min(TG_CH, n - c0);
-		__syncthreads();
-		for (int e = threadIdx.x; e < nc * F; e += 256) {
-			const int c = e / F, f = e - c * F;
-			xs[f * TG_CH + c] = X[(long)c0 * F + e];
-		}
-		__syncthreads();
-		float2 mn[RW][4];
-		if (c0 + TG_CH < cend) load(c0 + TG_CH, mn);
+	auto macc = [&](int c, const float2 (&d)[16], int cnt) {
 #pragma unroll
-		for (int u = 0; u < 4; u++) {
-			const int c = lane + 64 * u;
-			if (c >= nc) continue;
+		for (int j = 0; j < 16; j++) {
+			if (j < cnt) {
+				const double mre = d[j].x, mim = d[j].y;
+				const double2 *xp = X + (long)(c + j) * F;
 #pragma unroll
-			for (int f = 0; f < F; f++) {
-				const double2 x = xs[f * TG_CH + c];
-#pragma unroll
-				for (int q = 0; q < RW; q++) {
-					const double mr = m[q][u].x, mi = m[q][u].y;
-					ar[q][f] += (mr * x.x) - (mi * x.y);
-					ai[q][f] += (mr * x.y) + (mi * x.x);
+				for (int f = 0; f < F; f++) {
+					const double2 x = xp[f];
+					ar[f] = fma(mre, x.x, ar[f]);
+					ar[f] = fma(-mim, x.y, ar[f]);
+					ai[f] = fma(mre, x.y, ai[f]);
+					ai[f] = fma(mim, x.x, ai[f]);
 				}
 			}
 		}
+	};
+	const int nstep = (ce - cb) / 16;
+	float2 m[16], mn[16];
+	if (nstep > 0) load(cb, m);
+	for (int t = 0; t < nstep; t++) {
+		const int c = cb + 16 * t;
+		if (t + 1 < nstep) load(c + 16, mn);
+		macc(c, m, 16);
 #pragma unroll
-		for (int q = 0; q < RW; q++)
-#pragma unroll
-			for (int u = 0; u < 4; u++) m[q][u] = mn[q][u];
+		for (int j = 0; j < 16; j++) m[j] = mn[j];
 	}
+	const int ct = cb + 16 * nstep;
+	if (ct < ce) {
+		const int cnt = ce - ct;
 #pragma unroll
-	for (int q = 0; q < RW; q++) {
-		const int row = r0 + q;
+		for (int u = 0; u < 16; u++) m[u] = (act && (u < cnt)) ? tr[(long)(ct + u) * n] : make_float2(0, 0);
+		macc(ct, m, cnt);
+	}
+	if (act) {
 #pragma unroll
-		for (int f = 0; f < F; f++) {
-			double sr = ar[q][f], si = ai[q][f];
-			for (int off = 32; off > 0; off >>= 1) {
-				sr += __shfl_xor(sr, off);
-				si += __shfl_xor(si, off);
-			}
-			if ((lane == 0) && (row < n)) Y[(long)row * F + f] = make_double2(sr, si);
-		}
+		for (int f = 0; f < F; f++) part[((long)s * n + row) * F + f] = make_double2(ar[f], ai[f]);
 	}
 }
 
-// rows per wave of the triangular products (PEEC_TG_RW = 4 or 8; 4 by default)
-int tg_rw()
+// Y[row][f] = sum over the row's segments of part[s][row][f] (the items k_tgemv ran for the
+// row's 64-row block), in increasing s
+template <bool LOWER>
+__global__ void k_tgsum(const double2 *__restrict__ part, int n, int F, int cs, double2 *__restrict__ Y)
 {
-	static int rw = -1;
-	if (rw < 0) {
-		const char *e = getenv("PEEC_TG_RW");
-		rw = (e && (atoi(e) == 8)) ? 8 : 4;
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= (long)n * F) return;
+	const int row = (int)(e / F);
+	const int rlo = (row / TG_RB) * TG_RB, rhi = min(n, rlo + TG_RB);
+	const int s0 = LOWER ? 0 : rlo / cs;
+	const int s1 = LOWER ? (rhi - 1) / cs : (n - 1) / cs;
+	const long stride = (long)n * F;
+	double2 a = part[(long)s0 * stride + e];
+	for (int s = s0 + 1; s <= s1; s++) {
+		const double2 p = part[(long)s * stride + e];
+		a.x += p.x;
+		a.y += p.y;
 	}
-	return rw;
+	Y[e] = a;
+}
+
+// column segment of the triangular products (PEEC_TG_CS, a multiple of 64 in [64, 8192]; 512 by
+// default)
+int tg_cs()
+{
+	static int cs = -1;
+	if (cs < 0) {
+		const char *e = getenv("PEEC_TG_CS");
+		const int v = e ? atoi(e) : 512;
+		cs = ((v >= 64) && (v <= 8192) && (v % 64 == 0)) ? v : 512;
+	}
+	return cs;
+}
+
+// doubles of partial sums the products need for order n and F columns
+size_t tg_part_count(int n, int F)
+{
+	return (size_t)((n + tg_cs() - 1) / tg_cs()) * n * F;
 }
 
 template <bool LOWER, int F>
-void tgemv_f(hipStream_t s, const float2 *Mt, int n, const double2 *X, double2 *Y)
+void tgemv_f(hipStream_t s, const float2 *Mt, int n, const double2 *X, double2 *Y, double2 *part)
 {
-	if (tg_rw() == 8) hipLaunchKernelGGL((k_tgemv<LOWER, F, 8>), dim3((n + 31) / 32), dim3(256), 0, s, Mt, (long)n, n, X, Y);
-	else hipLaunchKernelGGL((k_tgemv<LOWER, F, 4>), dim3((n + 15) / 16), dim3(256), 0, s, Mt, (long)n, n, X, Y);
+	const int cs = tg_cs();
+	const dim3 grid((unsigned)((n + 4 * TG_RB - 1) / (4 * TG_RB)), (unsigned)((n + cs - 1) / cs));
+	hipLaunchKernelGGL((k_tgemv<LOWER, F>), grid, dim3(256), 0, s, Mt, n, cs, X, part);
+	hipLaunchKernelGGL(k_tgsum<LOWER>, dim3((unsigned)(((long)n * F + 255) / 256)), dim3(256), 0, s,
+	                   (const double2 *)part, n, F, cs, Y);
 }
 
 template <bool LOWER>
-void tgemv(hipStream_t s, const float2 *Mt, int n, const double2 *X, double2 *Y, int F)
+void tgemv(hipStream_t s, const float2 *Mt, int n, const double2 *X, double2 *Y, int F, double2 *part)
 {
 	switch (F) {
-	case 1: tgemv_f<LOWER, 1>(s, Mt, n, X, Y); break;
-	case 2: tgemv_f<LOWER, 2>(s, Mt, n, X, Y); break;
-	case 3: tgemv_f<LOWER, 3>(s, Mt, n, X, Y); break;
-	case 4: tgemv_f<LOWER, 4>(s, Mt, n, X, Y); break;
-	case 5: tgemv_f<LOWER, 5>(s, Mt, n, X, Y); break;
-	case 6: tgemv_f<LOWER, 6>(s, Mt, n, X, Y); break;
-	case 7: tgemv_f<LOWER, 7>(s, Mt, n, X, Y); break;
-	default: tgemv_f<LOWER, 8>(s, Mt, n, X, Y); break;
+	case 1: tgemv_f<LOWER, 1>(s, Mt, n, X, Y, part); break;
+	case 2: tgemv_f<LOWER, 2>(s, Mt, n, X, Y, part); break;
+	case 3: tgemv_f<LOWER, 3>(s, Mt, n, X, Y, part); break;
+	case 4: tgemv_f<LOWER, 4>(s, Mt, n, X, Y, part); break;
+	case 5: tgemv_f<LOWER, 5>(s, Mt, n, X, Y, part); break;
+	case 6: tgemv_f<LOWER, 6>(s, Mt, n, X, Y, part); break;
+	case 7: tgemv_f<LOWER, 7>(s, Mt, n, X, Y, part); break;
+	default: tgemv_f<LOWER, 8>(s, Mt, n, X, Y, part); break;
 	}
 }
 
@@ -509,20 +562,30 @@ struct fast {
 	double2 *luw = nullptr;   // FP64 factorisation workspace
 	double2 *inv = nullptr;   // FP64 inverse of one factor (built, then demoted)
 	double2 *scr = nullptr;   // GEMM scratch of the recursive inversion
-	float2 *ML = nullptr, *MU = nullptr;   // FP32 Linv, Uinv
+	float2 *ML = nullptr, *MU = nullptr;   // FP32 Linv, Uinv, stored by columns
 	double2 *dinvL = nullptr, *dinvU = nullptr;
 	int *piv = nullptr, *perm = nullptr;
 	double fref = -1;
 	bool stale = false;       // refactor before the next batch
 	int its_ref = -1;         // lock-step iterations of the first batch after a factorisation
+	int lock_ref = -1;        // the same summed over ports (cost rule)
+	double lu_ms = 0;         // wall time of the last factorisation and inversion
+	double excess_ms = 0;     // GMRES time beyond the best batch's rate since the last factorisation
+	std::vector<std::pair<int, double>> hist;   // per batch since the factorisation: lock-step
+	                                            // iterations, ms per iteration
+	int served = 0;           // points solved with the current factorisation
+	int span = 0;             // points the previous factorisation served (0: none yet)
+	slotbuf ahead;            // system at a point ahead of the current batch (stamps; its Lp and P
+	                          // live in the inverse workspace until the factorisation)
 	double2 *V = nullptr;     // (MKRYLOV + 1) x n x F Arnoldi basis
 	double2 *Z = nullptr;     // MKRYLOV x n x F preconditioned directions
 	double2 *W = nullptr, *Xv = nullptr, *Bv = nullptr, *R = nullptr, *T1 = nullptr;
+	double2 *tgp = nullptr;   // segment partial sums of the triangular products
 	double2 *partial = nullptr, *h = nullptr, *yd = nullptr;
 	double *scale = nullptr;
 };
 
-// both FP32 inverses from the FP64 factors in c.luw
+// both FP32 inverses from the FP64 factors in c.luw, stored by columns
 void build_inverses(fast &c, hipStream_t s)
 {
 	const int n = c.n;
@@ -530,8 +593,9 @@ void build_inverses(fast &c, hipStream_t s)
 		const bool lower = (pass == 0);
 		dlu_tri_inverse(s, n, (const d_complex_t *)c.luw, n, lower ? 1 : 0, (d_complex_t *)c.inv, (d_complex_t *)c.scr,
 		                (d_complex_t *)(lower ? c.dinvL : c.dinvU));
-		hipLaunchKernelGGL(k_demote, dim3(blocks((long)n * n)), dim3(256), 0, s, (const double2 *)c.inv, lower ? c.ML : c.MU,
-		                   (long)n * n);
+		const unsigned nt = (unsigned)((n + 31) / 32);
+		hipLaunchKernelGGL(k_transpose32<double2>, dim3(nt, nt), dim3(256), 0, s, (const double2 *)c.inv,
+		                   lower ? c.ML : c.MU, n);
 	}
 }
 
@@ -539,8 +603,8 @@ void build_inverses(fast &c, hipStream_t s)
 void precond(fast &c, hipStream_t s, const double2 *v, double2 *z, int F)
 {
 	hipLaunchKernelGGL(k_permute, dim3(blocks((long)c.n * F)), dim3(256), 0, s, v, z, c.perm, c.n, F);
-	tgemv<true>(s, (const float2 *)c.ML, c.n, (const double2 *)z, c.T1, F);
-	tgemv<false>(s, (const float2 *)c.MU, c.n, (const double2 *)c.T1, z, F);
+	tgemv<true>(s, (const float2 *)c.ML, c.n, (const double2 *)z, c.T1, F, c.tgp);
+	tgemv<false>(s, (const float2 *)c.MU, c.n, (const double2 *)c.T1, z, F, c.tgp);
 }
 
 int g_offS = 0, g_offQ = 0;
@@ -740,7 +804,12 @@ void free_fast(fast &c)
 		for (void *p : bb)
 			if (p) HIPCHK(hipFree(p));
 	}
-	void *bufs[] = {c.luw, c.inv, c.scr, c.ML, c.MU, c.dinvL, c.dinvU, c.piv, c.perm, c.V, c.Z, c.W, c.Xv, c.Bv, c.R, c.T1, c.partial, c.h, c.yd, c.scale};
+	{
+		void *bb[] = {c.ahead.rp, c.ahead.ci, c.ahead.ri, c.ahead.val};
+		for (void *p : bb)
+			if (p) HIPCHK(hipFree(p));
+	}
+	void *bufs[] = {c.luw, c.inv, c.scr, c.ML, c.MU, c.dinvL, c.dinvU, c.piv, c.perm, c.V, c.Z, c.W, c.Xv, c.Bv, c.R, c.T1, c.tgp, c.partial, c.h, c.yd, c.scale};
 	for (void *b : bufs)
 		if (b) HIPCHK(hipFree(b));
 	c = fast();
@@ -794,6 +863,20 @@ void mna_stamps_charge(const peec_t *p, double f, stamps_t *st);
 // PEEC_TRACE=1 prints the iteration counts of every batch to stderr
 static const bool g_trace = getenv("PEEC_TRACE") && getenv("PEEC_TRACE")[0] == '1';
 
+// refactor rule: PEEC_REFACTOR_ALPHA > 0 prices extra iterations against the factorisation time
+// (0.75 by default: iteration counts grow about linearly with the distance from the factorisation's
+// frequency, so refactoring somewhat before the break-even point pays); 0 selects the
+// iteration-ratio rule (more than twice the first batch's)
+static double refactor_alpha()
+{
+	static double a = -1;
+	if (a < 0) {
+		const char *e = getenv("PEEC_REFACTOR_ALPHA");
+		a = e ? std::max(0.0, atof(e)) : 0.75;
+	}
+	return a;
+}
+
 // PEEC_MIXED_LU64=1 factors and inverts the preconditioner in FP64 (default: FP32 storage, FP64
 // panels, FP32 MFMA updates and inversion GEMMs)
 static bool lu64()
@@ -833,6 +916,7 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 			c.sl[f].rp = dal<int>(n + 1);
 			hipLaunchKernelGGL(k_zero_u32, dim3(blocks(n + 1, 64)), dim3(64), 0, s, (unsigned *)c.sl[f].rp, n + 1);
 		}
+		c.ahead.rp = dal<int>(n + 1);
 		const int nblk = (n + TB - 1) / TB;
 		c.luw = dal<double2>((size_t)n * n);
 		c.inv = dal<double2>((size_t)n * n);
@@ -851,6 +935,7 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 		c.Bv = dal<double2>(nf);
 		c.R = dal<double2>(nf);
 		c.T1 = dal<double2>(nf);
+		c.tgp = dal<double2>(tg_part_count(n, FMAX));
 		c.partial = dal<double2>((size_t)((n + DOT_ROWS - 1) / DOT_ROWS) * (MKRYLOV + 1) * FMAX);
 		c.h = dal<double2>((size_t)(MKRYLOV + 1) * FMAX);
 		c.yd = dal<double2>((size_t)(MKRYLOV + 1) * FMAX);
@@ -887,11 +972,10 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 	for (int f0 = 0; (f0 < nfreq) && !rc; f0 += F) {
 		const int nb = std::min(F, nfreq - f0);
 		const auto t0 = std::chrono::steady_clock::now();
-		// fill each system of the batch into its slot
-		for (int b = 0; b < nb; b++) {
-			const double freq = freq_at(p, ifreq0 + f0 + b);
+		// the system of sweep point k (relative to this call) into slot buffers sb
+		auto fill_system = [&](slotbuf &sb, int k) {
+			const double freq = freq_at(p, ifreq0 + k);
 			const double kw = p->retardation ? (2 * PI * freq / C0) : 0;
-			slotbuf &sb = c.sl[b];
 			sb.omega = 2 * PI * freq;
 			if (kf) {
 				pgpu_kf_fill(g, kw, (d_complex_t *)sb.lp, nc ? (d_complex_t *)sb.P : nullptr);
@@ -906,13 +990,14 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 			mna_stamps_charge(p, freq, &sp);
 			upload_stamps(sb, sp, n, s);
 			mna_stamps_free(&sp);
-		}
+		};
+		// fill each system of the batch into its slot
+		for (int b = 0; b < nb; b++) fill_system(c.sl[b], f0 + b);
 		const auto t1 = std::chrono::steady_clock::now();
 		if (st) st->fill_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
-		// (re)factor the preconditioner at the frequency of slot b
-		auto refactor = [&](int b) -> int {
+		// (re)factor the preconditioner on the system in sb, that of sweep point k
+		auto refactor_on = [&](const slotbuf &sb, int k) -> int {
 			const auto ta = std::chrono::steady_clock::now();
-			const slotbuf &sb = c.sl[b];
 			hipLaunchKernelGGL(k_assemble_aug, dim3(blocks((long)n * n)), dim3(256), 0, s, c.luw, n, p->offS, p->nseg,
 			                   (const double2 *)sb.lp, sb.omega, offQ, nc, (const double2 *)sb.P);
 			if (sb.nnz > 0)
@@ -937,28 +1022,54 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 					build_inverses(c, s);
 				}
 				else {
-					slu_tri_inverse(s, n, a32, n, 1, c.ML, c.scr, c.dinvL);
-					slu_tri_inverse(s, n, a32, n, 0, c.MU, c.scr, c.dinvU);
+					// each inverse built row-major in the free half of the workspace, then stored
+					// by columns
+					float2 *tmp = a32 + (size_t)n * n;
+					const unsigned nt = (unsigned)((n + 31) / 32);
+					slu_tri_inverse(s, n, a32, n, 1, tmp, c.scr, c.dinvL);
+					hipLaunchKernelGGL(k_transpose32<float2>, dim3(nt, nt), dim3(256), 0, s, (const float2 *)tmp, c.ML, n);
+					slu_tri_inverse(s, n, a32, n, 0, tmp, c.scr, c.dinvU);
+					hipLaunchKernelGGL(k_transpose32<float2>, dim3(nt, nt), dim3(256), 0, s, (const float2 *)tmp, c.MU, n);
 				}
 			}
 			HIPCHK(hipStreamSynchronize(s));
-			c.fref = freq_at(p, ifreq0 + f0 + b);
+			c.fref = freq_at(p, ifreq0 + k);
+			c.lu_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ta).count();
 			if (st) {
 				st->n_lu++;
-				st->lu_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ta).count();
+				st->lu_ms += c.lu_ms;
 			}
+			c.served = 0;
+			c.its_ref = -1;
+			c.lock_ref = -1;
+			c.excess_ms = 0;
 			return 0;
 		};
-		// a fresh factorisation when there is none, or when the last batch needed markedly more
-		// iterations than the first batch after the last factorisation (the preconditioner has
-		// drifted too far from the current frequencies): at the middle of this batch
+		auto refactor = [&](int b) -> int { return refactor_on(c.sl[b], f0 + b); };
+		// a fresh factorisation when there is none, or when the refactor rule marked the current
+		// one stale.  Iteration counts grow with the distance from the factorisation's
+		// frequency on either side, so the factorisation goes to the middle of the run of points
+		// it is expected to serve: half the previous factorisation's run ahead of this batch
+		// (the middle of this batch for the first one, or under the iteration-ratio rule)
 		if ((c.fref < 0) || c.stale) {
-			rc = refactor(nb / 2);
+			if (c.served > 0) c.span = c.served;
+			const int k = ((c.span > 0) && (refactor_alpha() > 0)) ? std::min(nfreq - 1, f0 + c.span / 2) : f0 + nb / 2;
+			if (k < f0 + nb) {
+				rc = refactor(k - f0);
+			}
+			else {
+				// the system of point k: Lp and P in the inverse workspace (free until the
+				// factorisation demotes into it), stamps in their own buffers
+				c.ahead.lp = c.inv;
+				c.ahead.P = c.inv + (size_t)p->nseg * p->nseg;
+				fill_system(c.ahead, k);
+				rc = refactor_on(c.ahead, k);
+			}
 			if (rc) break;
 			c.stale = false;
-			c.its_ref = -1;
 		}
-		int its_batch = 0;
+		int its_batch = 0, its_lock = 0;
+		double batch_ms = 0;
 		// every port: batched GMRES over the nb systems
 		for (int j = 0; (j < np) && !rc; j++) {
 			std::vector<double2> hb((size_t)n * F, make_double2(0, 0));
@@ -972,6 +1083,7 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 			g_mvt.reset();
 			std::vector<int> its = gmres_batch(c, s, F, nb);
 			if (st) mv_collect(st);
+			int lock = 0;
 			for (int b = 0; b < nb; b++) {
 				if (its[b] < 0) {
 					// stale preconditioner: refactor at this frequency and solve the batch again
@@ -985,11 +1097,14 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 				}
 				if (st) st->gmres_its += its[b];
 				its_batch = std::max(its_batch, its[b]);
+				lock = std::max(lock, its[b]);
 			}
+			its_lock += lock;
+			batch_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tg).count();
 			if (g_trace) {
 				fprintf(stderr, "[sweep_fast] points %d..%d port %d its", ifreq0 + f0, ifreq0 + f0 + nb - 1, j);
 				for (int b = 0; b < nb; b++) fprintf(stderr, " %d", its[b]);
-				fprintf(stderr, " (ref %.4g Hz)\n", c.fref);
+				fprintf(stderr, " (ref %.4g Hz, excess %.0f ms of %.0f)\n", c.fref, c.excess_ms, c.lu_ms);
 			}
 			if (st) st->solve_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tg).count();
 			HIPCHK(hipMemcpyAsync(hx.data(), c.Xv, hx.size() * sizeof(double2), hipMemcpyDeviceToHost, s));
@@ -1001,8 +1116,27 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 					zmat[((size_t)(f0 + b) * np + i) * np + j] = d_complex(v1.x - v2.x, v1.y - v2.y);
 				}
 		}
-		if (c.its_ref < 0) c.its_ref = its_batch;
-		else if (its_batch > std::max(REFACTOR_MIN_ITS, 2 * c.its_ref)) c.stale = true;
+		c.served += nb;
+		if (c.its_ref < 0) {
+			c.its_ref = its_batch;
+			c.lock_ref = its_lock;
+			c.hist.assign(1, std::make_pair(its_lock, batch_ms / std::max(1, its_lock)));
+		}
+		else if (refactor_alpha() <= 0) {
+			// iteration-ratio rule
+			if (its_batch > std::max(REFACTOR_MIN_ITS, 2 * c.its_ref)) c.stale = true;
+		}
+		else if (its_lock > 0) {
+			// cost rule: the lock-step iterations beyond the fewest a batch needed since the
+			// factorisation (what a fresh factorisation would give), priced at this batch's time
+			// per iteration, accumulate until they reach alpha times the factorisation's own
+			// time (the ski-rental rule)
+			c.lock_ref = std::min(c.lock_ref, its_lock);
+			c.hist.push_back(std::make_pair(its_lock, batch_ms / its_lock));
+			c.excess_ms = 0;
+			for (const auto &h : c.hist) c.excess_ms += (double)(h.first - c.lock_ref) * h.second;
+			if (c.excess_ms >= refactor_alpha() * c.lu_ms) c.stale = true;
+		}
 		if (st) st->total_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 	}
 	return rc;
@@ -1014,4 +1148,8 @@ extern "C" void pgpu_fast_reset(void)
 	g_fast.fref = -1;
 	g_fast.stale = false;
 	g_fast.its_ref = -1;
+	g_fast.lock_ref = -1;
+	g_fast.excess_ms = 0;
+	g_fast.served = 0;
+	g_fast.span = 0;
 }
