@@ -135,126 +135,99 @@ struct slots {
 	slot s[FMAX];
 };
 
-// Y[:, f] = A_f X[:, f] for f < nact (vectors n x F row-major; columns f >= nact are zeroed):
-// one wave per pair of rows serving every system of the batch.  A lane reads the F entries of X
-// for its column as one contiguous run and uses them for both rows; the dense block rows of the
-// F systems stream lane-strided over the block's columns.  The two rows of a wave lie in the same
-// dense block except where a block starts; such a pair is handled row by row.
-struct rowref {
-	bool dense, lrow;
-	int c0, nn;
-	long roff;
-};
-
-__device__ __forceinline__ rowref row_of(int row, int offS, int nseg, int offQ, int nc)
+// Y[:, f] = A_f X[:, f] for f < nact (vectors n x F row-major; columns f >= nact are zeroed),
+// in two launches.
+//
+// k_mvdense: the dense blocks (Lp on the branch-current rows, P on the charge rows).  Both are
+// symmetric (the fills evaluate one triangle and mirror it), so B[c][r] serves for B[r][c] and a
+// lane can own a row while the wave reads columns: one load instruction takes 64 consecutive
+// entries of a stored row (1 KB).  Items are (64-row block of either dense block, column segment
+// of cs columns); each lane accumulates its row's sums over the segment for every active system
+// (the F entries of X at a column are the same for the whole wave: scalar loads) and writes them
+// to part[s][row][f].
+//
+// k_mvfin: per (row, f): the dense segments summed in increasing s, scaled by -jw on the Lp rows,
+// plus the row's sparse stamps (CSR).
+template <int F>
+__global__ __launch_bounds__(256) void k_mvdense(slots S, int nact, int offS, int nseg, int offQ, int nc, int cs, int nrbS,
+                                                 const double2 *__restrict__ X, double2 *__restrict__ part, int n)
 {
-	rowref r;
-	r.lrow = (row >= offS) && (row < offS + nseg);
-	const bool q = (row >= offQ) && (row < offQ + nc);
-	r.dense = r.lrow || q;
-	r.c0 = r.lrow ? offS : offQ;
-	r.nn = r.lrow ? nseg : (q ? nc : 0);
-	r.roff = r.lrow ? (long)(row - offS) * nseg : (q ? (long)(row - offQ) * nc : 0);
-	return r;
-}
-
-template <int NR>
-__device__ __forceinline__ void dense_rows(const slots &S, int nact, int F, const rowref *rr, const double2 *__restrict__ X,
-                                           int lane, double (&sr)[2][FMAX], double (&si)[2][FMAX])
-{
-	const double2 *a[NR][FMAX];
+	constexpr int CPS = (F >= 16) ? 1 : 16 / F;   // columns per step: 16 loads in flight per lane
+	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+	const int rb = blockIdx.x * 4 + wave, s = blockIdx.y;
+	const bool lblk = rb < nrbS;
+	const int nb = lblk ? nseg : nc, o = lblk ? offS : offQ;
+	const int rlo = (lblk ? rb : rb - nrbS) * 64;
+	const int cb = s * cs;
+	if ((rlo >= nb) || (cb >= nb)) return;
+	const int ce = min(nb, cb + cs);
+	const int lr = rlo + lane;
+	const bool act = lr < nb;
+	const double2 *B[F];
 #pragma unroll
-	for (int k = 0; k < NR; k++)
+	for (int f = 0; f < F; f++) B[f] = (f < nact) ? ((lblk ? S.s[f].lp : S.s[f].P) + (act ? lr : rlo)) : nullptr;
+	double ar[F], ai[F];
 #pragma unroll
-		for (int f = 0; f < FMAX; f++)
-			a[k][f] = ((f < nact) ? (rr[k].lrow ? S.s[f].lp : S.s[f].P) : S.s[0].lp) + rr[k].roff;
-	const int c0 = rr[0].c0, nn = rr[0].nn;
-	for (int c = lane; c < nn; c += 64) {
-		const double2 *xc = X + (long)(c0 + c) * F;
+	for (int f = 0; f < F; f++) ar[f] = ai[f] = 0;
+	for (int c = cb; c < ce; c += CPS) {
+		double2 m[CPS][F];
 #pragma unroll
-		for (int f = 0; f < FMAX; f++) {
-			if (f < nact) {
-				const double2 x = xc[f];
+		for (int u = 0; u < CPS; u++)
 #pragma unroll
-				for (int k = 0; k < NR; k++) {
-					const double2 p = zmul(a[k][f][c], x);
-					sr[k][f] += p.x;
-					si[k][f] += p.y;
+			for (int f = 0; f < F; f++)
+				m[u][f] = (act && (f < nact) && (c + u < ce)) ? B[f][(long)(c + u) * nb] : make_double2(0, 0);
+#pragma unroll
+		for (int u = 0; u < CPS; u++) {
+			if (c + u < ce) {
+				const double2 *xp = X + (long)(o + c + u) * F;
+#pragma unroll
+				for (int f = 0; f < F; f++) {
+					if (f < nact) {
+						const double2 x = xp[f];
+						ar[f] = fma(m[u][f].x, x.x, ar[f]);
+						ar[f] = fma(-m[u][f].y, x.y, ar[f]);
+						ai[f] = fma(m[u][f].x, x.y, ai[f]);
+						ai[f] = fma(m[u][f].y, x.x, ai[f]);
+					}
 				}
 			}
 		}
 	}
+	if (act) {
+#pragma unroll
+		for (int f = 0; f < F; f++) part[((long)s * n + o + lr) * F + f] = make_double2(ar[f], ai[f]);
+	}
 }
 
-__global__ __launch_bounds__(256) void k_matvec(slots S, int n, int F, int nact, int offS, int nseg, int offQ, int nc,
-                                                const double2 *__restrict__ X, double2 *__restrict__ Y)
+__global__ __launch_bounds__(256) void k_mvfin(slots S, int n, int F, int nact, int offS, int nseg, int offQ, int nc, int cs,
+                                               const double2 *__restrict__ part, const double2 *__restrict__ X,
+                                               double2 *__restrict__ Y)
 {
-	const int row0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 2;
-	const int lane = threadIdx.x & 63;
-	if (row0 >= n) return;
-	const int nr = min(2, n - row0);
-	rowref rr[2];
-	rr[0] = row_of(row0, offS, nseg, offQ, nc);
-	rr[1] = (nr > 1) ? row_of(row0 + 1, offS, nseg, offQ, nc) : rr[0];
-	double sr[2][FMAX], si[2][FMAX];
-#pragma unroll
-	for (int k = 0; k < 2; k++)
-#pragma unroll
-		for (int f = 0; f < FMAX; f++) sr[k][f] = si[k][f] = 0;
-	const bool paired = (nr == 2) && rr[0].dense && rr[1].dense && (rr[0].lrow == rr[1].lrow);
-	if (paired) {
-		dense_rows<2>(S, nact, F, rr, X, lane, sr, si);
-	}
-	else {
-		for (int k = 0; k < nr; k++) {
-			if (!rr[k].dense) continue;
-			double tr[2][FMAX], ti[2][FMAX];
-#pragma unroll
-			for (int f = 0; f < FMAX; f++) tr[0][f] = ti[0][f] = tr[1][f] = ti[1][f] = 0;
-			dense_rows<1>(S, nact, F, &rr[k], X, lane, tr, ti);
-#pragma unroll
-			for (int f = 0; f < FMAX; f++) {
-				sr[k][f] = tr[0][f];
-				si[k][f] = ti[0][f];
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= (long)n * F) return;
+	const int row = (int)(e / F), f = (int)(e - (long)row * F);
+	double2 y = make_double2(0, 0);
+	if (f < nact) {
+		const slot &sl = S.s[f];
+		const bool l = (row >= offS) && (row < offS + nseg);
+		const bool q = (row >= offQ) && (row < offQ + nc);
+		if (l || q) {
+			const int nsg = ((l ? nseg : nc) + cs - 1) / cs;
+			double tr = 0, ti = 0;
+			for (int sg = 0; sg < nsg; sg++) {
+				const double2 p = part[((long)sg * n + row) * F + f];
+				tr += p.x;
+				ti += p.y;
 			}
+			y = l ? make_double2(sl.omega * ti, -sl.omega * tr) : make_double2(tr, ti);   // Lp rows: -jw * sum
+		}
+		for (int k = sl.rp[row]; k < sl.rp[row + 1]; k++) {
+			const double2 p = zmul(sl.val[k], X[(long)sl.ci[k] * F + f]);
+			y.x += p.x;
+			y.y += p.y;
 		}
 	}
-#pragma unroll
-	for (int k = 0; k < 2; k++)
-#pragma unroll
-		for (int f = 0; f < FMAX; f++) {
-			if (f >= nact) continue;
-			for (int off = 32; off > 0; off >>= 1) {
-				sr[k][f] += __shfl_xor(sr[k][f], off);
-				si[k][f] += __shfl_xor(si[k][f], off);
-			}
-		}
-	if (lane < 2 * F) {
-		const int k = lane / F, f = lane - k * F;
-		if (k < nr) {
-			const int row = row0 + k;
-			double2 y = make_double2(0, 0);
-			if (f < nact) {
-				const slot &sl = S.s[f];
-				double tr = 0, ti = 0;
-#pragma unroll
-				for (int kk = 0; kk < 2; kk++)
-#pragma unroll
-					for (int g = 0; g < FMAX; g++)
-						if ((kk == k) && (g == f)) {
-							tr = sr[kk][g];
-							ti = si[kk][g];
-						}
-				y = rr[k].lrow ? make_double2(sl.omega * ti, -sl.omega * tr) : make_double2(tr, ti);   // Lp rows: -jw * sum
-				for (int e = sl.rp[row]; e < sl.rp[row + 1]; e++) {
-					const double2 p = zmul(sl.val[e], X[(long)sl.ci[e] * F + f]);
-					y.x += p.x;
-					y.y += p.y;
-				}
-			}
-			Y[(long)row * F + f] = y;
-		}
-	}
+	Y[e] = y;
 }
 
 // Y = B - Y (residual after a product), all columns
@@ -581,6 +554,7 @@ struct fast {
 	double2 *Z = nullptr;     // MKRYLOV x n x F preconditioned directions
 	double2 *W = nullptr, *Xv = nullptr, *Bv = nullptr, *R = nullptr, *T1 = nullptr;
 	double2 *tgp = nullptr;   // segment partial sums of the triangular products
+	double2 *mvp = nullptr;   // segment partial sums of the dense blocks of A x
 	double2 *partial = nullptr, *h = nullptr, *yd = nullptr;
 	double *scale = nullptr;
 };
@@ -641,7 +615,23 @@ void matvec(fast &c, hipStream_t s, const double2 *x, double2 *y, int F, int nac
 		const slotbuf &b = c.sl[(f < nact) ? f : 0];
 		S.s[f] = slot{b.rp, b.ci, b.val, b.lp, b.P, b.omega};
 	}
-	hipLaunchKernelGGL(k_matvec, dim3((c.n + 7) / 8), dim3(256), 0, s, S, c.n, F, nact, g_offS, c.nseg, g_offQ, c.nc, x, y);
+	const int cs = tg_cs();
+	const int nrbS = (c.nseg + 63) / 64, nrbQ = (c.nc + 63) / 64;
+	const dim3 grid((unsigned)((nrbS + nrbQ + 3) / 4), (unsigned)((std::max(c.nseg, c.nc) + cs - 1) / cs));
+	if ((nrbS + nrbQ) > 0) {
+		switch (F) {
+		case 1: hipLaunchKernelGGL(k_mvdense<1>, grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, cs, nrbS, x, c.mvp, c.n); break;
+		case 2: hipLaunchKernelGGL(k_mvdense<2>, grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, cs, nrbS, x, c.mvp, c.n); break;
+		case 3: hipLaunchKernelGGL(k_mvdense<3>, grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, cs, nrbS, x, c.mvp, c.n); break;
+		case 4: hipLaunchKernelGGL(k_mvdense<4>, grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, cs, nrbS, x, c.mvp, c.n); break;
+		case 5: hipLaunchKernelGGL(k_mvdense<5>, grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, cs, nrbS, x, c.mvp, c.n); break;
+		case 6: hipLaunchKernelGGL(k_mvdense<6>, grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, cs, nrbS, x, c.mvp, c.n); break;
+		case 7: hipLaunchKernelGGL(k_mvdense<7>, grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, cs, nrbS, x, c.mvp, c.n); break;
+		default: hipLaunchKernelGGL(k_mvdense<8>, grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, cs, nrbS, x, c.mvp, c.n); break;
+		}
+	}
+	hipLaunchKernelGGL(k_mvfin, dim3(blocks((long)c.n * F)), dim3(256), 0, s, S, c.n, F, nact, g_offS, c.nseg, g_offQ, c.nc, cs,
+	                   (const double2 *)c.mvp, x, y);
 	if (g_mvt.on) {
 		HIPCHK(hipEventRecord(g_mvt.ev[g_mvt.used + 1], s));
 		g_mvt.used += 2;
@@ -809,7 +799,7 @@ void free_fast(fast &c)
 		for (void *p : bb)
 			if (p) HIPCHK(hipFree(p));
 	}
-	void *bufs[] = {c.luw, c.inv, c.scr, c.ML, c.MU, c.dinvL, c.dinvU, c.piv, c.perm, c.V, c.Z, c.W, c.Xv, c.Bv, c.R, c.T1, c.tgp, c.partial, c.h, c.yd, c.scale};
+	void *bufs[] = {c.luw, c.inv, c.scr, c.ML, c.MU, c.dinvL, c.dinvU, c.piv, c.perm, c.V, c.Z, c.W, c.Xv, c.Bv, c.R, c.T1, c.tgp, c.mvp, c.partial, c.h, c.yd, c.scale};
 	for (void *b : bufs)
 		if (b) HIPCHK(hipFree(b));
 	c = fast();
@@ -936,6 +926,7 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 		c.R = dal<double2>(nf);
 		c.T1 = dal<double2>(nf);
 		c.tgp = dal<double2>(tg_part_count(n, FMAX));
+		c.mvp = dal<double2>((size_t)((std::max(p->nseg, nc) + tg_cs() - 1) / tg_cs()) * n * FMAX);
 		c.partial = dal<double2>((size_t)((n + DOT_ROWS - 1) / DOT_ROWS) * (MKRYLOV + 1) * FMAX);
 		c.h = dal<double2>((size_t)(MKRYLOV + 1) * FMAX);
 		c.yd = dal<double2>((size_t)(MKRYLOV + 1) * FMAX);
