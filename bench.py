@@ -9,17 +9,19 @@ of that sweep, device resident from the cell arrays to Z(f).
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--form mixed|plain] [--batch B] [--no-cpu]
 
 Forms.  mixed (default): the sweep's points 0..K-1 in one accelerated sweep -- k-factored Lp/P
-fill, charge-unknown system, one FP64 LU of A(f_ref) inverted into FP32 factors, FP64 flexible
-GMRES to a 1e-10 relative residual in lock-step batches of B frequencies; the LU, the
-inversion and the fill preparation run inside the timed region (cold start).  plain: every
+fill, charge-unknown system, an LU of A(f_ref) on FP32 storage (FP64 panels and pivoting, FP32
+MFMA trailing updates) whose factors are inverted in FP32, FP64 flexible GMRES to a 1e-10
+relative residual in lock-step batches of B frequencies, a fresh factorisation whenever the
+extra iterations have cost about as much as one; the LUs, the inversions and the fill
+preparation run inside the timed region (cold start).  plain: every
 point independently (fill, P LU and inverse, assembly, FP64 LU, solve), K points spread over
 the sweep.  Either way value = K / (wall time of the K points).
 
 Zin of the mixed form is compared with the plain FP64 form on three of the timed points (the
 metric is quoted at Zin rel err <= 1e-6); the plain form's own rate on those points is reported
-beside it.  The roofline block is the LU trailing-update GEMM (k_zgemm, FP64 MFMA) timed with HIP
-events on the library's stream around every launch; the GMRES matvec is reported in GB/s the
-same way.  The precision form (FP32-stored LU, FP32 MFMA trailing updates, FP64 refinement)
+beside it.  The roofline block is the LU trailing-update GEMM (rocBLAS cgemm on FP32 MFMA) timed
+with HIP events on the library's stream around every launch; the GMRES matvec is reported in
+GB/s the same way.  The precision form (FP32-stored LU, FP32 MFMA trailing updates, FP64 refinement)
 runs on the same three points and reports its rate and its Zin error against the plain form per
 point.  cpu_baseline times the oracle (the reference's arithmetic restated in C, OpenMP on
 all host cores) on a bounded sample and extrapolates to one frequency point.
@@ -192,17 +194,23 @@ def precision_points(m, idx, zp):
 
 def lu_roofline(achieved: float) -> dict:
     """Roofline of the accelerated sweep's dominant kernel, the trailing update of the LU of
-    A(f_ref): FP32 MFMA (k_cgemm32) by default, FP64 MFMA (k_zgemm) with PEEC_MIXED_LU64=1."""
+    A(f_ref): FP32 storage by default (rocBLAS cgemm on FP32 MFMA; the hand-written k_cgemm32 with
+    PEEC_GEMM=hip), FP64 with PEEC_MIXED_LU64=1 (rocBLAS zgemm / k_zgemm on FP64 MFMA)."""
+    hip = os.environ.get("PEEC_GEMM", "") == "hip"
     if os.environ.get("PEEC_MIXED_LU64", "0") == "1":
-        name, peak, prof = "k_zgemm (LU trailing update of A(f_ref), FP64 MFMA)", PEAK_FP64_MFMA, "k_zgemm_plate64_mixed_sweep.md"
-        alg = "16 (2 M N + (M + N) K) B"
+        peak, alg = PEAK_FP64_MFMA, "16 (2 M N + (M + N) K) B"
+        name = "k_zgemm" if hip else "rocBLAS zgemm"
+        prof = "k_zgemm_plate64_mixed_sweep.md" if hip else None
+        name += " (LU trailing update of A(f_ref), FP64 MFMA)"
     else:
-        name, peak, prof = "k_cgemm32 (LU trailing update of A(f_ref), FP32 MFMA)", PEAK_FP32_MFMA, "k_cgemm32_plate64_mixed_sweep.md"
-        alg = "8 (2 M N + (M + N) K) B"
+        peak, alg = PEAK_FP32_MFMA, "8 (2 M N + (M + N) K) B"
+        name = "k_cgemm32" if hip else "rocBLAS cgemm"
+        prof = "k_cgemm32_plate64_mixed_sweep.md" if hip else "cgemm_lu_update_plate64_mixed_sweep.md"
+        name += " (LU trailing update of A(f_ref), FP32 MFMA)"
     return {"kernel": name, "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-            "frac": achieved / peak, "traffic": lu_traffic(prof),
-            "traffic_note": f"HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a copy of known size, "
-                            f"profiles/{prof}); algorithmic {alg}"}
+            "frac": achieved / peak, "traffic": lu_traffic(prof) if prof else None,
+            "traffic_note": (f"HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a copy of known size, "
+                             f"profiles/{prof}); algorithmic {alg}") if prof else f"not profiled; algorithmic {alg}"}
 
 
 def lu_traffic(name: str) -> float | None:
@@ -514,7 +522,8 @@ def main():
             "fill_gflops": fill_flop / (fill_ms * 1e-3) / 1e9 if fill_ms > 0 else None,
             "fill_frac_of_fp64_vector_peak": (fill_flop / (fill_ms * 1e-3) / 1e12) / PEAK_FP64_VECTOR if fill_ms > 0 else None,
             "roofline": {
-                "kernel": "k_zgemm (LU trailing update, FP64 MFMA)",
+                "kernel": ("k_zgemm" if os.environ.get("PEEC_GEMM", "") == "hip" else "rocBLAS zgemm")
+                          + " (LU trailing update, FP64 MFMA)",
                 "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP64_MFMA, "traffic": None,
                 "launches": st.lu_gemm_launches, "avg_launch_ms": gemm_avg_ms, "avg_launch_flop": gemm_avg_flop,

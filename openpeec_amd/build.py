@@ -64,7 +64,7 @@ def build(verbose: bool = False) -> Path:
             f.result()
     objs = [str(_obj(s)) for s in c_srcs + hip_srcs]
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-Bsymbolic", "-o", str(tmp), *objs, "-lgomp", "-lm"])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-Bsymbolic", "-o", str(tmp), *objs, "-lrocblas", "-lgomp", "-lm"])
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")

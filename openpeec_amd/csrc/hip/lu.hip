@@ -14,12 +14,17 @@
 //   k_laswp     applies the panel's swaps to the columns left and right of the panel and to
 //               the running row permutation (used by the solves as one gather).
 //   k_trsm_lu   U12 = L11^-1 A12 (unit lower, LDS-staged).
-//   k_zgemm     A22 -= L21 U12 on FP64 MFMA (v_mfma_f64_16x16x4), complex as 4 real products.
+//   GEMM        A22 -= L21 U12: rocBLAS zgemm / cgemm by default; PEEC_GEMM=hip selects the
+//               hand-written k_zgemm (FP64 MFMA, v_mfma_f64_16x16x4) / k_cgemm32 (FP32 MFMA),
+//               complex as 4 real products.
 // The trailing update's summation order differs from the reference's rank-1 sequence, so the
 // factors agree with the reference to rounding, not bitwise; pivots agree except on near-ties.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 #include <climits>
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include "../peec_internal.h"
 #include "hip_check.hpp"
 
@@ -545,10 +550,50 @@ void ensure_ws(int n)
 	}
 }
 
+// The products C -= A B of the factorisation and of the triangular inversion are plain GEMMs:
+// rocBLAS (zgemm / cgemm, Tensile kernels tuned for gfx950) runs them by default; PEEC_GEMM=hip
+// selects the hand-written MFMA kernels k_zgemm / k_cgemm32.  Row-major C -= A B is issued as the
+// column-major C^T = -B^T A^T + C^T.
+bool gemm_hip()
+{
+	const char *e = getenv("PEEC_GEMM");   // read per call (a few hundred calls per factorisation)
+	return e && (strcmp(e, "hip") == 0);
+}
+
+rocblas_handle blas()
+{
+	static rocblas_handle h = nullptr;
+	if (h == nullptr) {
+		if (rocblas_create_handle(&h) != rocblas_status_success) {
+			fprintf(stderr, "*** rocblas_create_handle failed\n");
+			abort();
+		}
+	}
+	return h;
+}
+
+void blas_check(rocblas_status st, const char *what)
+{
+	if (st != rocblas_status_success) {
+		fprintf(stderr, "*** %s failed: %s\n", what, rocblas_status_to_string(st));
+		abort();
+	}
+}
+
 void launch_gemm(hipStream_t s, int M, int N, int K, const double2 *A, long lda, const double2 *B, long ldb,
                  double2 *C, long ldc)
 {
 	if ((M <= 0) || (N <= 0) || (K <= 0)) return;
+	if (!gemm_hip()) {
+		rocblas_handle h = blas();
+		blas_check(rocblas_set_stream(h, s), "rocblas_set_stream");
+		const rocblas_double_complex alpha{-1.0, 0.0}, beta{1.0, 0.0};
+		blas_check(rocblas_zgemm(h, rocblas_operation_none, rocblas_operation_none, N, M, K, &alpha,
+		                         (const rocblas_double_complex *)B, (rocblas_int)ldb, (const rocblas_double_complex *)A,
+		                         (rocblas_int)lda, &beta, (rocblas_double_complex *)C, (rocblas_int)ldc),
+		           "rocblas_zgemm");
+		return;
+	}
 	dim3 grid((N + 127) / 128, (M + 63) / 64);
 	hipLaunchKernelGGL(k_zgemm, grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc);
 }
@@ -630,6 +675,16 @@ void launch_update(hipStream_t s, int M, int N, int K, const float2 *A, long lda
                    long ldc)
 {
 	if ((M <= 0) || (N <= 0) || (K <= 0)) return;
+	if (!gemm_hip()) {
+		rocblas_handle h = blas();
+		blas_check(rocblas_set_stream(h, s), "rocblas_set_stream");
+		const rocblas_float_complex alpha{-1.0f, 0.0f}, beta{1.0f, 0.0f};
+		blas_check(rocblas_cgemm(h, rocblas_operation_none, rocblas_operation_none, N, M, K, &alpha,
+		                         (const rocblas_float_complex *)B, (rocblas_int)ldb, (const rocblas_float_complex *)A,
+		                         (rocblas_int)lda, &beta, (rocblas_float_complex *)C, (rocblas_int)ldc),
+		           "rocblas_cgemm");
+		return;
+	}
 	dim3 grid((N + 63) / 64, (M + 63) / 64);
 	hipLaunchKernelGGL(k_cgemm32, grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc);
 }

@@ -332,6 +332,26 @@ __global__ void k_add(double2 *X, const double2 *D, long cnt)
 	X[e] = make_double2(X[e].x + D[e].x, X[e].y + D[e].y);
 }
 
+// initial guesses of a batch from the previous batch's solutions of the same port (n x F each):
+// X[:, b] = P[:, ia] + t_b (P[:, ia] - P[:, ip]) for b < nb (linear extrapolation in frequency;
+// t_b = 0 when only one previous point exists), zero for the unused columns
+struct xcoef {
+	double t[FMAX];
+};
+__global__ void k_extrap(const double2 *P, double2 *X, int n, int F, int nb, int ia, int ip, xcoef tc)
+{
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= (long)n * F) return;
+	const int i = (int)(e / F), b = (int)(e - (long)i * F);
+	double2 x = make_double2(0, 0);
+	if (b < nb) {
+		const double2 a = P[(long)i * F + ia], p = P[(long)i * F + ip];
+		const double t = tc.t[b];
+		x = make_double2(a.x + t * (a.x - p.x), a.y + t * (a.y - p.y));
+	}
+	X[e] = x;
+}
+
 __global__ void k_zero(double2 *X, long cnt)
 {
 	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -548,6 +568,9 @@ struct fast {
 	                                            // iterations, ms per iteration
 	int served = 0;           // points solved with the current factorisation
 	int span = 0;             // points the previous factorisation served (0: none yet)
+	std::vector<double2 *> xw;   // per port: the previous batch's solutions (n x F)
+	int xw_nb = 0;            // points in the previous batch (0: none)
+	double xw_f[FMAX] = {};   // their frequencies
 	slotbuf ahead;            // system at a point ahead of the current batch (stamps; its Lp and P
 	                          // live in the inverse workspace until the factorisation)
 	double2 *V = nullptr;     // (MKRYLOV + 1) x n x F Arnoldi basis
@@ -668,7 +691,8 @@ void dots(fast &c, hipStream_t s, const double2 *V, long vstride, int nk, const 
 
 // batched restarted flexible GMRES on F systems A_f x_f = b_f (b in c.Bv, x in c.Xv); returns
 // per-system iteration counts, -1 for a system that did not converge
-std::vector<int> gmres_batch(fast &c, hipStream_t s, int F, int nactive)
+// warm: c.Xv already holds the initial guesses, else they are zero
+std::vector<int> gmres_batch(fast &c, hipStream_t s, int F, int nactive, bool warm)
 {
 	const int n = c.n;
 	const long nf = (long)n * F;
@@ -680,7 +704,7 @@ std::vector<int> gmres_batch(fast &c, hipStream_t s, int F, int nactive)
 	std::vector<double2> hh;
 	dots(c, s, c.Bv, 0, 1, c.Bv, F, hh);
 	for (int f = 0; f < F; f++) bnorm[f] = std::sqrt(hh[f].x);
-	hipLaunchKernelGGL(k_zero, dim3(blocks(nf)), dim3(256), 0, s, c.Xv, nf);
+	if (!warm) hipLaunchKernelGGL(k_zero, dim3(blocks(nf)), dim3(256), 0, s, c.Xv, nf);
 	for (int cyc = 0; cyc < MCYCLES; cyc++) {
 		// r = b - A x
 		matvec(c, s, c.Xv, c.R, F, nactive);
@@ -794,6 +818,8 @@ void free_fast(fast &c)
 		for (void *p : bb)
 			if (p) HIPCHK(hipFree(p));
 	}
+	for (double2 *p : c.xw)
+		if (p) HIPCHK(hipFree(p));
 	{
 		void *bb[] = {c.ahead.rp, c.ahead.ci, c.ahead.ri, c.ahead.val};
 		for (void *p : bb)
@@ -852,6 +878,17 @@ void mna_stamps_charge(const peec_t *p, double f, stamps_t *st);
 
 // PEEC_TRACE=1 prints the iteration counts of every batch to stderr
 static const bool g_trace = getenv("PEEC_TRACE") && getenv("PEEC_TRACE")[0] == '1';
+
+// initial guesses extrapolated from the previous batch (PEEC_WARM=0 starts every batch from zero)
+static bool warm_start()
+{
+	static int w = -1;
+	if (w < 0) {
+		const char *e = getenv("PEEC_WARM");
+		w = (e && (e[0] == '0')) ? 0 : 1;
+	}
+	return w == 1;
+}
 
 // refactor rule: PEEC_REFACTOR_ALPHA > 0 prices extra iterations against the factorisation time
 // (0.75 by default: iteration counts grow about linearly with the distance from the factorisation's
@@ -960,6 +997,7 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 			st->kf_direct_pairs = kf ? pgpu_kf_direct_pairs(g) : -1;
 		}
 	}
+	c.xw_nb = 0;   // initial guesses only from batches of this call
 	for (int f0 = 0; (f0 < nfreq) && !rc; f0 += F) {
 		const int nb = std::min(F, nfreq - f0);
 		const auto t0 = std::chrono::steady_clock::now();
@@ -1072,14 +1110,28 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 			const auto tg = std::chrono::steady_clock::now();
 			g_mvt.on = (st != nullptr);
 			g_mvt.reset();
-			std::vector<int> its = gmres_batch(c, s, F, nb);
+			// initial guesses: the previous batch's solutions of this port extrapolated to this
+			// batch's frequencies (the solution varies smoothly between sweep points)
+			bool warm = false;
+			if (warm_start() && (c.xw_nb > 0) && (j < (int)c.xw.size())) {
+				const int ia = c.xw_nb - 1, ip = (c.xw_nb > 1) ? c.xw_nb - 2 : ia;
+				xcoef tc{};
+				for (int b = 0; b < nb; b++) {
+					const double df = c.xw_f[ia] - c.xw_f[ip];
+					tc.t[b] = (df != 0) ? (freq_at(p, ifreq0 + f0 + b) - c.xw_f[ia]) / df : 0.0;
+				}
+				hipLaunchKernelGGL(k_extrap, dim3(blocks((long)n * F)), dim3(256), 0, s, (const double2 *)c.xw[j], c.Xv, n, F,
+				                   nb, ia, ip, tc);
+				warm = true;
+			}
+			std::vector<int> its = gmres_batch(c, s, F, nb, warm);
 			if (st) mv_collect(st);
 			int lock = 0;
 			for (int b = 0; b < nb; b++) {
 				if (its[b] < 0) {
 					// stale preconditioner: refactor at this frequency and solve the batch again
 					if ((rc = refactor(b))) break;
-					its = gmres_batch(c, s, F, nb);
+					its = gmres_batch(c, s, F, nb, true);
 					if (st) mv_collect(st);
 					if (its[b] < 0) {
 						rc = 4;
@@ -1098,6 +1150,9 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 				fprintf(stderr, " (ref %.4g Hz, excess %.0f ms of %.0f)\n", c.fref, c.excess_ms, c.lu_ms);
 			}
 			if (st) st->solve_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tg).count();
+			if ((int)c.xw.size() <= j) c.xw.resize(j + 1, nullptr);
+			if (c.xw[j] == nullptr) c.xw[j] = dal<double2>((size_t)n * FMAX);
+			HIPCHK(hipMemcpyAsync(c.xw[j], c.Xv, (size_t)n * F * sizeof(double2), hipMemcpyDeviceToDevice, s));
 			HIPCHK(hipMemcpyAsync(hx.data(), c.Xv, hx.size() * sizeof(double2), hipMemcpyDeviceToHost, s));
 			HIPCHK(hipStreamSynchronize(s));
 			for (int b = 0; b < nb; b++)
@@ -1107,6 +1162,8 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 					zmat[((size_t)(f0 + b) * np + i) * np + j] = d_complex(v1.x - v2.x, v1.y - v2.y);
 				}
 		}
+		c.xw_nb = nb;
+		for (int b = 0; b < nb; b++) c.xw_f[b] = freq_at(p, ifreq0 + f0 + b);
 		c.served += nb;
 		if (c.its_ref < 0) {
 			c.its_ref = its_batch;
@@ -1143,4 +1200,5 @@ extern "C" void pgpu_fast_reset(void)
 	g_fast.excess_ms = 0;
 	g_fast.served = 0;
 	g_fast.span = 0;
+	g_fast.xw_nb = 0;
 }

@@ -24,9 +24,13 @@ def _rand(n, seed):
     return (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))).astype(np.complex128)
 
 
+@pytest.mark.parametrize("gemm", ["rocblas", "hip"])
 @pytest.mark.parametrize("n", [1, 2, 7, 64, 65, 130, 300, 777])
-def test_lu_solve_matches_lapack(lib, n):
-    print(f"[lu] n={n}", flush=True)
+def test_lu_solve_matches_lapack(lib, n, gemm, monkeypatch):
+    """Both trailing-update GEMMs: rocBLAS zgemm (default) and the hand-written k_zgemm."""
+    if gemm == "hip":
+        monkeypatch.setenv("PEEC_GEMM", "hip")
+    print(f"[lu] n={n} gemm={gemm}", flush=True)
     a0 = _rand(n, 1000 + n)
     b0 = (np.arange(n) + 1j).astype(np.complex128)
     a = a0.copy()

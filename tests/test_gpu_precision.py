@@ -37,9 +37,13 @@ def _split(lu, piv):
     return L, U, perm
 
 
+@pytest.mark.parametrize("gemm", ["rocblas", "hip"])
 @pytest.mark.parametrize("n,seed,boost", [(64, 1, 0.0), (200, 2, 0.0), (333, 3, 5.0), (513, 4, 0.0)])
-def test_lu_decomp32_backward_error(lib, n, seed, boost):
-    print(f"[lu32] n={n} seed={seed}", flush=True)
+def test_lu_decomp32_backward_error(lib, n, seed, boost, gemm, monkeypatch):
+    """Both trailing-update GEMMs: rocBLAS cgemm (default) and the hand-written k_cgemm32."""
+    if gemm == "hip":
+        monkeypatch.setenv("PEEC_GEMM", "hip")
+    print(f"[lu32] n={n} seed={seed} gemm={gemm}", flush=True)
     rng = np.random.default_rng(seed)
     a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + boost * np.eye(n)
     a = np.ascontiguousarray(a, dtype=np.complex128)
