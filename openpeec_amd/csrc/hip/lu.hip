@@ -794,6 +794,40 @@ void scale_block(hipStream_t s, TS *X, long ld, int rows, int cols, double sc)
 // -inv(T22) T21 inv(T11) (lower) and -inv(T11) T12 inv(T22) (upper), two GEMMs on MFMA (FP64
 // for double2 storage, FP32 for float2).  Leaves are the diagonal-block inverses.  scr holds at
 // least ceil(n/2)^2 + TI n entries.
+// C = alpha * B * A (right) or alpha * A * B (left) for a triangular A, all column-major as
+// rocBLAS sees them (rocblas_?trmm, out of place); upper = A's fill in that view
+void trmm(hipStream_t s, bool right, bool upper, int m, int n, double alpha, const float2 *A, long lda, const float2 *B,
+          long ldb, float2 *C, long ldc)
+{
+	rocblas_handle h = blas();
+	blas_check(rocblas_set_stream(h, s), "rocblas_set_stream");
+	const rocblas_float_complex a{(float)alpha, 0.0f};
+	blas_check(rocblas_ctrmm(h, right ? rocblas_side_right : rocblas_side_left, upper ? rocblas_fill_upper : rocblas_fill_lower,
+	                         rocblas_operation_none, rocblas_diagonal_non_unit, m, n, &a, (const rocblas_float_complex *)A,
+	                         (rocblas_int)lda, (const rocblas_float_complex *)B, (rocblas_int)ldb, (rocblas_float_complex *)C,
+	                         (rocblas_int)ldc),
+	           "rocblas_ctrmm");
+}
+
+void trmm(hipStream_t s, bool right, bool upper, int m, int n, double alpha, const double2 *A, long lda, const double2 *B,
+          long ldb, double2 *C, long ldc)
+{
+	rocblas_handle h = blas();
+	blas_check(rocblas_set_stream(h, s), "rocblas_set_stream");
+	const rocblas_double_complex a{alpha, 0.0};
+	blas_check(rocblas_ztrmm(h, right ? rocblas_side_right : rocblas_side_left, upper ? rocblas_fill_upper : rocblas_fill_lower,
+	                         rocblas_operation_none, rocblas_diagonal_non_unit, m, n, &a, (const rocblas_double_complex *)A,
+	                         (rocblas_int)lda, (const rocblas_double_complex *)B, (rocblas_int)ldb, (rocblas_double_complex *)C,
+	                         (rocblas_int)ldc),
+	           "rocblas_ztrmm");
+}
+
+// Recursive inversion: inv([T11 0; T21 T22]) = [O11 0; -O22 T21 O11  O22] (lower; the upper case
+// mirrors it), the diagonal 64 x 64 blocks inverted directly (k_diaginv).  With rocBLAS both
+// products of a level are triangular multiplies (O11 and O22 are triangular), half the flops of
+// full GEMMs; row-major X is the column-major X^T, so a row-major lower triangle is an upper
+// one to rocBLAS and a right-hand product becomes a left-hand one.  PEEC_GEMM=hip runs them as
+// full products on the hand-written GEMM (the zeros above the diagonal included).
 template <typename TS>
 void tri_inv(hipStream_t s, bool lower, const TS *T, long ld, TS *Out, int i0, int n, const TS *dinv, TS *scr)
 {
@@ -807,6 +841,21 @@ void tri_inv(hipStream_t s, bool lower, const TS *T, long ld, TS *Out, int i0, i
 	tri_inv(s, lower, T, ld, Out, i0 + n1, n2, dinv, scr);
 	const TS *O11 = Out + (long)i0 * ld + i0;
 	const TS *O22 = Out + (long)(i0 + n1) * ld + (i0 + n1);
+	if (!gemm_hip()) {
+		if (lower) {
+			// scr = -T21 O11 (n2 x n1, ld n1); O21 = O22 scr
+			TS *O21 = Out + (long)(i0 + n1) * ld + i0;
+			trmm(s, false, true, n1, n2, -1.0, O11, ld, T + (long)(i0 + n1) * ld + i0, ld, scr, n1);
+			trmm(s, true, true, n1, n2, 1.0, O22, ld, (const TS *)scr, n1, O21, ld);
+		}
+		else {
+			// scr = -T12 O22 (n1 x n2, ld n2); O12 = O11 scr
+			TS *O12 = Out + (long)i0 * ld + (i0 + n1);
+			trmm(s, false, false, n2, n1, -1.0, O22, ld, T + (long)i0 * ld + (i0 + n1), ld, scr, n2);
+			trmm(s, true, false, n2, n1, 1.0, O11, ld, (const TS *)scr, n2, O12, ld);
+		}
+		return;
+	}
 	if (lower) {
 		// C = -T21 inv(T11) (n2 x n1); Out21 = -(inv(T22) C), then negated
 		scale_block(s, scr, n1, n2, n1, 0);

@@ -333,21 +333,25 @@ __global__ void k_add(double2 *X, const double2 *D, long cnt)
 }
 
 // initial guesses of a batch from the previous batch's solutions of the same port (n x F each):
-// X[:, b] = P[:, ia] + t_b (P[:, ia] - P[:, ip]) for b < nb (linear extrapolation in frequency;
-// t_b = 0 when only one previous point exists), zero for the unused columns
+// X[:, b] = sum_q w[b][q] P[:, col[q]] for b < nb (Lagrange extrapolation in frequency through
+// up to three previous points), zero for the unused columns
 struct xcoef {
-	double t[FMAX];
+	double w[FMAX][3];
+	int col[3];
+	int nq;
 };
-__global__ void k_extrap(const double2 *P, double2 *X, int n, int F, int nb, int ia, int ip, xcoef tc)
+__global__ void k_extrap(const double2 *P, double2 *X, int n, int F, int nb, xcoef tc)
 {
 	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (e >= (long)n * F) return;
 	const int i = (int)(e / F), b = (int)(e - (long)i * F);
 	double2 x = make_double2(0, 0);
 	if (b < nb) {
-		const double2 a = P[(long)i * F + ia], p = P[(long)i * F + ip];
-		const double t = tc.t[b];
-		x = make_double2(a.x + t * (a.x - p.x), a.y + t * (a.y - p.y));
+		for (int q = 0; q < tc.nq; q++) {
+			const double2 a = P[(long)i * F + tc.col[q]];
+			x.x += tc.w[b][q] * a.x;
+			x.y += tc.w[b][q] * a.y;
+		}
 	}
 	X[e] = x;
 }
@@ -495,6 +499,19 @@ int tg_cs()
 	return cs;
 }
 
+// column segment of the dense blocks of A x (PEEC_MV_CS, a multiple of 64 in [64, 8192]; 256 by
+// default: about 5000 items on the 64 x 64 plate, enough to fill every SIMD)
+int mv_cs()
+{
+	static int cs = -1;
+	if (cs < 0) {
+		const char *e = getenv("PEEC_MV_CS");
+		const int v = e ? atoi(e) : 256;
+		cs = ((v >= 64) && (v <= 8192) && (v % 64 == 0)) ? v : 256;
+	}
+	return cs;
+}
+
 // doubles of partial sums the products need for order n and F columns
 size_t tg_part_count(int n, int F)
 {
@@ -638,7 +655,7 @@ void matvec(fast &c, hipStream_t s, const double2 *x, double2 *y, int F, int nac
 		const slotbuf &b = c.sl[(f < nact) ? f : 0];
 		S.s[f] = slot{b.rp, b.ci, b.val, b.lp, b.P, b.omega};
 	}
-	const int cs = tg_cs();
+	const int cs = mv_cs();
 	const int nrbS = (c.nseg + 63) / 64, nrbQ = (c.nc + 63) / 64;
 	const dim3 grid((unsigned)((nrbS + nrbQ + 3) / 4), (unsigned)((std::max(c.nseg, c.nc) + cs - 1) / cs));
 	if ((nrbS + nrbQ) > 0) {
@@ -879,15 +896,16 @@ void mna_stamps_charge(const peec_t *p, double f, stamps_t *st);
 // PEEC_TRACE=1 prints the iteration counts of every batch to stderr
 static const bool g_trace = getenv("PEEC_TRACE") && getenv("PEEC_TRACE")[0] == '1';
 
-// initial guesses extrapolated from the previous batch (PEEC_WARM=0 starts every batch from zero)
-static bool warm_start()
+// initial guesses extrapolated from the previous batch through its last PEEC_WARM points (1..3;
+// 3, quadratic, by default; 0 starts every batch from zero)
+static int warm_start()
 {
 	static int w = -1;
 	if (w < 0) {
 		const char *e = getenv("PEEC_WARM");
-		w = (e && (e[0] == '0')) ? 0 : 1;
+		w = e ? std::min(3, std::max(0, atoi(e))) : 3;
 	}
-	return w == 1;
+	return w;
 }
 
 // refactor rule: PEEC_REFACTOR_ALPHA > 0 prices extra iterations against the factorisation time
@@ -963,7 +981,7 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 		c.R = dal<double2>(nf);
 		c.T1 = dal<double2>(nf);
 		c.tgp = dal<double2>(tg_part_count(n, FMAX));
-		c.mvp = dal<double2>((size_t)((std::max(p->nseg, nc) + tg_cs() - 1) / tg_cs()) * n * FMAX);
+		c.mvp = dal<double2>((size_t)((std::max(p->nseg, nc) + mv_cs() - 1) / mv_cs()) * n * FMAX);
 		c.partial = dal<double2>((size_t)((n + DOT_ROWS - 1) / DOT_ROWS) * (MKRYLOV + 1) * FMAX);
 		c.h = dal<double2>((size_t)(MKRYLOV + 1) * FMAX);
 		c.yd = dal<double2>((size_t)(MKRYLOV + 1) * FMAX);
@@ -1113,15 +1131,23 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 			// initial guesses: the previous batch's solutions of this port extrapolated to this
 			// batch's frequencies (the solution varies smoothly between sweep points)
 			bool warm = false;
-			if (warm_start() && (c.xw_nb > 0) && (j < (int)c.xw.size())) {
-				const int ia = c.xw_nb - 1, ip = (c.xw_nb > 1) ? c.xw_nb - 2 : ia;
+			if ((warm_start() > 0) && (c.xw_nb > 0) && (j < (int)c.xw.size())) {
 				xcoef tc{};
+				tc.nq = std::min(warm_start(), c.xw_nb);
+				for (int q = 0; q < tc.nq; q++) tc.col[q] = c.xw_nb - 1 - q;
+				for (int q = 1; q < tc.nq; q++)
+					if (c.xw_f[tc.col[q]] == c.xw_f[tc.col[q - 1]]) tc.nq = 1;   // repeated frequency: constant
 				for (int b = 0; b < nb; b++) {
-					const double df = c.xw_f[ia] - c.xw_f[ip];
-					tc.t[b] = (df != 0) ? (freq_at(p, ifreq0 + f0 + b) - c.xw_f[ia]) / df : 0.0;
+					const double fb = freq_at(p, ifreq0 + f0 + b);
+					for (int q = 0; q < tc.nq; q++) {
+						double w = 1;
+						for (int r = 0; r < tc.nq; r++)
+							if (r != q) w *= (fb - c.xw_f[tc.col[r]]) / (c.xw_f[tc.col[q]] - c.xw_f[tc.col[r]]);
+						tc.w[b][q] = w;
+					}
 				}
 				hipLaunchKernelGGL(k_extrap, dim3(blocks((long)n * F)), dim3(256), 0, s, (const double2 *)c.xw[j], c.Xv, n, F,
-				                   nb, ia, ip, tc);
+				                   nb, tc);
 				warm = true;
 			}
 			std::vector<int> its = gmres_batch(c, s, F, nb, warm);
