@@ -43,6 +43,11 @@ void kf_rho_launch(hipStream_t s, const dcell_t *C, const dedge_t *edge, int n, 
 void kf_setup_launch(hipStream_t s, int which, int km, const pe::fill_args &f, double klo, double khi, const double *rho,
                      double *rec, int T, int2 *list, unsigned *nlist);
 void kf_eval_launch(hipStream_t s, int which, const pe::fill_args &f, const double *rec, int T);
+// the same for nk <= KF_FMAX wavenumbers in one pass over the records (out[k]: matrix at kw[k],
+// leading dimension f.ld; f.kw and f.out unused)
+constexpr int KF_FMAX = 8;
+void kf_eval_multi_launch(hipStream_t s, int which, const pe::fill_args &f, const double *rec, int T, int nk,
+                          const double *kw, void *const *out);
 
 // cell set of a device mirror as fill arguments (gpu_runtime.hip); which 0 current cells, 1
 // charge cells (carea set, sub-cell CSR returned)

@@ -9,6 +9,7 @@
 // Only the Z entries (and optional distributions) come back over PCIe.
 // Reference: solve.c:239-363 (sweep), mna.c:175-308 (assembly), potential.c:258-281 (C = P^-1).
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -583,7 +584,7 @@ void pgpu_kf_fill(pgpu_t *g, double kw, d_complex_t *dlp, d_complex_t *dP)
 	double2 *lp_out = (double2 *)dlp, *p_out = (double2 *)dP;
 	const peec_t *p = g->p;
 	const int ns = g->seg.n, nh = g->chg.n, nc = p->ncell;
-	if (ns > 0) {
+	if ((ns > 0) && lp_out) {
 		pe::fill_args a = base_args(g->seg, ns, kw, p->gp, lp_out, ns);
 		kf_eval_launch(g->s, 0, a, g->kf_rec_lp, g->kf_T);
 		fill_list_launch(g->s, 0, g->seg.km, a, g->kf_list_lp, g->kf_nd_lp);
@@ -601,6 +602,43 @@ void pgpu_kf_fill(pgpu_t *g, double kw, d_complex_t *dlp, d_complex_t *dP)
 			kf_eval_launch(g->s, 1, a, g->kf_rec_ig, g->kf_T);
 			fill_list_launch(g->s, 1, g->chg.km, a, g->kf_list_ig, g->kf_nd_ig);
 			aggregate_launch(g->s, g->G, nh, g->csoff, g->csidx, g->carea, nc, 4 * PI * EPS0, p_out, nc);
+		}
+	}
+}
+
+// the same for nk wavenumbers (a batch of frequencies): the records are read once for all of them
+void pgpu_kf_fill_multi(pgpu_t *g, int nk, const double *kw, d_complex_t *const *dlp, d_complex_t *const *dP)
+{
+	const peec_t *p = g->p;
+	const int ns = g->seg.n, nh = g->chg.n, nc = p->ncell;
+	if (nk > KF_FMAX) {
+		for (int k0 = 0; k0 < nk; k0 += KF_FMAX)
+			pgpu_kf_fill_multi(g, std::min(KF_FMAX, nk - k0), kw + k0, dlp + k0, dP ? dP + k0 : nullptr);
+		return;
+	}
+	if (ns > 0) {
+		pe::fill_args a = base_args(g->seg, ns, kw[0], p->gp, dlp[0], ns);
+		kf_eval_multi_launch(g->s, 0, a, g->kf_rec_lp, g->kf_T, nk, kw, (void *const *)dlp);
+		for (int k = 0; k < nk; k++) {
+			a.kw = kw[k];
+			a.out = dlp[k];
+			fill_list_launch(g->s, 0, g->seg.km, a, g->kf_list_lp, g->kf_nd_lp);
+		}
+	}
+	if ((nh > 0) && (nc > 0) && dP) {
+		if (g->cells_identity) {
+			pe::fill_args a = base_args(g->chg, nh, kw[0], p->gp, dP[0], nc);
+			a.carea = g->carea;
+			kf_eval_multi_launch(g->s, 1, a, g->kf_rec_ig, g->kf_T, nk, kw, (void *const *)dP);
+			for (int k = 0; k < nk; k++) {
+				a.kw = kw[k];
+				a.out = dP[k];
+				fill_list_launch(g->s, 1, g->chg.km, a, g->kf_list_ig, g->kf_nd_ig);
+			}
+		}
+		else {
+			// sub-cell integrals aggregated per frequency through the one G buffer
+			for (int k = 0; k < nk; k++) pgpu_kf_fill(g, kw[k], nullptr, dP[k]);
 		}
 	}
 }

@@ -55,4 +55,27 @@ __device__ __forceinline__ void kf_value(const double *rec, long nslots, long sl
 	}
 }
 
+// one term of a record held in registers (q[0 .. KF_Q)) at wavenumber kw, added to (vr, vi)
+__device__ __forceinline__ void kf_term(const double (&q)[KF_Q], double kw, double &vr, double &vi)
+{
+	double zr = q[2 + KF_M], zi = 0;
+#pragma unroll
+	for (int m = KF_M - 1; m >= 0; m--) {
+		const double s = kw / (m + 1);
+		const double nr = zi * s, ni = -zr * s;
+		zr = q[2 + m] + nr;
+		zi = ni;
+	}
+	const double Rc = q[1];
+	double er = zr, ei = zi;
+	if (Rc != 0) {
+		double sn, cs;
+		sincos(kw * Rc, &sn, &cs);
+		er = (cs * zr) + (sn * zi);
+		ei = (cs * zi) - (sn * zr);
+	}
+	vr += q[0] + er;
+	vi += ei;
+}
+
 }  // namespace pe

@@ -475,6 +475,87 @@ void kf_setup_launch(hipStream_t s, int which, int km, const pe::fill_args &f, d
 	HIPCHK(hipGetLastError());
 }
 
+namespace pe {
+// wavenumbers and outputs of a batch of frequencies
+struct kf_multi {
+	double kw[KF_FMAX];
+	void *out[KF_FMAX];
+	int nk;
+};
+
+// k_kf_eval for nk wavenumbers at once: each record read once, nk entries (and their mirrors)
+// written, one output matrix per wavenumber (same ld)
+template <int T>
+__global__ __launch_bounds__(256) void k_kf_eval_multi(kf_args a, int which, kf_multi mk)
+{
+	const fill_args &f = a.f;
+	int ti, tj;
+	const long tile = (long)blockIdx.x + f.tile0;
+	tri_tile_kf(tile, f.nt, ti, tj);
+	const int i = ti * 16 + (threadIdx.x >> 4);
+	const int j = tj * 16 + (threadIdx.x & 15);
+	if ((i >= f.n) || (j >= f.n) || (j < i)) return;
+	const long slot = tile * 256 + threadIdx.x;
+	// the record's T terms (base, and the ground-plane image when present) held in registers,
+	// then one frequency at a time evaluated and stored
+	double q[T][KF_Q];
+#pragma unroll
+	for (int t = 0; t < T; t++)
+#pragma unroll
+		for (int m = 0; m < KF_Q; m++) q[t][m] = a.rec[(long)(t * KF_Q + m) * a.nslots + slot];
+	const bool scaled = (which == 1) && (f.carea != nullptr);
+	double sij = 1, sji = 1;
+	if (scaled) {
+		const double ai = f.carea[i], aj = f.carea[j];
+		sij = 1 / (f.k4pe * ai * aj);
+		sji = 1 / (f.k4pe * aj * ai);
+	}
+	for (int k = 0; k < mk.nk; k++) {
+		double vr = 0, vi = 0;
+#pragma unroll
+		for (int t = 0; t < T; t++) kf_term(q[t], mk.kw[k], vr, vi);
+		double2 *out = (double2 *)mk.out[k];
+		if (scaled) {
+			out[(long)i * f.ld + j] = make_double2(sij * (0 + vr), sij * (0 + vi));
+			if (i != j) out[(long)j * f.ld + i] = make_double2(sji * (0 + vr), sji * (0 + vi));
+		}
+		else {
+			out[(long)i * f.ld + j] = make_double2(vr, vi);
+			if (i != j) out[(long)j * f.ld + i] = make_double2(vr, vi);
+		}
+	}
+}
+}  // namespace pe
+
+void kf_eval_multi_launch(hipStream_t s, int which, const pe::fill_args &f, const double *rec, int T, int nk,
+                          const double *kw, void *const *out)
+{
+	pe::kf_args a;
+	a.f = f;
+	a.klo = a.khi = 0;
+	a.rho = nullptr;
+	a.rec = (double *)rec;
+	a.nslots = kf_slots(f.n);
+	a.T = T;
+	a.list = nullptr;
+	a.nlist = nullptr;
+	pe::kf_multi mk{};
+	mk.nk = nk;
+	for (int k = 0; k < nk; k++) {
+		mk.kw[k] = kw[k];
+		mk.out[k] = out[k];
+	}
+	const long ntiles = (long)f.nt * (f.nt + 1) / 2;
+	const long chunk = 1L << 20;
+	for (long t0 = 0; t0 < ntiles; t0 += chunk) {
+		a.f.tile0 = t0;
+		const unsigned nb = (unsigned)((ntiles - t0 < chunk) ? (ntiles - t0) : chunk);
+		if (T > 1) hipLaunchKernelGGL(pe::k_kf_eval_multi<2>, dim3(nb), dim3(256), 0, s, a, which, mk);
+		else hipLaunchKernelGGL(pe::k_kf_eval_multi<1>, dim3(nb), dim3(256), 0, s, a, which, mk);
+	}
+	HIPCHK(hipGetLastError());
+}
+
 void kf_eval_launch(hipStream_t s, int which, const pe::fill_args &f, const double *rec, int T)
 {
 	pe::kf_args a;

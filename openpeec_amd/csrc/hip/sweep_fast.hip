@@ -1019,12 +1019,16 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 	for (int f0 = 0; (f0 < nfreq) && !rc; f0 += F) {
 		const int nb = std::min(F, nfreq - f0);
 		const auto t0 = std::chrono::steady_clock::now();
-		// the system of sweep point k (relative to this call) into slot buffers sb
-		auto fill_system = [&](slotbuf &sb, int k) {
+		// the system of sweep point k (relative to this call) into slot buffers sb (its dense
+		// blocks too when blocks is set)
+		auto fill_system = [&](slotbuf &sb, int k, bool blocks) {
 			const double freq = freq_at(p, ifreq0 + k);
 			const double kw = p->retardation ? (2 * PI * freq / C0) : 0;
 			sb.omega = 2 * PI * freq;
-			if (kf) {
+			if (!blocks) {
+				// dense blocks already filled (batch fill)
+			}
+			else if (kf) {
 				pgpu_kf_fill(g, kw, (d_complex_t *)sb.lp, nc ? (d_complex_t *)sb.P : nullptr);
 			}
 			else {
@@ -1038,8 +1042,20 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 			upload_stamps(sb, sp, n, s);
 			mna_stamps_free(&sp);
 		};
-		// fill each system of the batch into its slot
-		for (int b = 0; b < nb; b++) fill_system(c.sl[b], f0 + b);
+		// fill each system of the batch into its slot; with records, the batch's Lp and P in one
+		// pass over them
+		if (kf) {
+			double kws[FMAX];
+			d_complex_t *lps[FMAX], *Ps[FMAX];
+			for (int b = 0; b < nb; b++) {
+				const double freq = freq_at(p, ifreq0 + f0 + b);
+				kws[b] = p->retardation ? (2 * PI * freq / C0) : 0;
+				lps[b] = (d_complex_t *)c.sl[b].lp;
+				Ps[b] = (d_complex_t *)c.sl[b].P;
+			}
+			pgpu_kf_fill_multi(g, nb, kws, lps, nc ? Ps : nullptr);
+		}
+		for (int b = 0; b < nb; b++) fill_system(c.sl[b], f0 + b, !kf);
 		const auto t1 = std::chrono::steady_clock::now();
 		if (st) st->fill_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
 		// (re)factor the preconditioner on the system in sb, that of sweep point k
@@ -1109,7 +1125,7 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 				// factorisation demotes into it), stamps in their own buffers
 				c.ahead.lp = c.inv;
 				c.ahead.P = c.inv + (size_t)p->nseg * p->nseg;
-				fill_system(c.ahead, k);
+				fill_system(c.ahead, k, true);
 				rc = refactor_on(c.ahead, k);
 			}
 			if (rc) break;
