@@ -50,14 +50,31 @@ __device__ __forceinline__ double2 cinv(double2 z)
 }
 
 // ------------------------------------------------------------------------------------------
-// grid barrier for cooperative launches: monotone counter, target = base + P * (step + 1)
-__device__ __forceinline__ void grid_barrier(unsigned *count, unsigned target)
+// grid barrier for cooperative launches, two levels: the P workgroups arrive at the counter
+// of their group of BAR_GROUP (one 64-byte line each); the last arrival of a group adds one to
+// the root counter, which every workgroup polls.  Same-address atomics serialise at the memory
+// side, so a flat counter cost about 40 ns per arrival (4.7 us per column with 117
+// workgroups); with groups the longest chain is a group plus the groups' arrivals at the root.
+// Counters are cleared before each launch and count up within it: step s (1, 2, ...) completes
+// when the root reaches ngroups * s.
+constexpr int BAR_GROUP = 16;
+constexpr int BAR_STRIDE = 16;   // unsigned per counter (one 64-byte line)
+constexpr int BAR_BYTES = (1 + PANEL_PMAX / BAR_GROUP) * BAR_STRIDE * (int)sizeof(unsigned);
+
+__device__ __forceinline__ void grid_barrier(unsigned *bar, int P, int w, unsigned step)
 {
 	__syncthreads();
 	if (threadIdx.x == 0) {
+		const int grp = w / BAR_GROUP;
+		const unsigned gsize = (unsigned)min(BAR_GROUP, P - grp * BAR_GROUP);
+		const unsigned ngrp = (unsigned)((P + BAR_GROUP - 1) / BAR_GROUP);
 		__threadfence();
-		atomicAdd(count, 1u);
-		while (__hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+		const unsigned old = atomicAdd(bar + BAR_STRIDE * (1 + grp), 1u);
+		if (old == gsize * step - 1) {
+			__threadfence();
+			atomicAdd(bar, 1u);
+		}
+		while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < ngrp * step) {
 			__builtin_amdgcn_s_sleep(1);
 		}
 		__threadfence();
@@ -99,7 +116,7 @@ constexpr int SROW = NB + 1;    // padded LDS row stride (complex): spreads row-
 // trips per column.
 template <typename TS>
 __global__ __launch_bounds__(PANEL_T) void k_panel(TS *A, long lda, int n, int k0, int nb, int R, int *piv,
-                                                    panel_ws ws, unsigned bar_base)
+                                                    panel_ws ws)
 {
 	extern __shared__ double2 S[];   // [R][SROW]
 	__shared__ double2 U[NB], D[NB];
@@ -142,7 +159,7 @@ __global__ __launch_bounds__(PANEL_T) void k_panel(TS *A, long lda, int n, int k
 			if ((bv >= 0) && (lane < nb)) ws.pubrow[((long)par * P + w) * NB + lane] = S[(bi - r0) * SROW + lane];
 			if ((kk >= r0) && (kk < r0 + rc) && (lane < nb)) ws.diagrow[par * NB + lane] = S[(kk - r0) * SROW + lane];
 		}
-		grid_barrier(ws.bar, bar_base + (unsigned)(P * (jj + 1)));
+		grid_barrier(ws.bar, P, w, (unsigned)(jj + 1));
 		if (wave == 0) {
 			double gv = -1;
 			int gi = INT_MAX, gw = 0;
@@ -182,18 +199,20 @@ __global__ __launch_bounds__(PANEL_T) void k_panel(TS *A, long lda, int n, int k
 		__syncthreads();
 		if (s_stop) return;
 		const int gi = s_gi;
+		// the interchange, one lane per column: row kk takes the pivot row, the pivot row's
+		// place takes row kk (a serial copy by the two owning threads was the column's longest path)
+		if (tid < nb) {
+			if ((kk >= r0) && (kk < r0 + rc)) S[(kk - r0) * SROW + tid] = U[tid];
+			if ((gi != kk) && (gi >= r0) && (gi < r0 + rc)) S[(gi - r0) * SROW + tid] = D[tid];
+		}
+		__syncthreads();
 		if (tid < rc) {
 			double2 *row = &S[tid * SROW];
 			const int g = r0 + tid;
-			if (g == kk) {
-				for (int c = 0; c < nb; c++) row[c] = U[c];
-			}
-			else if (g == gi) {
-				for (int c = 0; c < nb; c++) row[c] = D[c];
-			}
 			if (g > kk) {
 				const double2 l = cmul(row[jj], cinv(U[jj]));
 				row[jj] = l;
+#pragma unroll 4
 				for (int c = jj + 1; c < nb; c++) {
 					const double2 p = cmul(l, U[c]);
 					double2 a = row[c];
@@ -527,7 +546,7 @@ void ensure_ws(int n)
 		g_lw.attr = true;
 	}
 	const size_t need = 2 * PANEL_PMAX * sizeof(double) + 2 * PANEL_PMAX * sizeof(int) +
-	                    2 * (size_t)PANEL_PMAX * NB * sizeof(double2) + 2 * NB * sizeof(double2) + 256 + 256;
+	                    2 * (size_t)PANEL_PMAX * NB * sizeof(double2) + 2 * NB * sizeof(double2) + BAR_BYTES + 256;
 	if (g_lw.base == nullptr) {
 		HIPCHK(hipMalloc((void **)&g_lw.base, need));
 		char *p = g_lw.base;
@@ -540,7 +559,7 @@ void ensure_ws(int n)
 		g_lw.ws.pubi = (int *)p;
 		p += 2 * PANEL_PMAX * sizeof(int);
 		g_lw.ws.bar = (unsigned *)p;
-		p += 256;
+		p += BAR_BYTES;
 		g_lw.ws.info = (int *)p;
 	}
 	if (g_lw.nperm < n) {
@@ -900,7 +919,6 @@ int lu_factor_t(hipStream_t s, int n, TS *A, int lda, int *dpiv, int *dperm, pgp
 	if (n <= 0) return -1;
 	ensure_ws(n);
 	int *perm = (dperm != nullptr) ? dperm : g_lw.perm;
-	HIPCHK(hipMemsetAsync(g_lw.ws.bar, 0, sizeof(unsigned), s));
 	HIPCHK(hipMemsetAsync(g_lw.ws.info, 0xff, sizeof(int), s));   // -1
 	hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, s, perm, n);
 	hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, s, dpiv, n);
@@ -909,7 +927,6 @@ int lu_factor_t(hipStream_t s, int n, TS *A, int lda, int *dpiv, int *dperm, pgp
 		HIPCHK(hipEventCreate(&e0));
 		HIPCHK(hipEventCreate(&e1));
 	}
-	unsigned bar_base = 0;
 	const int pmax = (g_lw.ncu < PANEL_PMAX) ? g_lw.ncu : PANEL_PMAX;
 	const long ldal = lda;
 	for (int k0 = 0; k0 < n; k0 += NB) {
@@ -925,14 +942,13 @@ int lu_factor_t(hipStream_t s, int n, TS *A, int lda, int *dpiv, int *dperm, pgp
 		int *pivp = dpiv;
 		int nn = n, kk0 = k0, nbb = nb;
 		panel_ws ws = g_lw.ws;
-		unsigned bb = bar_base;
 		TS *Ap = A;
 		long ldp = ldal;
 		void *kargs[] = {(void *)&Ap, (void *)&ldp, (void *)&nn, (void *)&kk0, (void *)&nbb,
-		                 (void *)&R, (void *)&pivp, (void *)&ws, (void *)&bb};
+		                 (void *)&R, (void *)&pivp, (void *)&ws};
+		HIPCHK(hipMemsetAsync(g_lw.ws.bar, 0, BAR_BYTES, s));
 		HIPCHK(hipLaunchCooperativeKernel((const void *)k_panel<TS>, dim3(P), dim3(PANEL_T), kargs,
 		                                  (unsigned)(R * SROW * sizeof(double2)), s));
-		bar_base += (unsigned)(P * nb);
 		const int nout = n - nb;
 		hipLaunchKernelGGL(k_laswp<TS>, dim3((nout + 1 + 255) / 256), dim3(256), 0, s, A, ldal, n, k0, nb, dpiv, perm,
 		                   (const int *)g_lw.ws.info);
