@@ -19,9 +19,9 @@ the sweep.  Either way value = K / (wall time of the K points).
 
 Zin of the mixed form is compared with the plain FP64 form on three of the timed points (the
 metric is quoted at Zin rel err <= 1e-6); the plain form's own rate on those points is reported
-beside it.  The roofline block is the LU trailing-update GEMM (rocBLAS cgemm on FP32 MFMA) timed
-with HIP events on the library's stream around every launch; the GMRES matvec is reported in
-GB/s the same way.  The precision form (FP32-stored LU, FP32 MFMA trailing updates, FP64 refinement)
+beside it.  The roofline block is the sweep's dominant kernel, k_mvdense (the dense blocks of the
+GMRES product A x, HBM bound), timed with HIP events on the library's stream around every launch;
+the LU trailing update (rocBLAS cgemm on FP32 MFMA) is reported the same way in lu_update.  The precision form (FP32-stored LU, FP32 MFMA trailing updates, FP64 refinement)
 runs on the same three points and reports its rate and its Zin error against the plain form per
 point.  cpu_baseline times the oracle (the reference's arithmetic restated in C, OpenMP on
 all host cores) on a bounded sample and extrapolates to one frequency point.
@@ -45,6 +45,7 @@ METRIC = "freq points/s (Lp/P fill+solve) at Zin rel err ≤1e-6 vs FP64; fill G
 PEAK_FP64_MFMA = 78.6     # TFLOP/s, MI355X dense FP64 matrix
 PEAK_FP64_VECTOR = 78.6   # TFLOP/s, MI355X FP64 vector
 PEAK_FP32_MFMA = 157.3    # TFLOP/s, MI355X dense FP32 matrix
+PEAK_HBM_GBS = 8000.0     # GB/s, MI355X HBM3E
 
 
 def parse():
@@ -211,6 +212,18 @@ def lu_roofline(achieved: float) -> dict:
             "frac": achieved / peak, "traffic": lu_traffic(prof) if prof else None,
             "traffic_note": (f"HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a copy of known size, "
                              f"profiles/{prof}); algorithmic {alg}") if prof else f"not profiled; algorithmic {alg}"}
+
+
+def mv_roofline(gbs: float | None, launches: int, avg_ms: float, avg_bytes: float) -> dict:
+    """Roofline of the accelerated sweep's dominant kernel (most GPU time per sweep): k_mvdense,
+    the dense blocks of the GMRES product A x (Lp and P of every system of a batch, FP64), HBM
+    bound; timed with HIP events on the library's stream around every launch of the timed sweep."""
+    prof = "k_mvdense_plate64_mixed_sweep.md"
+    return {"kernel": "k_mvdense (GMRES product A x, dense Lp and P blocks of 4 systems, FP64)", "bound": "hbm",
+            "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": (gbs / PEAK_HBM_GBS) if gbs else None,
+            "traffic": lu_traffic(prof), "launches": launches, "avg_launch_ms": avg_ms, "avg_launch_bytes": avg_bytes,
+            "traffic_note": f"HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a copy of known size, "
+                            f"profiles/{prof}); algorithmic 16 B per entry of Lp and P per system, x once, segment sums"}
 
 
 def lu_traffic(name: str) -> float | None:
@@ -444,7 +457,9 @@ def main():
                               "gmres_its": st.gmres_its, "n_lu": st.n_lu,
                               "k_zgemm_avg_launch_ms": st.lu_gemm_ms / max(1, st.lu_gemm_launches),
                               "k_zgemm_avg_launch_flop": st.lu_gemm_flop / max(1, st.lu_gemm_launches),
-                              "k_zgemm_launches": st.lu_gemm_launches}))
+                              "k_zgemm_launches": st.lu_gemm_launches,
+                              "mv_avg_launch_ms": st.mv_ms / max(1, st.mv_launches), "mv_launches": st.mv_launches,
+                              "mv_GB_per_s": (st.mv_bytes / (st.mv_ms * 1e-3)) / 1e9 if st.mv_ms > 0 else None}))
             m.close()
             return
         zp, t_plain, stp = plain_points(m, idx)
@@ -477,17 +492,16 @@ def main():
             "stages_ms": {"fill_incl_kf_setup": st.fill_ms, "kf_setup": st.kf_setup_ms,
                           "lu_and_inverse": st.lu_ms, "gmres": st.solve_ms, "total": st.total_ms},
             "n_lu": st.n_lu, "gmres_its_per_point": st.gmres_its / steps, "kf_direct_pairs": st.kf_direct_pairs,
-            "gmres_matvec": {"GB_per_s": mv_gbs, "launches": st.mv_launches,
-                             "avg_launch_ms": st.mv_ms / max(1, st.mv_launches),
-                             "frac_of_hbm_peak": (mv_gbs / 8000.0) if mv_gbs else None},
+            "lu_update": (lu_roofline(achieved) | {
+                "launches": st.lu_gemm_launches, "avg_launch_ms": gemm_ms, "avg_launch_flop": gemm_flop,
+            }),
             "fill_nominal_gflop_per_point": fill_flop / 1e9,
             "plain_fill": {"gflops": fill_flop_top / (plain_fill_ms * 1e-3) / 1e9,
                            "frac_of_fp64_vector_peak": fill_flop_top / (plain_fill_ms * 1e-3) / 1e12 / PEAK_FP64_VECTOR,
                            "note": "element-by-element FP64 fill of the plain form at the last checked point"},
             "kfactored_fill_ms_per_point": (st.fill_ms - st.kf_setup_ms) / steps,
-            "roofline": (lu_roofline(achieved) | {
-                "launches": st.lu_gemm_launches, "avg_launch_ms": gemm_ms, "avg_launch_flop": gemm_flop,
-            }),
+            "roofline": mv_roofline(mv_gbs, st.mv_launches, st.mv_ms / max(1, st.mv_launches),
+                                    st.mv_bytes / max(1, st.mv_launches)),
         })
     else:
         form = PEEC_FORM_PLAIN

@@ -40,8 +40,8 @@ typedef struct {
 	long   gmres_its;       /* GMRES iterations summed over systems (mixed)   */
 	double kf_setup_ms;     /* k-factored fill preparation (mixed), in fill_ms */
 	long   kf_direct_pairs; /* pairs the k-factored fill evaluates directly   */
-	double mv_ms;           /* GMRES products A x: summed kernel time (mixed) */
-	double mv_bytes;        /* their algorithmic HBM bytes (matrix blocks read) */
+	double mv_ms;           /* GMRES products A x: summed time of their dense-block kernel (mixed) */
+	double mv_bytes;        /* its algorithmic HBM bytes (blocks read, x, segment sums written) */
 	long   mv_launches;
 	int    fp64_fallbacks;  /* points the precision form solved in FP64 instead */
 	int    pc_builds;       /* preconditioner builds (learned form)           */

@@ -670,16 +670,16 @@ void matvec(fast &c, hipStream_t s, const double2 *x, double2 *y, int F, int nac
 		default: hipLaunchKernelGGL(k_mvdense<8>, grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, cs, nrbS, x, c.mvp, c.n); break;
 		}
 	}
-	hipLaunchKernelGGL(k_mvfin, dim3(blocks((long)c.n * F)), dim3(256), 0, s, S, c.n, F, nact, g_offS, c.nseg, g_offQ, c.nc, cs,
-	                   (const double2 *)c.mvp, x, y);
 	if (g_mvt.on) {
+		// k_mvdense alone: the dense blocks of the nact systems (16 B per entry), x once, and the
+		// segment sums written
 		HIPCHK(hipEventRecord(g_mvt.ev[g_mvt.used + 1], s));
 		g_mvt.used += 2;
-		// matrix blocks of the nact systems (16 B per entry) + x, y and the CSR stamps
-		double b = 16.0 * nact * ((double)c.nseg * c.nseg + (double)c.nc * c.nc) + 32.0 * c.n * F;
-		for (int f = 0; f < nact; f++) b += 4.0 * (c.n + 1) + 20.0 * c.sl[f].nnz;
-		g_mvt.bytes += b;
+		const double nsum = (double)((c.nseg + cs - 1) / cs) * c.nseg + (double)((c.nc + cs - 1) / cs) * c.nc;
+		g_mvt.bytes += 16.0 * nact * ((double)c.nseg * c.nseg + (double)c.nc * c.nc) + 16.0 * c.n * F + 16.0 * nsum * F;
 	}
+	hipLaunchKernelGGL(k_mvfin, dim3(blocks((long)c.n * F)), dim3(256), 0, s, S, c.n, F, nact, g_offS, c.nseg, g_offQ, c.nc, cs,
+	                   (const double2 *)c.mvp, x, y);
 }
 
 // sum the recorded product timings into st (stream already synchronised by the caller)
