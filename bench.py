@@ -7,6 +7,7 @@ on, corner-to-corner port, sweep 10 MHz - 3 GHz in 201 points.  A step is one fr
 of that sweep, device resident from the cell arrays to Z(f).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--form mixed|plain] [--batch B] [--no-cpu]
+    python bench.py --only batch | learned | fullstack     (one side workload alone)
 
 Forms.  mixed (default): the sweep's points 0..K-1 in one accelerated sweep -- k-factored Lp/P
 fill, charge-unknown system, an LU of A(f_ref) on FP32 storage (FP64 panels and pivoting, FP32
@@ -65,7 +66,9 @@ def parse():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner workload")
     ap.add_argument("--learned-points", type=int, default=21,
                     help="frequency points of the learned workload (21: every fifth of its 101)")
-    ap.add_argument("--only", choices=["batch", "learned"], default=None, help="run one side workload alone")
+    ap.add_argument("--only", choices=["batch", "learned", "fullstack"], default=None,
+                    help="run one side workload alone (fullstack: the full-stack configuration, not in the default run)")
+    ap.add_argument("--fullstack-points", type=int, default=150, help="points of the full-stack sweep")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (calibrates the HBM byte counters)")
     return ap.parse_args()
@@ -397,6 +400,53 @@ def bench_learned(lib, points: int = 101) -> dict:
     return out
 
 
+def bench_fullstack(lib, npoints: int = 150) -> dict:
+    """The full-stack configuration (ground plane, 4-layer volume plate over a dielectric brick,
+    n = 29 361 unknowns): the accelerated form over npoints points of its 20 MHz-step sweep (cold,
+    one warm-up point first), the plain FP64 and the precision form at its first and last points,
+    and each form's Zin error against the plain form there."""
+    from openpeec_amd import PEEC_FORM_MIXED, PEEC_FORM_PLAIN, Peec, decks
+    t0 = time.perf_counter()
+    m = Peec.from_deck_text(decks.full_stack_deck(), lib)
+    m.setup()
+    log(f"full stack: {m.p.nseg} current cells, {m.p.ncell} capacitive cells, {m.p.nunknown} unknowns "
+        f"({time.perf_counter() - t0:.1f} s setup)")
+    nf = m.p.nfreq
+    npoints = max(2, min(npoints, nf))
+    m.sweep(0, 1, PEEC_FORM_MIXED, 4)
+    lib.peec_sweep_reset()
+    t0 = time.perf_counter()
+    z, st = m.sweep(0, npoints, PEEC_FORM_MIXED, 4)
+    elapsed = time.perf_counter() - t0
+    log(f"full stack: accelerated sweep of {npoints} points {elapsed:.1f} s ({st.n_lu} factorisations, "
+        f"{st.gmres_its} GMRES iterations)")
+    idx = [0, npoints - 1]
+    zp, t_plain, stp = plain_points(m, idx)
+    log(f"full stack: plain FP64 form {t_plain / len(idx):.1f} s per point")
+    prec = precision_points(m, idx, zp)
+    zm = np.array([z[i, 0, 0] for i in idx])
+    err = float(np.max(np.abs(zm - zp) / np.abs(zp)))
+    out = {
+        "workload": "ground plane 0.1 mm under an FR4 brick (eps_r 4.4, 60 x 60 x 1.6 mm, 20 x 20 x 4 cells); "
+                    "copper plate 60 x 60 mm, 35 um, 48 x 48 grid in 4 volume layers on the brick; port plate "
+                    f"corner to ground; retarded + capacitive; points 20 MHz + k 20 MHz, k = 0..{npoints - 1}",
+        "current_cells": m.p.nseg, "capacitive_cells": m.p.ncell, "unknowns": m.p.nunknown,
+        "accelerated": {"points": npoints, "s": elapsed, "points_per_s": npoints / elapsed, "n_lu": st.n_lu,
+                        "gmres_its_per_point": st.gmres_its / npoints, "kf_direct_pairs": st.kf_direct_pairs,
+                        "stages_ms": {"fill_incl_kf_setup": st.fill_ms, "kf_setup": st.kf_setup_ms,
+                                      "lu_and_inverse": st.lu_ms, "gmres": st.solve_ms},
+                        "zin_max_rel_err_vs_plain_fp64": err},
+        "plain_fp64_points_per_s": len(idx) / t_plain,
+        "speedup_vs_plain_fp64": (npoints / elapsed) / (len(idx) / t_plain),
+        "plain_stages_ms_last_point": {"lp_fill": stp.fill_ms, "charge_fill": stp.pot_ms, "p_lu_and_inverse": stp.cinv_ms,
+                                       "mna_lu": stp.lu_ms, "total": stp.total_ms},
+        "precision_form": prec,
+        "zin_points_hz": [m.freq(i) for i in idx],
+    }
+    m.close()
+    return out
+
+
 def copy_probe(mib: int) -> None:
     """A copy of known size through one elementwise kernel (y = 1.0 * x): its FETCH_SIZE /
     WRITE_SIZE per dispatch against mib MiB read and written scales the counters of the path's
@@ -430,6 +480,9 @@ def main():
         return
     if args.only == "learned":
         print(json.dumps({"learned_preconditioner": bench_learned(lib, args.learned_points)}))
+        return
+    if args.only == "fullstack":
+        print(json.dumps({"full_stack": bench_fullstack(lib, args.fullstack_points)}))
         return
     text = decks.plate_deck(n=args.grid)
     m = Peec.from_deck_text(text, lib)

@@ -493,13 +493,19 @@ int pgpu_kf_setup_launch(pgpu_t *g, double klo, double khi)
 	const int T = p->gp ? 2 : 1;
 	const long sl = kf_slots(ns), sh = (nh > 0) ? kf_slots(nh) : 0;
 	const double bytes = (double)T * kf_record_doubles() * 8.0 * (double)(sl + sh) + 8.0 * (double)(sl + sh);
-	if (bytes > 64e9) return 1;
+	// records of an earlier band go first; the new ones may take up to 70 % of the free device
+	// memory (the sweep's systems and factors are allocated before them; the rest is for the
+	// plain and precision forms' buffers); beyond that the caller fills element by element
 	dfree(g->kf_rec_lp);
 	dfree(g->kf_rec_ig);
 	dfree(g->kf_list_lp);
 	dfree(g->kf_list_ig);
 	g->kf_rec_lp = g->kf_rec_ig = nullptr;
 	g->kf_list_lp = g->kf_list_ig = nullptr;
+	g->kf_ready = false;
+	size_t mfree = 0, mtotal = 0;
+	HIPCHK(hipMemGetInfo(&mfree, &mtotal));
+	if (bytes > 0.7 * (double)mfree) return 1;
 	if (g->kf_cnt == nullptr) g->kf_cnt = dalloc<unsigned>(2);
 	if (g->kf_rho_seg == nullptr) g->kf_rho_seg = dalloc<double>(ns);
 	HIPCHK(hipMemsetAsync(g->kf_cnt, 0, 2 * sizeof(unsigned), g->s));

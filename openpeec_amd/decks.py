@@ -126,3 +126,25 @@ def disk_quad_deck(nr: int = 48, nsec: int = 32, qx: int = 32, qy: int = 16, rad
         f"quad = {x0!r} {y0!r} {gap!r}  {-x0!r} {y0!r} {gap!r}  {-x0!r} {-y0!r} {gap!r}  {x0!r} {-y0!r} {gap!r}  "
         f"{thick!r} 5.8e7 {qx} {qy}",
         "port = 1 2 50", f"frequency = {f0!r} {f1!r} {ndiv}", "end", ""])
+
+
+def full_stack_deck(grid: int = 48, ndivt: int = 4, diel: tuple = (20, 20, 4), gpz: float = -1e-4,
+                    f0: float = 2e7, f1: float = 3e9, nd: int = 149) -> str:
+    """The full-stack configuration: a copper plate 60 x 60 mm, 35 um thick, at z = 1.6 mm on a
+    grid x grid mesh split into ndivt volume layers, over an FR4-like brick (eps_r 4.4,
+    60 x 60 x 1.6 mm, diel cells) that fills the gap to a ground plane, port from the plate's
+    corner node to the ground node, sweep f0 .. f1 in nd + 1 points (20 MHz steps by default).
+
+    The ground plane sits gpz (default 0.1 mm) below the brick's bottom face: with the brick on
+    the image plane itself, its bottom charge cells coincide with their own images, their rows of
+    P vanish and the reference's pot_fill stops on a singular matrix (potential.c:228-233)."""
+    da, db, dt = diel
+    lines = ["OpenPEEC 1 0", f"title = full stack {grid}x{grid}x{ndivt} over brick {da}x{db}x{dt}",
+             "capacitance = 1", "retardation = 1", f"groundplane = {gpz!r}",
+             "node = 1 0 0 0.0016",
+             f"plate = 0 0 0.0016 0.06 0 0 0 0.06 0 3.5e-5 5.8e7 {grid} {grid} {ndivt}",
+             f"dielectric = 0 0 0 0.06 0 0 0 0.06 0 0.0016 4.4 {da} {db} {dt}",
+             "port = 1 0 50",
+             f"frequency = {f0!r} {f1!r} {nd}",
+             "end", ""]
+    return "\n".join(lines)
