@@ -223,3 +223,17 @@ def test_plate_capacitance_richardson():
         m.close()
     c_inf = 2 * cs[1] - cs[0]
     assert abs(c_inf - 4.08100e-11) <= 0.01 * 4.08100e-11
+
+
+def test_full_stack_deck_fix():
+    """The full-stack configuration as written puts the brick's bottom face on the ground plane:
+    those charge cells coincide with their own images, their P rows vanish and the sweep stops on
+    a singular potential-coefficient matrix (potential.c:228-233).  With the ground plane 0.1 mm
+    below the brick (decks.full_stack_deck's default) the same deck runs; small grids here."""
+    small = dict(grid=8, ndivt=2, diel=(4, 4, 2), nd=1)
+    with pytest.raises(RuntimeError):
+        oracle_zmat(decks.full_stack_deck(gpz=0.0, **small))
+    z = oracle_zmat(decks.full_stack_deck(**small))
+    assert z.shape == (2, 1, 1) and np.all(np.isfinite(z))
+    # a capacitive input at 20 MHz (plate over ground through the brick): Im Z < 0
+    assert z[0, 0, 0].imag < 0
