@@ -20,8 +20,8 @@ the sweep.  Either way value = K / (wall time of the K points).
 
 Zin of the mixed form is compared with the plain FP64 form on three of the timed points (the
 metric is quoted at Zin rel err <= 1e-6); the plain form's own rate on those points is reported
-beside it.  The roofline block is the sweep's dominant kernel, k_mvdense (the dense blocks of the
-GMRES product A x, HBM bound), timed with HIP events on the library's stream around every launch;
+beside it.  The roofline block is the sweep's dominant kernel, k_mvsym (the dense blocks of the
+GMRES product A x from their upper triangles, HBM bound), timed with HIP events on the library's stream around every launch;
 the LU trailing update (rocBLAS cgemm on FP32 MFMA) is reported the same way in lu_update.  The precision form (FP32-stored LU, FP32 MFMA trailing updates, FP64 refinement)
 runs on the same three points and reports its rate and its Zin error against the plain form per
 point.  cpu_baseline times the oracle (the reference's arithmetic restated in C, OpenMP on
@@ -218,15 +218,21 @@ def lu_roofline(achieved: float) -> dict:
 
 
 def mv_roofline(gbs: float | None, launches: int, avg_ms: float, avg_bytes: float) -> dict:
-    """Roofline of the accelerated sweep's dominant kernel (most GPU time per sweep): k_mvdense,
-    the dense blocks of the GMRES product A x (Lp and P of every system of a batch, FP64), HBM
-    bound; timed with HIP events on the library's stream around every launch of the timed sweep."""
-    prof = "k_mvdense_plate64_mixed_sweep.md"
-    return {"kernel": "k_mvdense (GMRES product A x, dense Lp and P blocks of 4 systems, FP64)", "bound": "hbm",
+    """Roofline of the accelerated sweep's dominant kernel (most GPU time per sweep): the dense
+    blocks of the GMRES product A x (Lp and P of every system of a batch, FP64), HBM bound --
+    k_mvsym, which reads the upper triangles only (k_mvdense, both triangles, with
+    PEEC_MV_SYM=0); timed with HIP events on the library's stream around every launch of the
+    timed sweep."""
+    sym = os.environ.get("PEEC_MV_SYM", "1") != "0"
+    prof = "k_mvsym_plate64_mixed_sweep.md" if sym else "k_mvdense_plate64_mixed_sweep.md"
+    name = ("k_mvsym (GMRES product A x: upper triangles of the symmetric Lp and P of 4 systems, FP64)" if sym
+            else "k_mvdense (GMRES product A x, dense Lp and P blocks of 4 systems, FP64)")
+    return {"kernel": name, "bound": "hbm",
             "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": (gbs / PEAK_HBM_GBS) if gbs else None,
             "traffic": lu_traffic(prof), "launches": launches, "avg_launch_ms": avg_ms, "avg_launch_bytes": avg_bytes,
             "traffic_note": f"HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a copy of known size, "
-                            f"profiles/{prof}); algorithmic 16 B per entry of Lp and P per system, x once, segment sums"}
+                            f"profiles/{prof}); algorithmic 16 B per entry read of Lp and P per system (upper "
+                            f"triangle with whole diagonal tiles for k_mvsym), x once, partial sums written"}
 
 
 def lu_traffic(name: str) -> float | None:

@@ -199,6 +199,178 @@ __global__ __launch_bounds__(256) void k_mvdense(slots S, int nact, int offS, in
 	}
 }
 
+// k_mvsym: the same product reading only the upper triangle of each symmetric block.  Items are
+// (64-row block I, run of js 64 x 64 tiles J >= I).  A wave covers 16 columns of a tile in four
+// row phases: lane l takes column 16 g + (l mod 16) and rows 4 q + (l / 16), q = 0..15, so one
+// load instruction reads four 256-byte runs of stored rows.  A tile above the diagonal serves
+// both triangles: B[r][c] x[c] summed over the 16 lanes of a row (four DPP steps inside the
+// row, no LDS) goes to row r, kept by the lane that owns r; B[r][c] x[r] accumulates per lane
+// for its column c and the four row phases are added once per 16 columns, written per (I, c)
+// to partT.  A diagonal tile is read whole and used once.  Row sums per item go to
+// partN[s][row][f].  Half the bytes of k_mvdense.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v)
+{
+	const long long b = __double_as_longlong(v);
+	const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
+	const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+	return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// sum over the 16 lanes of each row of the wave, in every lane of the row: quad_perm [1,0,3,2],
+// quad_perm [2,3,0,1], row_half_mirror, row_mirror
+__device__ __forceinline__ double row16_sum(double v)
+{
+	v += dpp_move<0xB1>(v);
+	v += dpp_move<0x4E>(v);
+	v += dpp_move<0x141>(v);
+	v += dpp_move<0x140>(v);
+	return v;
+}
+
+template <int F, int RU>
+__global__ __launch_bounds__(256) void k_mvsym(slots S, int nact, int offS, int nseg, int offQ, int nc, int js,
+                                               const double2 *__restrict__ X, double2 *__restrict__ partN,
+                                               double2 *__restrict__ partT, int n)
+{
+	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+	const int cg = lane & 15, ph = lane >> 4;
+	const int ntS = (nseg + 63) / 64;
+	const int ib = blockIdx.x * 4 + wave, s = blockIdx.y;
+	const bool lblk = ib < ntS;
+	const int nb = lblk ? nseg : nc, o = lblk ? offS : offQ;
+	const int nt = (nb + 63) / 64;
+	const int I = lblk ? ib : ib - ntS;
+	if (I >= nt) return;
+	const int J0 = I + s * js;
+	if (J0 >= nt) return;
+	const int J1 = min(nt, J0 + js);
+	const int rlo = I * 64, nr = min(64, nb - rlo);
+	const double2 *B[F];
+#pragma unroll
+	for (int f = 0; f < F; f++) B[f] = (f < nact) ? (lblk ? S.s[f].lp : S.s[f].P) : nullptr;
+	// lane l keeps row rlo + 4 (l mod 16) + l / 16
+	double nr_[F], ni_[F];
+#pragma unroll
+	for (int f = 0; f < F; f++) nr_[f] = ni_[f] = 0;
+	for (int J = J0; J < J1; J++) {
+		const bool diag = (J == I);
+		for (int g = 0; g < 4; g++) {
+			const int cl = J * 64 + g * 16 + cg;
+			const bool cact = cl < nb;
+			double2 xl[F];
+			double tr_[F], ti_[F];
+#pragma unroll
+			for (int f = 0; f < F; f++) {
+				xl[f] = (cact && (f < nact)) ? X[(long)(o + cl) * F + f] : make_double2(0, 0);
+				tr_[f] = ti_[f] = 0;
+			}
+			for (int q0 = 0; q0 < 16; q0 += RU) {
+				double2 b[RU][F];
+#pragma unroll
+				for (int u = 0; u < RU; u++) {
+					const int rr = 4 * (q0 + u) + ph;
+#pragma unroll
+					for (int f = 0; f < F; f++)
+						b[u][f] = (cact && (f < nact) && (rr < nr)) ? B[f][(long)(rlo + rr) * nb + cl] : make_double2(0, 0);
+				}
+#pragma unroll
+				for (int u = 0; u < RU; u++) {
+					const int q = q0 + u;
+					if (4 * q >= nr) continue;
+					const int rr = 4 * q + ph;
+					const double2 *xr = X + (long)(o + rlo + min(rr, nr - 1)) * F;
+#pragma unroll
+					for (int f = 0; f < F; f++) {
+						if (f >= nact) continue;
+						const double2 bb = b[u][f];
+						// row rlo + rr: sum over the 16 columns of the row's lanes
+						double pr = bb.x * xl[f].x;
+						pr = fma(-bb.y, xl[f].y, pr);
+						double pi = bb.x * xl[f].y;
+						pi = fma(bb.y, xl[f].x, pi);
+						pr = row16_sum(pr);
+						pi = row16_sum(pi);
+						if (cg == q) {
+							nr_[f] += pr;
+							ni_[f] += pi;
+						}
+						// the mirror entry B[c][r] = B[r][c]: column c takes B[r][c] x[r]
+						if (!diag) {
+							const double2 x = xr[f];
+							tr_[f] = fma(bb.x, x.x, tr_[f]);
+							tr_[f] = fma(-bb.y, x.y, tr_[f]);
+							ti_[f] = fma(bb.x, x.y, ti_[f]);
+							ti_[f] = fma(bb.y, x.x, ti_[f]);
+						}
+					}
+				}
+			}
+			if (!diag) {
+				// the four row phases of column cl
+#pragma unroll
+				for (int f = 0; f < F; f++) {
+					tr_[f] += __shfl_xor(tr_[f], 16);
+					ti_[f] += __shfl_xor(ti_[f], 16);
+					tr_[f] += __shfl_xor(tr_[f], 32);
+					ti_[f] += __shfl_xor(ti_[f], 32);
+				}
+				if (cact && (ph == 0)) {
+#pragma unroll
+					for (int f = 0; f < F; f++) partT[((long)I * n + o + cl) * F + f] = make_double2(tr_[f], ti_[f]);
+				}
+			}
+		}
+	}
+	const int myrow = 4 * cg + ph;
+	if (myrow < nr) {
+#pragma unroll
+		for (int f = 0; f < F; f++) partN[((long)s * n + o + rlo + myrow) * F + f] = make_double2(nr_[f], ni_[f]);
+	}
+}
+
+// per (row, f) of the symmetric product: the row's item sums (partN, increasing s), then the
+// mirror contributions of the tiles above it in its column (partT, increasing I), the -jw scale
+// on the Lp rows, and the row's sparse stamps
+__global__ __launch_bounds__(256) void k_mvfin_sym(slots S, int n, int F, int nact, int offS, int nseg, int offQ, int nc,
+                                                   int js, const double2 *__restrict__ partN,
+                                                   const double2 *__restrict__ partT, const double2 *__restrict__ X,
+                                                   double2 *__restrict__ Y)
+{
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= (long)n * F) return;
+	const int row = (int)(e / F), f = (int)(e - (long)row * F);
+	double2 y = make_double2(0, 0);
+	if (f < nact) {
+		const slot &sl = S.s[f];
+		const bool l = (row >= offS) && (row < offS + nseg);
+		const bool q = (row >= offQ) && (row < offQ + nc);
+		if (l || q) {
+			const int nb = l ? nseg : nc, o = l ? offS : offQ;
+			const int nt = (nb + 63) / 64, Ib = (row - o) / 64;
+			const int nsg = (nt - Ib + js - 1) / js;
+			double tr = 0, ti = 0;
+			for (int sg = 0; sg < nsg; sg++) {
+				const double2 p = partN[((long)sg * n + row) * F + f];
+				tr += p.x;
+				ti += p.y;
+			}
+			for (int I = 0; I < Ib; I++) {
+				const double2 p = partT[((long)I * n + row) * F + f];
+				tr += p.x;
+				ti += p.y;
+			}
+			y = l ? make_double2(sl.omega * ti, -sl.omega * tr) : make_double2(tr, ti);   // Lp rows: -jw * sum
+		}
+		for (int k = sl.rp[row]; k < sl.rp[row + 1]; k++) {
+			const double2 p = zmul(sl.val[k], X[(long)sl.ci[k] * F + f]);
+			y.x += p.x;
+			y.y += p.y;
+		}
+	}
+	Y[e] = y;
+}
+
 __global__ __launch_bounds__(256) void k_mvfin(slots S, int n, int F, int nact, int offS, int nseg, int offQ, int nc, int cs,
                                                const double2 *__restrict__ part, const double2 *__restrict__ X,
                                                double2 *__restrict__ Y)
@@ -512,6 +684,25 @@ int mv_cs()
 	return cs;
 }
 
+// dense blocks of A x read as symmetric, upper triangle only (PEEC_MV_SYM=0: both triangles,
+// k_mvdense); PEEC_MV_JS 64 x 64 tiles per item (1 by default: the most items)
+bool mv_sym()
+{
+	const char *e = getenv("PEEC_MV_SYM");   // read per product (a few per GMRES iteration)
+	return !(e && (e[0] == '0'));
+}
+
+int mv_js()
+{
+	static int v = -1;
+	if (v < 0) {
+		const char *e = getenv("PEEC_MV_JS");
+		const int j = e ? atoi(e) : 1;
+		v = ((j >= 1) && (j <= 64)) ? j : 1;
+	}
+	return v;
+}
+
 // doubles of partial sums the products need for order n and F columns
 size_t tg_part_count(int n, int F)
 {
@@ -595,6 +786,7 @@ struct fast {
 	double2 *W = nullptr, *Xv = nullptr, *Bv = nullptr, *R = nullptr, *T1 = nullptr;
 	double2 *tgp = nullptr;   // segment partial sums of the triangular products
 	double2 *mvp = nullptr;   // segment partial sums of the dense blocks of A x
+	double2 *mvt = nullptr;   // mirror partial sums of the symmetric product (per row block and column)
 	double2 *partial = nullptr, *h = nullptr, *yd = nullptr;
 	double *scale = nullptr;
 };
@@ -654,6 +846,42 @@ void matvec(fast &c, hipStream_t s, const double2 *x, double2 *y, int F, int nac
 	for (int f = 0; f < FMAX; f++) {
 		const slotbuf &b = c.sl[(f < nact) ? f : 0];
 		S.s[f] = slot{b.rp, b.ci, b.val, b.lp, b.P, b.omega};
+	}
+	if (mv_sym()) {
+		const int js = mv_js();
+		const int ntS = (c.nseg + 63) / 64, ntQ = (c.nc + 63) / 64, ntm = std::max(ntS, ntQ);
+		const dim3 grid((unsigned)((ntS + ntQ + 3) / 4), (unsigned)((ntm + js - 1) / js));
+		if ((ntS + ntQ) > 0) {
+			switch (F) {
+			case 1: hipLaunchKernelGGL((k_mvsym<1, 2>), grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, js, x, c.mvp, c.mvt, c.n); break;
+			case 2: hipLaunchKernelGGL((k_mvsym<2, 2>), grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, js, x, c.mvp, c.mvt, c.n); break;
+			case 3: hipLaunchKernelGGL((k_mvsym<3, 2>), grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, js, x, c.mvp, c.mvt, c.n); break;
+			case 4: hipLaunchKernelGGL((k_mvsym<4, 2>), grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, js, x, c.mvp, c.mvt, c.n); break;
+			case 5: hipLaunchKernelGGL((k_mvsym<5, 2>), grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, js, x, c.mvp, c.mvt, c.n); break;
+			case 6: hipLaunchKernelGGL((k_mvsym<6, 2>), grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, js, x, c.mvp, c.mvt, c.n); break;
+			case 7: hipLaunchKernelGGL((k_mvsym<7, 2>), grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, js, x, c.mvp, c.mvt, c.n); break;
+			default: hipLaunchKernelGGL((k_mvsym<8, 2>), grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, js, x, c.mvp, c.mvt, c.n); break;
+			}
+		}
+		if (g_mvt.on) {
+			// the upper triangle's tiles of the nact systems (16 B per entry), x once, the item and
+			// mirror sums written
+			HIPCHK(hipEventRecord(g_mvt.ev[g_mvt.used + 1], s));
+			g_mvt.used += 2;
+			double ent = 0, sums = 0;
+			for (int blk = 0; blk < 2; blk++) {
+				const int nb = blk ? c.nc : c.nseg, nt = (nb + 63) / 64;
+				for (int I = 0; I < nt; I++) {
+					const int rows = std::min(64, nb - 64 * I);
+					ent += (double)rows * (nb - 64 * I);
+					sums += (double)((nt - I + js - 1) / js) * rows + (double)(nb - 64 * (I + 1) > 0 ? nb - 64 * (I + 1) : 0);
+				}
+			}
+			g_mvt.bytes += 16.0 * nact * ent + 16.0 * c.n * F + 16.0 * F * sums;
+		}
+		hipLaunchKernelGGL(k_mvfin_sym, dim3(blocks((long)c.n * F)), dim3(256), 0, s, S, c.n, F, nact, g_offS, c.nseg, g_offQ,
+		                   c.nc, js, (const double2 *)c.mvp, (const double2 *)c.mvt, x, y);
+		return;
 	}
 	const int cs = mv_cs();
 	const int nrbS = (c.nseg + 63) / 64, nrbQ = (c.nc + 63) / 64;
@@ -842,7 +1070,7 @@ void free_fast(fast &c)
 		for (void *p : bb)
 			if (p) HIPCHK(hipFree(p));
 	}
-	void *bufs[] = {c.luw, c.inv, c.scr, c.ML, c.MU, c.dinvL, c.dinvU, c.piv, c.perm, c.V, c.Z, c.W, c.Xv, c.Bv, c.R, c.T1, c.tgp, c.mvp, c.partial, c.h, c.yd, c.scale};
+	void *bufs[] = {c.luw, c.inv, c.scr, c.ML, c.MU, c.dinvL, c.dinvU, c.piv, c.perm, c.V, c.Z, c.W, c.Xv, c.Bv, c.R, c.T1, c.tgp, c.mvp, c.mvt, c.partial, c.h, c.yd, c.scale};
 	for (void *b : bufs)
 		if (b) HIPCHK(hipFree(b));
 	c = fast();
@@ -981,7 +1209,13 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 		c.R = dal<double2>(nf);
 		c.T1 = dal<double2>(nf);
 		c.tgp = dal<double2>(tg_part_count(n, FMAX));
-		c.mvp = dal<double2>((size_t)((std::max(p->nseg, nc) + mv_cs() - 1) / mv_cs()) * n * FMAX);
+		{
+			const int ntm = (std::max(p->nseg, nc) + 63) / 64;
+			const size_t segs = std::max((size_t)((std::max(p->nseg, nc) + mv_cs() - 1) / mv_cs()),
+			                             (size_t)((ntm + mv_js() - 1) / mv_js()));
+			c.mvp = dal<double2>(segs * n * FMAX);
+			c.mvt = dal<double2>((size_t)std::max(1, ntm) * n * FMAX);
+		}
 		c.partial = dal<double2>((size_t)((n + DOT_ROWS - 1) / DOT_ROWS) * (MKRYLOV + 1) * FMAX);
 		c.h = dal<double2>((size_t)(MKRYLOV + 1) * FMAX);
 		c.yd = dal<double2>((size_t)(MKRYLOV + 1) * FMAX);

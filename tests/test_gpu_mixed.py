@@ -76,3 +76,24 @@ def test_mixed_fp64_preconditioner_build(lib, monkeypatch):
     assert _rel(z32, z64) <= 1e-6
     assert st32.gmres_its <= 1.25 * st64.gmres_its + 8
     m.close()
+
+
+@pytest.mark.parametrize("text_fn", [
+    lambda: decks.plate_deck(n=24, ndiv=8),          # Lp 1 200 (edge tiles of 48), P 625
+    lambda: deck_text("monopole_gp.peec"),           # ground-plane images
+])
+def test_mixed_symmetric_product_matches_full(lib, monkeypatch, text_fn):
+    """The GMRES product read from the upper triangles only (k_mvsym, default) against the
+    product over both triangles (k_mvdense): the same Zin to the GMRES tolerance."""
+    text = text_fn()
+    print(f"[mixed-sym] {text.splitlines()[1]}", flush=True)
+    m = Peec.from_deck_text(text, lib)
+    m.setup()
+    zs = {}
+    for sym in ("1", "0"):
+        monkeypatch.setenv("PEEC_MV_SYM", sym)
+        lib.peec_sweep_reset()
+        zs[sym], st = m.sweep(0, m.nfreq, PEEC_FORM_MIXED, 4)
+        print(f"  sym={sym}: {st.gmres_its} iterations", flush=True)
+    assert _rel(zs["1"], zs["0"]) <= 1e-8, _rel(zs["1"], zs["0"])
+    m.close()
