@@ -200,13 +200,13 @@ __global__ __launch_bounds__(256) void k_mvdense(slots S, int nact, int offS, in
 }
 
 // k_mvsym: the same product reading only the upper triangle of each symmetric block.  Items are
-// (64-row block I, run of js 64 x 64 tiles J >= I).  A wave covers 16 columns of a tile in four
-// row phases: lane l takes column 16 g + (l mod 16) and rows 4 q + (l / 16), q = 0..15, so one
-// load instruction reads four 256-byte runs of stored rows.  A tile above the diagonal serves
-// both triangles: B[r][c] x[c] summed over the 16 lanes of a row (four DPP steps inside the
-// row, no LDS) goes to row r, kept by the lane that owns r; B[r][c] x[r] accumulates per lane
-// for its column c and the four row phases are added once per 16 columns, written per (I, c)
-// to partT.  A diagonal tile is read whole and used once.  Row sums per item go to
+// (64-row block I, run of js 64 x 64 tiles J >= I), the systems of the batch one after another.
+// Lane l takes columns 16 g + (l mod 16), g = 0..3, of rows 4 q + (l / 16), q = 0..15: four
+// rows per step, each read whole by four back-to-back loads.  A tile above the diagonal serves
+// both triangles: B[r][c] x[c] summed over the lane's four columns and then the 16 lanes of a
+// row (four DPP steps inside the row, no LDS) goes to row r, kept by the lane that owns r;
+// B[r][c] x[r] accumulates per lane for its columns c and the four row phases are added once
+// per tile, written per (I, c) to partT.  A diagonal tile is read whole and used once.  Row sums per item go to
 // partN[s][row][f].  Half the bytes of k_mvdense.
 template <int CTRL>
 __device__ __forceinline__ double dpp_move(double v)
@@ -246,78 +246,80 @@ __global__ __launch_bounds__(256) void k_mvsym(slots S, int nact, int offS, int 
 	if (J0 >= nt) return;
 	const int J1 = min(nt, J0 + js);
 	const int rlo = I * 64, nr = min(64, nb - rlo);
-	const double2 *B[F];
-#pragma unroll
-	for (int f = 0; f < F; f++) B[f] = (f < nact) ? (lblk ? S.s[f].lp : S.s[f].P) : nullptr;
 	// lane l keeps row rlo + 4 (l mod 16) + l / 16
 	double nr_[F], ni_[F];
 #pragma unroll
 	for (int f = 0; f < F; f++) nr_[f] = ni_[f] = 0;
 	for (int J = J0; J < J1; J++) {
 		const bool diag = (J == I);
-		for (int g = 0; g < 4; g++) {
-			const int cl = J * 64 + g * 16 + cg;
-			const bool cact = cl < nb;
-			double2 xl[F];
-			double tr_[F], ti_[F];
+		// one system at a time: its tile rows are read whole (four 16-column groups per lane)
 #pragma unroll
-			for (int f = 0; f < F; f++) {
-				xl[f] = (cact && (f < nact)) ? X[(long)(o + cl) * F + f] : make_double2(0, 0);
-				tr_[f] = ti_[f] = 0;
+		for (int f = 0; f < F; f++) {
+			if (f >= nact) continue;
+			const double2 *Bf = lblk ? S.s[f].lp : S.s[f].P;
+			int cl[4];
+			bool cact[4];
+			double2 xl[4];
+			double tr_[4], ti_[4];
+#pragma unroll
+			for (int g = 0; g < 4; g++) {
+				cl[g] = J * 64 + g * 16 + cg;
+				cact[g] = cl[g] < nb;
+				xl[g] = cact[g] ? X[(long)(o + cl[g]) * F + f] : make_double2(0, 0);
+				tr_[g] = ti_[g] = 0;
 			}
 			for (int q0 = 0; q0 < 16; q0 += RU) {
-				double2 b[RU][F];
+				if (4 * q0 >= nr) break;
+				double2 b[RU][4];
 #pragma unroll
 				for (int u = 0; u < RU; u++) {
 					const int rr = 4 * (q0 + u) + ph;
 #pragma unroll
-					for (int f = 0; f < F; f++)
-						b[u][f] = (cact && (f < nact) && (rr < nr)) ? B[f][(long)(rlo + rr) * nb + cl] : make_double2(0, 0);
+					for (int g = 0; g < 4; g++)
+						b[u][g] = (cact[g] && (rr < nr)) ? Bf[(long)(rlo + rr) * nb + cl[g]] : make_double2(0, 0);
 				}
 #pragma unroll
 				for (int u = 0; u < RU; u++) {
 					const int q = q0 + u;
 					if (4 * q >= nr) continue;
 					const int rr = 4 * q + ph;
-					const double2 *xr = X + (long)(o + rlo + min(rr, nr - 1)) * F;
+					// row rlo + rr: sum over the tile's 64 columns (4 per lane, then the 16 lanes)
+					double pr = 0, pi = 0;
 #pragma unroll
-					for (int f = 0; f < F; f++) {
-						if (f >= nact) continue;
-						const double2 bb = b[u][f];
-						// row rlo + rr: sum over the 16 columns of the row's lanes
-						double pr = bb.x * xl[f].x;
-						pr = fma(-bb.y, xl[f].y, pr);
-						double pi = bb.x * xl[f].y;
-						pi = fma(bb.y, xl[f].x, pi);
-						pr = row16_sum(pr);
-						pi = row16_sum(pi);
-						if (cg == q) {
-							nr_[f] += pr;
-							ni_[f] += pi;
-						}
-						// the mirror entry B[c][r] = B[r][c]: column c takes B[r][c] x[r]
-						if (!diag) {
-							const double2 x = xr[f];
-							tr_[f] = fma(bb.x, x.x, tr_[f]);
-							tr_[f] = fma(-bb.y, x.y, tr_[f]);
-							ti_[f] = fma(bb.x, x.y, ti_[f]);
-							ti_[f] = fma(bb.y, x.x, ti_[f]);
+					for (int g = 0; g < 4; g++) {
+						pr = fma(b[u][g].x, xl[g].x, pr);
+						pr = fma(-b[u][g].y, xl[g].y, pr);
+						pi = fma(b[u][g].x, xl[g].y, pi);
+						pi = fma(b[u][g].y, xl[g].x, pi);
+					}
+					pr = row16_sum(pr);
+					pi = row16_sum(pi);
+					if (cg == q) {
+						nr_[f] += pr;
+						ni_[f] += pi;
+					}
+					// the mirror entries B[c][r] = B[r][c]: each column c takes B[r][c] x[r]
+					if (!diag) {
+						const double2 x = X[(long)(o + rlo + min(rr, nr - 1)) * F + f];
+#pragma unroll
+						for (int g = 0; g < 4; g++) {
+							tr_[g] = fma(b[u][g].x, x.x, tr_[g]);
+							tr_[g] = fma(-b[u][g].y, x.y, tr_[g]);
+							ti_[g] = fma(b[u][g].x, x.y, ti_[g]);
+							ti_[g] = fma(b[u][g].y, x.x, ti_[g]);
 						}
 					}
 				}
 			}
 			if (!diag) {
-				// the four row phases of column cl
+				// the four row phases of each column
 #pragma unroll
-				for (int f = 0; f < F; f++) {
-					tr_[f] += __shfl_xor(tr_[f], 16);
-					ti_[f] += __shfl_xor(ti_[f], 16);
-					tr_[f] += __shfl_xor(tr_[f], 32);
-					ti_[f] += __shfl_xor(ti_[f], 32);
-				}
-				if (cact && (ph == 0)) {
-#pragma unroll
-					for (int f = 0; f < F; f++) partT[((long)I * n + o + cl) * F + f] = make_double2(tr_[f], ti_[f]);
+				for (int g = 0; g < 4; g++) {
+					tr_[g] += __shfl_xor(tr_[g], 16);
+					ti_[g] += __shfl_xor(ti_[g], 16);
+					tr_[g] += __shfl_xor(tr_[g], 32);
+					ti_[g] += __shfl_xor(ti_[g], 32);
+					if (cact[g] && (ph == 0)) partT[((long)I * n + o + cl[g]) * F + f] = make_double2(tr_[g], ti_[g]);
 				}
 			}
 		}
