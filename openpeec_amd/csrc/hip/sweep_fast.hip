@@ -484,6 +484,18 @@ __global__ void k_scale(const double2 *Win, double2 *Vout, const double *scale, 
 	Vout[e] = make_double2(s * w.x, s * w.y);
 }
 
+// Vout[:, f] = Win[:, f] / sqrt(nn[f].x) (0 where the norm is 0): the next Arnoldi vector from
+// the device-side norm, the same arithmetic as 1 / sqrt on the host
+__global__ void k_scale_inv_norm(const double2 *Win, double2 *Vout, const double2 *nn, int n, int F)
+{
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= (long)n * F) return;
+	const double hn = sqrt(nn[e % F].x);
+	const double s = (hn > 0) ? 1.0 / hn : 0.0;
+	const double2 w = Win[e];
+	Vout[e] = make_double2(s * w.x, s * w.y);
+}
+
 // U[:, f] = sum_k y[k][f] V_k[:, f]
 __global__ void k_combine(const double2 *V, long vstride, int nk, const double2 *y, double2 *U, int n, int F)
 {
@@ -790,6 +802,8 @@ struct fast {
 	double2 *mvp = nullptr;   // segment partial sums of the dense blocks of A x
 	double2 *mvt = nullptr;   // mirror partial sums of the symmetric product (per row block and column)
 	double2 *partial = nullptr, *h = nullptr, *yd = nullptr;
+	double2 *h2 = nullptr, *nn = nullptr;   // second Gram-Schmidt pass and norms, device side
+	double2 *hp = nullptr;    // pinned host copy of h, h2, nn (one transfer per Arnoldi step)
 	double *scale = nullptr;
 };
 
@@ -936,6 +950,14 @@ void dots(fast &c, hipStream_t s, const double2 *V, long vstride, int nk, const 
 	HIPCHK(hipStreamSynchronize(s));
 }
 
+// h[k][f] = V_k^H W for k < nk into the device buffer hout (no transfer)
+void dots_dev(fast &c, hipStream_t s, const double2 *V, long vstride, int nk, const double2 *W, int F, double2 *hout)
+{
+	const int nb = (c.n + DOT_ROWS - 1) / DOT_ROWS;
+	hipLaunchKernelGGL(k_dots, dim3(nb, nk), dim3(256), 0, s, V, vstride, nk, W, c.n, F, c.partial);
+	hipLaunchKernelGGL(k_dots_final, dim3(blocks((long)nk * F)), dim3(256), 0, s, c.partial, nb, nk, F, hout);
+}
+
 // batched restarted flexible GMRES on F systems A_f x_f = b_f (b in c.Bv, x in c.Xv); returns
 // per-system iteration counts, -1 for a system that did not converge
 // warm: c.Xv already holds the initial guesses, else they are zero
@@ -981,20 +1003,24 @@ std::vector<int> gmres_batch(fast &c, hipStream_t s, int F, int nactive, bool wa
 			double2 *zj = c.Z + (long)j * vstride;
 			precond(c, s, vj, zj, F);
 			matvec(c, s, zj, c.W, F, nactive);
-			std::vector<double2> h1, h2;
-			dots(c, s, c.V, vstride, j + 1, c.W, F, h1);
-			HIPCHK(hipMemcpyAsync(c.h, h1.data(), h1.size() * sizeof(double2), hipMemcpyHostToDevice, s));
+			// classical Gram-Schmidt, two passes, and the new vector's norm, all on the device;
+			// the coefficients reach the host in one transfer for the Hessenberg column
+			dots_dev(c, s, c.V, vstride, j + 1, c.W, F, c.h);
 			hipLaunchKernelGGL(k_orth, dim3(blocks(nf)), dim3(256), 0, s, c.V, vstride, j + 1, c.h, c.W, n, F);
-			dots(c, s, c.V, vstride, j + 1, c.W, F, h2);
-			HIPCHK(hipMemcpyAsync(c.h, h2.data(), h2.size() * sizeof(double2), hipMemcpyHostToDevice, s));
-			hipLaunchKernelGGL(k_orth, dim3(blocks(nf)), dim3(256), 0, s, c.V, vstride, j + 1, c.h, c.W, n, F);
-			std::vector<double2> nn;
-			dots(c, s, c.W, 0, 1, c.W, F, nn);
-			std::vector<double> inv(F);
+			dots_dev(c, s, c.V, vstride, j + 1, c.W, F, c.h2);
+			hipLaunchKernelGGL(k_orth, dim3(blocks(nf)), dim3(256), 0, s, c.V, vstride, j + 1, c.h2, c.W, n, F);
+			dots_dev(c, s, c.W, 0, 1, c.W, F, c.nn);
+			hipLaunchKernelGGL(k_scale_inv_norm, dim3(blocks(nf)), dim3(256), 0, s, c.W, c.V + (long)(j + 1) * vstride, c.nn,
+			                   n, F);
+			const size_t nh = (size_t)(j + 1) * F;
+			const double2 *h1 = c.hp, *h2 = c.hp + nh, *nn = c.hp + 2 * nh;
+			HIPCHK(hipMemcpyAsync(c.hp, c.h, nh * sizeof(double2), hipMemcpyDeviceToHost, s));
+			HIPCHK(hipMemcpyAsync(c.hp + nh, c.h2, nh * sizeof(double2), hipMemcpyDeviceToHost, s));
+			HIPCHK(hipMemcpyAsync(c.hp + 2 * nh, c.nn, (size_t)F * sizeof(double2), hipMemcpyDeviceToHost, s));
+			HIPCHK(hipStreamSynchronize(s));
 			bool all_conv = true;
 			for (int f = 0; f < F; f++) {
 				const double hn = std::sqrt(nn[f].x);
-				inv[f] = (hn > 0) ? 1.0 / hn : 0.0;
 				if (done[f] || (kdone[f] >= 0)) continue;
 				zc *Hc = &H[f][(size_t)j * (MKRYLOV + 1)];
 				for (int i = 0; i <= j; i++) Hc[i] = zc(h1[(size_t)i * F + f].x + h2[(size_t)i * F + f].x, h1[(size_t)i * F + f].y + h2[(size_t)i * F + f].y);
@@ -1019,8 +1045,6 @@ std::vector<int> gmres_batch(fast &c, hipStream_t s, int F, int nactive, bool wa
 				if (std::abs(g[f][j + 1]) <= TOL * bnorm[f]) kdone[f] = j + 1;
 			}
 			for (int f = 0; f < F; f++) all_conv = all_conv && (done[f] || (kdone[f] >= 0));
-			HIPCHK(hipMemcpyAsync(c.scale, inv.data(), F * sizeof(double), hipMemcpyHostToDevice, s));
-			hipLaunchKernelGGL(k_scale, dim3(blocks(nf)), dim3(256), 0, s, c.W, c.V + (long)(j + 1) * vstride, c.scale, n, F);
 			if (all_conv) {
 				j++;
 				break;
@@ -1075,6 +1099,9 @@ void free_fast(fast &c)
 	void *bufs[] = {c.luw, c.inv, c.scr, c.ML, c.MU, c.dinvL, c.dinvU, c.piv, c.perm, c.V, c.Z, c.W, c.Xv, c.Bv, c.R, c.T1, c.tgp, c.mvp, c.mvt, c.partial, c.h, c.yd, c.scale};
 	for (void *b : bufs)
 		if (b) HIPCHK(hipFree(b));
+	if (c.h2) HIPCHK(hipFree(c.h2));
+	if (c.nn) HIPCHK(hipFree(c.nn));
+	if (c.hp) HIPCHK(hipHostFree(c.hp));
 	c = fast();
 }
 
@@ -1220,6 +1247,9 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 		}
 		c.partial = dal<double2>((size_t)((n + DOT_ROWS - 1) / DOT_ROWS) * (MKRYLOV + 1) * FMAX);
 		c.h = dal<double2>((size_t)(MKRYLOV + 1) * FMAX);
+		c.h2 = dal<double2>((size_t)(MKRYLOV + 1) * FMAX);
+		c.nn = dal<double2>(FMAX);
+		HIPCHK(hipHostMalloc((void **)&c.hp, (size_t)(2 * (MKRYLOV + 1) + 1) * FMAX * sizeof(double2), hipHostMallocDefault));
 		c.yd = dal<double2>((size_t)(MKRYLOV + 1) * FMAX);
 		c.scale = dal<double>(FMAX);
 	}
