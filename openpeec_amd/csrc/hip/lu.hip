@@ -25,6 +25,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <utility>
+#include <vector>
 #include "../peec_internal.h"
 #include "hip_check.hpp"
 
@@ -579,15 +581,19 @@ bool gemm_hip()
 	return e && (strcmp(e, "hip") == 0);
 }
 
-rocblas_handle blas()
+// one rocBLAS handle per stream (a handle's workspace must not serve two streams at once), bound
+// to that stream
+rocblas_handle blas(hipStream_t s)
 {
-	static rocblas_handle h = nullptr;
-	if (h == nullptr) {
-		if (rocblas_create_handle(&h) != rocblas_status_success) {
-			fprintf(stderr, "*** rocblas_create_handle failed\n");
-			abort();
-		}
+	static std::vector<std::pair<hipStream_t, rocblas_handle>> hs;
+	for (const auto &e : hs)
+		if (e.first == s) return e.second;
+	rocblas_handle h = nullptr;
+	if ((rocblas_create_handle(&h) != rocblas_status_success) || (rocblas_set_stream(h, s) != rocblas_status_success)) {
+		fprintf(stderr, "*** rocblas_create_handle failed\n");
+		abort();
 	}
+	hs.push_back(std::make_pair(s, h));
 	return h;
 }
 
@@ -604,8 +610,7 @@ void launch_gemm(hipStream_t s, int M, int N, int K, const double2 *A, long lda,
 {
 	if ((M <= 0) || (N <= 0) || (K <= 0)) return;
 	if (!gemm_hip()) {
-		rocblas_handle h = blas();
-		blas_check(rocblas_set_stream(h, s), "rocblas_set_stream");
+		rocblas_handle h = blas(s);
 		const rocblas_double_complex alpha{-1.0, 0.0}, beta{1.0, 0.0};
 		blas_check(rocblas_zgemm(h, rocblas_operation_none, rocblas_operation_none, N, M, K, &alpha,
 		                         (const rocblas_double_complex *)B, (rocblas_int)ldb, (const rocblas_double_complex *)A,
@@ -695,8 +700,7 @@ void launch_update(hipStream_t s, int M, int N, int K, const float2 *A, long lda
 {
 	if ((M <= 0) || (N <= 0) || (K <= 0)) return;
 	if (!gemm_hip()) {
-		rocblas_handle h = blas();
-		blas_check(rocblas_set_stream(h, s), "rocblas_set_stream");
+		rocblas_handle h = blas(s);
 		const rocblas_float_complex alpha{-1.0f, 0.0f}, beta{1.0f, 0.0f};
 		blas_check(rocblas_cgemm(h, rocblas_operation_none, rocblas_operation_none, N, M, K, &alpha,
 		                         (const rocblas_float_complex *)B, (rocblas_int)ldb, (const rocblas_float_complex *)A,
@@ -807,19 +811,12 @@ void scale_block(hipStream_t s, TS *X, long ld, int rows, int cols, double sc)
 	hipLaunchKernelGGL((k_scale_block<TS>), dim3(tiblocks((long)rows * cols)), dim3(256), 0, s, X, ld, rows, cols, sc);
 }
 
-// Out[i0.., i0..] (n x n, leading dimension ld, zero on entry) = inverse of the unit-lower (lower)
-// or upper part of the packed factor T over the same rows/columns.  Halves split on TI
-// boundaries; with T = [T11 0; T21 T22] the off-diagonal block of the inverse is
-// -inv(T22) T21 inv(T11) (lower) and -inv(T11) T12 inv(T22) (upper), two GEMMs on MFMA (FP64
-// for double2 storage, FP32 for float2).  Leaves are the diagonal-block inverses.  scr holds at
-// least ceil(n/2)^2 + TI n entries.
 // C = alpha * B * A (right) or alpha * A * B (left) for a triangular A, all column-major as
 // rocBLAS sees them (rocblas_?trmm, out of place); upper = A's fill in that view
 void trmm(hipStream_t s, bool right, bool upper, int m, int n, double alpha, const float2 *A, long lda, const float2 *B,
           long ldb, float2 *C, long ldc)
 {
-	rocblas_handle h = blas();
-	blas_check(rocblas_set_stream(h, s), "rocblas_set_stream");
+	rocblas_handle h = blas(s);
 	const rocblas_float_complex a{(float)alpha, 0.0f};
 	blas_check(rocblas_ctrmm(h, right ? rocblas_side_right : rocblas_side_left, upper ? rocblas_fill_upper : rocblas_fill_lower,
 	                         rocblas_operation_none, rocblas_diagonal_non_unit, m, n, &a, (const rocblas_float_complex *)A,
@@ -831,8 +828,7 @@ void trmm(hipStream_t s, bool right, bool upper, int m, int n, double alpha, con
 void trmm(hipStream_t s, bool right, bool upper, int m, int n, double alpha, const double2 *A, long lda, const double2 *B,
           long ldb, double2 *C, long ldc)
 {
-	rocblas_handle h = blas();
-	blas_check(rocblas_set_stream(h, s), "rocblas_set_stream");
+	rocblas_handle h = blas(s);
 	const rocblas_double_complex a{alpha, 0.0};
 	blas_check(rocblas_ztrmm(h, right ? rocblas_side_right : rocblas_side_left, upper ? rocblas_fill_upper : rocblas_fill_lower,
 	                         rocblas_operation_none, rocblas_diagonal_non_unit, m, n, &a, (const rocblas_double_complex *)A,
@@ -841,8 +837,11 @@ void trmm(hipStream_t s, bool right, bool upper, int m, int n, double alpha, con
 	           "rocblas_ztrmm");
 }
 
-// Recursive inversion: inv([T11 0; T21 T22]) = [O11 0; -O22 T21 O11  O22] (lower; the upper case
-// mirrors it), the diagonal 64 x 64 blocks inverted directly (k_diaginv).  With rocBLAS both
+// Out[i0.., i0..] (n x n, leading dimension ld, zero on entry) = inverse of the unit-lower (lower)
+// or upper part of the packed factor T over the same rows/columns; scr holds at least
+// ceil(n/2)^2 + TI n entries.  Recursive: inv([T11 0; T21 T22]) = [O11 0; -O22 T21 O11  O22]
+// (lower; the upper case mirrors it), halves split on TI boundaries, the diagonal 64 x 64 blocks
+// inverted directly (k_diaginv).  With rocBLAS both
 // products of a level are triangular multiplies (O11 and O22 are triangular), half the flops of
 // full GEMMs; row-major X is the column-major X^T, so a row-major lower triangle is an upper
 // one to rocBLAS and a right-hand product becomes a left-hand one.  PEEC_GEMM=hip runs them as

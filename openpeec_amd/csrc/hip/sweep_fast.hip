@@ -777,6 +777,9 @@ struct fast {
 	double2 *luw = nullptr;   // FP64 factorisation workspace
 	double2 *inv = nullptr;   // FP64 inverse of one factor (built, then demoted)
 	double2 *scr = nullptr;   // GEMM scratch of the recursive inversion
+	double2 *scr2 = nullptr;  // the same for the second triangle (inverted concurrently)
+	hipStream_t s2 = nullptr; // second stream of the inversion (default priority)
+	hipEvent_t ev_f = nullptr, ev_u = nullptr;
 	float2 *ML = nullptr, *MU = nullptr;   // FP32 Linv, Uinv, stored by columns
 	double2 *dinvL = nullptr, *dinvU = nullptr;
 	int *piv = nullptr, *perm = nullptr;
@@ -1099,6 +1102,10 @@ void free_fast(fast &c)
 	void *bufs[] = {c.luw, c.inv, c.scr, c.ML, c.MU, c.dinvL, c.dinvU, c.piv, c.perm, c.V, c.Z, c.W, c.Xv, c.Bv, c.R, c.T1, c.tgp, c.mvp, c.mvt, c.partial, c.h, c.yd, c.scale};
 	for (void *b : bufs)
 		if (b) HIPCHK(hipFree(b));
+	if (c.scr2) HIPCHK(hipFree(c.scr2));
+	if (c.s2) HIPCHK(hipStreamDestroy(c.s2));
+	if (c.ev_f) HIPCHK(hipEventDestroy(c.ev_f));
+	if (c.ev_u) HIPCHK(hipEventDestroy(c.ev_u));
 	if (c.h2) HIPCHK(hipFree(c.h2));
 	if (c.nn) HIPCHK(hipFree(c.nn));
 	if (c.hp) HIPCHK(hipHostFree(c.hp));
@@ -1223,6 +1230,10 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 		c.luw = dal<double2>((size_t)n * n);
 		c.inv = dal<double2>((size_t)n * n);
 		c.scr = dal<double2>((size_t)((n + 1) / 2 + TB) * ((n + 1) / 2 + TB));
+		c.scr2 = dal<double2>((size_t)((n + 1) / 2 + TB) * ((n + 1) / 2 + TB));
+		HIPCHK(hipStreamCreate(&c.s2));
+		HIPCHK(hipEventCreateWithFlags(&c.ev_f, hipEventDisableTiming));
+		HIPCHK(hipEventCreateWithFlags(&c.ev_u, hipEventDisableTiming));
 		c.ML = dal<float2>((size_t)n * n);
 		c.MU = dal<float2>((size_t)n * n);
 		c.dinvL = dal<double2>((size_t)nblk * TB * TB);
@@ -1351,14 +1362,20 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 					build_inverses(c, s);
 				}
 				else {
-					// each inverse built row-major in the free half of the workspace, then stored
-					// by columns
-					float2 *tmp = a32 + (size_t)n * n;
+					// each inverse built row-major in free workspace, then stored by columns; the two
+					// are independent and run on two streams (the recursion's small levels leave
+					// most of the GPU idle): Linv from the free half of the inverse workspace on s,
+					// Uinv from the FP64 assembly buffer (no longer needed) on the second stream
+					float2 *tmpL = a32 + (size_t)n * n, *tmpU = (float2 *)c.luw;
 					const unsigned nt = (unsigned)((n + 31) / 32);
-					slu_tri_inverse(s, n, a32, n, 1, tmp, c.scr, c.dinvL);
-					hipLaunchKernelGGL(k_transpose32<float2>, dim3(nt, nt), dim3(256), 0, s, (const float2 *)tmp, c.ML, n);
-					slu_tri_inverse(s, n, a32, n, 0, tmp, c.scr, c.dinvU);
-					hipLaunchKernelGGL(k_transpose32<float2>, dim3(nt, nt), dim3(256), 0, s, (const float2 *)tmp, c.MU, n);
+					HIPCHK(hipEventRecord(c.ev_f, s));
+					HIPCHK(hipStreamWaitEvent(c.s2, c.ev_f, 0));
+					slu_tri_inverse(c.s2, n, a32, n, 0, tmpU, c.scr2, c.dinvU);
+					hipLaunchKernelGGL(k_transpose32<float2>, dim3(nt, nt), dim3(256), 0, c.s2, (const float2 *)tmpU, c.MU, n);
+					HIPCHK(hipEventRecord(c.ev_u, c.s2));
+					slu_tri_inverse(s, n, a32, n, 1, tmpL, c.scr, c.dinvL);
+					hipLaunchKernelGGL(k_transpose32<float2>, dim3(nt, nt), dim3(256), 0, s, (const float2 *)tmpL, c.ML, n);
+					HIPCHK(hipStreamWaitEvent(s, c.ev_u, 0));
 				}
 			}
 			HIPCHK(hipStreamSynchronize(s));
