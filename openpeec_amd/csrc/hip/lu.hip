@@ -8,9 +8,10 @@
 // Structure (right-looking, blocked by NB columns):
 //   k_panel     one cooperative launch per panel: the panel rows are split over P workgroups
 //               held in LDS; per column every workgroup publishes its local pivot candidate
-//               (value, row index, row contents) and the current diagonal row, one grid
-//               barrier, then every workgroup derives the same winner, swaps and eliminates.
-//               One grid barrier per column is the whole latency cost of partial pivoting.
+//               (value, row index, row contents) and the current diagonal row, reads every
+//               workgroup's candidate value from a ring of slots (the wait and the read in one
+//               poll), then every workgroup derives the same winner, swaps and eliminates.
+//               That exchange per column is the whole latency cost of partial pivoting.
 //   k_laswp     applies the panel's swaps to the columns left and right of the panel and to
 //               the running row permutation (used by the solves as one gather).
 //   k_trsm_lu   U12 = L11^-1 A12 (unit lower, LDS-staged).
@@ -36,6 +37,8 @@ constexpr int NB = 64;          // panel width (GEMM K)
 constexpr int PANEL_T = 256;    // threads per panel workgroup
 constexpr int PANEL_RMAX = 144; // panel rows per workgroup held in LDS (144 * 65 * 16 B = 146 KiB)
 constexpr int PANEL_PMAX = 256; // workgroups in a panel launch (at most one per CU)
+constexpr int PANEL_RING = 4;   // columns of candidate slots in flight (see k_panel)
+constexpr unsigned PANEL_SPIN_LIMIT = 1u << 22;   // polls before the exchange gives up (seconds)
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
@@ -52,37 +55,18 @@ __device__ __forceinline__ double2 cinv(double2 z)
 }
 
 // ------------------------------------------------------------------------------------------
-// grid barrier for cooperative launches, two levels: the P workgroups arrive at the counter
-// of their group of BAR_GROUP (one 64-byte line each); the last arrival of a group adds one to
-// the root counter, which every workgroup polls.  Same-address atomics serialise at the memory
-// side, so a flat counter cost about 40 ns per arrival (4.7 us per column with 117
-// workgroups); with groups the longest chain is a group plus the groups' arrivals at the root.
-// Counters are cleared before each launch and count up within it: step s (1, 2, ...) completes
-// when the root reaches ngroups * s.
-constexpr int BAR_GROUP = 16;
-constexpr int BAR_STRIDE = 16;   // unsigned per counter (one 64-byte line)
-constexpr int BAR_BYTES = (1 + PANEL_PMAX / BAR_GROUP) * BAR_STRIDE * (int)sizeof(unsigned);
-
-__device__ __forceinline__ void grid_barrier(unsigned *bar, int P, int w, unsigned step)
-{
-	__syncthreads();
-	if (threadIdx.x == 0) {
-		const int grp = w / BAR_GROUP;
-		const unsigned gsize = (unsigned)min(BAR_GROUP, P - grp * BAR_GROUP);
-		const unsigned ngrp = (unsigned)((P + BAR_GROUP - 1) / BAR_GROUP);
-		__threadfence();
-		const unsigned old = atomicAdd(bar + BAR_STRIDE * (1 + grp), 1u);
-		if (old == gsize * step - 1) {
-			__threadfence();
-			atomicAdd(bar, 1u);
-		}
-		while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < ngrp * step) {
-			__builtin_amdgcn_s_sleep(1);
-		}
-		__threadfence();
-	}
-	__syncthreads();
-}
+// Pivot exchange of the cooperative panel: per column every workgroup publishes its candidate
+// (row index, row contents, and the value max |re| + |im|) and reads all candidates' values.
+// The values go to a ring of PANEL_RING columns of slots, one per workgroup, NaN meaning "not
+// yet written" (the ring is filled with 0xFF bytes, a NaN, before each launch): the reader polls
+// the slots with coherent loads until none is NaN, so the wait and the read of the values are
+// one round trip, where a counter barrier needed an atomic, a poll and then the read.  The
+// writer publishes the row with plain stores and a release fence before its slot; the reader
+// fences (acquire) after its poll.  A workgroup clears its own slot two columns ahead once it
+// has seen every slot of the current column: all workgroups have then left the column two
+// back.  A two-level counter barrier measured 5.2 us per column at 117 workgroups; the first
+// flat counter 4.7-6.5 us (same-address atomics serialise at the memory side).
+constexpr int SLOT_BYTES = PANEL_RING * PANEL_PMAX * (int)sizeof(double);
 
 // (value, index) arg-max with ties to the smaller index; value < 0 marks "no candidate"
 __device__ __forceinline__ void amax_merge(double &v, int &i, double ov, int oi)
@@ -101,11 +85,10 @@ __device__ __forceinline__ void st2(double2 &d, double2 v) { d = v; }
 __device__ __forceinline__ void st2(float2 &d, double2 v) { d = make_float2((float)v.x, (float)v.y); }
 
 struct panel_ws {
-	double  *pubv;      // [2][P]     local max |re|+|im|
+	double  *slot;      // [PANEL_RING][PANEL_PMAX] local max |re|+|im| per column, NaN until written
 	int     *pubi;      // [2][P]     its global row
 	double2 *pubrow;    // [2][P][NB] that row's panel entries
 	double2 *diagrow;   // [2][NB]    current row kk
-	unsigned *bar;      // barrier counter
 	int     *info;      // singular row or -1
 };
 
@@ -138,8 +121,9 @@ __global__ __launch_bounds__(PANEL_T) void k_panel(TS *A, long lda, int n, int k
 	__syncthreads();
 
 	for (int jj = 0; jj < nb; jj++) {
-		const int kk = k0 + jj, par = jj & 1;
+		const int kk = k0 + jj, par = jj & 1, ring = jj & (PANEL_RING - 1);
 		if (wave == 0) {
+			// local candidate: the first row of largest |re| + |im| among this workgroup's rows
 			double bv = -1;
 			int bi = INT_MAX;
 			for (int r = lane; r < rc; r += 64) {
@@ -154,44 +138,71 @@ __global__ __launch_bounds__(PANEL_T) void k_panel(TS *A, long lda, int n, int k
 				const int oi = __shfl_xor(bi, off);
 				amax_merge(bv, bi, ov, oi);
 			}
-			if (lane == 0) {
-				ws.pubv[par * P + w] = bv;
-				ws.pubi[par * P + w] = bi;
-			}
+			// publish: row index and row contents (and the current diagonal row), then, after a
+			// release fence, the candidate's value in this workgroup's slot of the ring
+			if (lane == 0) ws.pubi[par * P + w] = bi;
 			if ((bv >= 0) && (lane < nb)) ws.pubrow[((long)par * P + w) * NB + lane] = S[(bi - r0) * SROW + lane];
 			if ((kk >= r0) && (kk < r0 + rc) && (lane < nb)) ws.diagrow[par * NB + lane] = S[(kk - r0) * SROW + lane];
-		}
-		grid_barrier(ws.bar, P, w, (unsigned)(jj + 1));
-		if (wave == 0) {
+			__threadfence();
+			if (lane == 0) __hip_atomic_store(&ws.slot[ring * PANEL_PMAX + w], bv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			// wait until every workgroup's slot of this column holds a value (NaN: not yet),
+			// reading the values themselves; bounded, so a fault in the exchange ends the
+			// factorisation (reported as a zero pivot) instead of hanging
+			double qv[PANEL_PMAX / 64];
+			unsigned spins = 0;
+			bool timeout = false;
+			for (;;) {
+				bool ok = true;
+#pragma unroll
+				for (int i = 0; i < PANEL_PMAX / 64; i++) {
+					const int q = lane + 64 * i;
+					qv[i] = -1;
+					if (q < P) {
+						qv[i] = __hip_atomic_load(&ws.slot[ring * PANEL_PMAX + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+						if (qv[i] != qv[i]) ok = false;
+					}
+				}
+				if (__all(ok)) break;
+				if (++spins > PANEL_SPIN_LIMIT) {
+					timeout = true;
+					break;
+				}
+				__builtin_amdgcn_s_sleep(1);
+			}
+			__threadfence();
+			// the winner: largest value, ties to the smaller workgroup (whose rows come first)
 			double gv = -1;
-			int gi = INT_MAX, gw = 0;
-			for (int q = lane; q < P; q += 64) {
-				const double qv = ws.pubv[par * P + q];
-				const int qi = ws.pubi[par * P + q];
-				if ((qv > gv) || ((qv == gv) && (qi < gi))) {
-					gv = qv;
-					gi = qi;
+			int gw = INT_MAX;
+#pragma unroll
+			for (int i = 0; i < PANEL_PMAX / 64; i++) {
+				const int q = lane + 64 * i;
+				if ((q < P) && ((qv[i] > gv) || ((qv[i] == gv) && (q < gw)))) {
+					gv = qv[i];
 					gw = q;
 				}
 			}
 			for (int off = 32; off > 0; off >>= 1) {
 				const double ov = __shfl_xor(gv, off);
-				const int oi = __shfl_xor(gi, off);
 				const int ow = __shfl_xor(gw, off);
-				if ((ov > gv) || ((ov == gv) && (oi < gi))) {
+				if ((ov > gv) || ((ov == gv) && (ow < gw))) {
 					gv = ov;
-					gi = oi;
 					gw = ow;
 				}
 			}
-			const int stop = !(gv >= 1e-30);
+			const int stop = timeout || !(gv >= 1e-30);
+			const int gi = (gw < P) ? ws.pubi[par * P + gw] : INT_MAX;
 			if (!stop && (lane < nb)) {
 				U[lane] = ws.pubrow[((long)par * P + gw) * NB + lane];
 				D[lane] = ws.diagrow[par * NB + lane];
 			}
+			// this workgroup's slot for column jj + 2 back to "not yet": every workgroup has
+			// published column jj, so none still reads column jj - 2's values
 			if (lane == 0) {
+				__hip_atomic_store(&ws.slot[((jj + 2) & (PANEL_RING - 1)) * PANEL_PMAX + w], __builtin_nan(""),
+				                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 				s_gi = gi;
 				s_stop = stop;
+				if (timeout) printf("*** LU panel: pivot exchange timed out (column %d)\n", kk);
 				if (w == 0) {
 					piv[kk] = stop ? ((gi == INT_MAX) ? kk : gi) : gi;
 					if (stop) *ws.info = kk;
@@ -547,8 +558,8 @@ void ensure_ws(int n)
 		HIPCHK(hipDeviceGetAttribute(&g_lw.ncu, hipDeviceAttributeMultiprocessorCount, dev));
 		g_lw.attr = true;
 	}
-	const size_t need = 2 * PANEL_PMAX * sizeof(double) + 2 * PANEL_PMAX * sizeof(int) +
-	                    2 * (size_t)PANEL_PMAX * NB * sizeof(double2) + 2 * NB * sizeof(double2) + BAR_BYTES + 256;
+	const size_t need = 2 * PANEL_PMAX * sizeof(int) + 2 * (size_t)PANEL_PMAX * NB * sizeof(double2) +
+	                    2 * NB * sizeof(double2) + SLOT_BYTES + 256;
 	if (g_lw.base == nullptr) {
 		HIPCHK(hipMalloc((void **)&g_lw.base, need));
 		char *p = g_lw.base;
@@ -556,12 +567,10 @@ void ensure_ws(int n)
 		p += 2 * (size_t)PANEL_PMAX * NB * sizeof(double2);
 		g_lw.ws.diagrow = (double2 *)p;
 		p += 2 * NB * sizeof(double2);
-		g_lw.ws.pubv = (double *)p;
-		p += 2 * PANEL_PMAX * sizeof(double);
+		g_lw.ws.slot = (double *)p;
+		p += SLOT_BYTES;
 		g_lw.ws.pubi = (int *)p;
 		p += 2 * PANEL_PMAX * sizeof(int);
-		g_lw.ws.bar = (unsigned *)p;
-		p += BAR_BYTES;
 		g_lw.ws.info = (int *)p;
 	}
 	if (g_lw.nperm < n) {
@@ -945,7 +954,7 @@ int lu_factor_t(hipStream_t s, int n, TS *A, int lda, int *dpiv, int *dperm, pgp
 		long ldp = ldal;
 		void *kargs[] = {(void *)&Ap, (void *)&ldp, (void *)&nn, (void *)&kk0, (void *)&nbb,
 		                 (void *)&R, (void *)&pivp, (void *)&ws};
-		HIPCHK(hipMemsetAsync(g_lw.ws.bar, 0, BAR_BYTES, s));
+		HIPCHK(hipMemsetAsync(g_lw.ws.slot, 0xff, SLOT_BYTES, s));   // every slot NaN
 		HIPCHK(hipLaunchCooperativeKernel((const void *)k_panel<TS>, dim3(P), dim3(PANEL_T), kargs,
 		                                  (unsigned)(R * SROW * sizeof(double2)), s));
 		const int nout = n - nb;
