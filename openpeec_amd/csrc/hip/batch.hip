@@ -53,6 +53,7 @@ namespace {
 
 constexpr int BMAXN = 4096;    // largest system order of a batched job (LDS holds b and pivots)
 constexpr int BMAXPORT = 8;
+constexpr int BSW = 8;          // sub-panel width inside a batched panel (s_L holds BSW^2 + BSW NB)
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
@@ -482,75 +483,151 @@ __device__ __forceinline__ int blu_factor(TS *A, int N, int *piv, double2 *s_row
 	for (int k0 = 0; (k0 < N) && (info < 0); k0 += NB) {
 		const int nb = (N - k0 < NB) ? (N - k0) : NB;
 		const int kend = k0 + nb;
-		for (int c = k0; c < kend; c++) {
-			double best = -1;
-			int bi = N;
-			for (int r = c + tid; r < N; r += 256) {
-				const double2 v = ld2(A[(long)r * N + c]);
-				const double a = fabs(v.x) + fabs(v.y);
-				if (a > best) {
-					best = a;
-					bi = r;
-				}
-			}
-			wg_argmax(best, bi, s_rv, s_ri);
-			if (best < 1e-30) {
-				info = c;
-				break;
-			}
-			if (tid == 0) piv[c] = bi;
-			if (bi != c)
-				for (int col = tid; col < N; col += 256) {
-					const TS t = A[(long)c * N + col];
-					A[(long)c * N + col] = A[(long)bi * N + col];
-					A[(long)bi * N + col] = t;
-				}
-			__syncthreads();
-			const double2 inv = crecip(ld2(A[(long)c * N + c]));
-			for (int r = c + 1 + tid; r < N; r += 256) st2(A[(long)r * N + c], cmul(ld2(A[(long)r * N + c]), inv));
-			const int w = kend - c - 1;
-			if (tid < w) s_row[tid] = ld2(A[(long)c * N + c + 1 + tid]);
-			__syncthreads();
-			if (w > 0) {
-				// rank-1 update of the panel's remaining columns, every row below c
-				for (int r = c + 1 + tid; r < N; r += 256) {
-					TS *ar = A + (long)r * N;
-					const double2 l = ld2(ar[c]);
-#pragma unroll 4
-					for (int t = 0; t < w; t++) {
-						const double2 u = s_row[t];
-						double2 x = ld2(ar[c + 1 + t]);
-						x.x -= (l.x * u.x) - (l.y * u.y);
-						x.y -= (l.x * u.y) + (l.y * u.x);
-						st2(ar[c + 1 + t], x);
+		// the panel in sub-panels of BSW columns: inside a sub-panel the unblocked steps update
+		// only its own columns; then its U rows for the rest of the panel (forward substitution
+		// with its unit-lower part) and one rank-BSW update of the rest of the panel's rows, so
+		// the panel is swept once per sub-panel instead of once per column
+		for (int c0 = k0; (c0 < kend) && (info < 0); c0 += BSW) {
+			const int ce = min(c0 + BSW, kend);
+			for (int c = c0; c < ce; c++) {
+				double best = -1;
+				int bi = N;
+				for (int r = c + tid; r < N; r += 256) {
+					const double2 v = ld2(A[(long)r * N + c]);
+					const double a = fabs(v.x) + fabs(v.y);
+					if (a > best) {
+						best = a;
+						bi = r;
 					}
 				}
+				wg_argmax(best, bi, s_rv, s_ri);
+				if (best < 1e-30) {
+					info = c;
+					break;
+				}
+				if (tid == 0) piv[c] = bi;
+				if (bi != c)
+					for (int col = tid; col < N; col += 256) {
+						const TS t = A[(long)c * N + col];
+						A[(long)c * N + col] = A[(long)bi * N + col];
+						A[(long)bi * N + col] = t;
+					}
+				__syncthreads();
+				const double2 inv = crecip(ld2(A[(long)c * N + c]));
+				for (int r = c + 1 + tid; r < N; r += 256) st2(A[(long)r * N + c], cmul(ld2(A[(long)r * N + c]), inv));
+				const int w = ce - c - 1;
+				if (tid < w) s_row[tid] = ld2(A[(long)c * N + c + 1 + tid]);
+				__syncthreads();
+				if (w > 0) {
+					// rank-1 update of the sub-panel's remaining columns, every row below c
+					for (int r = c + 1 + tid; r < N; r += 256) {
+						TS *ar = A + (long)r * N;
+						const double2 l = ld2(ar[c]);
+						for (int t = 0; t < w; t++) {
+							const double2 u = s_row[t];
+							double2 x = ld2(ar[c + 1 + t]);
+							x.x -= (l.x * u.x) - (l.y * u.y);
+							x.y -= (l.x * u.y) + (l.y * u.x);
+							st2(ar[c + 1 + t], x);
+						}
+					}
+				}
+				__syncthreads();
 			}
-			__syncthreads();
+			if (info >= 0) break;
+			const int wr = kend - ce, sw = ce - c0;
+			if (wr > 0) {
+				// s_L[0 .. BSW^2): the sub-panel's unit-lower part; s_U = s_L + BSW^2: its U rows
+				// for the panel's remaining columns (stride NB)
+				double2 *s_U = s_L + BSW * BSW;
+				for (int e = tid; e < sw * sw; e += 256) {
+					const int i = e / sw, j = e - i * sw;
+					s_L[i * BSW + j] = ld2(A[(long)(c0 + i) * N + c0 + j]);
+				}
+				__syncthreads();
+				if (tid < wr) {
+					const int col = ce + tid;
+					double2 u[BSW];
+#pragma unroll
+					for (int i = 0; i < BSW; i++) {
+						if (i >= sw) break;
+						double2 x = ld2(A[(long)(c0 + i) * N + col]);
+#pragma unroll
+						for (int j = 0; j < BSW; j++) {
+							if (j >= i) break;
+							const double2 l = s_L[i * BSW + j];
+							x.x -= (l.x * u[j].x) - (l.y * u[j].y);
+							x.y -= (l.x * u[j].y) + (l.y * u[j].x);
+						}
+						u[i] = x;
+						st2(A[(long)(c0 + i) * N + col], x);
+						s_U[i * NB + tid] = x;
+					}
+				}
+				__syncthreads();
+				// rank-sw update of the rest of the panel, every row below the sub-panel
+				for (int r = ce + tid; r < N; r += 256) {
+					TS *ar = A + (long)r * N;
+					double2 l[BSW];
+#pragma unroll
+					for (int j = 0; j < BSW; j++) l[j] = (j < sw) ? ld2(ar[c0 + j]) : make_double2(0, 0);
+					for (int t = 0; t < wr; t++) {
+						double2 x = ld2(ar[ce + t]);
+#pragma unroll
+						for (int j = 0; j < BSW; j++) {
+							if (j >= sw) break;
+							const double2 u = s_U[j * NB + t];
+							x.x -= (l[j].x * u.x) - (l[j].y * u.y);
+							x.y -= (l[j].x * u.y) + (l[j].y * u.x);
+						}
+						st2(ar[ce + t], x);
+					}
+				}
+				__syncthreads();
+			}
 		}
 		if (info >= 0) break;
 		const int m = N - kend;
 		if (m <= 0) break;
 		// U12 = L11^-1 A12 once the panel's interchanges are all applied: every thread owns
-		// columns, eliminating down the panel rows (coalesced across threads, row by row)
+		// columns (loads and stores coalesced across threads, row by row)
 		for (int e = tid; e < nb * nb; e += 256) {
 			const int r = e / nb, cc = e - r * nb;
 			s_L[r * NB + cc] = ld2(A[(long)(k0 + r) * N + k0 + cc]);
 		}
 		__syncthreads();
-		for (int col = kend + tid; col < N; col += 256)
+		// blocked forward substitution down a thread's column, 8 rows in registers at a time:
+		// the rows already solved are read back once per block, each row stored once (about
+		// 3 nb + nb^2 / 16 accesses per column instead of nb^2 read-modify-writes); rows from nb
+		// on, in a short last panel, are computed from stale s_L entries and never stored
+		for (int col = kend + tid; col < N; col += 256) {
+			for (int i0 = 0; i0 < nb; i0 += 8) {
+				double2 u[8];
+#pragma unroll
+				for (int q = 0; q < 8; q++) u[q] = (i0 + q < nb) ? ld2(A[(long)(k0 + i0 + q) * N + col]) : make_double2(0, 0);
 #pragma unroll 1
-			for (int c = 0; c < nb - 1; c++) {
-				const double2 u = ld2(A[(long)(k0 + c) * N + col]);
-#pragma unroll 4
-				for (int r = c + 1; r < nb; r++) {
-					const double2 l = s_L[r * NB + c];
-					double2 x = ld2(A[(long)(k0 + r) * N + col]);
-					x.x -= (l.x * u.x) - (l.y * u.y);
-					x.y -= (l.x * u.y) + (l.y * u.x);
-					st2(A[(long)(k0 + r) * N + col], x);
+				for (int c = 0; c < i0; c++) {
+					const double2 x = ld2(A[(long)(k0 + c) * N + col]);
+#pragma unroll
+					for (int q = 0; q < 8; q++) {
+						const double2 l = s_L[(i0 + q) * NB + c];
+						u[q].x -= (l.x * x.x) - (l.y * x.y);
+						u[q].y -= (l.x * x.y) + (l.y * x.x);
+					}
 				}
+#pragma unroll
+				for (int c = 0; c < 7; c++)
+#pragma unroll
+					for (int q = c + 1; q < 8; q++) {
+						const double2 l = s_L[(i0 + q) * NB + i0 + c];
+						u[q].x -= (l.x * u[c].x) - (l.y * u[c].y);
+						u[q].y -= (l.x * u[c].y) + (l.y * u[c].x);
+					}
+#pragma unroll
+				for (int q = 0; q < 8; q++)
+					if (i0 + q < nb) st2(A[(long)(k0 + i0 + q) * N + col], u[q]);
 			}
+		}
 		__syncthreads();
 		trailing_update<NB, CB>(A, N, k0, nb, m);
 		__syncthreads();
