@@ -612,6 +612,23 @@ void pgpu_kf_fill(pgpu_t *g, double kw, d_complex_t *dlp, d_complex_t *dP)
 	}
 }
 
+// Lp (which 0) or P (which 1) at kw from the records into out (original numbering, leading
+// dimension the cell count); 1 when the records do not cover kw or the sub-cell matrix that P
+// is aggregated from would not fit in a quarter of free memory (nothing launched then)
+int pgpu_kf_fill_kind(pgpu_t *g, int which, double kw, d_complex_t *out)
+{
+	long ns = 0;
+	int T = 0;
+	if (pgpu_kf_records(g, which, kw, &ns, &T) == nullptr) return 1;
+	if ((which == 1) && !g->cells_identity && (g->G == nullptr)) {
+		size_t fr = 0, tot = 0;
+		HIPCHK(hipMemGetInfo(&fr, &tot));
+		if ((double)g->chg.n * g->chg.n * sizeof(double2) > 0.25 * (double)fr) return 1;
+	}
+	pgpu_kf_fill(g, kw, (which == 0) ? out : nullptr, (which == 1) ? out : nullptr);
+	return 0;
+}
+
 // the same for nk wavenumbers (a batch of frequencies): the records are read once for all of them
 void pgpu_kf_fill_multi(pgpu_t *g, int nk, const double *kw, d_complex_t *const *dlp, d_complex_t *const *dP)
 {

@@ -9,6 +9,13 @@
 //     the norms are spread over the workgroup, reductions run in a fixed tree order, pivot
 //     searches keep the reference's first-extremum rule;
 //   * near blocks and far blocks that did not compress are evaluated densely (k_hm_dense);
+//   * when the whole matrix fits in a quarter of free device memory, every entry is evaluated
+//     first, in bulk, and ACA and the dense blocks read them from there: over a sweep by the
+//     k-factored fill of the band (records read in slot order, P aggregated from the sub-cell
+//     matrix in the order hm_entry sums), otherwise one thread per entry (k_hm_full).  The
+//     entries are the same numbers either way (same device arithmetic, same summation order);
+//     what changes is that the costly part of an entry (a Horner per sub-cell pair, or the
+//     element code) runs at full occupancy instead of inside ACA's sequential rank steps;
 //   * y = H x in two stages as the reference's hmat_matvec (hmatrix.c:680-778): t_b = V_b^T x
 //     per far block, then one workgroup per leaf row cluster adds U_b t_b and the dense rows of
 //     its blocks in the registered order, so every output row is written by one workgroup.
@@ -18,6 +25,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 #include "elements.hpp"
@@ -74,6 +82,18 @@ __device__ cplx hm_entry(const hent &E, int i, int j)
 	}
 	const double s = 1 / (E.f.k4pe * E.f.carea[i] * E.f.carea[j]);
 	return mk(s * sr, s * si);
+}
+
+// entry at permuted positions (a, b): A(perm[a], perm[b]), from the bulk-evaluated matrix when
+// there is one
+template <int KM>
+__device__ __forceinline__ cplx hm_pos(const hent &E, const int *perm, int a, int b)
+{
+	if (E.full != nullptr) {
+		const double2 v = E.full[(long)perm[a] * E.nfull + perm[b]];
+		return mk(v.x, v.y);
+	}
+	return hm_entry<KM>(E, perm[a], perm[b]);
 }
 
 constexpr int AT = 256;
@@ -202,9 +222,8 @@ __global__ __launch_bounds__(AT) void k_aca(aca_args a)
 		// residual row  v = A(irow, :) - sum_l u_l[irow] v_l
 		double amax = 0, rn2 = 0;
 		int jmax = -1;
-		const int gi = a.perm[r0 + irow];
 		for (int j = tid; j < nn; j += AT) {
-			const cplx e = hm_entry<KM>(a.E, gi, a.perm[c0 + j]);
+			const cplx e = hm_pos<KM>(a.E, a.perm, r0 + irow, c0 + j);
 			double ar = e.r, ai = e.i;
 			for (int l = 0; l < k; l++) sub_uv(ar, ai, U[(long)l * m + irow], V[(long)l * nn + j]);
 			V[(long)k * nn + j] = make_double2(ar, ai);
@@ -239,9 +258,8 @@ __global__ __launch_bounds__(AT) void k_aca(aca_args a)
 		}
 		__syncthreads();
 		// residual column  u = A(:, jmax) - sum_l u_l v_l[jmax]
-		const int gj = a.perm[c0 + jmax];
 		for (int i = tid; i < m; i += AT) {
-			const cplx e = hm_entry<KM>(a.E, a.perm[r0 + i], gj);
+			const cplx e = hm_pos<KM>(a.E, a.perm, r0 + i, c0 + jmax);
 			double ar = e.r, ai = e.i;
 			for (int l = 0; l < k; l++) sub_uv(ar, ai, U[(long)l * m + i], V[(long)l * nn + jmax]);
 			U[(long)k * m + i] = make_double2(ar, ai);
@@ -338,9 +356,22 @@ __global__ __launch_bounds__(AT) void k_hm_dense(hent E, const int *perm, const 
 	const long tot = (long)B.m * B.nn;
 	for (long e = threadIdx.x; e < tot; e += AT) {
 		const int i = (int)(e / B.nn), j = (int)(e - (long)i * B.nn);
-		const cplx v = hm_entry<KM>(E, perm[B.r0 + i], perm[B.c0 + j]);
+		const cplx v = hm_pos<KM>(E, perm, B.r0 + i, B.c0 + j);
 		D[B.off + e] = make_double2(v.r, v.i);
 	}
+}
+
+// every entry of the matrix: F[i * n + j] = A(i, j); one thread per entry, workgroup q covers
+// row q / nchunk, columns 256 (q mod nchunk) onwards
+template <int KM>
+__global__ __launch_bounds__(AT) void k_hm_full(hent E, int n, int nchunk, double2 *F)
+{
+	const long q = blockIdx.x;
+	const int i = (int)(q / nchunk);
+	const int j = (int)(q - (long)i * nchunk) * AT + threadIdx.x;
+	if (j >= n) return;
+	const cplx v = hm_entry<KM>(E, i, j);
+	F[(long)i * n + j] = make_double2(v.r, v.i);
 }
 
 // compact the ACA factors of block b from the scratch into the U / V pools
@@ -539,6 +570,36 @@ void aca_dispatch(int km, hipStream_t s, const pe::aca_args &a, int nfar)
 	HIPCHK(hipGetLastError());
 }
 
+template <int KM>
+void launch_full(hipStream_t s, const pe::hent &E, int n, double2 *F)
+{
+	const int nchunk = (n + pe::AT - 1) / pe::AT;
+	hipLaunchKernelGGL(pe::k_hm_full<KM>, dim3((unsigned)((long)n * nchunk)), dim3(pe::AT), 0, s, E, n, nchunk, F);
+}
+
+void full_dispatch(int km, hipStream_t s, const pe::hent &E, int n, double2 *F)
+{
+	switch (km_class_hm(km)) {
+	case pe::KM_WIRE: launch_full<pe::KM_WIRE>(s, E, n, F); break;
+	case pe::KM_RIBBON: launch_full<pe::KM_RIBBON>(s, E, n, F); break;
+	case pe::KM_RIBBON | pe::KM_BAR: launch_full<pe::KM_RIBBON | pe::KM_BAR>(s, E, n, F); break;
+	case pe::KM_WIRE | pe::KM_RIBBON | pe::KM_BAR: launch_full<pe::KM_WIRE | pe::KM_RIBBON | pe::KM_BAR>(s, E, n, F); break;
+	default: launch_full<15>(s, E, n, F); break;
+	}
+	HIPCHK(hipGetLastError());
+}
+
+// bulk evaluation of all entries before the build (PEEC_HM_FULL=0 evaluates them inside ACA
+// and the dense blocks instead): only when n^2 entries take at most a quarter of free memory
+bool hm_full_wanted(int n)
+{
+	const char *e = getenv("PEEC_HM_FULL");
+	if ((e != nullptr) && (atoi(e) == 0)) return false;
+	size_t fr = 0, tot = 0;
+	if (hipMemGetInfo(&fr, &tot) != hipSuccess) return false;
+	return (double)n * n * sizeof(double2) <= 0.25 * (double)fr;
+}
+
 void dense_dispatch(int km, hipStream_t s, const pe::hent &E, const int *perm, const pe::hdense *blk, int nb, double2 *D)
 {
 	switch (km_class_hm(km)) {
@@ -581,8 +642,19 @@ struct hmat_t *hm_build_dev(pgpu_t *g, int kind, int n, const double *pts, doubl
 	E.csoff = csoff;
 	E.csidx = csidx;
 	E.rec = pgpu_kf_records(g, kind, kw, &E.nslots, &E.T);
+	E.full = nullptr;
+	E.nfull = n;
 	h->km = km;
 	h->d_perm = up(s, std::vector<int>(t.perm, t.perm + n));
+	double2 *d_full = nullptr;
+	if (hm_full_wanted(n) && (hipMalloc((void **)&d_full, (size_t)n * n * sizeof(double2)) == hipSuccess)) {
+		if ((E.rec == nullptr) || (pgpu_kf_fill_kind(g, kind, kw, (d_complex_t *)d_full) != 0)) full_dispatch(km, s, E, n, d_full);
+		E.full = d_full;
+	}
+	else {
+		(void)hipGetLastError();
+		d_full = nullptr;
+	}
 	// far blocks and their ACA scratch
 	std::vector<pe::hfar> far(t.nfar);
 	std::vector<long> soff(t.nfar + 1, 0), moff(t.nfar + 1, 0);
@@ -681,6 +753,7 @@ struct hmat_t *hm_build_dev(pgpu_t *g, int kind, int n, const double *pts, doubl
 	h->d_near = up(s, near);
 	HIPCHK(hipStreamSynchronize(s));
 	HIPCHK(hipFree(d_dense));
+	if (d_full != nullptr) HIPCHK(hipFree(d_full));
 	h->rank = rank;
 	h->far = far;
 	h->near = near;

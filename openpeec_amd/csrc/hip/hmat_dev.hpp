@@ -16,6 +16,10 @@ struct hent {
 	const double *rec;        // k-factored records of the cell set covering f.kw, or null
 	long nslots;
 	int T;
+	// every entry at this wavenumber in original numbering, full[i * nfull + j] = A(i, j),
+	// evaluated in bulk before the build, or null
+	const double2 *full;
+	int nfull;
 };
 
 // far block: row cluster [r0, r0+m), column cluster [c0, c0+nn) of the permutation; rank > 0:

@@ -112,6 +112,9 @@ int  pgpu_kf_setup(pgpu_t *g, double klo, double khi);
 /* records of the frequency band [f0, f1] for the compressed builds (no-op when static or PEEC_KF=0) */
 void pgpu_kf_band(pgpu_t *g, const peec_t *p, double f0, double f1);
 void pgpu_kf_fill(pgpu_t *g, double kw, d_complex_t *dlp, d_complex_t *dP);
+/* Lp (which 0) or P (which 1) alone at kw into out (cell order); 1 when the band's records do
+ * not cover kw or P's sub-cell matrix would not fit in a quarter of free memory */
+int  pgpu_kf_fill_kind(pgpu_t *g, int which, double kw, d_complex_t *out);
 /* the same for nk frequencies at once (the records read once): dlp[k], dP[k] at kw[k] */
 void pgpu_kf_fill_multi(pgpu_t *g, int nk, const double *kw, d_complex_t *const *dlp, d_complex_t *const *dP);
 int  pgpu_kf_direct_pairs(pgpu_t *g);

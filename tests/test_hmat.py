@@ -103,6 +103,48 @@ def test_hmatrix_ranks_and_product(lib, name):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", sorted(DECKS))
+def test_bulk_entries_give_the_same_hmatrix(lib, name, monkeypatch):
+    """The entries evaluated in bulk before the build (default) and inside ACA and the dense
+    blocks (PEEC_HM_FULL=0) are the same numbers, so ranks must be identical and H x must agree
+    to rounding; a bulk matrix stored transposed, or indexed by original instead of permuted
+    positions, changes the pivots and the product at O(1)."""
+    m = Peec.from_deck_text(DECKS[name], lib)
+    m.setup()
+    f = m.freq(m.nfreq - 1)
+    rng = np.random.default_rng(7)
+    for kind, build in (("lp", lib.hmat_build), ("p", lib.hmat_build_pot)):
+        out = []
+        for mode in ("0", "1"):
+            monkeypatch.setenv("PEEC_HM_FULL", mode)
+            print(f"[hmat bulk] {name} {kind} PEEC_HM_FULL={mode}", flush=True)
+            h = build(C.byref(m.p), f, None)
+            if not h:
+                pytest.skip(f"{name}: no {kind} matrix")
+            cnt = (C.c_int * 4)()
+            lib.hmat_info(h, cnt, None, None)
+            ranks = np.zeros(max(1, cnt[0]), dtype=np.int32)
+            lib.hmat_info(h, cnt, None, ranks.ctypes.data_as(P(C.c_int)))
+            n = cnt[3]
+            x = np.ascontiguousarray(rng.standard_normal(n) + 1j * rng.standard_normal(n)) if mode == "0" else x
+            y = np.zeros_like(x)
+            lib.hmat_matvec(h, x.ctypes.data_as(P(abi.d_complex_t)), y.ctypes.data_as(P(abi.d_complex_t)))
+            out.append((ranks[:cnt[0]].copy(), y))
+            lib.hmat_free(h)
+        assert np.array_equal(out[0][0], out[1][0]), (name, kind)
+        assert _rel(out[1][1], out[0][1]) <= 1e-13, (name, kind, _rel(out[1][1], out[0][1]))
+    # over a sweep the bulk entries come from the band's k-factored fill instead
+    zs = []
+    for mode in ("0", "1"):
+        monkeypatch.setenv("PEEC_HM_FULL", mode)
+        print(f"[hmat bulk] {name} sweep PEEC_HM_FULL={mode}", flush=True)
+        z, _ = m.sweep(0, m.nfreq)
+        zs.append(z)
+    assert np.max(np.abs(zs[1] - zs[0]) / np.abs(zs[0])) <= 1e-12
+    m.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(DECKS))
 def test_preconditioner_matches_reference(lib, name):
     m = Peec.from_deck_text(DECKS[name], lib)
     m.setup()
