@@ -416,48 +416,81 @@ __global__ __launch_bounds__(AT) void k_hm_vt(const hfar *far, const double2 *Vp
 	}
 }
 
-// stage 2: one workgroup per leaf row cluster; a wave per row; blocks in registered order
+// stage 2: one workgroup per leaf row cluster (at most HM_LEAF = 32 rows).  The leaf's blocks
+// (far blocks in registered order, then near blocks) are dealt round-robin to the 4 waves; in a
+// wave, lane & 31 is the row and lane >> 5 takes every other rank (U is stored rank by rank, so
+// the 32 rows of one rank are one coalesced read) or every other column of a dense row.  The 8
+// partial sums of a row are added in a fixed order, so the product is deterministic.
 __global__ __launch_bounds__(AT) void k_hm_leaf(hleafs L, const hfar *far, const hdense *near, const double2 *Upool,
                                                 const double2 *Dpool, const double2 *ft, const double2 *xp, const int *perm,
                                                 double2 *y)
 {
+	__shared__ double2 part[2 * (AT / 64)][32];
 	const int r = blockIdx.x;
 	const int i0 = L.lo[r], i1 = L.hi[r];
 	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-	for (int i = i0 + wave; i < i1; i += AT / 64) {
+	const int row = lane & 31, sub = lane >> 5;
+	const int f0 = L.rfoff[r], nf = L.rfoff[r + 1] - f0;
+	const int n0 = L.rnoff[r], nb = nf + (L.rnoff[r + 1] - n0);
+	for (int c0 = i0; c0 < i1; c0 += 32) {
+		const int i = c0 + row;
+		const bool live = i < i1;
 		double sr = 0, si = 0;
-		for (int e = L.rfoff[r]; e < L.rfoff[r + 1]; e++) {
-			const hfar B = far[L.rfidx[e]];
-			if (B.rank > 0) {
-				for (int l = lane; l < B.rank; l += 64) {
-					const double2 u = Upool[B.uo + (long)l * B.m + (i - B.r0)], t = ft[B.fo + l];
-					sr += (u.x * t.x) - (u.y * t.y);
-					si += (u.x * t.y) + (u.y * t.x);
+		for (int e = wave; e < nb; e += AT / 64) {
+			int rk, br0, bnn, bc0;
+			long off;
+			if (e < nf) {
+				const int b = __builtin_amdgcn_readfirstlane(L.rfidx[f0 + e]);
+				const hfar B = far[b];
+				rk = B.rank;
+				br0 = B.r0;
+				bnn = B.nn;
+				bc0 = B.c0;
+				off = B.uo;
+				if (rk > 0) {
+					if (live) {
+						const double2 *u = Upool + off + (i - br0);
+						const double2 *t = ft + B.fo;
+						const long m = B.m;
+						for (int l = sub; l < rk; l += 2) {
+							const double2 a = u[l * m], x = t[l];
+							sr += (a.x * x.x) - (a.y * x.y);
+							si += (a.x * x.y) + (a.y * x.x);
+						}
+					}
+					continue;
 				}
 			}
 			else {
-				const double2 *row = Dpool + B.uo + (long)(i - B.r0) * B.nn;
-				for (int j = lane; j < B.nn; j += 64) {
-					const double2 a = row[j], x = xp[B.c0 + j];
-					sr += (a.x * x.x) - (a.y * x.y);
-					si += (a.x * x.y) + (a.y * x.x);
+				const int b = __builtin_amdgcn_readfirstlane(L.rnidx[n0 + (e - nf)]);
+				const hdense B = near[b];
+				br0 = B.r0;
+				bnn = B.nn;
+				bc0 = B.c0;
+				off = B.off;
+			}
+			if (live) {
+				const double2 *a = Dpool + off + (long)(i - br0) * bnn;
+				const double2 *x = xp + bc0;
+				for (int j = sub; j < bnn; j += 2) {
+					const double2 v = a[j], w = x[j];
+					sr += (v.x * w.x) - (v.y * w.y);
+					si += (v.x * w.y) + (v.y * w.x);
 				}
 			}
 		}
-		for (int e = L.rnoff[r]; e < L.rnoff[r + 1]; e++) {
-			const hdense B = near[L.rnidx[e]];
-			const double2 *row = Dpool + B.off + (long)(i - B.r0) * B.nn;
-			for (int j = lane; j < B.nn; j += 64) {
-				const double2 a = row[j], x = xp[B.c0 + j];
-				sr += (a.x * x.x) - (a.y * x.y);
-				si += (a.x * x.y) + (a.y * x.x);
+		part[2 * wave + sub][row] = make_double2(sr, si);
+		__syncthreads();
+		if ((threadIdx.x < 32) && live) {
+			double tr = 0, ti = 0;
+#pragma unroll
+			for (int q = 0; q < 2 * (AT / 64); q++) {
+				tr += part[q][threadIdx.x].x;
+				ti += part[q][threadIdx.x].y;
 			}
+			y[perm[i]] = make_double2(tr, ti);
 		}
-		for (int off = 32; off > 0; off >>= 1) {
-			sr += __shfl_xor(sr, off);
-			si += __shfl_xor(si, off);
-		}
-		if (lane == 0) y[perm[i]] = make_double2(sr, si);
+		__syncthreads();
 	}
 }
 
@@ -466,26 +499,44 @@ __global__ __launch_bounds__(AT) void k_hm_leaf(hleafs L, const hfar *far, const
 __global__ __launch_bounds__(AT) void k_hm_near_off(hleafs L, const hdense *near, const double2 *Dpool, const double2 *xp,
                                                     const int *perm, double2 *y)
 {
+	// the layout of k_hm_leaf: blocks round-robin over the waves, a lane per row, lane >> 5
+	// takes every other column, partial sums added in a fixed order
+	__shared__ double2 part[2 * (AT / 64)][32];
 	const int r = blockIdx.x;
 	const int i0 = L.lo[r], i1 = L.hi[r];
 	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-	for (int i = i0 + wave; i < i1; i += AT / 64) {
+	const int row = lane & 31, sub = lane >> 5;
+	const int n0 = L.rnoff[r], nb = L.rnoff[r + 1] - n0;
+	for (int c0 = i0; c0 < i1; c0 += 32) {
+		const int i = c0 + row;
+		const bool live = i < i1;
 		double sr = 0, si = 0;
-		for (int e = L.rnoff[r]; e < L.rnoff[r + 1]; e++) {
-			const hdense B = near[L.rnidx[e]];
+		for (int e = wave; e < nb; e += AT / 64) {
+			const int b = __builtin_amdgcn_readfirstlane(L.rnidx[n0 + e]);
+			const hdense B = near[b];
 			if ((B.r0 == B.c0) && (B.m == B.nn)) continue;
-			const double2 *row = Dpool + B.off + (long)(i - B.r0) * B.nn;
-			for (int j = lane; j < B.nn; j += 64) {
-				const double2 a = row[j], x = xp[B.c0 + j];
-				sr += (a.x * x.x) - (a.y * x.y);
-				si += (a.x * x.y) + (a.y * x.x);
+			if (live) {
+				const double2 *a = Dpool + B.off + (long)(i - B.r0) * B.nn;
+				const double2 *x = xp + B.c0;
+				for (int j = sub; j < B.nn; j += 2) {
+					const double2 v = a[j], w = x[j];
+					sr += (v.x * w.x) - (v.y * w.y);
+					si += (v.x * w.y) + (v.y * w.x);
+				}
 			}
 		}
-		for (int off = 32; off > 0; off >>= 1) {
-			sr += __shfl_xor(sr, off);
-			si += __shfl_xor(si, off);
+		part[2 * wave + sub][row] = make_double2(sr, si);
+		__syncthreads();
+		if ((threadIdx.x < 32) && live) {
+			double tr = 0, ti = 0;
+#pragma unroll
+			for (int q = 0; q < 2 * (AT / 64); q++) {
+				tr += part[q][threadIdx.x].x;
+				ti += part[q][threadIdx.x].y;
+			}
+			y[perm[i]] = make_double2(tr, ti);
 		}
-		if (lane == 0) y[perm[i]] = make_double2(sr, si);
+		__syncthreads();
 	}
 }
 
