@@ -416,43 +416,48 @@ __global__ __launch_bounds__(AT) void k_hm_vt(const hfar *far, const double2 *Vp
 	}
 }
 
-// stage 2: one workgroup per leaf row cluster (at most HM_LEAF = 32 rows).  The leaf's blocks
-// (far blocks in registered order, then near blocks) are dealt round-robin to the 4 waves; in a
-// wave, lane & 31 is the row and lane >> 5 takes every other rank (U is stored rank by rank, so
-// the 32 rows of one rank are one coalesced read) or every other column of a dense row.  The 8
-// partial sums of a row are added in a fixed order, so the product is deterministic.
-__global__ __launch_bounds__(AT) void k_hm_leaf(hleafs L, const hfar *far, const hdense *near, const double2 *Upool,
-                                                const double2 *Dpool, const double2 *ft, const double2 *xp, const int *perm,
-                                                double2 *y)
+// stage 2: the rows of each leaf row cluster (at most HM_LEAF = 32 rows) times the blocks
+// listed for it (far blocks in registered order, then near blocks; NEAR_OFF: the near blocks
+// other than the leaf's self-block).  HM_SPLIT workgroups share a leaf: workgroup s of leaf r
+// takes blocks e = 4 s + wave, 4 s + wave + 4 HM_SPLIT, ...; in a wave lane & 31 is the row and
+// lane >> 5 takes every other rank (U is stored rank by rank, so the 32 rows of one rank are one
+// coalesced read) or every other column of a dense row.  The workgroup's 8 partial sums of a
+// row are added in a fixed order into part[s][row]; k_hm_rowsum adds the HM_SPLIT of them in
+// order, so the product is deterministic.
+constexpr int HM_SPLIT = 8;
+
+template <bool NEAR_OFF>
+__global__ __launch_bounds__(AT) void k_hm_rows(hleafs L, const hfar *far, const hdense *near, const double2 *Upool,
+                                                const double2 *Dpool, const double2 *ft, const double2 *xp, int n,
+                                                double2 *part)
 {
-	__shared__ double2 part[2 * (AT / 64)][32];
-	const int r = blockIdx.x;
+	__shared__ double2 lp[2 * (AT / 64)][32];
+	const int r = blockIdx.x / HM_SPLIT, sp = blockIdx.x - r * HM_SPLIT;
 	const int i0 = L.lo[r], i1 = L.hi[r];
 	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 	const int row = lane & 31, sub = lane >> 5;
-	const int f0 = L.rfoff[r], nf = L.rfoff[r + 1] - f0;
+	const int f0 = L.rfoff[r], nf = NEAR_OFF ? 0 : (L.rfoff[r + 1] - f0);
 	const int n0 = L.rnoff[r], nb = nf + (L.rnoff[r + 1] - n0);
 	for (int c0 = i0; c0 < i1; c0 += 32) {
 		const int i = c0 + row;
 		const bool live = i < i1;
 		double sr = 0, si = 0;
-		for (int e = wave; e < nb; e += AT / 64) {
-			int rk, br0, bnn, bc0;
+		for (int e = (AT / 64) * sp + wave; e < nb; e += (AT / 64) * HM_SPLIT) {
+			int br0, bnn, bc0;
 			long off;
 			if (e < nf) {
 				const int b = __builtin_amdgcn_readfirstlane(L.rfidx[f0 + e]);
 				const hfar B = far[b];
-				rk = B.rank;
 				br0 = B.r0;
 				bnn = B.nn;
 				bc0 = B.c0;
 				off = B.uo;
-				if (rk > 0) {
+				if (B.rank > 0) {
 					if (live) {
 						const double2 *u = Upool + off + (i - br0);
 						const double2 *t = ft + B.fo;
 						const long m = B.m;
-						for (int l = sub; l < rk; l += 2) {
+						for (int l = sub; l < B.rank; l += 2) {
 							const double2 a = u[l * m], x = t[l];
 							sr += (a.x * x.x) - (a.y * x.y);
 							si += (a.x * x.y) + (a.y * x.x);
@@ -464,6 +469,7 @@ __global__ __launch_bounds__(AT) void k_hm_leaf(hleafs L, const hfar *far, const
 			else {
 				const int b = __builtin_amdgcn_readfirstlane(L.rnidx[n0 + (e - nf)]);
 				const hdense B = near[b];
+				if (NEAR_OFF && (B.r0 == B.c0) && (B.m == B.nn)) continue;
 				br0 = B.r0;
 				bnn = B.nn;
 				bc0 = B.c0;
@@ -479,65 +485,34 @@ __global__ __launch_bounds__(AT) void k_hm_leaf(hleafs L, const hfar *far, const
 				}
 			}
 		}
-		part[2 * wave + sub][row] = make_double2(sr, si);
+		lp[2 * wave + sub][row] = make_double2(sr, si);
 		__syncthreads();
 		if ((threadIdx.x < 32) && live) {
 			double tr = 0, ti = 0;
 #pragma unroll
 			for (int q = 0; q < 2 * (AT / 64); q++) {
-				tr += part[q][threadIdx.x].x;
-				ti += part[q][threadIdx.x].y;
+				tr += lp[q][threadIdx.x].x;
+				ti += lp[q][threadIdx.x].y;
 			}
-			y[perm[i]] = make_double2(tr, ti);
+			part[(long)sp * n + i] = make_double2(tr, ti);
 		}
 		__syncthreads();
 	}
 }
 
-// y = N x with N the near-field blocks off the block diagonal (every near block except the leaf
-// self-blocks r0 == c0, m == nn): one workgroup per leaf row range, output in cell order
-__global__ __launch_bounds__(AT) void k_hm_near_off(hleafs L, const hdense *near, const double2 *Dpool, const double2 *xp,
-                                                    const int *perm, double2 *y)
+// y[perm[i]] = sum over s of part[s][i], s in increasing order
+__global__ void k_hm_rowsum(const double2 *part, int n, const int *perm, double2 *y)
 {
-	// the layout of k_hm_leaf: blocks round-robin over the waves, a lane per row, lane >> 5
-	// takes every other column, partial sums added in a fixed order
-	__shared__ double2 part[2 * (AT / 64)][32];
-	const int r = blockIdx.x;
-	const int i0 = L.lo[r], i1 = L.hi[r];
-	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-	const int row = lane & 31, sub = lane >> 5;
-	const int n0 = L.rnoff[r], nb = L.rnoff[r + 1] - n0;
-	for (int c0 = i0; c0 < i1; c0 += 32) {
-		const int i = c0 + row;
-		const bool live = i < i1;
-		double sr = 0, si = 0;
-		for (int e = wave; e < nb; e += AT / 64) {
-			const int b = __builtin_amdgcn_readfirstlane(L.rnidx[n0 + e]);
-			const hdense B = near[b];
-			if ((B.r0 == B.c0) && (B.m == B.nn)) continue;
-			if (live) {
-				const double2 *a = Dpool + B.off + (long)(i - B.r0) * B.nn;
-				const double2 *x = xp + B.c0;
-				for (int j = sub; j < B.nn; j += 2) {
-					const double2 v = a[j], w = x[j];
-					sr += (v.x * w.x) - (v.y * w.y);
-					si += (v.x * w.y) + (v.y * w.x);
-				}
-			}
-		}
-		part[2 * wave + sub][row] = make_double2(sr, si);
-		__syncthreads();
-		if ((threadIdx.x < 32) && live) {
-			double tr = 0, ti = 0;
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	double tr = 0, ti = 0;
 #pragma unroll
-			for (int q = 0; q < 2 * (AT / 64); q++) {
-				tr += part[q][threadIdx.x].x;
-				ti += part[q][threadIdx.x].y;
-			}
-			y[perm[i]] = make_double2(tr, ti);
-		}
-		__syncthreads();
+	for (int sp = 0; sp < HM_SPLIT; sp++) {
+		const double2 v = part[(long)sp * n + i];
+		tr += v.x;
+		ti += v.y;
 	}
+	y[perm[i]] = make_double2(tr, ti);
 }
 
 // sum of |entry|^2 of each near block (out[2 q] diagonal self-blocks, out[2 q + 1] the others)
@@ -822,6 +797,7 @@ struct hmat_t *hm_build_dev(pgpu_t *g, int kind, int n, const double *pts, doubl
 	h->L.rnidx = up(s, std::vector<int>(t.rnidx, t.rnidx + std::max(1, t.rnoff[t.nleaf])));
 	h->d_ft = dal<double2>(std::max(1L, h->ftlen));
 	h->d_xp = dal<double2>(n);
+	h->d_part = dal<double2>((size_t)pe::HM_SPLIT * n);
 	h->d_x = dal<double2>(n);
 	h->d_y = dal<double2>(n);
 	HIPCHK(hipStreamSynchronize(s));
@@ -843,9 +819,11 @@ void hm_matvec_dev(const hmat_t *h, const double2 *x, double2 *y)
 	if (h->t.nfar > 0)
 		hipLaunchKernelGGL(pe::k_hm_vt, dim3(h->t.nfar), dim3(pe::AT), 0, s, (const pe::hfar *)h->d_far, (const double2 *)h->d_V,
 		                   (const double2 *)h->d_xp, h->d_ft);
-	hipLaunchKernelGGL(pe::k_hm_leaf, dim3(h->t.nleaf), dim3(pe::AT), 0, s, h->L, (const pe::hfar *)h->d_far,
+	hipLaunchKernelGGL(pe::k_hm_rows<false>, dim3(h->t.nleaf * pe::HM_SPLIT), dim3(pe::AT), 0, s, h->L, (const pe::hfar *)h->d_far,
 	                   (const pe::hdense *)h->d_near, (const double2 *)h->d_U, (const double2 *)h->d_D, (const double2 *)h->d_ft,
-	                   (const double2 *)h->d_xp, (const int *)h->d_perm, y);
+	                   (const double2 *)h->d_xp, n, h->d_part);
+	hipLaunchKernelGGL(pe::k_hm_rowsum, dim3((n + 255) / 256), dim3(256), 0, s, (const double2 *)h->d_part, n,
+	                   (const int *)h->d_perm, y);
 }
 
 // y = N x: near-field blocks off the block diagonal (x, y in cell order, device)
@@ -854,8 +832,11 @@ void hm_near_off_dev(const hmat_t *h, const double2 *x, double2 *y)
 	hipStream_t s = h->s;
 	const int n = h->n;
 	hipLaunchKernelGGL(pe::k_hm_gather, dim3((n + 255) / 256), dim3(256), 0, s, (const int *)h->d_perm, x, h->d_xp, n);
-	hipLaunchKernelGGL(pe::k_hm_near_off, dim3(h->t.nleaf), dim3(pe::AT), 0, s, h->L, (const pe::hdense *)h->d_near,
-	                   (const double2 *)h->d_D, (const double2 *)h->d_xp, (const int *)h->d_perm, y);
+	hipLaunchKernelGGL(pe::k_hm_rows<true>, dim3(h->t.nleaf * pe::HM_SPLIT), dim3(pe::AT), 0, s, h->L, (const pe::hfar *)h->d_far,
+	                   (const pe::hdense *)h->d_near, (const double2 *)h->d_U, (const double2 *)h->d_D, (const double2 *)h->d_ft,
+	                   (const double2 *)h->d_xp, n, h->d_part);
+	hipLaunchKernelGGL(pe::k_hm_rowsum, dim3((n + 255) / 256), dim3(256), 0, s, (const double2 *)h->d_part, n,
+	                   (const int *)h->d_perm, y);
 }
 
 // Frobenius norms of the near field: off the block diagonal and of the leaf self-blocks
@@ -924,7 +905,7 @@ void hmat_free(struct hmat_t *h)
 	if (h == nullptr) return;
 	HIPCHK(hipStreamSynchronize(h->s));
 	const void *bufs[] = {h->d_perm, h->d_far, h->d_near, h->d_U, h->d_V, h->d_D, h->L.lo, h->L.hi, h->L.rfoff, h->L.rfidx,
-	                      h->L.rnoff, h->L.rnidx, h->d_ft, h->d_xp, h->d_x, h->d_y};
+	                      h->L.rnoff, h->L.rnidx, h->d_ft, h->d_xp, h->d_x, h->d_y, h->d_part};
 	for (const void *b : bufs)
 		if (b) HIPCHK(hipFree(const_cast<void *>(b)));
 	hm_tree_free(&h->t);

@@ -68,6 +68,7 @@ struct hmat_t {
 	long nU = 0, nV = 0, nD = 0, ftlen = 0;
 	pe::hleafs L{};
 	double2 *d_ft = nullptr, *d_xp = nullptr, *d_x = nullptr, *d_y = nullptr;
+	double2 *d_part = nullptr;   // per-row partial sums of the leaf products (HM_SPLIT x n)
 	std::vector<int> rank;
 	std::vector<pe::hfar> far;
 	std::vector<pe::hdense> near;
