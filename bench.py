@@ -377,6 +377,9 @@ def bench_learned(lib, points: int = 101) -> dict:
            "training": {"samples": S, "geometries": len(LPC_TRAIN_GEOMS), "sample_s": t_samples, "fit_s": t_fit,
                         "loss_first": float(hist[0]), "loss_last": float(hist[-1])}}
     runs = {}
+    # untimed: the band's k-factored records and the device buffers are set up once per
+    # geometry, by whichever sweep comes first
+    m.sweep(0, points, PEEC_FORM_PLAIN)
     lib.peec_lpc_set_model(W.ctypes.data_as(PD))
     t0 = time.perf_counter()
     z_l, st = m.sweep(0, points, PEEC_FORM_LEARNED)
@@ -403,6 +406,31 @@ def bench_learned(lib, points: int = 101) -> dict:
                                       "time": runs["reuse_without_network"][0] / runs["learned"][0]}
     out["speedup_vs_reference_rule"] = runs["reference_rule"][0] / runs["learned"][0]
     m.close()
+    # the same geometry uncompressed: the accelerated form (all points, after an untimed run)
+    # and the plain dense FP64 form (first 5 points), against each other and against the
+    # compressed result (which carries the ACA truncation)
+    import re
+    from openpeec_amd import PEEC_FORM_MIXED
+    dm = Peec.from_deck_text(re.sub(r"compression = 1 \S+", "compression = 0", text), lib)
+    dm.setup()
+    kp = min(points, 5)
+    dm.sweep(0, points, PEEC_FORM_MIXED)
+    t0 = time.perf_counter()
+    z_m, _ = dm.sweep(0, points, PEEC_FORM_MIXED)
+    t_m = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    z_p, _ = dm.sweep(0, kp, PEEC_FORM_PLAIN)
+    t_p = time.perf_counter() - t0
+    rel = lambda a, b: float(np.max(np.abs(a - b) / np.abs(b)))
+    out["uncompressed_same_geometry"] = {
+        "unknowns": dm.p.nunknown, "accelerated_points_per_s": points / t_m, "plain_fp64_points_per_s": kp / t_p,
+        "plain_points": kp, "accelerated_vs_plain_fp64": (points / t_m) / (kp / t_p),
+        "accelerated_vs_compressed_reference_rule": (points / t_m) / out["reference_rule"]["points_per_s"],
+        "zin_max_rel_err": {"accelerated_vs_plain_fp64": rel(z_m[:kp], z_p),
+                            "compressed_vs_plain_fp64": rel(z_r[:kp], z_p),
+                            "compressed_vs_accelerated_all_points": rel(z_r, z_m)}}
+    log(f"learned: same geometry uncompressed, accelerated {points / t_m:.1f} points/s, plain FP64 {kp / t_p:.2f} points/s")
+    dm.close()
     return out
 
 

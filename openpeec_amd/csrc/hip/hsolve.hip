@@ -222,16 +222,27 @@ __global__ void k_schur_rows(int n1, const int *roff, const int *rleaf, const in
 	}
 }
 
-// z[cells of leaf] = D^-1 u[cells of leaf], one thread per cell row (block-local numbering)
+// z[cells of leaf] = D^-1 u[cells of leaf], one thread per cell row (block-local numbering); the
+// leaf's entries of u are gathered into LDS first (leaves of up to LEAF_LDS cells), so a row is
+// a chain of independent loads of D^-1; the sum runs over b in increasing order either way
+constexpr int LEAF_LDS = 128;
 __global__ void k_leaf_apply(int nleaf, const leafd *L, const int *lidx, const double2 *Dinv, const double2 *u, double2 *z)
 {
+	__shared__ double2 us[LEAF_LDS];
 	const int r = blockIdx.x;
 	if (r >= nleaf) return;
 	const leafd l = L[r];
+	const bool staged = l.m <= LEAF_LDS;
+	if (staged) {
+		for (int b = threadIdx.x; b < l.m; b += blockDim.x) us[b] = u[lidx[l.idx + b]];
+		__syncthreads();
+	}
 	for (int a = threadIdx.x; a < l.m; a += blockDim.x) {
+		const double2 *d = Dinv + l.doff + (long)a * l.m;
 		double2 acc = make_double2(0, 0);
+#pragma unroll 8
 		for (int b = 0; b < l.m; b++) {
-			const double2 p = cm(Dinv[l.doff + (long)a * l.m + b], u[lidx[l.idx + b]]);
+			const double2 p = cm(d[b], staged ? us[b] : u[lidx[l.idx + b]]);
 			acc.x += p.x;
 			acc.y += p.y;
 		}
