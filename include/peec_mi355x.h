@@ -147,7 +147,7 @@ int  peec_batch_trials(const peec_batch_t *b, int *nb, int *tile_cols, int *wg_p
 void peec_batch_free(peec_batch_t *b);
 /* The batched factorisation alone (tests): nmat row-major matrices of orders n[j],
  * concatenated in a, factored in place as lu_decomp (src/lu.c:16-66) does, one workgroup each
- * (panel width nb = 16 or 32); piv receives each matrix's interchanges, concatenated.  Returns
+ * (panel width nb = 16, 32 or 64); piv receives each matrix's interchanges, concatenated.  Returns
  * -1, the first matrix with a pivot below 1e-30, or -2 for bad arguments. */
 int  peec_batch_lu_decomp(int nmat, const int *n, d_complex_t *a, int *piv, int nb);
 

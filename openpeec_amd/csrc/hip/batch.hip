@@ -932,6 +932,8 @@ void launch_lu(hipStream_t s, batch &B, const tune_t &t, int nj, size_t lds, boo
 		float2 *a = B.arena32.p;
 		if ((t.nb == 16) && (t.cb == 2)) launch_lu_t<16, 2>(s, B, grid, nj, lds, a, order);
 		else if ((t.nb == 16) && (t.cb == 4)) launch_lu_t<16, 4>(s, B, grid, nj, lds, a, order);
+		else if ((t.nb == 64) && (t.cb == 2)) launch_lu_t<64, 2>(s, B, grid, nj, lds, a, order);
+		else if ((t.nb == 64) && (t.cb == 4)) launch_lu_t<64, 4>(s, B, grid, nj, lds, a, order);
 		else if (t.cb == 4) launch_lu_t<32, 4>(s, B, grid, nj, lds, a, order);
 		else launch_lu_t<32, 2>(s, B, grid, nj, lds, a, order);
 	}
@@ -939,6 +941,8 @@ void launch_lu(hipStream_t s, batch &B, const tune_t &t, int nj, size_t lds, boo
 		double2 *a = B.arena.p;
 		if ((t.nb == 16) && (t.cb == 2)) launch_lu_t<16, 2>(s, B, grid, nj, lds, a, order);
 		else if ((t.nb == 16) && (t.cb == 4)) launch_lu_t<16, 4>(s, B, grid, nj, lds, a, order);
+		else if ((t.nb == 64) && (t.cb == 2)) launch_lu_t<64, 2>(s, B, grid, nj, lds, a, order);
+		else if ((t.nb == 64) && (t.cb == 4)) launch_lu_t<64, 4>(s, B, grid, nj, lds, a, order);
 		else if (t.cb == 4) launch_lu_t<32, 4>(s, B, grid, nj, lds, a, order);
 		else launch_lu_t<32, 2>(s, B, grid, nj, lds, a, order);
 	}
@@ -1163,8 +1167,10 @@ int prepare(batch &B, peec_t **models, int nm, int ifreq0, int nfreq)
 		const int lds = (int)lu_lds(BMAXN);
 		const void *fns[] = {(const void *)k_b_lu<16, 2, double2>, (const void *)k_b_lu<16, 4, double2>,
 		                     (const void *)k_b_lu<32, 2, double2>, (const void *)k_b_lu<32, 4, double2>,
+		                     (const void *)k_b_lu<64, 2, double2>, (const void *)k_b_lu<64, 4, double2>,
 		                     (const void *)k_b_lu<16, 2, float2>,  (const void *)k_b_lu<16, 4, float2>,
-		                     (const void *)k_b_lu<32, 2, float2>,  (const void *)k_b_lu<32, 4, float2>};
+		                     (const void *)k_b_lu<32, 2, float2>,  (const void *)k_b_lu<32, 4, float2>,
+		                     (const void *)k_b_lu<64, 2, float2>,  (const void *)k_b_lu<64, 4, float2>};
 		for (const void *fn : fns) HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
 	}
 	B.nm = nm;
@@ -1281,7 +1287,7 @@ void autotune(batch &B, const std::vector<std::pair<int, int>> &first)
 	std::vector<std::pair<int, int>> sample;
 	const size_t stride = std::max<size_t>(1, first.size() / 512);
 	for (size_t k = 0; k < first.size(); k += stride) sample.push_back(first[k]);
-	const int nbs[2] = {16, 32};
+	const int nbs[3] = {16, 32, 64};
 	const int cbs[2] = {2, 4};
 	const int wpcs[2] = {1, 2};
 	B.trials.clear();
@@ -1448,7 +1454,7 @@ int peec_batch_sweep(peec_batch_t *h, d_complex_t **zmat, peec_batch_stats_t *st
 
 int peec_batch_lu_decomp(int nmat, const int *n, d_complex_t *a, int *piv, int nb)
 {
-	if ((nmat <= 0) || ((nb != 16) && (nb != 32))) return -2;
+	if ((nmat <= 0) || ((nb != 16) && (nb != 32) && (nb != 64))) return -2;
 	std::vector<long> aoff(nmat), poff(nmat);
 	long na = 0, np = 0;
 	for (int j = 0; j < nmat; j++) {
@@ -1472,6 +1478,7 @@ int peec_batch_lu_decomp(int nmat, const int *n, d_complex_t *a, int *piv, int n
 	HIPCHK(hipMemcpy(dpoff, poff.data(), (size_t)nmat * sizeof(long), hipMemcpyHostToDevice));
 	HIPCHK(hipMemcpy(dn, n, (size_t)nmat * sizeof(int), hipMemcpyHostToDevice));
 	if (nb == 16) hipLaunchKernelGGL((k_b_lu_only<16>), dim3(nmat), dim3(256), 0, 0, dA, daoff, dn, dpiv, dpoff, dinfo);
+	else if (nb == 64) hipLaunchKernelGGL((k_b_lu_only<64>), dim3(nmat), dim3(256), 0, 0, dA, daoff, dn, dpiv, dpoff, dinfo);
 	else hipLaunchKernelGGL((k_b_lu_only<32>), dim3(nmat), dim3(256), 0, 0, dA, daoff, dn, dpiv, dpoff, dinfo);
 	HIPCHK(hipGetLastError());
 	std::vector<int> info(nmat);

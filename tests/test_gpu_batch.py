@@ -27,7 +27,7 @@ def _rel(a, b):
     return float(np.max(np.abs(a - b) / np.abs(b)))
 
 
-@pytest.mark.parametrize("nb", [16, 32])
+@pytest.mark.parametrize("nb", [16, 32, 64])
 def test_batched_lu_factors(lib, nb):
     """P A = L U for matrices of uneven order (1 .. 300, edges around the panel width) factored
     together, one workgroup each, and the same interchanges as lu_decomp (random matrices have no
@@ -160,7 +160,7 @@ def test_batch_replay_and_autotune(lib):
     b = Batch(ms, 0, 8)
     st0 = b.tune()
     trials = b.trials()
-    assert len(trials) == st0.tune_trials == 16
+    assert len(trials) == st0.tune_trials == 24
     best = min(trials, key=lambda t: t[4])
     assert (st0.nb, st0.tile_cols, st0.wg_per_cu, st0.order) == best[:4]
     z1, st1 = b.sweep()
