@@ -394,26 +394,29 @@ __global__ void k_hm_gather(const int *perm, const double2 *x, double2 *xp, int 
 	if (e < n) xp[e] = x[perm[e]];
 }
 
-// stage 1: ft[ftoff_b + l] = v_l^T xp over the block's columns, one wave per rank
-__global__ __launch_bounds__(AT) void k_hm_vt(const hfar *far, const double2 *Vpool, const double2 *xp, double2 *ft)
+// stage 1: ft[fo_b + l] = v_l^T xp over the block's columns, one wave per (far block, rank)
+// pair: item q = fo_b + l of block b = item_blk[q], lanes over columns, a fixed xor reduction
+// (the pairs of one large high-rank block run in parallel instead of queueing in one workgroup)
+__global__ __launch_bounds__(AT) void k_hm_vt_items(const hfar *far, const int *item_blk, long nitem, const double2 *Vpool,
+                                                    const double2 *xp, double2 *ft)
 {
-	const hfar B = far[blockIdx.x];
-	if (B.rank <= 0) return;
-	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-	for (int l = wave; l < B.rank; l += AT / 64) {
-		const double2 *v = Vpool + B.vo + (long)l * B.nn;
-		double sr = 0, si = 0;
-		for (int j = lane; j < B.nn; j += 64) {
-			const double2 a = v[j], x = xp[B.c0 + j];
-			sr += (a.x * x.x) - (a.y * x.y);
-			si += (a.x * x.y) + (a.y * x.x);
-		}
-		for (int off = 32; off > 0; off >>= 1) {
-			sr += __shfl_xor(sr, off);
-			si += __shfl_xor(si, off);
-		}
-		if (lane == 0) ft[B.fo + l] = make_double2(sr, si);
+	const long q = (long)blockIdx.x * (AT / 64) + (threadIdx.x >> 6);
+	const int lane = threadIdx.x & 63;
+	if (q >= nitem) return;
+	const hfar B = far[item_blk[q]];
+	const int l = (int)(q - B.fo);
+	const double2 *v = Vpool + B.vo + (long)l * B.nn;
+	double sr = 0, si = 0;
+	for (int j = lane; j < B.nn; j += 64) {
+		const double2 a = v[j], x = xp[B.c0 + j];
+		sr += (a.x * x.x) - (a.y * x.y);
+		si += (a.x * x.y) + (a.y * x.x);
 	}
+	for (int off = 32; off > 0; off >>= 1) {
+		sr += __shfl_xor(sr, off);
+		si += __shfl_xor(si, off);
+	}
+	if (lane == 0) ft[q] = make_double2(sr, si);
 }
 
 // stage 2: the rows of each leaf row cluster (at most HM_LEAF = 32 rows) times the blocks
@@ -796,6 +799,11 @@ struct hmat_t *hm_build_dev(pgpu_t *g, int kind, int n, const double *pts, doubl
 	h->L.rnoff = up(s, std::vector<int>(t.rnoff, t.rnoff + t.nleaf + 1));
 	h->L.rnidx = up(s, std::vector<int>(t.rnidx, t.rnidx + std::max(1, t.rnoff[t.nleaf])));
 	h->d_ft = dal<double2>(std::max(1L, h->ftlen));
+	// far block of each V^T x item (kept alive until the stream is synchronised below)
+	std::vector<int> item((size_t)std::max(1L, h->ftlen), 0);
+	for (int b = 0; b < t.nfar; b++)
+		for (int l = 0; l < far[b].rank; l++) item[(size_t)far[b].fo + l] = b;
+	h->d_item = up(s, item);
 	h->d_xp = dal<double2>(n);
 	h->d_part = dal<double2>((size_t)pe::HM_SPLIT * n);
 	h->d_x = dal<double2>(n);
@@ -816,8 +824,9 @@ void hm_matvec_dev(const hmat_t *h, const double2 *x, double2 *y)
 	hipStream_t s = h->s;
 	const int n = h->n;
 	hipLaunchKernelGGL(pe::k_hm_gather, dim3((n + 255) / 256), dim3(256), 0, s, (const int *)h->d_perm, x, h->d_xp, n);
-	if (h->t.nfar > 0)
-		hipLaunchKernelGGL(pe::k_hm_vt, dim3(h->t.nfar), dim3(pe::AT), 0, s, (const pe::hfar *)h->d_far, (const double2 *)h->d_V,
+	if (h->ftlen > 0)
+		hipLaunchKernelGGL(pe::k_hm_vt_items, dim3((unsigned)((h->ftlen + pe::AT / 64 - 1) / (pe::AT / 64))), dim3(pe::AT), 0, s,
+		                   (const pe::hfar *)h->d_far, (const int *)h->d_item, h->ftlen, (const double2 *)h->d_V,
 		                   (const double2 *)h->d_xp, h->d_ft);
 	hipLaunchKernelGGL(pe::k_hm_rows<false>, dim3(h->t.nleaf * pe::HM_SPLIT), dim3(pe::AT), 0, s, h->L, (const pe::hfar *)h->d_far,
 	                   (const pe::hdense *)h->d_near, (const double2 *)h->d_U, (const double2 *)h->d_D, (const double2 *)h->d_ft,
@@ -905,7 +914,7 @@ void hmat_free(struct hmat_t *h)
 	if (h == nullptr) return;
 	HIPCHK(hipStreamSynchronize(h->s));
 	const void *bufs[] = {h->d_perm, h->d_far, h->d_near, h->d_U, h->d_V, h->d_D, h->L.lo, h->L.hi, h->L.rfoff, h->L.rfidx,
-	                      h->L.rnoff, h->L.rnidx, h->d_ft, h->d_xp, h->d_x, h->d_y, h->d_part};
+	                      h->L.rnoff, h->L.rnidx, h->d_ft, h->d_xp, h->d_x, h->d_y, h->d_part, h->d_item};
 	for (const void *b : bufs)
 		if (b) HIPCHK(hipFree(const_cast<void *>(b)));
 	hm_tree_free(&h->t);

@@ -69,6 +69,7 @@ struct hmat_t {
 	pe::hleafs L{};
 	double2 *d_ft = nullptr, *d_xp = nullptr, *d_x = nullptr, *d_y = nullptr;
 	double2 *d_part = nullptr;   // per-row partial sums of the leaf products (HM_SPLIT x n)
+	int *d_item = nullptr;       // far block of each (block, rank) pair of V^T x, by ft index
 	std::vector<int> rank;
 	std::vector<pe::hfar> far;
 	std::vector<pe::hdense> near;
