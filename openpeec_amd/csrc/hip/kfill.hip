@@ -14,9 +14,9 @@
 //
 // Rc is the centre distance (wires: sqrt(rc^2 + a^2)); |R_p - Rc| <= rho1 + rho2, the cells'
 // bounding radii about their centres.  A pair takes the shifted form when k_hi (rho1 + rho2)
-// <= 1/2 and the cells' bounding spheres are apart, else the unshifted form when k_hi R_max <=
-// 1/2; with moments to m = 14 the truncated tail is below (1/2)^15 / 15! ~ 2e-17 of the leading
-// term.  A pair that fits neither, or whose quadrature order changes inside [k_lo, k_hi], is
+// <= 0.8 and the cells' bounding spheres are apart, else the unshifted form when k_hi R_max <=
+// 0.8; with moments to m = 14 every term of the series is at most 1 in size and the truncated
+// tail is below 0.8^15 e^0.8 / 15! ~ 6e-14 of the pair's static magnitude sum_p w_p / R_p.  A pair that fits neither, or whose quadrature order changes inside [k_lo, k_hi], is
 // listed and evaluated directly at every frequency by the element code.
 //
 // One record per term (direct pair, ground-plane image): base = mult (I_static - fac N_0),
@@ -32,7 +32,7 @@
 
 namespace pe {
 
-constexpr double KF_X0 = 0.5;
+constexpr double KF_X0 = 0.8;
 
 __device__ __forceinline__ void tri_tile_kf(long L, int nt, int &ti, int &tj)
 {
