@@ -345,6 +345,22 @@ __global__ void k_maxpy(double2 *w, const double2 *V, long n, int nv, const doub
 	w[r] = acc;
 }
 
+// coefficients of one Gram-Schmidt pass from the multi-vector dot partials, summed over the
+// partial blocks in increasing order: h[i] = sum_b part[i][b]; acc[i] = h[i] on the first pass,
+// acc[i] + h[i] on the second
+__global__ void k_mdot_fin(const double2 *part, int nv, int nblk, double2 *h, double2 *acc, int pass)
+{
+	for (int i = threadIdx.x; i < nv; i += blockDim.x) {
+		double sr = 0, si = 0;
+		for (int b = 0; b < nblk; b++) {
+			sr += part[(size_t)i * nblk + b].x;
+			si += part[(size_t)i * nblk + b].y;
+		}
+		h[i] = make_double2(sr, si);
+		acc[i] = (pass == 0) ? make_double2(sr, si) : make_double2(acc[i].x + sr, acc[i].y + si);
+	}
+}
+
 // w -= h v  and  x += y v  style updates
 __global__ void k_axpy(double2 *w, const double2 *v, double2 h, int n)
 {
@@ -389,6 +405,7 @@ struct hop {
 	double2 *partial = nullptr;
 	std::vector<double2> hp;
 	double2 *mpart = nullptr, *mh = nullptr;   // multi-vector dots: partials, coefficients
+	double2 *hacc = nullptr;                    // both passes' coefficients summed
 	std::vector<double2> hmp, hmh;
 };
 
@@ -474,13 +491,14 @@ void hop_init(hop &o, const peec_t *p, const mna_sparse_t *sp, const hblock_t *b
 	o.hp.resize(DOT_BLK);
 	o.mpart = dal<double2>((size_t)(GM_M + 1) * MD_BLK);
 	o.mh = dal<double2>(GM_M + 1);
+	o.hacc = dal<double2>(GM_M + 1);
 	o.hmp.resize((size_t)(GM_M + 1) * MD_BLK);
 	o.hmh.resize(GM_M + 1);
 }
 
 void hop_free(hop &o)
 {
-	void *b[] = {o.rp, o.col, o.val, o.work, o.partial, o.mpart, o.mh};
+	void *b[] = {o.rp, o.col, o.val, o.work, o.partial, o.mpart, o.mh, o.hacc};
 	for (void *x : b)
 		if (x) HIPCHK(hipFree(x));
 	o = hop();
@@ -501,27 +519,25 @@ std::complex<double> hdot(hop &o, const double2 *a, const double2 *c, int n)
 }
 
 // classical Gram-Schmidt with one reorthogonalisation of w against V_0 .. V_j: per pass one
-// multi-vector dot kernel, one copy each way and one multi-vector update; Hc[i] += the
-// coefficients of both passes
-void orth_cgs2(hop &o, const double2 *V, int j, double2 *w, std::complex<double> *Hc)
+// multi-vector dot kernel, the coefficients summed on the device and one multi-vector update;
+// then |w|^2, and one copy to the host for both: Hc[i] += the coefficients of both passes, and
+// the return value is |w|^2 (the partials summed in the order hdot sums them)
+double orth_cgs2(hop &o, const double2 *V, int j, double2 *w, std::complex<double> *Hc)
 {
 	const int n = o.n, nv = j + 1;
 	for (int pass = 0; pass < 2; pass++) {
 		hipLaunchKernelGGL(k_mdot, dim3(MD_BLK, nv), dim3(RD), 0, o.s, V, (long)n, nv, (const double2 *)w, o.mpart);
-		HIPCHK(hipMemcpyAsync(o.hmp.data(), o.mpart, (size_t)nv * MD_BLK * sizeof(double2), hipMemcpyDeviceToHost, o.s));
-		HIPCHK(hipStreamSynchronize(o.s));
-		for (int i = 0; i < nv; i++) {
-			double sr = 0, si = 0;
-			for (int b = 0; b < MD_BLK; b++) {
-				sr += o.hmp[(size_t)i * MD_BLK + b].x;
-				si += o.hmp[(size_t)i * MD_BLK + b].y;
-			}
-			o.hmh[i] = make_double2(sr, si);
-			Hc[i] += std::complex<double>(sr, si);
-		}
-		HIPCHK(hipMemcpyAsync(o.mh, o.hmh.data(), (size_t)nv * sizeof(double2), hipMemcpyHostToDevice, o.s));
+		hipLaunchKernelGGL(k_mdot_fin, dim3(1), dim3(64), 0, o.s, (const double2 *)o.mpart, nv, (int)MD_BLK, o.mh, o.hacc, pass);
 		hipLaunchKernelGGL(k_maxpy, dim3(nb(n)), dim3(256), 0, o.s, w, V, (long)n, nv, (const double2 *)o.mh);
 	}
+	hipLaunchKernelGGL(k_dotp, dim3(DOT_BLK), dim3(RD), 0, o.s, (const double2 *)w, (const double2 *)w, n, o.partial);
+	HIPCHK(hipMemcpyAsync(o.hmh.data(), o.hacc, (size_t)nv * sizeof(double2), hipMemcpyDeviceToHost, o.s));
+	HIPCHK(hipMemcpyAsync(o.hp.data(), o.partial, DOT_BLK * sizeof(double2), hipMemcpyDeviceToHost, o.s));
+	HIPCHK(hipStreamSynchronize(o.s));
+	for (int i = 0; i < nv; i++) Hc[i] += std::complex<double>(o.hmh[i].x, o.hmh[i].y);
+	double sr = 0;
+	for (int b = 0; b < DOT_BLK; b++) sr += o.hp[b].x;
+	return sr;
 }
 
 // z = M^-1 v on the device (v, z length n; v is not modified)
@@ -596,8 +612,7 @@ int gmres_dev(hop &o, precond_t *pc, const double2 *b, double2 *x, double tol)
 			else hipLaunchKernelGGL(k_copy, dim3(nb(n)), dim3(256), 0, s, vj, zj, n);
 			hop_apply(o, zj, w);
 			zc *Hc = &H[(size_t)j * (GM_M + 1)];
-			orth_cgs2(o, V, j, w, Hc);
-			const double hn = std::sqrt(hdot(o, w, w, n).real());
+			const double hn = std::sqrt(orth_cgs2(o, V, j, w, Hc));
 			Hc[j + 1] = hn;
 			if (hn > 0) hipLaunchKernelGGL(k_scal, dim3(nb(n)), dim3(256), 0, s, (const double2 *)w, V + (long)(j + 1) * n, 1.0 / hn, n);
 			for (int i = 0; i < j; i++) {
@@ -1036,8 +1051,7 @@ int gmres_fn(hop &o, const std::function<void(const double2 *, double2 *)> &prec
 			prec(vj, zj);
 			hop_apply(o, zj, w);
 			zc *Hc = &H[(size_t)j * (GM_M + 1)];
-			orth_cgs2(o, V, j, w, Hc);
-			const double hn = std::sqrt(hdot(o, w, w, n).real());
+			const double hn = std::sqrt(orth_cgs2(o, V, j, w, Hc));
 			Hc[j + 1] = hn;
 			if (hn > 0) hipLaunchKernelGGL(k_scal, dim3(nb(n)), dim3(256), 0, s, (const double2 *)w, V + (long)(j + 1) * n, 1.0 / hn, n);
 			for (int i = 0; i < j; i++) {
