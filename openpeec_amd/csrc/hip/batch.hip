@@ -863,14 +863,16 @@ struct chunk {                 // jobs of one pass through the arena
 	int Nmax = 0;
 };
 
+// everything a captured pair of graphs depends on: launch shapes, the tuned variant, and the
+// largest order (the LU kernel's dynamic LDS is sized for it when the graph is recorded)
 struct gkey {
 	int nj;
 	long nst, arena_n, tilesS, tilesP, tilesG, nlist;
-	int nb, cb, wpc, fp32;
+	int nb, cb, wpc, fp32, Nmax;
 	bool operator<(const gkey &o) const
 	{
-		return std::tie(nj, nst, arena_n, tilesS, tilesP, tilesG, nlist, nb, cb, wpc, fp32) <
-		       std::tie(o.nj, o.nst, o.arena_n, o.tilesS, o.tilesP, o.tilesG, o.nlist, o.nb, o.cb, o.wpc, o.fp32);
+		return std::tie(nj, nst, arena_n, tilesS, tilesP, tilesG, nlist, nb, cb, wpc, fp32, Nmax) <
+		       std::tie(o.nj, o.nst, o.arena_n, o.tilesS, o.tilesP, o.tilesG, o.nlist, o.nb, o.cb, o.wpc, o.fp32, o.Nmax);
 	}
 };
 
@@ -1123,7 +1125,7 @@ void direct_blocks(batch &B, const chunk &c)
 
 std::pair<hipGraphExec_t, hipGraphExec_t> graphs_for(batch &B, const chunk &c, const tune_t &t)
 {
-	const gkey k{(int)c.J.size(), c.nst, c.arena_n, c.tilesS, c.tilesP, c.tilesG, c.nlist, t.nb, t.cb, t.wpc, B.fp32};
+	const gkey k{(int)c.J.size(), c.nst, c.arena_n, c.tilesS, c.tilesP, c.tilesG, c.nlist, t.nb, t.cb, t.wpc, B.fp32, c.Nmax};
 	auto it = B.graphs.find(k);
 	if (it != B.graphs.end()) return it->second;
 	hipGraphExec_t gf = capture(B.s, [&](hipStream_t s) { record_fill(s, B, c); });

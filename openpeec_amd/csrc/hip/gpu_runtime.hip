@@ -71,6 +71,36 @@ std::mutex g_mu;
 std::vector<pgpu *> g_tab;
 bool g_dev_init = false;
 
+// Process-exit teardown.  Device buffers, streams and library handles held by the library's
+// side tables are released by one atexit handler, registered after the first HIP call of the
+// process: handlers run in reverse order of registration, so this one runs before the HIP
+// runtime's and a profiler's own exit handlers, while the runtime is still whole.  Modules
+// with process-lifetime device state (LU workspace and rocBLAS handles, the accelerated form's
+// cache) add a hook; the hooks run last-registered first, then every device mirror is dropped.
+constexpr int EXIT_HOOKS_MAX = 16;
+void (*g_exit_hooks[EXIT_HOOKS_MAX])(void) = {};
+int g_nexit_hooks = 0;
+bool g_exit_registered = false, g_exiting = false;
+
+void release_mirror(pgpu *g);
+
+void teardown_at_exit()
+{
+	g_exiting = true;
+	if (hipDeviceSynchronize() != hipSuccess) return;   // a failed device: leave it to the runtime
+	for (int k = g_nexit_hooks - 1; k >= 0; k--) g_exit_hooks[k]();
+	std::lock_guard<std::mutex> lk(g_mu);
+	for (pgpu *g : g_tab) release_mirror(g);
+	g_tab.clear();
+}
+
+void register_teardown()
+{
+	if (g_exit_registered) return;
+	g_exit_registered = true;
+	atexit(teardown_at_exit);
+}
+
 template <typename T>
 T *dalloc(size_t n)
 {
@@ -98,6 +128,7 @@ void init_device()
 	const int dev = env ? atoi(env) : 0;
 	HIPCHK(hipSetDevice(dev));
 	g_dev_init = true;
+	register_teardown();
 }
 
 void upload_set(pset &ps, const seg_t *src, int n, int gp, double gpz)
@@ -402,12 +433,9 @@ pgpu_t *pgpu_get(const peec_t *p)
 	return g;
 }
 
-void peec_gpu_release(peec_t *p)
+namespace {
+void release_mirror(pgpu *g)
 {
-	std::lock_guard<std::mutex> lk(g_mu);
-	for (size_t k = 0; k < g_tab.size(); k++) {
-		pgpu *g = g_tab[k];
-		if (g->p != p) continue;
 		HIPCHK(hipStreamSynchronize(g->s));
 		free_set(g->seg);
 		free_set(g->chg);
@@ -444,6 +472,27 @@ void peec_gpu_release(peec_t *p)
 		dfree(g->qbuf);
 		HIPCHK(hipStreamDestroy(g->s));
 		delete g;
+}
+}  // namespace
+
+void pgpu_at_exit(void (*fn)(void))
+{
+	std::lock_guard<std::mutex> lk(g_mu);
+	register_teardown();
+	for (int k = 0; k < g_nexit_hooks; k++)
+		if (g_exit_hooks[k] == fn) return;
+	if (g_nexit_hooks == EXIT_HOOKS_MAX) pgpu_fatal("too many exit hooks");
+	g_exit_hooks[g_nexit_hooks++] = fn;
+}
+
+void peec_gpu_release(peec_t *p)
+{
+	std::lock_guard<std::mutex> lk(g_mu);
+	if (g_exiting) return;   // the exit handler has released every mirror
+	for (size_t k = 0; k < g_tab.size(); k++) {
+		pgpu *g = g_tab[k];
+		if (g->p != p) continue;
+		release_mirror(g);
 		g_tab.erase(g_tab.begin() + (long)k);
 		return;
 	}

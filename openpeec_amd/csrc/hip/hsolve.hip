@@ -1454,10 +1454,10 @@ int peec_lpc_samples(peec_t *p, const double *fref, const double *f, int npairs,
 
 // Compressed sweep with the learned correction: the preconditioner is built at a reference
 // frequency and kept while GMRES needs at most max(1.5 x, +10) the iterations of the first solve
-// after the build, unless a build costs less than half the last point's solves (then it is
-// rebuilt at every point); theta from the network (0 without a model: precond.c's method under
-// the same rules); tolerance tol on the true residual; a solve that fails is redone with a fresh
-// preconditioner and theta = 0.
+// after the build; theta from the network (0 without a model: precond.c's method under the same
+// rule); tolerance tol on the true residual; a solve that fails is redone with a fresh
+// preconditioner and theta = 0.  The schedule depends on iteration counts only, so the
+// preconditioner used at each point (and Z at the tolerance level) is the same on every run.
 int pgpu_sweep_learned(pgpu_t *g, peec_t *p, int ifreq0, int nfreq, d_complex_t *zmat, pgpu_stats_t *st, double tol)
 {
 	hipStream_t s = pgpu_hip_stream(g);
@@ -1480,7 +1480,6 @@ int pgpu_sweep_learned(pgpu_t *g, peec_t *p, int ifreq0, int nfreq, d_complex_t 
 	double fref = 0;
 	int its_fresh = -1;
 	bool rebuild = true;
-	double t_build = 0, t_solve = 0;   // ms of the last build and of the last point's solves
 	int rc = 0;
 	for (int k = 0; (k < nfreq) && !rc; k++) {
 		const double f = freq_at(p, ifreq0 + k);
@@ -1492,11 +1491,9 @@ int pgpu_sweep_learned(pgpu_t *g, peec_t *p, int ifreq0, int nfreq, d_complex_t 
 			break;
 		}
 		const auto t1 = std::chrono::steady_clock::now();
-		if (rebuild || (pc == nullptr) || (t_build < 0.5 * t_solve)) {
+		if (rebuild || (pc == nullptr)) {
 			if (pc) pc_free(pc);
-			const auto tb = std::chrono::steady_clock::now();
 			pc = pc_build_dev(p, &S.sp, S.blk, S.nblk, s);
-			t_build = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tb).count();
 			if (pc == nullptr) {
 				printf("*** compression : preconditioner failed at f = %.5e Hz (floating node?)\n", f);
 				rc = 6;
@@ -1550,11 +1547,10 @@ int pgpu_sweep_learned(pgpu_t *g, peec_t *p, int ifreq0, int nfreq, d_complex_t 
 			}
 		}
 		const auto t3 = std::chrono::steady_clock::now();
-		t_solve = std::chrono::duration<double, std::milli>(t3 - t2).count();
 		if (st) {
 			st->fill_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
 			st->lu_ms += std::chrono::duration<double, std::milli>(t2 - t1).count();
-			st->solve_ms += t_solve;
+			st->solve_ms += std::chrono::duration<double, std::milli>(t3 - t2).count();
 			st->total_ms += std::chrono::duration<double, std::milli>(t3 - t0).count();
 		}
 	}

@@ -89,7 +89,7 @@ struct panel_ws {
 	int     *pubi;      // [2][P]     its global row
 	double2 *pubrow;    // [2][P][NB] that row's panel entries
 	double2 *diagrow;   // [2][NB]    current row kk
-	int     *info;      // singular row or -1
+	int     *info;      // [0] singular row or -1; [1] column whose pivot exchange timed out, or -1
 };
 
 constexpr int SROW = NB + 1;    // padded LDS row stride (complex): spreads row-per-thread access over banks
@@ -206,6 +206,7 @@ __global__ __launch_bounds__(PANEL_T) void k_panel(TS *A, long lda, int n, int k
 				if (w == 0) {
 					piv[kk] = stop ? ((gi == INT_MAX) ? kk : gi) : gi;
 					if (stop) *ws.info = kk;
+					if (timeout) ws.info[1] = kk;   // a failed exchange, not a zero pivot
 				}
 			}
 		}
@@ -546,6 +547,8 @@ struct lu_work {
 };
 lu_work g_lw;
 
+void lu_release_at_exit();
+
 void ensure_ws(int n)
 {
 	if (!g_lw.attr) {
@@ -561,6 +564,7 @@ void ensure_ws(int n)
 	const size_t need = 2 * PANEL_PMAX * sizeof(int) + 2 * (size_t)PANEL_PMAX * NB * sizeof(double2) +
 	                    2 * NB * sizeof(double2) + SLOT_BYTES + 256;
 	if (g_lw.base == nullptr) {
+		pgpu_at_exit(lu_release_at_exit);
 		HIPCHK(hipMalloc((void **)&g_lw.base, need));
 		char *p = g_lw.base;
 		g_lw.ws.pubrow = (double2 *)p;
@@ -591,18 +595,39 @@ bool gemm_hip()
 }
 
 // one rocBLAS handle per stream (a handle's workspace must not serve two streams at once), bound
-// to that stream
+// to that stream; destroyed with the LU workspace at process exit
+constexpr int BLAS_MAX = 32;
+std::pair<hipStream_t, rocblas_handle> g_blas[BLAS_MAX];
+int g_nblas = 0;
+
+void lu_release_at_exit()
+{
+	for (int k = 0; k < g_nblas; k++) rocblas_destroy_handle(g_blas[k].second);
+	g_nblas = 0;
+	if (g_lw.base) HIPCHK(hipFree(g_lw.base));
+	if (g_lw.perm) HIPCHK(hipFree(g_lw.perm));
+	g_lw.base = nullptr;
+	g_lw.perm = nullptr;
+	g_lw.nperm = 0;
+}
+
 rocblas_handle blas(hipStream_t s)
 {
-	static std::vector<std::pair<hipStream_t, rocblas_handle>> hs;
-	for (const auto &e : hs)
-		if (e.first == s) return e.second;
+	for (int k = 0; k < g_nblas; k++)
+		if (g_blas[k].first == s) return g_blas[k].second;
+	if (g_nblas == BLAS_MAX) {
+		// streams come and go with device mirrors: recycle the oldest handle's slot
+		rocblas_destroy_handle(g_blas[0].second);
+		for (int k = 1; k < BLAS_MAX; k++) g_blas[k - 1] = g_blas[k];
+		g_nblas--;
+	}
 	rocblas_handle h = nullptr;
 	if ((rocblas_create_handle(&h) != rocblas_status_success) || (rocblas_set_stream(h, s) != rocblas_status_success)) {
 		fprintf(stderr, "*** rocblas_create_handle failed\n");
 		abort();
 	}
-	hs.push_back(std::make_pair(s, h));
+	pgpu_at_exit(lu_release_at_exit);
+	g_blas[g_nblas++] = std::make_pair(s, h);
 	return h;
 }
 
@@ -927,7 +952,7 @@ int lu_factor_t(hipStream_t s, int n, TS *A, int lda, int *dpiv, int *dperm, pgp
 	if (n <= 0) return -1;
 	ensure_ws(n);
 	int *perm = (dperm != nullptr) ? dperm : g_lw.perm;
-	HIPCHK(hipMemsetAsync(g_lw.ws.info, 0xff, sizeof(int), s));   // -1
+	HIPCHK(hipMemsetAsync(g_lw.ws.info, 0xff, 2 * sizeof(int), s));   // -1, -1
 	hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, s, perm, n);
 	hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, s, dpiv, n);
 	hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -943,8 +968,10 @@ int lu_factor_t(hipStream_t s, int n, TS *A, int lda, int *dpiv, int *dperm, pgp
 		int P = (m + PANEL_RMAX - 1) / PANEL_RMAX;
 		if (P < 1) P = 1;
 		if (P > pmax) {
+			// a size limit, not a property of the matrix: reported as the step n, which no
+			// zero pivot can have (those are 0 .. n-1), so callers that test info >= 0 stop
 			fprintf(stderr, "*** LU factorisation: n = %d exceeds the panel capacity (%d rows)\n", n, pmax * PANEL_RMAX);
-			return 0;
+			return n;
 		}
 		int R = (m + P - 1) / P;
 		int *pivp = dpiv;
@@ -984,10 +1011,13 @@ int lu_factor_t(hipStream_t s, int n, TS *A, int lda, int *dpiv, int *dperm, pgp
 		HIPCHK(hipEventDestroy(e0));
 		HIPCHK(hipEventDestroy(e1));
 	}
-	int info = -1;
-	HIPCHK(hipMemcpyAsync(&info, g_lw.ws.info, sizeof(int), hipMemcpyDeviceToHost, s));
+	int info[2] = {-1, -1};
+	HIPCHK(hipMemcpyAsync(info, g_lw.ws.info, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
 	HIPCHK(hipStreamSynchronize(s));
-	return info;
+	// a pivot exchange that gave up means the cooperative grid was not co-resident: a runtime
+	// failure like a failed HIP call, never reported as a singular matrix
+	if (info[1] >= 0) pgpu_fatal("LU panel: pivot exchange between workgroups timed out");
+	return info[0];
 }
 
 }  // namespace

@@ -1112,6 +1112,12 @@ void free_fast(fast &c)
 	c = fast();
 }
 
+// the cached preconditioner and its buffers live for the process: released at exit
+void fast_release_at_exit()
+{
+	free_fast(g_fast);
+}
+
 // stamps of one slot -> device triplets sorted by row (stable) + CSR row pointer
 void upload_stamps(slotbuf &b, const stamps_t &sp, int n, hipStream_t s)
 {
@@ -1231,6 +1237,7 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 		c.inv = dal<double2>((size_t)n * n);
 		c.scr = dal<double2>((size_t)((n + 1) / 2 + TB) * ((n + 1) / 2 + TB));
 		c.scr2 = dal<double2>((size_t)((n + 1) / 2 + TB) * ((n + 1) / 2 + TB));
+		pgpu_at_exit(fast_release_at_exit);
 		HIPCHK(hipStreamCreate(&c.s2));
 		HIPCHK(hipEventCreateWithFlags(&c.ev_f, hipEventDisableTiming));
 		HIPCHK(hipEventCreateWithFlags(&c.ev_u, hipEventDisableTiming));

@@ -94,6 +94,9 @@ pgpu_t *pgpu_get(const peec_t *p);          /* find or create; uploads cells on 
 void    peec_gpu_release(peec_t *p);        /* drop the mirror of p (called by peec_free) */
 int     pgpu_device_ok(void);               /* 1 if a HIP device is usable                */
 void    pgpu_fatal(const char *what);       /* print and abort: no host fallback exists   */
+/* fn runs once at process exit, before the HIP runtime's own teardown (hooks last-registered
+ * first, then every device mirror is released); for process-lifetime device state */
+void    pgpu_at_exit(void (*fn)(void));
 
 /* Fill (results stay on the device; *_to_host copies them into the peec_t arrays).
  * Return 0 on success. */
@@ -151,7 +154,8 @@ int  pgpu_sweep_learned(pgpu_t *g, peec_t *p, int ifreq0, int nfreq, d_complex_t
 /* ---------------------------------------------------------------------------------------- */
 /* Dense complex LU on device buffers (lu.hip).  Matrices are row-major, leading dim lda.   */
 /* factor in place; dpiv[k] as lu_decomp; dperm (may be NULL) receives the row permutation
- * (row k of the factors is original row dperm[k]).  Returns -1 or the singular step. */
+ * (row k of the factors is original row dperm[k]).  Returns -1, the singular step, or n when
+ * n exceeds the cooperative panel's capacity (256 workgroups x 144 rows on MI355X). */
 int  dlu_factor(void *stream, int n, d_complex_t *dA, int lda, int *dpiv, int *dperm, pgpu_stats_t *st);
 /* B <- A^-1 B (nrhs columns, row-major, leading dim ldb); dwork holds n * nrhs entries */
 void dlu_solve_perm(void *stream, int n, const d_complex_t *dA, int lda, const int *dperm,

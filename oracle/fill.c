@@ -671,7 +671,7 @@ int orc_pot_fill(const peec_t *p, double kw, d_complex_t *pmat, d_complex_t *c)
 	if (orc_lu_decomp(nc, P, piv) >= 0) {
 		rc = 1;
 	}
-	else {
+	else if (c != NULL) {
 		d_complex_t *b = (d_complex_t *)malloc((size_t)nc * sizeof(d_complex_t));
 		for (int j = 0; j < nc; j++) {
 			memset(b, 0, (size_t)nc * sizeof(d_complex_t));

@@ -39,7 +39,7 @@ void        orc_seg_mirror(const seg_t *s, double gpz, seg_t *out);
 /* matrix fills: lp[nseg*nseg], c[ncell*ncell] (= P^-1); cell maps built internally */
 d_complex_t orc_lp_entry(const peec_t *p, int i, int j, double kw);
 void        orc_lp_fill(const peec_t *p, double kw, d_complex_t *lp);
-int         orc_pot_fill(const peec_t *p, double kw, d_complex_t *pmat, d_complex_t *c);   /* 0 ok, 1 singular */
+int         orc_pot_fill(const peec_t *p, double kw, d_complex_t *pmat, d_complex_t *c);   /* 0 ok, 1 singular; pmat (P before inversion) and c may be NULL */
 
 /* dense LU (lu.c) */
 int  orc_lu_decomp(int n, d_complex_t *a, int *piv);

@@ -1,8 +1,9 @@
 """Dense complex LU on the GPU (lu_decomp / lu_solve through the C ABI).
 
 Checks: the solution of A x = b against numpy's LAPACK solve (backward error), the factors'
-reconstruction P A = L U, the pivot sequence against the reference lu_decomp (same |re|+|im|
-rule; random matrices have no near-ties), and the singular-pivot return code.
+reconstruction P A = L U, the pivot sequence against the oracle's restatement of lu_decomp
+(lu.c, pinned to the reference and to LAPACK by test_oracle.py; same |re|+|im| rule, random
+matrices have no near-ties), and the singular-pivot return code.
 """
 import ctypes as C
 
@@ -10,7 +11,7 @@ import numpy as np
 import pytest
 
 from openpeec_amd import abi
-from refharness import ref_available, ref_lib
+from oracle_py import oracle_lib
 
 pytestmark = pytest.mark.gpu
 
@@ -53,7 +54,6 @@ def test_lu_solve_matches_lapack(lib, n, gemm, monkeypatch):
     assert np.abs(L @ U - pa).max() / np.abs(a0).max() < 1e-12
 
 
-@pytest.mark.skipif(not ref_available(), reason="reference build not present")
 @pytest.mark.parametrize("n", [50, 200])
 def test_lu_pivots_match_reference(lib, n):
     a0 = _rand(n, 7 * n)
@@ -62,7 +62,7 @@ def test_lu_pivots_match_reference(lib, n):
     assert lib.lu_decomp(n, _zptr(a), piv.ctypes.data_as(C.POINTER(C.c_int))) == -1
     r = a0.copy()
     rpiv = np.zeros(n, dtype=np.int32)
-    assert ref_lib().lu_decomp(n, _zptr(r), rpiv.ctypes.data_as(C.POINTER(C.c_int))) == -1
+    assert oracle_lib().orc_lu_decomp(n, _zptr(r), rpiv.ctypes.data_as(C.POINTER(C.c_int))) == -1
     assert np.array_equal(piv, rpiv)
     assert np.abs(a - r).max() / np.abs(r).max() < 1e-11
 
