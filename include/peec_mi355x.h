@@ -171,6 +171,17 @@ int  peec_lpc_mlp_grad(const double *W, const double *X, const double *coef, int
 int  peec_lpc_fit(const double *X, const double *coef, int S, int steps, double lr, unsigned long long seed, double *W,
                   double *loss_hist);
 
+/* lu_decomp on a device-resident matrix, allocating nothing.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP stream capture, serving loops,
+ * harnesses that forbid hipMalloc).  dA (n x n, row-major, leading dimension lda >= n) and dpiv
+ * (n ints) are device pointers; the matrix is factored in place exactly as lu_decomp
+ * (src/lu.c:16-66) factors it, dpiv in lu_decomp's convention; stream is a hipStream_t (NULL:
+ * the default stream) and is synchronised before returning.  The panel's exchange buffers and
+ * the permutation scratch come from one static device array per GPU (668 KiB, held until the
+ * process ends), so n is at most 32768 (-2 with a message beyond it); calls that overlap on one
+ * GPU race on that array.  Returns -1, the first zero pivot's row, or -2 for invalid arguments. */
+int lu_decomp_device_noalloc(int n, d_complex_t *dA, int lda, int *dpiv, void *stream);
+
 /* Sizes for bindings */
 size_t peec_sizeof_peec(void);
 size_t peec_sizeof_seg(void);

@@ -235,6 +235,7 @@ def declare(lib: C.CDLL, reference: bool = False) -> None:
             "peec_lpc_fit": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int, C.c_int, C.c_double,
                                        C.c_ulonglong, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
             "peec_batch_lu_decomp": (C.c_int, [C.c_int, C.POINTER(C.c_int), PZ, C.POINTER(C.c_int), C.c_int]),
+            "lu_decomp_device_noalloc": (C.c_int, [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
         })
     for name, (res, args) in sig.items():
         fn = getattr(lib, name, None)

@@ -26,6 +26,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <utility>
 #include <vector>
 #include "../peec_internal.h"
@@ -549,7 +550,27 @@ lu_work g_lw;
 
 void lu_release_at_exit();
 
-void ensure_ws(int n)
+// the panel's exchange buffers, laid out in one block of PANEL_WS_BYTES
+constexpr size_t PANEL_WS_BYTES = 2 * (size_t)PANEL_PMAX * NB * sizeof(double2) + 2 * NB * sizeof(double2) + SLOT_BYTES +
+                                  2 * PANEL_PMAX * sizeof(int) + 256;
+
+panel_ws carve_panel_ws(char *p)
+{
+	panel_ws w;
+	w.pubrow = (double2 *)p;
+	p += 2 * (size_t)PANEL_PMAX * NB * sizeof(double2);
+	w.diagrow = (double2 *)p;
+	p += 2 * NB * sizeof(double2);
+	w.slot = (double *)p;
+	p += SLOT_BYTES;
+	w.pubi = (int *)p;
+	p += 2 * PANEL_PMAX * sizeof(int);
+	w.info = (int *)p;
+	return w;
+}
+
+// per device: the panel's dynamic-LDS attribute and the CU count (no allocation)
+void ensure_attr()
 {
 	if (!g_lw.attr) {
 		HIPCHK(hipFuncSetAttribute((const void *)k_panel<double2>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -561,21 +582,15 @@ void ensure_ws(int n)
 		HIPCHK(hipDeviceGetAttribute(&g_lw.ncu, hipDeviceAttributeMultiprocessorCount, dev));
 		g_lw.attr = true;
 	}
-	const size_t need = 2 * PANEL_PMAX * sizeof(int) + 2 * (size_t)PANEL_PMAX * NB * sizeof(double2) +
-	                    2 * NB * sizeof(double2) + SLOT_BYTES + 256;
+}
+
+void ensure_ws(int n)
+{
+	ensure_attr();
 	if (g_lw.base == nullptr) {
 		pgpu_at_exit(lu_release_at_exit);
-		HIPCHK(hipMalloc((void **)&g_lw.base, need));
-		char *p = g_lw.base;
-		g_lw.ws.pubrow = (double2 *)p;
-		p += 2 * (size_t)PANEL_PMAX * NB * sizeof(double2);
-		g_lw.ws.diagrow = (double2 *)p;
-		p += 2 * NB * sizeof(double2);
-		g_lw.ws.slot = (double *)p;
-		p += SLOT_BYTES;
-		g_lw.ws.pubi = (int *)p;
-		p += 2 * PANEL_PMAX * sizeof(int);
-		g_lw.ws.info = (int *)p;
+		HIPCHK(hipMalloc((void **)&g_lw.base, PANEL_WS_BYTES));
+		g_lw.ws = carve_panel_ws(g_lw.base);
 	}
 	if (g_lw.nperm < n) {
 		if (g_lw.perm) HIPCHK(hipFree(g_lw.perm));
@@ -724,16 +739,22 @@ __global__ __launch_bounds__(256) void k_cgemm32(int M, int N, int K, const floa
 }
 
 void launch_update(hipStream_t s, int M, int N, int K, const double2 *A, long lda, const double2 *B, long ldb, double2 *C,
-                   long ldc)
+                   long ldc, bool own)
 {
+	if (own) {
+		if ((M <= 0) || (N <= 0) || (K <= 0)) return;
+		dim3 grid((N + 127) / 128, (M + 63) / 64);
+		hipLaunchKernelGGL(k_zgemm, grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc);
+		return;
+	}
 	launch_gemm(s, M, N, K, A, lda, B, ldb, C, ldc);
 }
 
 void launch_update(hipStream_t s, int M, int N, int K, const float2 *A, long lda, const float2 *B, long ldb, float2 *C,
-                   long ldc)
+                   long ldc, bool own)
 {
 	if ((M <= 0) || (N <= 0) || (K <= 0)) return;
-	if (!gemm_hip()) {
+	if (!own && !gemm_hip()) {
 		rocblas_handle h = blas(s);
 		const rocblas_float_complex alpha{-1.0f, 0.0f}, beta{1.0f, 0.0f};
 		blas_check(rocblas_cgemm(h, rocblas_operation_none, rocblas_operation_none, N, M, K, &alpha,
@@ -911,17 +932,17 @@ void tri_inv(hipStream_t s, bool lower, const TS *T, long ld, TS *Out, int i0, i
 	if (lower) {
 		// C = -T21 inv(T11) (n2 x n1); Out21 = -(inv(T22) C), then negated
 		scale_block(s, scr, n1, n2, n1, 0);
-		launch_update(s, n2, n1, n1, T + (long)(i0 + n1) * ld + i0, ld, O11, ld, scr, n1);
+		launch_update(s, n2, n1, n1, T + (long)(i0 + n1) * ld + i0, ld, O11, ld, scr, n1, false);
 		TS *O21 = Out + (long)(i0 + n1) * ld + i0;
-		launch_update(s, n2, n1, n2, O22, ld, (const TS *)scr, n1, O21, ld);
+		launch_update(s, n2, n1, n2, O22, ld, (const TS *)scr, n1, O21, ld, false);
 		scale_block(s, O21, ld, n2, n1, -1);
 	}
 	else {
 		// C = -T12 inv(T22) (n1 x n2); Out12 = -(inv(T11) C), then negated
 		scale_block(s, scr, n2, n1, n2, 0);
-		launch_update(s, n1, n2, n2, T + (long)i0 * ld + (i0 + n1), ld, O22, ld, scr, n2);
+		launch_update(s, n1, n2, n2, T + (long)i0 * ld + (i0 + n1), ld, O22, ld, scr, n2, false);
 		TS *O12 = Out + (long)i0 * ld + (i0 + n1);
-		launch_update(s, n1, n2, n1, O11, ld, (const TS *)scr, n2, O12, ld);
+		launch_update(s, n1, n2, n1, O11, ld, (const TS *)scr, n2, O12, ld, false);
 		scale_block(s, O12, ld, n1, n2, -1);
 	}
 }
@@ -946,13 +967,24 @@ extern "C" void dlu_zgemm_sub(void *stream, int M, int N, int K, const d_complex
 
 
 namespace {
+// fills n 32-bit words with 0xFFFFFFFF: -1 as int, NaN as either half of a double (a kernel
+// rather than hipMemsetAsync, so the sequence stays valid under stream capture)
+__global__ void k_fill_ones(unsigned *p, int n)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) p[i] = 0xFFFFFFFFu;
+}
+
+// The blocked factorisation on a given workspace: panel exchange buffers ws, permutation
+// scratch perm_ws (n ints, used when dperm is NULL); own_gemm selects the hand-written trailing
+// update (no library call, no allocation) whatever PEEC_GEMM says.
 template <typename TS>
-int lu_factor_t(hipStream_t s, int n, TS *A, int lda, int *dpiv, int *dperm, pgpu_stats_t *st)
+int lu_factor_ws(hipStream_t s, int n, TS *A, int lda, int *dpiv, int *dperm, pgpu_stats_t *st, const panel_ws &wsp,
+                 int *perm_ws, bool own_gemm)
 {
 	if (n <= 0) return -1;
-	ensure_ws(n);
-	int *perm = (dperm != nullptr) ? dperm : g_lw.perm;
-	HIPCHK(hipMemsetAsync(g_lw.ws.info, 0xff, 2 * sizeof(int), s));   // -1, -1
+	int *perm = (dperm != nullptr) ? dperm : perm_ws;
+	hipLaunchKernelGGL(k_fill_ones, dim3(1), dim3(64), 0, s, (unsigned *)wsp.info, 2);   // -1, -1
 	hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, s, perm, n);
 	hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, s, dpiv, n);
 	hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -976,17 +1008,18 @@ int lu_factor_t(hipStream_t s, int n, TS *A, int lda, int *dpiv, int *dperm, pgp
 		int R = (m + P - 1) / P;
 		int *pivp = dpiv;
 		int nn = n, kk0 = k0, nbb = nb;
-		panel_ws ws = g_lw.ws;
+		panel_ws ws = wsp;
 		TS *Ap = A;
 		long ldp = ldal;
 		void *kargs[] = {(void *)&Ap, (void *)&ldp, (void *)&nn, (void *)&kk0, (void *)&nbb,
 		                 (void *)&R, (void *)&pivp, (void *)&ws};
-		HIPCHK(hipMemsetAsync(g_lw.ws.slot, 0xff, SLOT_BYTES, s));   // every slot NaN
+		hipLaunchKernelGGL(k_fill_ones, dim3((SLOT_BYTES / 4 + 255) / 256), dim3(256), 0, s, (unsigned *)wsp.slot,
+		                   SLOT_BYTES / 4);   // every slot NaN
 		HIPCHK(hipLaunchCooperativeKernel((const void *)k_panel<TS>, dim3(P), dim3(PANEL_T), kargs,
 		                                  (unsigned)(R * SROW * sizeof(double2)), s));
 		const int nout = n - nb;
 		hipLaunchKernelGGL(k_laswp<TS>, dim3((nout + 1 + 255) / 256), dim3(256), 0, s, A, ldal, n, k0, nb, dpiv, perm,
-		                   (const int *)g_lw.ws.info);
+		                   (const int *)wsp.info);
 		const int nr = n - k0 - nb;
 		if (nr > 0) {
 			TS *A11 = A + (long)k0 * lda + k0;
@@ -995,7 +1028,7 @@ int lu_factor_t(hipStream_t s, int n, TS *A, int lda, int *dpiv, int *dperm, pgp
 			TS *A22 = A + (long)(k0 + nb) * lda + k0 + nb;
 			hipLaunchKernelGGL((k_trsm<true, TS>), dim3((nr + 63) / 64), dim3(64), 0, s, A11, ldal, A12, ldal, nb, nr);
 			if (st) HIPCHK(hipEventRecord(e0, s));
-			launch_update(s, nr, nr, nb, (const TS *)A21, ldal, (const TS *)A12, ldal, A22, ldal);
+			launch_update(s, nr, nr, nb, (const TS *)A21, ldal, (const TS *)A12, ldal, A22, ldal, own_gemm);
 			if (st) {
 				HIPCHK(hipEventRecord(e1, s));
 				HIPCHK(hipEventSynchronize(e1));
@@ -1012,12 +1045,20 @@ int lu_factor_t(hipStream_t s, int n, TS *A, int lda, int *dpiv, int *dperm, pgp
 		HIPCHK(hipEventDestroy(e1));
 	}
 	int info[2] = {-1, -1};
-	HIPCHK(hipMemcpyAsync(info, g_lw.ws.info, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+	HIPCHK(hipMemcpyAsync(info, wsp.info, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
 	HIPCHK(hipStreamSynchronize(s));
 	// a pivot exchange that gave up means the cooperative grid was not co-resident: a runtime
 	// failure like a failed HIP call, never reported as a singular matrix
 	if (info[1] >= 0) pgpu_fatal("LU panel: pivot exchange between workgroups timed out");
 	return info[0];
+}
+
+template <typename TS>
+int lu_factor_t(hipStream_t s, int n, TS *A, int lda, int *dpiv, int *dperm, pgpu_stats_t *st)
+{
+	if (n <= 0) return -1;
+	ensure_ws(n);
+	return lu_factor_ws<TS>(s, n, A, lda, dpiv, dperm, st, g_lw.ws, g_lw.perm, false);
 }
 
 }  // namespace
@@ -1137,4 +1178,57 @@ extern "C" void slu_tri_inverse(void *stream, int n, const void *LU, int lda, in
 {
 	tri_inverse_t<float2>((hipStream_t)stream, n, (const float2 *)LU, lda, lower, (float2 *)Out, (float2 *)scr,
 	                      (float2 *)dinv);
+}
+
+// ---------------------------------------------------------------- no-allocation entry point
+// lu_decomp on a device-resident matrix for callers that cannot allocate (code under stream
+// capture, serving loops): the panel's exchange buffers and the permutation scratch live in a
+// file-scope device array, one copy per GPU, whose address is looked up once per device.  The
+// trailing update is the hand-written k_zgemm (a library GEMM would allocate its handle).
+namespace {
+constexpr int NOALLOC_NMAX = 32768;   // largest order served from the static array
+constexpr size_t NOALLOC_BYTES = PANEL_WS_BYTES + (size_t)NOALLOC_NMAX * sizeof(int);   // 668 KiB
+constexpr int NOALLOC_MAXDEV = 16;
+}  // namespace
+
+// In host code `scratch` names a host-side stand-in: only the address from hipGetSymbolAddress
+// (per device, below) is passed to kernels.
+static __device__ __align__(256) char scratch[NOALLOC_BYTES];
+
+namespace {
+char *g_scratch_dev[NOALLOC_MAXDEV] = {};   // per device: the array's device address
+std::mutex g_scratch_mu;
+
+char *scratch_for_current_device()
+{
+	int dev = 0;
+	if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+	if ((dev < 0) || (dev >= NOALLOC_MAXDEV)) return nullptr;
+	std::lock_guard<std::mutex> lk(g_scratch_mu);
+	if (g_scratch_dev[dev] == nullptr) {
+		void *ptr = nullptr;
+		if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+		g_scratch_dev[dev] = (char *)ptr;
+	}
+	return g_scratch_dev[dev];
+}
+}  // namespace
+
+extern "C" int lu_decomp_device_noalloc(int n, d_complex_t *dA, int lda, int *dpiv, void *stream)
+{
+	if (n <= 0) return -1;
+	if ((n > NOALLOC_NMAX) || (lda < n) || (dA == nullptr) || (dpiv == nullptr)) {
+		fprintf(stderr, "*** lu_decomp_device_noalloc: n = %d (at most %d), lda = %d: invalid arguments\n", n, NOALLOC_NMAX,
+		        lda);
+		return -2;
+	}
+	char *base = scratch_for_current_device();
+	if (base == nullptr) {
+		fprintf(stderr, "*** lu_decomp_device_noalloc: the scratch array's device address is unavailable\n");
+		return -2;
+	}
+	ensure_attr();
+	const panel_ws ws = carve_panel_ws(base);
+	int *perm = (int *)(base + PANEL_WS_BYTES);
+	return lu_factor_ws<double2>((hipStream_t)stream, n, (double2 *)dA, lda, dpiv, nullptr, nullptr, ws, perm, true);
 }
