@@ -1,0 +1,88 @@
+"""The disk + quad compressed workload at a reduced size (disk 12 rings x 24 sectors under a
+12 x 6 quad, 21 points 100 MHz - 4 GHz), through the C ABI, against the reference's own solve()
+on the same decks (tests/golden/c4_vectors.npz, make_c4_golden.py):
+
+* dense path (compression = 0): the plain device form within 1e-9 of the reference's dense Z,
+  the accelerated form within 1e-6 (the metric's bar) of it;
+* compressed path at ACA tolerance 1e-10: the device's compressed sweep (the reference's
+  algorithm) and the learned form (preconditioner kept across points, the network's
+  correction; GMRES's true residual decides convergence) within 1e-6 of the dense Z and within
+  1e-8 of the reference's compressed Z;
+* compressed path at ACA tolerance 1e-6: the truncation itself puts the reference's compressed Z
+  up to 1.7e-4 from its dense Z; the device's compressed Z is held to the same truncation level
+  (per point within 10x the reference's own distance from dense, plus 1e-7).
+"""
+import numpy as np
+import pytest
+
+from conftest import DECKS
+from openpeec_amd import PEEC_FORM_LEARNED, PEEC_FORM_MIXED, PEEC_FORM_PLAIN, Peec, decks
+from refharness import rel_err
+
+pytestmark = pytest.mark.gpu
+
+GOLD = np.load(DECKS.parent / "c4_vectors.npz")
+C4_SMALL = dict(nr=12, nsec=24, qx=12, qy=6, f0=1e8, f1=4e9, ndiv=20)
+NP_LPC = 450
+
+
+def _deck(tol):
+    text = decks.disk_quad_deck(ctol=1e-6 if tol is None else tol, **C4_SMALL)
+    return text.replace("compression = 1 1e-06", "compression = 0") if tol is None else text
+
+
+def _per_point(z, zr):
+    return np.abs(z[:, 0, 0] - zr[:, 0, 0]) / np.abs(zr[:, 0, 0])
+
+
+def _sweep(lib, text, form):
+    m = Peec.from_deck_text(text, lib)
+    m.setup()
+    z, st = m.sweep(0, m.nfreq, form)
+    m.close()
+    return z, st
+
+
+@pytest.mark.parametrize("form,tol", [(PEEC_FORM_PLAIN, 1e-9), (PEEC_FORM_MIXED, 1e-6)])
+def test_dense_forms_match_reference_dense(lib, form, tol):
+    print(f"[c4] dense form {form}", flush=True)
+    lib.peec_sweep_reset()
+    z, _ = _sweep(lib, _deck(None), form)
+    e = rel_err(z, GOLD["c4/dense/zmat"])
+    assert e <= tol, e
+
+
+def test_compressed_tight_aca_meets_the_bar(lib):
+    print("[c4] compressed, ACA 1e-10", flush=True)
+    z, st = _sweep(lib, _deck(1e-10), PEEC_FORM_PLAIN)
+    assert rel_err(z, GOLD["c4/dense/zmat"]) <= 1e-6, rel_err(z, GOLD["c4/dense/zmat"])
+    assert rel_err(z, GOLD["c4/comp_1e-10/zmat"]) <= 1e-8, rel_err(z, GOLD["c4/comp_1e-10/zmat"])
+    assert st.gmres_its > 0
+
+
+@pytest.mark.parametrize("with_model", [False, True])
+def test_learned_tight_aca_meets_the_bar(lib, with_model):
+    """Without a model (theta = 0: precond.c's preconditioner reused across points) and with an
+    arbitrary untrained network (theta far from the trained one): the answer is the same."""
+    import ctypes as C
+    print(f"[c4] learned, ACA 1e-10, model {with_model}", flush=True)
+    if with_model:
+        W = np.random.default_rng(3).standard_normal(NP_LPC) * 0.3
+        lib.peec_lpc_set_model(W.ctypes.data_as(C.POINTER(C.c_double)))
+    try:
+        z, st = _sweep(lib, _deck(1e-10), PEEC_FORM_LEARNED)
+    finally:
+        lib.peec_lpc_set_model(None)
+    assert rel_err(z, GOLD["c4/dense/zmat"]) <= 1e-6, rel_err(z, GOLD["c4/dense/zmat"])
+    assert rel_err(z, GOLD["c4/comp_1e-10/zmat"]) <= 1e-6
+    assert st.pc_builds >= 1
+
+
+def test_compressed_default_aca_is_truncation_limited_like_the_reference(lib):
+    print("[c4] compressed, ACA 1e-6", flush=True)
+    z, _ = _sweep(lib, _deck(1e-6), PEEC_FORM_PLAIN)
+    e_dev = _per_point(z, GOLD["c4/dense/zmat"])
+    e_ref = _per_point(GOLD["c4/comp_1e-06/zmat"], GOLD["c4/dense/zmat"])
+    print("device vs dense", np.array2string(e_dev, precision=1))
+    print("reference vs dense", np.array2string(e_ref, precision=1))
+    assert np.all(e_dev <= 10 * e_ref + 1e-7)
