@@ -1,31 +1,33 @@
 #!/usr/bin/env python3
 """Benchmark: frequency points per second of the full-wave PEEC path on one MI355X.
 
-Workload (the precision configuration): copper plate 100 x 100 mm, 35 um, 64 x 64 grid
-(8320 ribbon branches, 4225 charge cells, 12545 MNA unknowns), retardation and capacitance
-on, corner-to-corner port, sweep 10 MHz - 3 GHz in 201 points.  A step is one frequency point
-of that sweep, device resident from the cell arrays to Z(f).
+Headline workload (the full-stack configuration, the largest that runs on one GPU): ground plane
+0.1 mm below an FR4-like brick (eps_r 4.4, 60 x 60 x 1.6 mm, 20 x 20 x 4 cells) under a copper
+plate 60 x 60 mm, 35 um, on a 48 x 48 grid split into 4 volume layers; port from the plate's
+corner to ground; retarded + capacitive; 150 points f_k = k 20 MHz (24 780 current cells,
+4 581 capacitive cells, 29 361 MNA unknowns).  The timed region is that whole sweep, cold (the
+k-factored fill records, factorisations and inversions are inside it), split into K steps:
+step i is a contiguous run of the sweep's points (whole lock-step batches where K allows), so
+ms_per_step = wall time / K and value = 150 / wall time.  W warm-up points run first and the
+cached preconditioner and records are dropped before timing.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--form mixed|plain] [--batch B] [--no-cpu]
-    python bench.py --only batch | learned | fullstack     (one side workload alone)
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu] ...
+    python bench.py --only batch | learned | plate     (one side workload alone)
 
-Forms.  mixed (default): the sweep's points 0..K-1 in one accelerated sweep -- k-factored Lp/P
-fill, charge-unknown system, an LU of A(f_ref) on FP32 storage (FP64 panels and pivoting, FP32
-MFMA trailing updates) whose factors are inverted in FP32, FP64 flexible GMRES to a 1e-10
-relative residual in lock-step batches of B frequencies, a fresh factorisation whenever the
-extra iterations have cost about as much as one; the LUs, the inversions and the fill
-preparation run inside the timed region (cold start).  plain: every
-point independently (fill, P LU and inverse, assembly, FP64 LU, solve), K points spread over
-the sweep.  Either way value = K / (wall time of the K points).
-
-Zin of the mixed form is compared with the plain FP64 form on three of the timed points (the
-metric is quoted at Zin rel err <= 1e-6); the plain form's own rate on those points is reported
-beside it.  The roofline block is the sweep's dominant kernel, k_mvsym (the dense blocks of the
-GMRES product A x from their upper triangles, HBM bound), timed with HIP events on the library's stream around every launch;
-the LU trailing update (rocBLAS cgemm on FP32 MFMA) is reported the same way in lu_update.  The precision form (FP32-stored LU, FP32 MFMA trailing updates, FP64 refinement)
-runs on the same three points and reports its rate and its Zin error against the plain form per
-point.  cpu_baseline times the oracle (the reference's arithmetic restated in C, OpenMP on
-all host cores) on a bounded sample and extrapolates to one frequency point.
+Form: the accelerated sweep -- k-factored Lp/P fill, charge-unknown system, an LU of A(f_ref) on
+FP32 storage (FP64 panels and pivoting, FP32 MFMA trailing updates) whose factors are inverted in
+FP32, FP64 flexible GMRES to a 1e-10 relative residual in lock-step batches of B frequencies, a
+fresh factorisation when the extra iterations have cost about as much as one.  Zin of the
+first, middle and last points is compared with the plain FP64 form (the reference's algorithm:
+element fill, P^-1, dense FP64 LU); the run fails when the error exceeds 1e-6.  The precision
+form (FP32-stored LU with FP64 refinement) runs on the same points.  roofline: the kernel with
+the most GPU time in the timed window among the sweep's two heavy kernels -- the GMRES product
+k_mvsym (HBM bound) and the LU trailing update (FP32 MFMA bound) -- timed with HIP events on the
+library's stream around every launch; the other is reported beside it.  Side objects: the
+precision workload (64 x 64 plate, 201 points), the batching workload (256 random wire
+structures x 64 points), the learned-preconditioner workload (disk + quad, compressed, 101
+points).  cpu_baseline times the oracle (the reference's arithmetic restated in C, OpenMP on
+all host cores) on a bounded sample of one full-stack point and extrapolates.
 """
 from __future__ import annotations
 
@@ -52,30 +54,34 @@ PEAK_HBM_GBS = 8000.0     # GB/s, MI355X HBM3E
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="points timed (default: 201 mixed, 3 plain)")
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--form", choices=["plain", "mixed"], default="mixed")
-    ap.add_argument("--batch", type=int, default=4, help="mixed form: frequencies per lock-step batch (1..8)")
-    ap.add_argument("--grid", type=int, default=64, help="plate grid (64 = the precision workload)")
+    ap.add_argument("--steps", type=int, default=20, help="steps the full-stack sweep is split into (all timed)")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed warm-up points before the cold sweep")
+    ap.add_argument("--batch", type=int, default=4, help="frequencies per lock-step batch (1..8)")
+    ap.add_argument("--grid", type=int, default=64, help="plate grid of the precision workload (64)")
+    ap.add_argument("--no-plate", action="store_true", help="skip the precision workload (64 x 64 plate)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--no-check", action="store_true", help="skip the small-deck oracle check")
     ap.add_argument("--no-batch", action="store_true", help="skip the many-structure batching workload")
     ap.add_argument("--no-compare", action="store_true",
-                    help="accelerated form only, without the plain / precision comparison points (profiling runs)")
+                    help="headline sweep only, without the plain / precision comparison points (profiling runs)")
     ap.add_argument("--batch-structs", type=int, default=256, help="structures of the batching workload")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner workload")
-    ap.add_argument("--learned-points", type=int, default=21,
-                    help="frequency points of the learned workload (21: every fifth of its 101)")
-    ap.add_argument("--only", choices=["batch", "learned", "fullstack"], default=None,
-                    help="run one side workload alone (fullstack: the full-stack configuration, not in the default run)")
-    ap.add_argument("--fullstack-points", type=int, default=150, help="points of the full-stack sweep")
+    ap.add_argument("--learned-points", type=int, default=101, help="frequency points of the learned workload (101)")
+    ap.add_argument("--aca-tol", type=float, default=1e-12,
+                    help="ACA tolerance of the learned workload (1e-12: Zin within 1e-6 of dense over the whole band; "
+                         "at 1e-6 and 1e-10 the truncation alone exceeds that)")
+    ap.add_argument("--only", choices=["batch", "learned", "plate"], default=None,
+                    help="run one side workload alone")
+    ap.add_argument("--fullstack-points", type=int, default=150, help="points of the full-stack sweep (150)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (calibrates the HBM byte counters)")
     return ap.parse_args()
 
 
-def cpu_baseline(m, budget_s: float = 20.0) -> dict:
-    """Oracle timing on a bounded sample of one C2 frequency point, extrapolated."""
+def cpu_baseline(m, f: float, label: str) -> dict:
+    """Oracle timing on a bounded sample of one frequency point of the workload in m (the plain
+    form's stages: Lp and charge-cell fills, P LU and inverse, MNA LU, port solve), extrapolated
+    to the whole point by pair count and by n^3 / n^2."""
     sys.path.insert(0, str(ROOT / "tests"))
     from oracle_py import oracle_lib
     from openpeec_amd import abi
@@ -83,7 +89,6 @@ def cpu_baseline(m, budget_s: float = 20.0) -> dict:
     orc = oracle_lib()
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     p = m.p
-    f = 3e9
     kw = 2 * np.pi * f / 2.99792458e8
     t_total = 0.0
     parts = {}
@@ -100,8 +105,8 @@ def cpu_baseline(m, budget_s: float = 20.0) -> dict:
             sampled += sum(n - i for i in range(s0, s0 + rows))
         total_pairs = n * (n + 1) // 2
         parts["lp_fill_s" if pot == 0 else "p_fill_s"] = t * total_pairs / sampled
-    # dense LU (12545) and P^-1 (LU + ncell unit-column solves at 4225) from a 3072 sample,
-    # large enough (151 MB) to stream from DRAM like the full-size factorisation
+    # dense LU and P^-1 (LU + ncell unit-column solves) from a 3072 sample, large enough
+    # (151 MB) to stream from DRAM like the full-size factorisation
     ns = 3072
     rng = np.random.default_rng(1)
     a = (rng.standard_normal((ns, ns)) + 1j * rng.standard_normal((ns, ns))).astype(np.complex128)
@@ -124,10 +129,10 @@ def cpu_baseline(m, budget_s: float = 20.0) -> dict:
     t_total = sum(parts.values())
     return {
         "value": 1.0 / t_total, "unit": "freq points/s", "cores": threads, "kind": "port",
-        "sample": (f"oracle (reference arithmetic, bitwise-equal on the sample decks) at 3 GHz: "
+        "sample": (f"oracle (reference arithmetic, bitwise-equal on the sample decks) at {f / 1e9:g} GHz: "
                    f"3 x 24 rows of the Lp and charge-cell triangles, LU and 16 solves of a {ns}^2 "
                    f"complex matrix; extrapolated by pair count and n^3 / n^2 to one point of the "
-                   f"64 x 64 plate (n = {n}, ncell = {nc})"),
+                   f"{label} (n = {n}, ncell = {nc})"),
         "seconds_per_point": t_total, "parts_s": {k: round(v, 3) for k, v in parts.items()},
     }
 
@@ -196,43 +201,47 @@ def precision_points(m, idx, zp):
     }
 
 
-def lu_roofline(achieved: float) -> dict:
-    """Roofline of the accelerated sweep's dominant kernel, the trailing update of the LU of
-    A(f_ref): FP32 storage by default (rocBLAS cgemm on FP32 MFMA; the hand-written k_cgemm32 with
-    PEEC_GEMM=hip), FP64 with PEEC_MIXED_LU64=1 (rocBLAS zgemm / k_zgemm on FP64 MFMA)."""
+def lu_roofline(achieved: float, workload: str) -> dict:
+    """Roofline of the trailing update of the accelerated form's LU of A(f_ref): FP32 storage by
+    default (rocBLAS cgemm on FP32 MFMA; the hand-written k_cgemm32 with PEEC_GEMM=hip), FP64 with
+    PEEC_MIXED_LU64=1 (rocBLAS zgemm / k_zgemm on FP64 MFMA).  Traffic from the workload's profile
+    (profiles/<kernel>_<workload>.md) when one was taken."""
     hip = os.environ.get("PEEC_GEMM", "") == "hip"
     if os.environ.get("PEEC_MIXED_LU64", "0") == "1":
         peak, alg = PEAK_FP64_MFMA, "16 (2 M N + (M + N) K) B"
         name = "k_zgemm" if hip else "rocBLAS zgemm"
-        prof = "k_zgemm_plate64_mixed_sweep.md" if hip else None
+        prof = f"{'k_zgemm' if hip else 'zgemm_lu_update'}_{workload}.md"
         name += " (LU trailing update of A(f_ref), FP64 MFMA)"
     else:
         peak, alg = PEAK_FP32_MFMA, "8 (2 M N + (M + N) K) B"
         name = "k_cgemm32" if hip else "rocBLAS cgemm"
-        prof = "k_cgemm32_plate64_mixed_sweep.md" if hip else "cgemm_lu_update_plate64_mixed_sweep.md"
+        prof = f"{'k_cgemm32' if hip else 'cgemm_lu_update'}_{workload}.md"
         name += " (LU trailing update of A(f_ref), FP32 MFMA)"
+    traffic = lu_traffic(prof)
     return {"kernel": name, "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-            "frac": achieved / peak, "traffic": lu_traffic(prof) if prof else None,
+            "frac": achieved / peak, "traffic": traffic,
             "traffic_note": (f"HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a copy of known size, "
-                             f"profiles/{prof}); algorithmic {alg}") if prof else f"not profiled; algorithmic {alg}"}
+                             f"profiles/{prof}); algorithmic {alg}") if traffic else f"not profiled here; algorithmic {alg}"}
 
 
-def mv_roofline(gbs: float | None, launches: int, avg_ms: float, avg_bytes: float) -> dict:
-    """Roofline of the accelerated sweep's dominant kernel (most GPU time per sweep): the dense
-    blocks of the GMRES product A x (Lp and P of every system of a batch, FP64), HBM bound --
-    k_mvsym, which reads the upper triangles only (k_mvdense, both triangles, with
+def mv_roofline(gbs: float | None, launches: int, avg_ms: float, avg_bytes: float, prof: str, what: str) -> dict:
+    """Roofline of the GMRES product's dense blocks (Lp and P of every system of a batch, FP64),
+    HBM bound -- k_mvsym, which reads the upper triangles only (k_mvdense, both triangles, with
     PEEC_MV_SYM=0); timed with HIP events on the library's stream around every launch of the
     timed sweep."""
     sym = os.environ.get("PEEC_MV_SYM", "1") != "0"
-    prof = "k_mvsym_plate64_mixed_sweep.md" if sym else "k_mvdense_plate64_mixed_sweep.md"
-    name = ("k_mvsym (GMRES product A x: upper triangles of the symmetric Lp and P of 4 systems, FP64)" if sym
-            else "k_mvdense (GMRES product A x, dense Lp and P blocks of 4 systems, FP64)")
+    if not sym:
+        prof = prof.replace("k_mvsym", "k_mvdense")
+    name = (f"k_mvsym (GMRES product A x: upper triangles of the symmetric Lp and P {what}, FP64)" if sym
+            else f"k_mvdense (GMRES product A x, dense Lp and P blocks {what}, FP64)")
+    traffic = lu_traffic(prof)
     return {"kernel": name, "bound": "hbm",
             "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": (gbs / PEAK_HBM_GBS) if gbs else None,
-            "traffic": lu_traffic(prof), "launches": launches, "avg_launch_ms": avg_ms, "avg_launch_bytes": avg_bytes,
-            "traffic_note": f"HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a copy of known size, "
-                            f"profiles/{prof}); algorithmic 16 B per entry read of Lp and P per system (upper "
-                            f"triangle with whole diagonal tiles for k_mvsym), x once, partial sums written"}
+            "traffic": traffic, "launches": launches, "avg_launch_ms": avg_ms, "avg_launch_bytes": avg_bytes,
+            "traffic_note": (f"HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a copy of known size, "
+                             f"profiles/{prof}); " if traffic else "not profiled here; ") +
+                            "algorithmic 16 B per entry read of Lp and P per system (upper triangle with whole "
+                            "diagonal tiles for k_mvsym), x once, partial sums written"}
 
 
 def lu_traffic(name: str) -> float | None:
@@ -334,12 +343,16 @@ LPC_TRAIN_GEOMS = [dict(nr=8, nsec=16, qx=8, qy=4), dict(nr=10, nsec=20, qx=10, 
 LPC_TRAIN_PAIRS = [(fr, fr * s) for fr in (4e8, 1e9, 2e9) for s in (1.0, 1.3, 1.8, 2.5)]
 
 
-def bench_learned(lib, points: int = 101) -> dict:
-    """The learned-preconditioner workload: disk 48 x 32 under a 32 x 16 quad, compressed, swept
-    over `points` of 100 MHz - 4 GHz three ways: the reference's rule (precond.c's preconditioner
-    rebuilt at every frequency, reference GMRES termination), the same preconditioner reused
-    across frequencies without the network (theta = 0), and with the network trained on the
-    device on small geometries (training time reported separately)."""
+def bench_learned(lib, points: int = 101, aca_tol: float = 1e-12) -> dict:
+    """The learned-preconditioner workload: disk 48 x 32 under a 32 x 16 quad, compressed at ACA
+    tolerance aca_tol, swept over `points` of 100 MHz - 4 GHz three ways: the reference's rule
+    (precond.c's preconditioner rebuilt at every frequency, reference GMRES termination), the
+    same preconditioner reused across frequencies without the network (theta = 0), and with the
+    network trained on the device on small geometries (training time reported separately).  Zin
+    of every point is compared with the same geometry uncompressed (accelerated dense form, itself
+    checked against the plain dense FP64 form on 5 points across the band).  The truncation alone
+    leaves the reference's own compressed solve up to 1.7e-4 from dense at ACA 1e-6 (reduced
+    deck), and 4.1e-6 from its ACA-1e-12 answer at 1e-10 (this deck at 4 GHz, |Zin| = 6.6 ohm)."""
     import ctypes as C
     from openpeec_amd import PEEC_FORM_LEARNED, PEEC_FORM_PLAIN, Peec, decks
     PD = C.POINTER(C.c_double)
@@ -368,11 +381,11 @@ def bench_learned(lib, points: int = 101) -> dict:
     lib.peec_lpc_fit(X.ctypes.data_as(PD), coef.ctypes.data_as(PD), S, 1500, 3e-3, 7, W.ctypes.data_as(PD),
                      hist.ctypes.data_as(PD))
     t_fit = time.perf_counter() - t0
-    text = decks.disk_quad_deck(ndiv=points - 1)
+    text = decks.disk_quad_deck(ndiv=points - 1, ctol=aca_tol)
     m = Peec.from_deck_text(text, lib)
     m.setup()
     out = {"workload": f"disk r = 50 mm, 48 rings x 32 sectors, under a 80 x 40 mm quad (32 x 16) 5 mm above, "
-                       f"compressed (ACA 1e-6), retarded + capacitive, {points} points 100 MHz-4 GHz",
+                       f"compressed (ACA {aca_tol:g}), retarded + capacitive, {points} points 100 MHz-4 GHz",
            "unknowns": m.p.nunknown, "branches": m.p.nseg, "charge_cells": m.p.ncell,
            "training": {"samples": S, "geometries": len(LPC_TRAIN_GEOMS), "sample_s": t_samples, "fit_s": t_fit,
                         "loss_first": float(hist[0]), "loss_last": float(hist[-1])}}
@@ -413,70 +426,195 @@ def bench_learned(lib, points: int = 101) -> dict:
     from openpeec_amd import PEEC_FORM_MIXED
     dm = Peec.from_deck_text(re.sub(r"compression = 1 \S+", "compression = 0", text), lib)
     dm.setup()
-    kp = min(points, 5)
+    pidx = sorted({int(round(x)) for x in np.linspace(0, points - 1, min(points, 5))})
     dm.sweep(0, points, PEEC_FORM_MIXED)
     t0 = time.perf_counter()
     z_m, _ = dm.sweep(0, points, PEEC_FORM_MIXED)
     t_m = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    z_p, _ = dm.sweep(0, kp, PEEC_FORM_PLAIN)
-    t_p = time.perf_counter() - t0
+    z_p, t_p, _ = plain_points(dm, pidx)
     rel = lambda a, b: float(np.max(np.abs(a - b) / np.abs(b)))
+    per_point = np.abs(z_l[:, 0, 0] - z_m[:, 0, 0]) / np.abs(z_m[:, 0, 0])
     out["uncompressed_same_geometry"] = {
-        "unknowns": dm.p.nunknown, "accelerated_points_per_s": points / t_m, "plain_fp64_points_per_s": kp / t_p,
-        "plain_points": kp, "accelerated_vs_plain_fp64": (points / t_m) / (kp / t_p),
+        "unknowns": dm.p.nunknown, "accelerated_points_per_s": points / t_m, "plain_fp64_points_per_s": len(pidx) / t_p,
+        "plain_points_hz": [dm.freq(i) for i in pidx], "accelerated_vs_plain_fp64": (points / t_m) / (len(pidx) / t_p),
         "accelerated_vs_compressed_reference_rule": (points / t_m) / out["reference_rule"]["points_per_s"],
-        "zin_max_rel_err": {"accelerated_vs_plain_fp64": rel(z_m[:kp], z_p),
-                            "compressed_vs_plain_fp64": rel(z_r[:kp], z_p),
-                            "compressed_vs_accelerated_all_points": rel(z_r, z_m)}}
-    log(f"learned: same geometry uncompressed, accelerated {points / t_m:.1f} points/s, plain FP64 {kp / t_p:.2f} points/s")
+        "zin_max_rel_err": {"accelerated_vs_plain_fp64": rel(z_m[pidx, 0, 0], z_p),
+                            "learned_vs_plain_fp64": rel(z_l[pidx, 0, 0], z_p),
+                            "compressed_reference_rule_vs_accelerated_all_points": rel(z_r, z_m),
+                            "learned_vs_accelerated_all_points": float(per_point.max())}}
+    out["zin_check"] = {"learned_vs_dense_per_point_max": float(per_point.max()),
+                        "worst_point_hz": dm.freq(int(np.argmax(per_point))), "bar": 1e-6,
+                        "ok": bool(per_point.max() <= 1e-6 and rel(z_m[pidx, 0, 0], z_p) <= 1e-6)}
+    log(f"learned: same geometry uncompressed, accelerated {points / t_m:.1f} points/s, plain FP64 {len(pidx) / t_p:.2f} points/s")
     dm.close()
     return out
 
 
-def bench_fullstack(lib, npoints: int = 150) -> dict:
-    """The full-stack configuration (ground plane, 4-layer volume plate over a dielectric brick,
-    n = 29 361 unknowns): the accelerated form over npoints points of its 20 MHz-step sweep (cold,
-    one warm-up point first), the plain FP64 and the precision form at its first and last points,
-    and each form's Zin error against the plain form there."""
-    from openpeec_amd import PEEC_FORM_MIXED, PEEC_FORM_PLAIN, Peec, decks
+def sweep_chunks(npts: int, k: int, batch: int) -> list[tuple[int, int]]:
+    """The sweep's points 0..npts-1 as k contiguous runs (i0, count), whole lock-step batches of
+    `batch` points each where k allows (the last run takes the remainder)."""
+    nbat = (npts + batch - 1) // batch
+    if k <= nbat:
+        sizes = [(nbat // k + (1 if i < nbat % k else 0)) * batch for i in range(k)]
+    else:
+        k = min(k, npts)
+        sizes = [npts // k + (1 if i < npts % k else 0) for i in range(k)]
+    out, i0 = [], 0
+    for sz in sizes:
+        c = min(sz, npts - i0)
+        if c > 0:
+            out.append((i0, c))
+        i0 += c
+    return out
+
+
+def headline_roofline(st, workload: str = "fullstack_sweep", what: str = "of 4 systems of the full stack") -> tuple[dict, dict]:
+    """The timed window's two heavy kernels with their HIP-event times: the GMRES product
+    k_mvsym (HBM) and the LU trailing update (rocBLAS cgemm on FP32 MFMA, k_cgemm32 with
+    PEEC_GEMM=hip); returns (dominant, other)."""
+    mv_gbs = (st.mv_bytes / (st.mv_ms * 1e-3)) / 1e9 if st.mv_ms > 0 else None
+    mv = mv_roofline(mv_gbs, st.mv_launches, st.mv_ms / max(1, st.mv_launches), st.mv_bytes / max(1, st.mv_launches),
+                     f"k_mvsym_{workload}.md", what)
+    mv["total_ms_in_timed_window"] = st.mv_ms
+    gemm_ms = st.lu_gemm_ms / max(1, st.lu_gemm_launches)
+    gemm_flop = st.lu_gemm_flop / max(1, st.lu_gemm_launches)
+    achieved = (gemm_flop / (gemm_ms * 1e-3)) / 1e12 if gemm_ms > 0 else 0.0
+    lu = lu_roofline(achieved, workload) | {"launches": st.lu_gemm_launches, "avg_launch_ms": gemm_ms,
+                                                       "avg_launch_flop": gemm_flop,
+                                                       "total_ms_in_timed_window": st.lu_gemm_ms}
+    return (mv, lu) if st.mv_ms >= st.lu_gemm_ms else (lu, mv)
+
+
+def bench_headline(lib, args) -> dict:
+    """The full-stack workload swept cold in K steps (see the module docstring)."""
+    from openpeec_amd import PEEC_FORM_MIXED, Peec, decks
     t0 = time.perf_counter()
     m = Peec.from_deck_text(decks.full_stack_deck(), lib)
     m.setup()
     log(f"full stack: {m.p.nseg} current cells, {m.p.ncell} capacitive cells, {m.p.nunknown} unknowns "
         f"({time.perf_counter() - t0:.1f} s setup)")
     nf = m.p.nfreq
-    npoints = max(2, min(npoints, nf))
-    m.sweep(0, 1, PEEC_FORM_MIXED, 4)
+    npts = max(2, min(args.fullstack_points, nf))
+    chunks = sweep_chunks(npts, max(1, args.steps), args.batch)
+    if args.warmup > 0:
+        m.sweep(0, min(args.warmup, npts), PEEC_FORM_MIXED, args.batch)
     lib.peec_sweep_reset()
+    z = np.zeros((npts, m.p.nport, m.p.nport), dtype=np.complex128)
+    tot = None
+    step_ms = []
     t0 = time.perf_counter()
-    z, st = m.sweep(0, npoints, PEEC_FORM_MIXED, 4)
+    for (i0, c) in chunks:
+        ts = time.perf_counter()
+        zc, st = m.sweep(i0, c, PEEC_FORM_MIXED, args.batch)
+        step_ms.append(1e3 * (time.perf_counter() - ts))
+        z[i0:i0 + c] = zc
+        tot = st if tot is None else add_stats(tot, st)
     elapsed = time.perf_counter() - t0
-    log(f"full stack: accelerated sweep of {npoints} points {elapsed:.1f} s ({st.n_lu} factorisations, "
-        f"{st.gmres_its} GMRES iterations)")
-    idx = [0, npoints - 1]
+    log(f"full stack: cold sweep of {npts} points in {len(chunks)} steps, {elapsed:.1f} s ({tot.n_lu} factorisations, "
+        f"{tot.gmres_its} GMRES iterations)")
+    if args.no_compare:
+        roof, other = headline_roofline(tot)
+        return {"value": npts / elapsed, "unit": "freq points/s", "steps": len(chunks),
+                "ms_per_step": 1e3 * elapsed / len(chunks), "step_ms": step_ms, "n_lu": tot.n_lu,
+                "gmres_its_per_point": tot.gmres_its / npts, "zin_check": {"skipped": True, "ok": True},
+                "stages_ms": {"fill_incl_kf_setup": tot.fill_ms, "kf_setup": tot.kf_setup_ms, "lu_and_inverse": tot.lu_ms,
+                              "gmres": tot.solve_ms, "total": tot.total_ms},
+                "roofline": roof, "second_kernel": other}, m
+    idx = [0, npts // 2, npts - 1]
     zp, t_plain, stp = plain_points(m, idx)
     log(f"full stack: plain FP64 form {t_plain / len(idx):.1f} s per point")
     prec = precision_points(m, idx, zp)
     zm = np.array([z[i, 0, 0] for i in idx])
-    err = float(np.max(np.abs(zm - zp) / np.abs(zp)))
+    errs = np.abs(zm - zp) / np.abs(zp)
+    err = float(np.max(errs))
+    fill_flop_top = lib.peec_fill_flops(C.byref(m.p), m.freq(idx[-1]))
+    plain_fill_ms = stp.fill_ms + stp.pot_ms
+    roof, other = headline_roofline(tot)
     out = {
-        "workload": "ground plane 0.1 mm under an FR4 brick (eps_r 4.4, 60 x 60 x 1.6 mm, 20 x 20 x 4 cells); "
-                    "copper plate 60 x 60 mm, 35 um, 48 x 48 grid in 4 volume layers on the brick; port plate "
-                    f"corner to ground; retarded + capacitive; points 20 MHz + k 20 MHz, k = 0..{npoints - 1}",
-        "current_cells": m.p.nseg, "capacitive_cells": m.p.ncell, "unknowns": m.p.nunknown,
-        "accelerated": {"points": npoints, "s": elapsed, "points_per_s": npoints / elapsed, "n_lu": st.n_lu,
-                        "gmres_its_per_point": st.gmres_its / npoints, "kf_direct_pairs": st.kf_direct_pairs,
-                        "stages_ms": {"fill_incl_kf_setup": st.fill_ms, "kf_setup": st.kf_setup_ms,
-                                      "lu_and_inverse": st.lu_ms, "gmres": st.solve_ms},
-                        "zin_max_rel_err_vs_plain_fp64": err},
+        "value": npts / elapsed, "unit": "freq points/s", "n_gpus": 1, "steps": len(chunks), "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / len(chunks), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (fill, GMRES, residuals); preconditioner factors stored f32",
+        "data": "synthetic deck: full-stack geometry generated in-process (no dataset)",
+        "config": {
+            "workload": "full stack: ground plane 0.1 mm under an FR4 brick (eps_r 4.4, 60 x 60 x 1.6 mm, 20 x 20 x 4 "
+                        "cells); copper plate 60 x 60 mm, 35 um, 48 x 48 grid in 4 volume layers on the brick; port "
+                        f"plate corner to ground; retarded + capacitive; {npts} points f_k = k 20 MHz; one step = one "
+                        f"contiguous run of {npts / len(chunks):.1f} points on average (the whole sweep is timed)",
+            "unknowns": m.p.nunknown, "current_cells": m.p.nseg, "capacitive_cells": m.p.ncell, "form": "accelerated",
+            "batch": args.batch, "points_hz": [m.freq(0), m.freq(npts - 1)],
+            "step_points": [c for _, c in chunks]},
+        "zin_check": {"points_hz": [m.freq(i) for i in idx], "rel_err_vs_plain_fp64": [float(e) for e in errs],
+                      "max_rel_err_vs_plain_fp64": err, "bar": 1e-6, "ok": err <= 1e-6},
         "plain_fp64_points_per_s": len(idx) / t_plain,
-        "speedup_vs_plain_fp64": (npoints / elapsed) / (len(idx) / t_plain),
-        "plain_stages_ms_last_point": {"lp_fill": stp.fill_ms, "charge_fill": stp.pot_ms, "p_lu_and_inverse": stp.cinv_ms,
-                                       "mna_lu": stp.lu_ms, "total": stp.total_ms},
+        "speedup_vs_plain_fp64": (npts / elapsed) / (len(idx) / t_plain),
         "precision_form": prec,
-        "zin_points_hz": [m.freq(i) for i in idx],
+        "stages_ms": {"fill_incl_kf_setup": tot.fill_ms, "kf_setup": tot.kf_setup_ms, "lu_and_inverse": tot.lu_ms,
+                      "gmres": tot.solve_ms, "total": tot.total_ms},
+        "step_ms": step_ms,
+        "n_lu": tot.n_lu, "gmres_its_per_point": tot.gmres_its / npts, "kf_direct_pairs": tot.kf_direct_pairs,
+        "fill_nominal_gflop_per_point": lib.peec_fill_flops(C.byref(m.p), m.freq(npts - 1)) / 1e9,
+        "plain_fill": {"gflops": fill_flop_top / (plain_fill_ms * 1e-3) / 1e9,
+                       "frac_of_fp64_vector_peak": fill_flop_top / (plain_fill_ms * 1e-3) / 1e12 / PEAK_FP64_VECTOR,
+                       "note": "element-by-element FP64 fill (Lp and charge cells) of the plain form at 3 GHz"},
+        "plain_stages_ms_last_point": {"lp_fill": stp.fill_ms, "charge_fill": stp.pot_ms, "p_lu_and_inverse": stp.cinv_ms,
+                                       "mna_assembly": stp.asm_ms, "mna_lu": stp.lu_ms, "total": stp.total_ms},
+        "roofline": roof,
+        "second_kernel": other,
     }
+    return out, m
+
+
+def add_stats(a, b):
+    """Sum of two peec_stats_t of consecutive sweep calls (times, counts, flops, bytes)."""
+    from openpeec_amd import abi
+    r = abi.peec_stats_t()
+    for name, _ in abi.peec_stats_t._fields_:
+        va, vb = getattr(a, name), getattr(b, name)
+        if name in ("form", "ising", "kf_direct_pairs"):
+            setattr(r, name, vb)
+        elif name == "berr":
+            setattr(r, name, max(va, vb))
+        else:
+            setattr(r, name, va + vb)
+    return r
+
+
+def bench_plate(lib, grid: int = 64, batch: int = 4) -> dict:
+    """The precision workload (64 x 64 plate, 201 points 10 MHz - 3 GHz): the accelerated form
+    over the whole sweep, cold; the plain FP64 and precision forms at its first, middle and last
+    points, with the accelerated form's Zin error against the plain one."""
+    from openpeec_amd import PEEC_FORM_MIXED, Peec, decks
+    m = Peec.from_deck_text(decks.plate_deck(n=grid), lib)
+    m.setup()
+    nf = m.p.nfreq
+    m.sweep(0, 1, PEEC_FORM_MIXED, batch)
+    lib.peec_sweep_reset()
+    t0 = time.perf_counter()
+    z, st = m.sweep(0, nf, PEEC_FORM_MIXED, batch)
+    elapsed = time.perf_counter() - t0
+    idx = [0, nf // 2, nf - 1]
+    zp, t_plain, stp = plain_points(m, idx)
+    prec = precision_points(m, idx, zp)
+    zm = np.array([z[i, 0, 0] for i in idx])
+    err = float(np.max(np.abs(zm - zp) / np.abs(zp)))
+    fill_flop_top = lib.peec_fill_flops(C.byref(m.p), m.freq(idx[-1]))
+    plain_fill_ms = stp.fill_ms + stp.pot_ms
+    roof, other = headline_roofline(st, "plate64_mixed_sweep", "of 4 systems of the 64 x 64 plate")
+    out = {
+        "workload": f"copper plate 100x100 mm, 35 um, {grid}x{grid} grid, retarded + capacitive, corner-to-corner port, "
+                    f"10 MHz-3 GHz sweep of {nf} points, accelerated form, cold",
+        "unknowns": m.p.nunknown, "points": nf, "s": elapsed, "points_per_s": nf / elapsed,
+        "zin_max_rel_err_vs_plain_fp64": err, "plain_fp64_points_per_s": len(idx) / t_plain,
+        "speedup_vs_plain_fp64": (nf / elapsed) / (len(idx) / t_plain), "precision_form": prec,
+        "stages_ms": {"fill_incl_kf_setup": st.fill_ms, "kf_setup": st.kf_setup_ms, "lu_and_inverse": st.lu_ms,
+                      "gmres": st.solve_ms, "total": st.total_ms},
+        "n_lu": st.n_lu, "gmres_its_per_point": st.gmres_its / nf,
+        "plain_fill": {"gflops": fill_flop_top / (plain_fill_ms * 1e-3) / 1e9,
+                       "frac_of_fp64_vector_peak": fill_flop_top / (plain_fill_ms * 1e-3) / 1e12 / PEAK_FP64_VECTOR},
+        "gmres_product": roof if roof["unit"] == "GB/s" else other,
+        "lu_update": roof if roof["unit"] == "TFLOP/s" else other,
+    }
+
     m.close()
     return out
 
@@ -504,7 +642,7 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU frequency sharding is "
                                    "described in DESIGN.md but not part of this build", "n_gpus": args.gpus}))
         sys.exit(2)
-    from openpeec_amd import PEEC_FORM_MIXED, PEEC_FORM_PLAIN, Peec, decks, load_library
+    from openpeec_amd import load_library
 
     lib = load_library(build_if_missing=True)
     if not lib.peec_device_available():
@@ -513,137 +651,35 @@ def main():
         print(json.dumps({"batching": bench_batch(lib, args.batch_structs)}))
         return
     if args.only == "learned":
-        print(json.dumps({"learned_preconditioner": bench_learned(lib, args.learned_points)}))
+        print(json.dumps({"learned_preconditioner": bench_learned(lib, args.learned_points, args.aca_tol)}))
         return
-    if args.only == "fullstack":
-        print(json.dumps({"full_stack": bench_fullstack(lib, args.fullstack_points)}))
+    if args.only == "plate":
+        print(json.dumps({"precision_workload": bench_plate(lib, args.grid, args.batch)}))
         return
-    text = decks.plate_deck(n=args.grid)
-    m = Peec.from_deck_text(text, lib)
-    m.setup()
-    nf = m.p.nfreq
-    n = m.p.nunknown
-    mixed = args.form == "mixed"
-    steps = args.steps if args.steps is not None else (nf if mixed else 3)
-    steps = max(1, min(steps, nf))
     result = {"metric": METRIC}
-    if mixed:
-        # warm-up (kernel loading, allocations), then a cold sweep: preconditioner and fill
-        # records are dropped so the timed region rebuilds them
-        if args.warmup > 0:
-            m.sweep(0, min(args.warmup, nf), PEEC_FORM_MIXED, args.batch)
-        lib.peec_sweep_reset()
-        t0 = time.perf_counter()
-        z, st = m.sweep(0, steps, PEEC_FORM_MIXED, args.batch)
-        elapsed = time.perf_counter() - t0
-        idx = sorted({0, steps // 2, steps - 1})
-        zm = np.array([z[i, 0, 0] for i in idx])
-        if args.no_compare:
-            print(json.dumps({"steps": steps, "ms_per_step": 1e3 * elapsed / steps, "value": steps / elapsed,
-                              "stages_ms": {"fill": st.fill_ms, "lu_and_inverse": st.lu_ms, "gmres": st.solve_ms},
-                              "gmres_its": st.gmres_its, "n_lu": st.n_lu,
-                              "k_zgemm_avg_launch_ms": st.lu_gemm_ms / max(1, st.lu_gemm_launches),
-                              "k_zgemm_avg_launch_flop": st.lu_gemm_flop / max(1, st.lu_gemm_launches),
-                              "k_zgemm_launches": st.lu_gemm_launches,
-                              "mv_avg_launch_ms": st.mv_ms / max(1, st.mv_launches), "mv_launches": st.mv_launches,
-                              "mv_GB_per_s": (st.mv_bytes / (st.mv_ms * 1e-3)) / 1e9 if st.mv_ms > 0 else None}))
-            m.close()
-            return
-        zp, t_plain, stp = plain_points(m, idx)
-        prec = precision_points(m, idx, zp)
-        fill_flop_top = lib.peec_fill_flops(C.byref(m.p), m.freq(idx[-1]))
-        plain_fill_ms = stp.fill_ms + stp.pot_ms
-        err = float(np.max(np.abs(zm - zp) / np.abs(zp)))
-        gemm_ms = st.lu_gemm_ms / max(1, st.lu_gemm_launches)
-        gemm_flop = st.lu_gemm_flop / max(1, st.lu_gemm_launches)
-        achieved = (gemm_flop / (gemm_ms * 1e-3)) / 1e12 if gemm_ms > 0 else 0.0
-        mv_gbs = (st.mv_bytes / (st.mv_ms * 1e-3)) / 1e9 if st.mv_ms > 0 else None
-        fill_flop = lib.peec_fill_flops(C.byref(m.p), m.freq(steps - 1))
-        result.update({
-            "value": steps / elapsed, "unit": "freq points/s", "n_gpus": 1, "steps": steps,
-            "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / steps, "higher_is_better": True,
-            "scaling": "weak", "vs_baseline": None,
-            "dtype": "f64 (GMRES, residuals, fill); preconditioner factors stored f32",
-            "data": "synthetic deck: plate geometry generated in-process (no dataset)",
-            "config": {
-                "workload": f"copper plate 100x100 mm, 35 um, {args.grid}x{args.grid} grid, retarded + capacitive, "
-                            f"corner-to-corner port, 10 MHz-3 GHz sweep of {nf} points; one step = one frequency point",
-                "unknowns": n, "charge_unknowns": m.p.ncell, "branches": m.p.nseg, "form": "mixed",
-                "batch": args.batch, "points": [m.freq(0), m.freq(steps - 1)],
-            },
-            "zin_check": {"points_hz": [m.freq(i) for i in idx], "max_rel_err_vs_plain_fp64": err,
-                          "bar": 1e-6, "ok": err <= 1e-6},
-            "plain_fp64_points_per_s": len(idx) / t_plain,
-            "speedup_vs_plain_fp64": (steps / elapsed) / (len(idx) / t_plain),
-            "precision_form": prec,
-            "stages_ms": {"fill_incl_kf_setup": st.fill_ms, "kf_setup": st.kf_setup_ms,
-                          "lu_and_inverse": st.lu_ms, "gmres": st.solve_ms, "total": st.total_ms},
-            "n_lu": st.n_lu, "gmres_its_per_point": st.gmres_its / steps, "kf_direct_pairs": st.kf_direct_pairs,
-            "lu_update": (lu_roofline(achieved) | {
-                "launches": st.lu_gemm_launches, "avg_launch_ms": gemm_ms, "avg_launch_flop": gemm_flop,
-            }),
-            "fill_nominal_gflop_per_point": fill_flop / 1e9,
-            "plain_fill": {"gflops": fill_flop_top / (plain_fill_ms * 1e-3) / 1e9,
-                           "frac_of_fp64_vector_peak": fill_flop_top / (plain_fill_ms * 1e-3) / 1e12 / PEAK_FP64_VECTOR,
-                           "note": "element-by-element FP64 fill of the plain form at the last checked point"},
-            "kfactored_fill_ms_per_point": (st.fill_ms - st.kf_setup_ms) / steps,
-            "roofline": mv_roofline(mv_gbs, st.mv_launches, st.mv_ms / max(1, st.mv_launches),
-                                    st.mv_bytes / max(1, st.mv_launches)),
-        })
-    else:
-        form = PEEC_FORM_PLAIN
-        idx = [int(round(x)) for x in np.linspace(0, nf - 1, steps + 2)[1:-1]]
-        for w in range(args.warmup):
-            m.sweep(nf - 1 - w, 1, form)
-        t0 = time.perf_counter()
-        for i in idx:
-            m.sweep(i, 1, form)
-        elapsed = time.perf_counter() - t0
-        z_top, st = m.sweep(nf - 1, 1, form)
-        fill_flop = lib.peec_fill_flops(C.byref(m.p), m.freq(nf - 1))
-        fill_ms = st.fill_ms + st.pot_ms
-        gemm_avg_ms = st.lu_gemm_ms / max(1, st.lu_gemm_launches)
-        gemm_avg_flop = st.lu_gemm_flop / max(1, st.lu_gemm_launches)
-        achieved = (gemm_avg_flop / (gemm_avg_ms * 1e-3)) / 1e12 if gemm_avg_ms > 0 else 0.0
-        result.update({
-            "value": len(idx) / elapsed, "unit": "freq points/s", "n_gpus": 1, "steps": len(idx),
-            "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / len(idx), "higher_is_better": True,
-            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-            "data": "synthetic deck: plate geometry generated in-process (no dataset)",
-            "config": {
-                "workload": f"copper plate 100x100 mm, 35 um, {args.grid}x{args.grid} grid, retarded + capacitive, "
-                            f"corner-to-corner port, 10 MHz-3 GHz sweep; one step = one frequency point",
-                "unknowns": n, "branches": m.p.nseg, "charge_cells": m.p.ncell, "form": "plain",
-                "frequencies_timed_hz": [m.freq(i) for i in idx],
-            },
-            "stages_ms_top_freq": {
-                "lp_fill": st.fill_ms, "charge_fill": st.pot_ms, "p_lu_and_inverse": st.cinv_ms,
-                "mna_assembly": st.asm_ms, "mna_lu": st.lu_ms, "port_solve": st.solve_ms, "total": st.total_ms,
-            },
-            "fill_gflops": fill_flop / (fill_ms * 1e-3) / 1e9 if fill_ms > 0 else None,
-            "fill_frac_of_fp64_vector_peak": (fill_flop / (fill_ms * 1e-3) / 1e12) / PEAK_FP64_VECTOR if fill_ms > 0 else None,
-            "roofline": {
-                "kernel": ("k_zgemm" if os.environ.get("PEEC_GEMM", "") == "hip" else "rocBLAS zgemm")
-                          + " (LU trailing update, FP64 MFMA)",
-                "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_MFMA, "traffic": None,
-                "launches": st.lu_gemm_launches, "avg_launch_ms": gemm_avg_ms, "avg_launch_flop": gemm_avg_flop,
-            },
-        })
+    head, m = bench_headline(lib, args)
+    result.update(head)
     if not args.no_check:
         result["small_deck_check"] = small_check(lib)
+    if not args.no_plate:
+        log("precision workload (64 x 64 plate)")
+        result["precision_workload"] = bench_plate(lib, args.grid, args.batch)
     if not args.no_batch:
         result["batching"] = bench_batch(lib, args.batch_structs)
     if not args.no_learned:
-        result["learned_preconditioner"] = bench_learned(lib, args.learned_points)
+        result["learned_preconditioner"] = bench_learned(lib, args.learned_points, args.aca_tol)
     if not args.no_cpu:
         sys.path.insert(0, str(ROOT / "tests"))
         from oracle_py import oracle_peec
-        m_cpu = oracle_peec(text)
-        result["cpu_baseline"] = cpu_baseline(m_cpu)
+        from openpeec_amd import decks
+        m_cpu = oracle_peec(decks.full_stack_deck())
+        result["cpu_baseline"] = cpu_baseline(m_cpu, m.freq(m.p.nfreq - 1), "full stack (29 361 unknowns)")
         m_cpu.close()
     m.close()
     print(json.dumps(result))
+    if not result["zin_check"]["ok"]:
+        log(f"Zin check failed: {result['zin_check']['max_rel_err_vs_plain_fp64']:.3e} > 1e-6")
+        sys.exit(3)
 
 
 if __name__ == "__main__":

@@ -988,9 +988,9 @@ int lu_factor_ws(hipStream_t s, int n, TS *A, int lda, int *dpiv, int *dperm, pg
 	hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, s, perm, n);
 	hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, s, dpiv, n);
 	hipEvent_t e0 = nullptr, e1 = nullptr;
-	if (st) {
-		HIPCHK(hipEventCreate(&e0));
-		HIPCHK(hipEventCreate(&e1));
+	if (st) {   // timing-only events: no system-scope fence around the timed GEMM
+		HIPCHK(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence));
+		HIPCHK(hipEventCreateWithFlags(&e1, hipEventDisableSystemFence));
 	}
 	const int pmax = (g_lw.ncu < PANEL_PMAX) ? g_lw.ncu : PANEL_PMAX;
 	const long ldal = lda;

@@ -853,9 +853,11 @@ void matvec(fast &c, hipStream_t s, const double2 *x, double2 *y, int F, int nac
 {
 	if (g_mvt.on) {
 		if ((int)g_mvt.ev.size() < g_mvt.used + 2) {
+			// timing-only events: no system-scope fence (its cache writeback would be charged to
+			// the kernel between them)
 			hipEvent_t e0, e1;
-			HIPCHK(hipEventCreate(&e0));
-			HIPCHK(hipEventCreate(&e1));
+			HIPCHK(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence));
+			HIPCHK(hipEventCreateWithFlags(&e1, hipEventDisableSystemFence));
 			g_mvt.ev.push_back(e0);
 			g_mvt.ev.push_back(e1);
 		}
@@ -1406,7 +1408,9 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 		// (the middle of this batch for the first one, or under the iteration-ratio rule)
 		if ((c.fref < 0) || c.stale) {
 			if (c.served > 0) c.span = c.served;
-			const int k = ((c.span > 0) && (refactor_alpha() > 0)) ? std::min(nfreq - 1, f0 + c.span / 2) : f0 + nb / 2;
+			// (any point of the deck's sweep: a sweep called in runs of points still looks ahead)
+			const int last = std::max(nfreq - 1, p->nfreq - 1 - ifreq0);
+			const int k = ((c.span > 0) && (refactor_alpha() > 0)) ? std::min(last, f0 + c.span / 2) : f0 + nb / 2;
 			if (k < f0 + nb) {
 				rc = refactor(k - f0);
 			}
