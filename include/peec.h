@@ -279,6 +279,8 @@ void mna_sparse_free(mna_sparse_t *sp);                                         
 void mna_apply(const peec_t *p, const mna_sparse_t *sp, const hblock_t *blk, int nblk,
 	const d_complex_t *x, d_complex_t *y, d_complex_t *work);                     /* mna.c:345 */
 void mna_rhs_port(const peec_t *p, int iport, d_complex_t *b);                  /* mna.c:367 */
+void mna_rhs_sources(const peec_t *p, d_complex_t *b);                          /* mna.c:471 */
+void mna_rhs_planewave(const peec_t *p, double f, d_complex_t *b);              /* mna.c:433 */
 
 /* ---- leaf-block + circuit-Schur preconditioner (reference peec.h:358-364); device resident ---- */
 struct precond_t;
@@ -288,6 +290,8 @@ void pc_apply(const struct precond_t *pc, d_complex_t *b);                      
 double pc_memory_mb(const struct precond_t *pc);                                /* precond.c:435 */
 
 /* ---- iterative solvers (reference peec.h:366-371) ---- */
+int  gmres_solve(int n, const d_complex_t *a, const d_complex_t *alu, const int *piv,
+	d_complex_t *b, double tol);                                                  /* iterative.c:230 */
 int  gmres_hmat(const peec_t *p, const mna_sparse_t *sp, const hblock_t *blk, int nblk,
 	const struct precond_t *pc, d_complex_t *b, double tol);                      /* iterative.c:285 */
 
@@ -297,6 +301,16 @@ void lu_solve(int n, const d_complex_t *a, const int *piv, d_complex_t *b);     
 
 /* ---- sweep (reference peec.h:374-375) ---- */
 int  solve(peec_t *p, FILE *fp_log);                                            /* solve.c:171 */
+
+/* ---- result files and tables (reference peec.h:377-389) ---- */
+void output_zin(const peec_t *p, FILE *fp_log);                                 /* output.c:28   */
+int  output_csv(const peec_t *p, const char *fn);                               /* output.c:34   */
+void output_spara(const peec_t *p, FILE *fp_log);                               /* output.c:85   */
+int  output_touchstone(const peec_t *p);                                        /* output.c:103  */
+int  output_dist(const peec_t *p, const char *fn);                              /* output.c:229  */
+int  output_pw(const peec_t *p, const char *fn, FILE *fp_log);                  /* output.c:178  */
+int  output_far(const peec_t *p, const char *fn, FILE *fp_log);                 /* farfield.c:42 */
+int  output_tran(const peec_t *p, const char *fn, FILE *fp_log);                /* transient.c:92 */
 
 #ifdef __cplusplus
 }

@@ -198,6 +198,17 @@ def declare(lib: C.CDLL, reference: bool = False) -> None:
         "poly_corr": (d_complex_t, [PS, PS, C.c_double]),
         "seg_mirror": (None, [PS, C.c_double, PS]),
         "mna_rhs_port": (None, [PP, C.c_int, PZ]),
+        "mna_rhs_sources": (None, [PP, PZ]),
+        "mna_rhs_planewave": (None, [PP, C.c_double, PZ]),
+        "gmres_solve": (C.c_int, [C.c_int, PZ, PZ, C.POINTER(C.c_int), PZ, C.c_double]),
+        "output_zin": (None, [PP, C.c_void_p]),
+        "output_csv": (C.c_int, [PP, C.c_char_p]),
+        "output_spara": (None, [PP, C.c_void_p]),
+        "output_touchstone": (C.c_int, [PP]),
+        "output_dist": (C.c_int, [PP, C.c_char_p]),
+        "output_pw": (C.c_int, [PP, C.c_char_p, C.c_void_p]),
+        "output_far": (C.c_int, [PP, C.c_char_p, C.c_void_p]),
+        "output_tran": (C.c_int, [PP, C.c_char_p, C.c_void_p]),
     }
     if not reference:
         sig.update({

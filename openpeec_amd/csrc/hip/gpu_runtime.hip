@@ -64,6 +64,14 @@ struct pgpu {
 	unsigned *kf_cnt = nullptr;
 	int kf_nd_lp = 0, kf_nd_ig = 0;
 	double *kf_rho_seg = nullptr, *kf_rho_chg = nullptr;
+	// solve() points: right-hand sides, and acceleration = 1's kept LU and GMRES workspace
+	double2 *xr = nullptr;       // n x xr_cap solution vectors (one per right-hand side)
+	int xr_cap = 0;
+	int *rowofcell = nullptr;    // [ncell] MNA row of each capacitive cell's node, or -1
+	double2 *alu = nullptr;      // n x n factors of the kept LU
+	int *alu_piv = nullptr, *alu_perm = nullptr;
+	bool have_lu = false;
+	dgmres_ws *gw = nullptr;
 };
 
 namespace {
@@ -371,6 +379,10 @@ void ensure_mna(pgpu *g, const peec_t *p)
 		g->cellrow = dalloc<int>(n);
 		HIPCHK(hipMemcpy(g->cellrow, cr.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
 		g->qbuf = dalloc<double2>((size_t)p->ncell * np);
+		std::vector<int> rc(p->ncell);
+		for (int l = 0; l < p->ncell; l++) rc[l] = p->nodemap[p->cellid[l]];
+		g->rowofcell = dalloc<int>(p->ncell);
+		HIPCHK(hipMemcpy(g->rowofcell, rc.data(), (size_t)p->ncell * sizeof(int), hipMemcpyHostToDevice));
 	}
 }
 
@@ -470,6 +482,12 @@ void release_mirror(pgpu *g)
 		dfree(g->d_st_ci);
 		dfree(g->d_st_v);
 		dfree(g->qbuf);
+		dfree(g->xr);
+		dfree(g->rowofcell);
+		dfree(g->alu);
+		dfree(g->alu_piv);
+		dfree(g->alu_perm);
+		dgmres_ws_free(g->gw);
 		HIPCHK(hipStreamDestroy(g->s));
 		delete g;
 }
@@ -853,6 +871,71 @@ d_complex_t pgpu_pot_entry(pgpu_t *g, int i, int j, double kw)
 }
 
 // ------------------------------------------------------------------------------------------
+namespace {
+// The MNA matrix of one frequency point in g->A (solve.c:239-299 up to the factorisation):
+// Lp and, with capacitive cells, P -> C = P^-1 when retarded (the static setup's matrices
+// otherwise), then the dense blocks and the sparse stamps.  ev (may be NULL) receives events
+// 0..4 around the stages.  Returns 1 on a singular P.
+int point_system(pgpu *g, const peec_t *p, double freq, bool usecap, hipEvent_t *ev)
+{
+	const int n = p->nunknown;
+	const double kw = p->retardation ? (2 * PI * freq / C0) : 0;
+	const double omega = 2 * PI * freq;
+	auto mark = [&](int k) {
+		if (ev) HIPCHK(hipEventRecord(ev[k], g->s));
+	};
+	mark(0);
+	if (p->retardation) {
+		pgpu_fill_lp(g, kw);
+		mark(1);
+		if (usecap) {
+			pgpu_fill_pot(g, kw);
+			mark(2);
+			int ising = -1;
+			if (pgpu_cmat(g, &ising)) {
+				printf("*** singular potential coefficient matrix (row %d)\n", ising);
+				return 1;
+			}
+		}
+		else {
+			mark(2);
+		}
+	}
+	else {
+		mark(1);
+		mark(2);
+	}
+	mark(3);
+	stamps_t sp;
+	mna_stamps(p, freq, &sp);
+	if (sp.n > g->st_cap) {
+		dfree(g->d_st_ri);
+		dfree(g->d_st_ci);
+		dfree(g->d_st_v);
+		g->st_cap = sp.n;
+		g->d_st_ri = dalloc<int>(sp.n);
+		g->d_st_ci = dalloc<int>(sp.n);
+		g->d_st_v = dalloc<double2>(sp.n);
+	}
+	if (sp.n > 0) {
+		HIPCHK(hipMemcpyAsync(g->d_st_ri, sp.ri, (size_t)sp.n * sizeof(int), hipMemcpyHostToDevice, g->s));
+		HIPCHK(hipMemcpyAsync(g->d_st_ci, sp.ci, (size_t)sp.n * sizeof(int), hipMemcpyHostToDevice, g->s));
+		HIPCHK(hipMemcpyAsync(g->d_st_v, sp.val, (size_t)sp.n * sizeof(double2), hipMemcpyHostToDevice, g->s));
+	}
+	const long tot = (long)n * n;
+	hipLaunchKernelGGL(k_assemble_dense, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, g->s, g->A, (long)n, n,
+	                   p->offS, p->nseg, g->lp, omega, usecap ? g->cellrow : nullptr, g->C, p->ncell);
+	if (sp.n > 0)
+		hipLaunchKernelGGL(k_assemble_sparse, dim3((sp.n + 255) / 256), dim3(256), 0, g->s, g->A, (long)n, g->d_st_ri,
+		                   g->d_st_ci, g->d_st_v, sp.n);
+	mark(4);
+	HIPCHK(hipStreamSynchronize(g->s));   // the stamps' host buffers are released next
+	mna_stamps_free(&sp);
+	return 0;
+}
+
+}  // namespace
+
 int pgpu_sweep(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq, d_complex_t *zmat, d_complex_t *segi,
                d_complex_t *cellq, pgpu_stats_t *st, int precision)
 {
@@ -888,58 +971,11 @@ int pgpu_sweep(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq, d_complex_t *z
 	for (int f = 0; f < nfreq; f++) {
 		const int ifreq = ifreq0 + f;
 		const double freq = freq_at(p, ifreq);
-		const double kw = p->retardation ? (2 * PI * freq / C0) : 0;
-		const double omega = 2 * PI * freq;
 		const auto t0 = std::chrono::steady_clock::now();
-		HIPCHK(hipEventRecord(ev[0], g->s));
-		if (p->retardation) {
-			pgpu_fill_lp(g, kw);
-			HIPCHK(hipEventRecord(ev[1], g->s));
-			if (usecap) {
-				pgpu_fill_pot(g, kw);
-				HIPCHK(hipEventRecord(ev[2], g->s));
-				int ising = -1;
-				if (pgpu_cmat(g, &ising)) {
-					printf("*** singular potential coefficient matrix (row %d)\n", ising);
-					rc = 1;
-					break;
-				}
-			}
-			else {
-				HIPCHK(hipEventRecord(ev[2], g->s));
-			}
+		if (point_system(g, p, freq, usecap, ev)) {
+			rc = 1;
+			break;
 		}
-		else {
-			HIPCHK(hipEventRecord(ev[1], g->s));
-			HIPCHK(hipEventRecord(ev[2], g->s));
-		}
-		HIPCHK(hipEventRecord(ev[3], g->s));
-		// assembly
-		stamps_t sp;
-		mna_stamps(p, freq, &sp);
-		if (sp.n > g->st_cap) {
-			dfree(g->d_st_ri);
-			dfree(g->d_st_ci);
-			dfree(g->d_st_v);
-			g->st_cap = sp.n;
-			g->d_st_ri = dalloc<int>(sp.n);
-			g->d_st_ci = dalloc<int>(sp.n);
-			g->d_st_v = dalloc<double2>(sp.n);
-		}
-		if (sp.n > 0) {
-			HIPCHK(hipMemcpyAsync(g->d_st_ri, sp.ri, (size_t)sp.n * sizeof(int), hipMemcpyHostToDevice, g->s));
-			HIPCHK(hipMemcpyAsync(g->d_st_ci, sp.ci, (size_t)sp.n * sizeof(int), hipMemcpyHostToDevice, g->s));
-			HIPCHK(hipMemcpyAsync(g->d_st_v, sp.val, (size_t)sp.n * sizeof(double2), hipMemcpyHostToDevice, g->s));
-		}
-		const long tot = (long)n * n;
-		hipLaunchKernelGGL(k_assemble_dense, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, g->s, g->A, (long)n, n,
-		                   p->offS, p->nseg, g->lp, omega, usecap ? g->cellrow : nullptr, g->C, p->ncell);
-		if (sp.n > 0)
-			hipLaunchKernelGGL(k_assemble_sparse, dim3((sp.n + 255) / 256), dim3(256), 0, g->s, g->A, (long)n,
-			                   g->d_st_ri, g->d_st_ci, g->d_st_v, sp.n);
-		HIPCHK(hipEventRecord(ev[4], g->s));
-		HIPCHK(hipStreamSynchronize(g->s));   // stamps host buffers are released below
-		mna_stamps_free(&sp);
 		// factor: in place in FP64, or an FP32 copy (the FP64 matrix is kept for the refinement)
 		int info;
 		bool fp64 = !precision;
@@ -1070,6 +1106,100 @@ int pgpu_sweep(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq, d_complex_t *z
 	dfree(d_i2);
 	dfree(d_rowofcell);
 	return rc;
+}
+
+void pgpu_accel_reset(pgpu_t *g)
+{
+	g->have_lu = false;
+}
+
+// One point of the reference's solve() on the dense path (solve.c:239-405): the system of this
+// frequency, then nr right-hand sides (host, one n-vector each) solved on the device and copied
+// back into x (nr x n).  accel 0: a fresh LU of A, as lu_decomp / lu_solve.  accel 1 (accel_rhs,
+// solve.c:47-64): the LU kept from an earlier point preconditions one GMRES(60) cycle per
+// right-hand side to a residual of 1e-10 |b| (gmres_solve); the first cycle that does not
+// converge refactors at this point, and the fresh LU then solves that right-hand side and the
+// rest.  q (may be NULL): the cell charges q = C v of the first nq solutions (ncell each, dense
+// capacitive decks).  Returns 0, 1 (singular P) or 2 (singular MNA matrix).
+int pgpu_solve_point(pgpu_t *g, const peec_t *p, int ifreq, int nr, const d_complex_t *rhs, d_complex_t *x, int nq,
+                     d_complex_t *q, int accel, pgpu_accel_stats_t *as)
+{
+	const int n = p->nunknown;
+	const double freq = freq_at(p, ifreq);
+	ensure_cells(g);
+	ensure_mna(g, p);
+	const bool usecap = p->capacitance && (p->ncell > 0) && (g->cellrow != nullptr);
+	if (!p->retardation) {
+		if (p->nseg > 0) pgpu_lp_from_host(g, p->lp);
+		if (usecap) pgpu_cmat_from_host(g, p->cmat);
+	}
+	if (point_system(g, p, freq, usecap, nullptr)) return 1;
+	if (nr > g->xr_cap) {
+		dfree(g->xr);
+		g->xr = dalloc<double2>((size_t)n * nr);
+		g->xr_cap = nr;
+	}
+	HIPCHK(hipMemcpyAsync(g->xr, rhs, (size_t)n * nr * sizeof(double2), hipMemcpyHostToDevice, g->s));
+	auto vec = [&](int r) { return (d_complex_t *)(g->xr + (size_t)r * n); };
+	if (!accel) {
+		const int info = dlu_factor(g->s, n, (d_complex_t *)g->A, n, g->apiv, g->aperm, nullptr);
+		if (info >= 0) {
+			printf("*** singular matrix at f = %.5e Hz (row %d : floating node or ideal voltage source loop?)\n", freq, info);
+			return 2;
+		}
+		for (int r = 0; r < nr; r++)
+			dlu_solve_perm(g->s, n, (const d_complex_t *)g->A, n, g->aperm, vec(r), 1, 1, (d_complex_t *)g->Bw);
+	}
+	else {
+		if (g->alu == nullptr) {
+			g->alu = dalloc<double2>((size_t)n * n);
+			g->alu_piv = dalloc<int>(n);
+			g->alu_perm = dalloc<int>(n);
+			g->gw = dgmres_ws_create(n);
+		}
+		auto refactor = [&]() -> int {
+			HIPCHK(hipMemcpyAsync(g->alu, g->A, (size_t)n * n * sizeof(double2), hipMemcpyDeviceToDevice, g->s));
+			const int info = dlu_factor(g->s, n, (d_complex_t *)g->alu, n, g->alu_piv, g->alu_perm, nullptr);
+			if (info >= 0) {
+				g->have_lu = false;
+				printf("*** singular matrix at f = %.5e Hz (row %d : floating node or ideal voltage source loop?)\n", freq, info);
+				return 2;
+			}
+			g->have_lu = true;
+			if (as) as->nlu++;
+			return 0;
+		};
+		bool fresh = false;
+		if (!g->have_lu) {
+			if (refactor()) return 2;
+			fresh = true;
+		}
+		for (int r = 0; r < nr; r++) {
+			if (!fresh) {
+				const int it = dgmres_cycle(g->s, g->gw, n, (const d_complex_t *)g->A, n, (const d_complex_t *)g->alu, g->alu_perm,
+				                            vec(r), 1e-10);
+				if (it >= 0) {
+					if (as) {
+						as->ngm++;
+						as->itsum += it;
+					}
+					continue;
+				}
+				if (refactor()) return 2;   // the preconditioner is stale: LU at this point
+				fresh = true;
+			}
+			dlu_solve_perm(g->s, n, (const d_complex_t *)g->alu, n, g->alu_perm, vec(r), 1, 1, (d_complex_t *)g->Bw);
+		}
+	}
+	if (usecap && (q != nullptr))
+		for (int j = 0; j < nq; j++)
+			hipLaunchKernelGGL(k_cell_charge, dim3((unsigned)((p->ncell + 255) / 256)), dim3(256), 0, g->s, g->C, p->ncell,
+			                   g->rowofcell, (const double2 *)vec(j), 1, g->qbuf + (size_t)j * p->ncell);
+	HIPCHK(hipMemcpyAsync(x, g->xr, (size_t)n * nr * sizeof(double2), hipMemcpyDeviceToHost, g->s));
+	if (usecap && (q != nullptr) && (nq > 0))
+		HIPCHK(hipMemcpyAsync(q, g->qbuf, (size_t)p->ncell * nq * sizeof(double2), hipMemcpyDeviceToHost, g->s));
+	HIPCHK(hipStreamSynchronize(g->s));
+	return 0;
 }
 
 }  // extern "C"

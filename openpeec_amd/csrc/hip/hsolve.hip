@@ -1269,11 +1269,85 @@ int gmres_hmat(const peec_t *p, const mna_sparse_t *sp, const hblock_t *blk, int
 	return it;
 }
 
-// Compressed sweep (solve.c:246-300 with compress = 1), device resident: per frequency the H
-// matrices of Lp (and P), the sparse part, the preconditioner, then one GMRES per port.
-int pgpu_sweep_compressed(pgpu_t *g, peec_t *p, int ifreq0, int nfreq, d_complex_t *zmat, pgpu_stats_t *st)
+// One point of the reference's solve() on the compressed path (solve.c:246-300, 319-405 with
+// compress = 1), device resident: the H matrices of Lp (and P), the sparse part and the
+// preconditioner at this frequency, then one GMRES (true residual 1e-10, gmres_hmat) per
+// right-hand side: rhs holds nr host vectors of n, x receives their solutions.  its (may be NULL)
+// accumulates GMRES iterations, mem_mb (may be NULL) receives the point's H-matrix and
+// preconditioner memory (hmat_memory_mb + pc_memory_mb).  Returns 0, 5 (H-matrix build), 6
+// (preconditioner) or 4 (GMRES did not converge).
+int pgpu_solve_point_compressed(pgpu_t *g, peec_t *p, int ifreq, int nr, const d_complex_t *rhs, d_complex_t *x, long *its,
+                                double *mem_mb, pgpu_stats_t *st)
 {
 	hipStream_t s = pgpu_hip_stream(g);
+	const int n = p->nunknown;
+	const double f = freq_at(p, ifreq);
+	const auto t0 = std::chrono::steady_clock::now();
+	hblock_t blk[HBLK_MAX];
+	int nblk = 0;
+	hmat_t *hl = hmat_build(p, f, nullptr);
+	if (hl == nullptr) {
+		printf("*** compression : Lp H-matrix build failed at f = %.5e Hz\n", f);
+		return 5;
+	}
+	blk[nblk++] = hblock_t{p->offS, p->nseg, hl, d_complex(0, -2 * PI * f)};
+	hmat_t *hp = nullptr;
+	if (p->capacitance && (p->ncell > 0)) {
+		hp = hmat_build_pot(p, f, nullptr);
+		if (hp == nullptr) {
+			printf("*** compression : P H-matrix build failed at f = %.5e Hz\n", f);
+			hmat_free(hl);
+			return 5;
+		}
+		blk[nblk++] = hblock_t{p->offQ, p->ncell, hp, d_complex(1, 0)};
+	}
+	const auto t1 = std::chrono::steady_clock::now();
+	mna_sparse_t sp;
+	mna_sparse(p, f, &sp);
+	precond_t *pc = pc_build_dev(p, &sp, blk, nblk, s);
+	const auto t2 = std::chrono::steady_clock::now();
+	int rc = 0;
+	if (pc == nullptr) {
+		printf("*** compression : preconditioner failed at f = %.5e Hz (floating node?)\n", f);
+		rc = 6;
+	}
+	hop o;
+	if (!rc) hop_init(o, p, &sp, blk, nblk, s);
+	double2 *db = dal<double2>(n), *dx = dal<double2>(n);
+	for (int r = 0; (r < nr) && !rc; r++) {
+		HIPCHK(hipMemcpyAsync(db, rhs + (size_t)r * n, (size_t)n * sizeof(double2), hipMemcpyHostToDevice, s));
+		const int it = gmres_dev(o, pc, db, dx, 1e-10);
+		if (it < 0) {
+			printf("*** compression : GMRES did not converge at f = %.5e Hz\n", f);
+			rc = 4;
+			break;
+		}
+		if (its) *its += it;
+		if (st) st->gmres_its += it;
+		HIPCHK(hipMemcpyAsync(x + (size_t)r * n, dx, (size_t)n * sizeof(double2), hipMemcpyDeviceToHost, s));
+		HIPCHK(hipStreamSynchronize(s));
+	}
+	const auto t3 = std::chrono::steady_clock::now();
+	if (st) {
+		st->fill_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+		st->lu_ms += std::chrono::duration<double, std::milli>(t2 - t1).count();
+		st->solve_ms += std::chrono::duration<double, std::milli>(t3 - t2).count();
+		st->total_ms += std::chrono::duration<double, std::milli>(t3 - t0).count();
+	}
+	if (mem_mb) *mem_mb = hmat_memory_mb(hl) + hmat_memory_mb(hp) + pc_memory_mb(pc);
+	HIPCHK(hipFree(db));
+	HIPCHK(hipFree(dx));
+	if (pc != nullptr) hop_free(o);
+	pc_free(pc);
+	mna_sparse_free(&sp);
+	hmat_free(hl);
+	hmat_free(hp);
+	return rc;
+}
+
+// Compressed sweep (solve.c:246-300 with compress = 1) of the ports alone: Z of every point.
+int pgpu_sweep_compressed(pgpu_t *g, peec_t *p, int ifreq0, int nfreq, d_complex_t *zmat, pgpu_stats_t *st)
+{
 	{
 		const auto t0 = std::chrono::steady_clock::now();
 		pgpu_kf_band(g, p, freq_at(p, ifreq0), freq_at(p, ifreq0 + nfreq - 1));
@@ -1284,77 +1358,20 @@ int pgpu_sweep_compressed(pgpu_t *g, peec_t *p, int ifreq0, int nfreq, d_complex
 		}
 	}
 	const int np = p->nport, n = p->nunknown;
-	std::vector<d_complex_t> hb(n);
-	double2 *db = dal<double2>(n), *dx = dal<double2>(n);
-	std::vector<double2> hx(n);
-	int rc = 0;
-	for (int k = 0; (k < nfreq) && !rc; k++) {
-		const double f = freq_at(p, ifreq0 + k);
-		const auto t0 = std::chrono::steady_clock::now();
-		hblock_t blk[HBLK_MAX];
-		int nblk = 0;
-		hmat_t *hl = hmat_build(p, f, nullptr);
-		if (hl == nullptr) {
-			rc = 5;
-			break;
-		}
-		blk[nblk++] = hblock_t{p->offS, p->nseg, hl, d_complex(0, -2 * PI * f)};
-		hmat_t *hp = nullptr;
-		if (p->capacitance && (p->ncell > 0)) {
-			hp = hmat_build_pot(p, f, nullptr);
-			if (hp == nullptr) {
-				hmat_free(hl);
-				rc = 5;
-				break;
-			}
-			blk[nblk++] = hblock_t{p->offQ, p->ncell, hp, d_complex(1, 0)};
-		}
-		const auto t1 = std::chrono::steady_clock::now();
-		mna_sparse_t sp;
-		mna_sparse(p, f, &sp);
-		precond_t *pc = pc_build_dev(p, &sp, blk, nblk, s);
-		const auto t2 = std::chrono::steady_clock::now();
-		if (pc == nullptr) {
-			printf("*** compression : preconditioner failed at f = %.5e Hz (floating node?)\n", f);
-			rc = 6;
-		}
-		hop o;
-		if (!rc) hop_init(o, p, &sp, blk, nblk, s);
-		for (int j = 0; (j < np) && !rc; j++) {
-			mna_rhs_port(p, j, hb.data());
-			HIPCHK(hipMemcpyAsync(db, hb.data(), (size_t)n * sizeof(double2), hipMemcpyHostToDevice, s));
-			const int it = gmres_dev(o, pc, db, dx, 1e-10);
-			if (it < 0) {
-				printf("*** GMRES did not converge at f = %.5e Hz\n", f);
-				rc = 4;
-				break;
-			}
-			if (st) st->gmres_its += it;
-			HIPCHK(hipMemcpyAsync(hx.data(), dx, (size_t)n * sizeof(double2), hipMemcpyDeviceToHost, s));
-			HIPCHK(hipStreamSynchronize(s));
+	std::vector<d_complex_t> rhs((size_t)n * np), x((size_t)n * np);
+	for (int j = 0; j < np; j++) mna_rhs_port(p, j, &rhs[(size_t)j * n]);
+	for (int k = 0; k < nfreq; k++) {
+		const int rc = pgpu_solve_point_compressed(g, p, ifreq0 + k, np, rhs.data(), x.data(), nullptr, nullptr, st);
+		if (rc) return rc;
+		for (int j = 0; j < np; j++)
 			for (int i = 0; i < np; i++) {
 				const int i1 = p->nodemap[p->port[i].n1], i2 = p->nodemap[p->port[i].n2];
-				const double2 v1 = (i1 < 0) ? make_double2(0, 0) : hx[i1];
-				const double2 v2 = (i2 < 0) ? make_double2(0, 0) : hx[i2];
-				zmat[((size_t)k * np + i) * np + j] = d_complex(v1.x - v2.x, v1.y - v2.y);
+				const d_complex_t v1 = (i1 < 0) ? d_complex(0, 0) : x[(size_t)j * n + i1];
+				const d_complex_t v2 = (i2 < 0) ? d_complex(0, 0) : x[(size_t)j * n + i2];
+				zmat[((size_t)k * np + i) * np + j] = d_sub(v1, v2);
 			}
-		}
-		const auto t3 = std::chrono::steady_clock::now();
-		if (st) {
-			st->fill_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
-			st->lu_ms += std::chrono::duration<double, std::milli>(t2 - t1).count();
-			st->solve_ms += std::chrono::duration<double, std::milli>(t3 - t2).count();
-			st->total_ms += std::chrono::duration<double, std::milli>(t3 - t0).count();
-		}
-		if (!rc) hop_free(o);
-		pc_free(pc);
-		mna_sparse_free(&sp);
-		hmat_free(hl);
-		hmat_free(hp);
 	}
-	HIPCHK(hipFree(db));
-	HIPCHK(hipFree(dx));
-	return rc;
+	return 0;
 }
 
 

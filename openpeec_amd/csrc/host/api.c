@@ -258,62 +258,128 @@ out:
 	return rc;
 }
 
+/* node voltages of the sources' solution, every used node id (solve.c:101-112) */
+static void log_node_voltages(const peec_t *p, const d_complex_t *x, double f, FILE *fp_log)
+{
+	if (fp_log == NULL) return;
+	fprintf(fp_log, "=== node voltage === (f = %.5e Hz)\n", f);
+	fprintf(fp_log, "  %s\n", "node      real[V]      imag[V]       |V|[V]");
+	for (int id = 0; id <= p->maxid; id++) {
+		const int row = p->nodemap[id];
+		if (row == -2) continue;
+		const d_complex_t v = (row < 0) ? d_complex(0, 0) : x[row];
+		fprintf(fp_log, "%6d %12.4e %12.4e %12.4e\n", id, v.r, v.i, d_abs(v));
+	}
+}
+
+/* port i's terminal voltage V(n1) - V(n2) in the solution x */
+static d_complex_t port_voltage(const peec_t *p, int i, const d_complex_t *x)
+{
+	const int r1 = p->nodemap[p->port[i].n1], r2 = p->nodemap[p->port[i].n2];
+	const d_complex_t v1 = (r1 < 0) ? d_complex(0, 0) : x[r1];
+	const d_complex_t v2 = (r2 < 0) ? d_complex(0, 0) : x[r2];
+	return d_sub(v1, v2);
+}
+
+static void say(FILE *fp_log, const char *msg, double f)
+{
+	printf(msg, f);
+	if (fp_log) fprintf(fp_log, msg, f);
+}
+
+/* The sweep of the reference's solve() (solve.c:171-430), device resident per point.  Right-hand
+ * sides of a point: each port driven by 1 A (Z column j, distributions), then the plane wave
+ * (open-circuit port voltages, induced currents) and the independent sources (node voltages to
+ * the log), all solved with the point's factorisation -- a fresh LU, the kept LU with GMRES
+ * (acceleration = 1) or the compressed operator with its preconditioner (compression = 1). */
 int solve(peec_t *p, FILE *fp_log)
 {
-	const int np = p->nport;
-	if (p->pw || (p->nsrc > 0)) {
-		printf("%s\n", "*** planewave / independent sources are not available in this build");
-		return 1;
-	}
-	const size_t nmat = (size_t)p->nfreq * np * np;
-	p->zin = (d_complex_t *)malloc((size_t)np * p->nfreq * sizeof(d_complex_t));
+	const int np = p->nport, n = p->nunknown, nf = p->nfreq;
+	const int npw = p->pw ? 1 : 0, nsr = (p->nsrc > 0) ? 1 : 0;
+	const int nr = np + npw + nsr;
+	const size_t nmat = (size_t)nf * np * np;
+	p->zin = (d_complex_t *)malloc((size_t)np * nf * sizeof(d_complex_t));
 	p->zmat = (d_complex_t *)malloc(nmat * sizeof(d_complex_t));
 	p->smat = (d_complex_t *)malloc(nmat * sizeof(d_complex_t));
+	if (p->pw) {
+		p->voc = (d_complex_t *)malloc((size_t)np * nf * sizeof(d_complex_t));
+		if (p->dist && (p->nseg > 0)) p->segipw = (d_complex_t *)malloc((size_t)nf * p->nseg * sizeof(d_complex_t));
+	}
 	if ((p->dist || (p->ffnth > 0)) && (p->nseg > 0))
-		p->segi = (d_complex_t *)malloc((size_t)p->nfreq * np * p->nseg * sizeof(d_complex_t));
+		p->segi = (d_complex_t *)malloc((size_t)nf * np * p->nseg * sizeof(d_complex_t));
 	if (p->dist && p->capacitance && (p->ncell > 0))
-		p->cellq = (d_complex_t *)malloc((size_t)p->nfreq * np * p->ncell * sizeof(d_complex_t));
-	if ((p->zin == NULL) || (p->zmat == NULL) || (p->smat == NULL)) {
+		p->cellq = (d_complex_t *)malloc((size_t)nf * np * p->ncell * sizeof(d_complex_t));
+	d_complex_t *rhs = (d_complex_t *)malloc((size_t)n * (nr > 0 ? nr : 1) * sizeof(d_complex_t));
+	d_complex_t *x = (d_complex_t *)malloc((size_t)n * (nr > 0 ? nr : 1) * sizeof(d_complex_t));
+	d_complex_t *q = (p->cellq && !p->compress) ? (d_complex_t *)malloc((size_t)p->ncell * np * sizeof(d_complex_t)) : NULL;
+	int ret = 1;
+	if ((p->zin == NULL) || (p->zmat == NULL) || (p->smat == NULL) || (rhs == NULL) || (x == NULL) ||
+	    (p->pw && (p->voc == NULL)) || (p->cellq && !p->compress && (q == NULL))) {
 		printf("%s\n", "*** memory allocation error (matrix)");
-		return 1;
+		goto done;
 	}
 	pgpu_t *g = pgpu_get(p);
-	if (p->compress && (p->nfreq > 0)) pgpu_kf_band(g, p, freq_at(p, 0), freq_at(p, p->nfreq - 1));
-	for (int ifreq = 0; ifreq < p->nfreq; ifreq++) {
+	if (p->compress && (nf > 0)) pgpu_kf_band(g, p, freq_at(p, 0), freq_at(p, nf - 1));
+	pgpu_accel_reset(g);
+	pgpu_accel_stats_t as = {0, 0, 0};
+	long citsum = 0;
+	int cnrhs = 0;
+	double cmem = 0;
+	for (int ifreq = 0; ifreq < nf; ifreq++) {
 		const double f = freq_at(p, ifreq);
+		for (int j = 0; j < np; j++) mna_rhs_port(p, j, rhs + (size_t)j * n);
+		if (npw) mna_rhs_planewave(p, f, rhs + (size_t)np * n);
+		if (nsr) mna_rhs_sources(p, rhs + (size_t)(np + npw) * n);
+		int rc;
 		if (p->compress) {
-			/* solve.c:246-300: H matrices, leaf/Schur preconditioner, GMRES per port */
-			if (pgpu_sweep_compressed(g, p, ifreq, 1, &p->zmat[ZIDX(p, ifreq, 0, 0)], NULL)) {
-				if (fp_log) fprintf(fp_log, "*** compressed solve failed at f = %.5e Hz\n", f);
-				return 1;
-			}
-			for (int j = 0; j < np; j++) p->zin[(size_t)j * p->nfreq + ifreq] = p->zmat[ZIDX(p, ifreq, j, j)];
-			if (z_to_s(p, ifreq)) {
-				printf("*** singular (Z + Z0) at f = %.5e Hz\n", f);
-				if (fp_log) fprintf(fp_log, "*** singular (Z + Z0) at f = %.5e Hz\n", f);
-				return 1;
-			}
-			continue;
+			rc = pgpu_solve_point_compressed(g, p, ifreq, nr, rhs, x, &citsum, &cmem, NULL);
+			cnrhs += nr;
 		}
-		const int rc = pgpu_sweep(g, p, ifreq, 1, &p->zmat[ZIDX(p, ifreq, 0, 0)],
-			p->segi ? &p->segi[DIDX(p, ifreq, 0, 0)] : NULL,
-			p->cellq ? &p->cellq[QIDX(p, ifreq, 0, 0)] : NULL, NULL, 0);
+		else {
+			rc = pgpu_solve_point(g, p, ifreq, nr, rhs, x, q ? np : 0, q, p->accel, &as);
+		}
 		if (rc == 1) {
-			if (fp_log) fprintf(fp_log, "*** singular potential coefficient matrix at f = %.5e Hz\n", f);
-			return 1;
+			say(fp_log, "*** singular potential coefficient matrix at f = %.5e Hz\n", f);
+			goto done;
 		}
 		if (rc) {
-			if (fp_log) fprintf(fp_log, "*** singular matrix at f = %.5e Hz\n", f);
-			return 1;
+			say(fp_log, p->compress ? "*** compression : solve failed at f = %.5e Hz\n" : "*** singular matrix at f = %.5e Hz\n", f);
+			goto done;
 		}
-		for (int j = 0; j < np; j++) p->zin[(size_t)j * p->nfreq + ifreq] = p->zmat[ZIDX(p, ifreq, j, j)];
+		for (int j = 0; j < np; j++) {
+			const d_complex_t *xj = x + (size_t)j * n;
+			for (int i = 0; i < np; i++) p->zmat[ZIDX(p, ifreq, i, j)] = port_voltage(p, i, xj);
+			p->zin[(size_t)j * nf + ifreq] = p->zmat[ZIDX(p, ifreq, j, j)];
+			if (p->segi != NULL) memcpy(&p->segi[DIDX(p, ifreq, j, 0)], xj + p->offS, (size_t)p->nseg * sizeof(d_complex_t));
+			if (p->cellq != NULL)
+				memcpy(&p->cellq[QIDX(p, ifreq, j, 0)], p->compress ? (xj + p->offQ) : (q + (size_t)j * p->ncell),
+				       (size_t)p->ncell * sizeof(d_complex_t));
+		}
 		if (z_to_s(p, ifreq)) {
-			printf("*** singular (Z + Z0) at f = %.5e Hz\n", f);
-			if (fp_log) fprintf(fp_log, "*** singular (Z + Z0) at f = %.5e Hz\n", f);
-			return 1;
+			say(fp_log, "*** singular (Z + Z0) at f = %.5e Hz\n", f);
+			goto done;
 		}
+		if (npw) {
+			const d_complex_t *xw = x + (size_t)np * n;
+			for (int i = 0; i < np; i++) p->voc[(size_t)i * nf + ifreq] = port_voltage(p, i, xw);
+			if (p->segipw != NULL) memcpy(&p->segipw[(size_t)ifreq * p->nseg], xw + p->offS, (size_t)p->nseg * sizeof(d_complex_t));
+		}
+		if (nsr) log_node_voltages(p, x + (size_t)(np + npw) * n, f, fp_log);
 	}
-	return 0;
+	if (p->accel && fp_log)
+		fprintf(fp_log, "acceleration : %d LU for %d frequencies, %d GMRES solves (avg %.1f iterations)\n", as.nlu, nf, as.ngm,
+		        (as.ngm > 0) ? ((double)as.itsum / as.ngm) : 0.0);
+	if (p->compress && (cnrhs > 0) && fp_log) {
+		const double dm = hmat_dense_mb(n) + hmat_dense_mb(p->nseg) + (p->capacitance ? hmat_dense_mb(p->ncell) : 0);
+		fprintf(fp_log, "compression : %d GMRES solves (avg %.1f iterations), %.2f MB (dense MNA + Lp %.2f MB, %.1fx)\n", cnrhs,
+		        (double)citsum / cnrhs, cmem, dm, (cmem > 0) ? (dm / cmem) : 0);
+	}
+	ret = 0;
+done:
+	free(rhs);
+	free(x);
+	free(q);
+	return ret;
 }
 
 /* ---------------------------------------------------------------- extensions ----------- */

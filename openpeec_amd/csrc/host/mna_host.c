@@ -255,3 +255,84 @@ void mna_rhs_port(const peec_t *p, int iport, d_complex_t *b)
 	if (i1 >= 0) b[i1] = d_add(b[i1], d_complex(1, 0));
 	if (i2 >= 0) b[i2] = d_sub(b[i2], d_complex(1, 0));
 }
+
+/* Independent sources as a right-hand side (mna.c:471-489): a vsource sets its branch row to
+ * amp / phase; an isource drives its current from n1 to n2 through the source (SPICE sign). */
+void mna_rhs_sources(const peec_t *p, d_complex_t *b)
+{
+	memset(b, 0, (size_t)p->nunknown * sizeof(d_complex_t));
+	for (int k = 0; k < p->nsrc; k++) {
+		const src_t *e = &p->src[k];
+		const d_complex_t v = d_polar_deg(e->amp, e->phase);
+		if (e->isvsrc) {
+			b[p->offV + e->ibr] = v;
+			continue;
+		}
+		const int r1 = p->nodemap[e->n1], r2 = p->nodemap[e->n2];
+		if (r1 >= 0) b[r1] = d_sub(b[r1], v);
+		if (r2 >= 0) b[r2] = d_add(b[r2], v);
+	}
+}
+
+/* sin(u) / u, 1 for |u| < 1e-8 */
+static double pw_sinc(double u)
+{
+	return (fabs(u) < 1e-8) ? 1.0 : (sin(u) / u);
+}
+
+static double dot3(const double *a, const double *b)
+{
+	return (a[0] * b[0]) + (a[1] * b[1]) + (a[2] * b[2]);
+}
+
+/* Line integral of a uniform plane wave E0 e exp(+j k dir . r) along one current cell, exact for
+ * a straight cell of constant direction: (E0 e . t) len exp(j k dir . c) sinc(k len (dir . t) / 2),
+ * with the cross-width average sinc(k wid (dir . w) / 2) for ribbons and bars (mna.c:402-431). */
+static d_complex_t pw_line_integral(const seg_t *s, const double *dir, const double *e, d_complex_t e0, double kw)
+{
+	double t[3], c[3];
+	for (int q = 0; q < 3; q++) {
+		t[q] = (s->x2[q] - s->x1[q]) / s->len;
+		c[q] = 0.5 * (s->x1[q] + s->x2[q]);
+	}
+	double amp = dot3(e, t) * s->len * pw_sinc(0.5 * kw * s->len * dot3(dir, t));
+	const double wn = sqrt(dot3(s->wv, s->wv));
+	if ((s->wid > 0) && (wn > 0)) amp *= pw_sinc(0.5 * kw * s->wid * (dot3(dir, s->wv) / wn));
+	const double ph = kw * dot3(dir, c);
+	return d_rmul(amp, d_mul(e0, d_complex(cos(ph), sin(ph))));
+}
+
+/* Plane-wave excitation as a right-hand side (mna.c:433-469): each branch row receives minus
+ * the incident field's line integral along its cell.  The wave arrives from (pwth, pwph)
+ * degrees with theta (pwpol 1) or phi (pwpol 2) polarisation; over a ground plane the reflected
+ * wave (arrival mirrored in z, horizontal field reversed, phase matched on the plane) is added. */
+void mna_rhs_planewave(const peec_t *p, double f, d_complex_t *b)
+{
+	memset(b, 0, (size_t)p->nunknown * sizeof(d_complex_t));
+	if (!p->pw) return;
+	const double kw = 2 * PI * f / C0;
+	const double th = p->pwth * PI / 180, ph = p->pwph * PI / 180;
+	const double st = sin(th), ct = cos(th), sp = sin(ph), cp = cos(ph);
+	const double dir[3] = {st * cp, st * sp, ct};
+	double e[3];
+	if (p->pwpol == 2) {
+		e[0] = -sp;
+		e[1] = cp;
+		e[2] = 0;
+	}
+	else {
+		e[0] = ct * cp;
+		e[1] = ct * sp;
+		e[2] = -st;
+	}
+	const d_complex_t e0 = d_polar_deg(p->pwamp, p->pwphase);
+	const double dir_r[3] = {dir[0], dir[1], -dir[2]};
+	const double e_r[3] = {-e[0], -e[1], e[2]};
+	const double psi = 2 * kw * dir[2] * p->gpz;
+	const d_complex_t e0_r = d_mul(e0, d_complex(cos(psi), sin(psi)));
+	for (int m = 0; m < p->nseg; m++) {
+		d_complex_t v = pw_line_integral(&p->seg[m], dir, e, e0, kw);
+		if (p->gp) v = d_add(v, pw_line_integral(&p->seg[m], dir_r, e_r, e0_r, kw));
+		b[p->offS + m] = d_rmul(-1, v);
+	}
+}

@@ -89,6 +89,7 @@ void hm_tree_free(hm_tree_t *t);
 /* ---------------------------------------------------------------------------------------- */
 /* Device mirror of one peec_t (gpu_runtime.hip).                                            */
 typedef struct pgpu pgpu_t;
+typedef struct dgmres_ws dgmres_ws;   /* GMRES workspace of dgmres.hip */
 
 pgpu_t *pgpu_get(const peec_t *p);          /* find or create; uploads cells on creation */
 void    peec_gpu_release(peec_t *p);        /* drop the mirror of p (called by peec_free) */
@@ -129,12 +130,25 @@ void pgpu_fill_to_host(pgpu_t *g, double kw, int mode, d_complex_t *lp, d_comple
 d_complex_t pgpu_lp_entry(pgpu_t *g, int i, int j, double kw);
 d_complex_t pgpu_pot_entry(pgpu_t *g, int i, int j, double kw);
 
+typedef peec_stats_t pgpu_stats_t;
+
+/* one point of solve() on the dense path: nr right-hand sides (host, n each) -> solutions x
+ * (nr x n); accel 1 keeps an LU across points and solves by GMRES with it (acceleration = 1);
+ * q: cell charges of the first nq solutions.  0, 1 (singular P) or 2 (singular A). */
+typedef struct { int nlu, ngm; long itsum; } pgpu_accel_stats_t;
+int  pgpu_solve_point(pgpu_t *g, const peec_t *p, int ifreq, int nr, const d_complex_t *rhs, d_complex_t *x, int nq,
+	d_complex_t *q, int accel, pgpu_accel_stats_t *as);
+void pgpu_accel_reset(pgpu_t *g);   /* forget the kept LU (a new solve() starts fresh) */
+/* the same on the compressed path (H matrices, leaf/Schur preconditioner, GMRES to 1e-10 per
+ * right-hand side); its: GMRES iterations summed; mem_mb: the point's H-matrix + preconditioner
+ * memory.  0, 5 (H-matrix build), 6 (preconditioner) or 4 (GMRES did not converge). */
+int  pgpu_solve_point_compressed(pgpu_t *g, peec_t *p, int ifreq, int nr, const d_complex_t *rhs, d_complex_t *x, long *its,
+	double *mem_mb, pgpu_stats_t *st);
+
 /* sweep of frequency points, device resident: fills, assembles, factors, solves every port.
  * Writes zmat (nport x nport per point), and optionally segi/cellq rows.  Returns 0 or an error
  * code (1 singular P, 2 singular MNA, 3 refinement stalled); precision = 1 factors an FP32 copy and
  * refines in FP64 (PEEC_FORM_PRECISION). */
-typedef peec_stats_t pgpu_stats_t;
-
 int pgpu_sweep(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq, d_complex_t *zmat,
 	d_complex_t *segi, d_complex_t *cellq, pgpu_stats_t *st, int precision);
 
@@ -171,6 +185,13 @@ void slu_tri_inverse(void *stream, int n, const void *LU, int lda, int lower, vo
 int  slu_factor(void *stream, int n, void *dA, int lda, int *dpiv, int *dperm, pgpu_stats_t *st);
 void slu_solve_perm(void *stream, int n, const void *dA, int lda, const int *dperm,
 	d_complex_t *dB, int ldb, int nrhs, d_complex_t *dwork);
+/* one right-preconditioned GMRES(60) cycle on a dense device matrix A with the LU factors LU /
+ * perm (dlu_factor's) of a nearby matrix as preconditioner (dgmres.hip); b (device) becomes the
+ * solution when the cycle reaches tol |b|.  Returns iterations or -1.  ws: dgmres_ws_create(n). */
+dgmres_ws *dgmres_ws_create(int n);
+void dgmres_ws_free(dgmres_ws *ws);
+int  dgmres_cycle(void *stream, dgmres_ws *ws, int n, const d_complex_t *A, int lda, const d_complex_t *LU, const int *perm,
+	d_complex_t *b, double tol);
 /* C[M x N] -= A[M x K] B[K x N] (complex FP64 MFMA) */
 void dlu_zgemm_sub(void *stream, int M, int N, int K, const d_complex_t *A, long lda,
 	const d_complex_t *B, long ldb, d_complex_t *C, long ldc);

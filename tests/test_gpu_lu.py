@@ -75,3 +75,27 @@ def test_lu_singular_returns_row(lib):
     piv = np.zeros(n, dtype=np.int32)
     info = lib.lu_decomp(n, _zptr(a), piv.ctypes.data_as(C.POINTER(C.c_int)))
     assert info == 37
+
+
+def test_gmres_solve_with_a_nearby_lu(lib):
+    """gmres_solve (iterative.c:230-252): one GMRES(60) cycle on A preconditioned by the LU of a
+    nearby matrix converges to tol |b| and returns the solution (checked against LAPACK); with the
+    LU of an unrelated matrix the cycle fails, returns -1 and leaves b untouched."""
+    n = 300
+    a = _rand(n, 21)
+    near = a + 1e-3 * _rand(n, 22)
+    b0 = (np.arange(n) * 0.01 + 1j).astype(np.complex128)
+    for m0, expect in ((near, True), (_rand(n, 23), False)):
+        alu = m0.copy()
+        piv = np.zeros(n, dtype=np.int32)
+        assert lib.lu_decomp(n, _zptr(alu), piv.ctypes.data_as(C.POINTER(C.c_int))) == -1
+        b = b0.copy()
+        it = lib.gmres_solve(n, _zptr(a), _zptr(alu), piv.ctypes.data_as(C.POINTER(C.c_int)), _zptr(b), 1e-10)
+        if expect:
+            assert 1 <= it <= 60
+            x = np.linalg.solve(a, b0)
+            assert np.abs(a @ b - b0).max() <= 1e-9 * np.abs(b0).max() * 10
+            assert np.abs(b - x).max() <= 1e-7 * np.abs(x).max()
+        else:
+            assert it == -1
+            assert np.array_equal(b, b0)

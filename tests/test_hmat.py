@@ -205,3 +205,46 @@ def test_compressed_sweep_matches_reference_and_dense(lib, name):
     assert float(np.max(np.abs(z - zo) / np.abs(zo))) <= 2e-5
     assert st.gmres_its > 0
     m.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(DECKS))
+def test_leaf_blocks(lib, name):
+    """hmat_nleaf / hmat_leaf_block (hmatrix.c:807-824): the leaves tile [0, n) in the
+    reference's cluster order (the golden partition's diagonal near blocks), and each leaf's
+    diagonal block holds the entries lp_entry / pot_entry give (never approximated)."""
+    m = Peec.from_deck_text(DECKS[name], lib)
+    m.setup()
+    f = m.freq(0)
+    kw = m.kw(f)
+    for kind, build, entry in (("lp", lib.hmat_build, lib.lp_entry), ("p", lib.hmat_build_pot, lib.pot_entry)):
+        key = f"hm/{name}/{kind}"
+        if f"{key}/perm" not in GOLD.files:
+            continue
+        print(f"[leaf] {name} {kind}", flush=True)
+        h = build(C.byref(m.p), f, None)
+        assert h
+        nleaf = lib.hmat_nleaf(h)
+        gperm, gnear = GOLD[f"{key}/perm"], GOLD[f"{key}/near"]
+        diag = sorted((int(r0), int(mm)) for r0, mm, c0, nn in gnear if r0 == c0 and mm == nn)
+        assert nleaf == len(diag)
+        seen = []
+        for r in range(nleaf):
+            idx = P(C.c_int)()
+            mm = C.c_int()
+            blk = lib.hmat_leaf_block(h, r, C.byref(idx), C.byref(mm))
+            assert blk
+            ids = [idx[a] for a in range(mm.value)]
+            seen += ids
+            assert any(sz == mm.value and list(gperm[r0:r0 + sz]) == ids for r0, sz in diag), (name, kind, r)
+            if r not in (0, nleaf // 2, nleaf - 1):
+                continue   # entries checked on three leaves (one device call per entry)
+            for a in range(mm.value):
+                for b in range(mm.value):
+                    e = entry(C.byref(m.p), ids[a], ids[b], kw)
+                    want = complex(e.r, e.i)
+                    got = complex(blk[a * mm.value + b].r, blk[a * mm.value + b].i)
+                    assert abs(got - want) <= 1e-12 * max(abs(want), 1e-300), (name, kind, r, a, b, got, want)
+        assert sorted(seen) == list(range(len(gperm)))
+        lib.hmat_free(h)
+    m.close()
