@@ -237,10 +237,8 @@ int    pot_fill(peec_t *p, double f, FILE *fp_log);                             
 int    pot_cells(peec_t *p);                                                    /* potential.c:168 */
 d_complex_t pot_entry(const peec_t *p, int i, int j, double kw);                /* potential.c:124 */
 
-/* ---- skin effect (reference peec.h:315-317) ---- */
-d_complex_t zint_round(double len, double a, double sigma, double freq);
-d_complex_t zint_bar(double len, double area, double perim, double sigma, double freq);
-d_complex_t zint_seg(const seg_t *s, double freq);
+/* skin effect (reference peec.h:315-317, skin.c) is not part of this path: the deck reader
+ * refuses skineffect = 1 and the zint_* functions are not exported. */
 
 /* ---- H-matrix compression (reference peec.h:277-296); device resident ---- */
 struct hmat_t;

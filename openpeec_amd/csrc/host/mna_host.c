@@ -208,9 +208,8 @@ static void stamps_any(const peec_t *p, double f, stamps_t *st, int charge_off)
 		stamp(&a, rk, i1, d_complex(1, 0));
 		stamp(&a, i2, rk, d_complex(-1, 0));
 		stamp(&a, rk, i2, d_complex(-1, 0));
-		const d_complex_t zint = p->seg[k].diel ? zint_dielectric(&p->seg[k], omega, f)
-		                       : p->skin        ? zint_seg(&p->seg[k], f)
-		                                        : d_complex(p->seg[k].res, 0);
+		/* internal impedance: the DC resistance (skin effect is refused by the deck reader) */
+		const d_complex_t zint = p->seg[k].diel ? zint_dielectric(&p->seg[k], omega, f) : d_complex(p->seg[k].res, 0);
 		stamp(&a, rk, rk, d_rmul(-1, zint));
 	}
 	/* compressed capacitive form: KCL + jw q and charge-row -v couplings (mna.c:271-283) */

@@ -22,6 +22,7 @@ _libc.fmemopen.argtypes = [C.c_void_p, C.c_size_t, C.c_char_p]
 _libc.fopen.restype = C.c_void_p
 _libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
 _libc.fclose.argtypes = [C.c_void_p]
+_libc.fflush.argtypes = [C.c_void_p]
 
 _ref = None
 

@@ -24,9 +24,15 @@ def seg_hash(p, which):
     return hashlib.sha256(raw).hexdigest()
 
 
+def without_skin(text):
+    """The deck with skineffect switched off: the library refuses skin effect (outside this path),
+    and the cells do not depend on it."""
+    return "\n".join(ln for ln in text.splitlines() if not ln.strip().startswith("skineffect"))
+
+
 @pytest.mark.parametrize("name", GEOM)
 def test_cells_bitwise_equal_reference_hash(lib, name):
-    m = Peec.from_deck_text(deck_text(name + ".peec"), lib)
+    m = Peec.from_deck_text(without_skin(deck_text(name + ".peec")), lib)
     m.wire_build()
     want = GOLD[f"geom/{name}"]
     assert seg_hash(m.p, "seg") == want[0]
@@ -78,3 +84,86 @@ def test_deck_errors_match_reference_rules(lib, capfd):
     disk96 = "OpenPEEC 1 0\ndisk = 0 0 0 0 0 1 0.05 3.5e-5 5.8e7 4 96\nport = 1 2 50\nfrequency = 1e6 1e6 0\nend\n"
     with pytest.raises(ValueError):
         Peec.from_deck_text(disk96, lib)
+
+
+def _c_stdout(capfd):
+    """what the library printed (C stdio is flushed first)"""
+    from refharness import _libc
+    _libc.fflush(None)
+    return capfd.readouterr().out
+
+
+def test_skin_effect_is_refused(lib, capfd):
+    with pytest.raises(ValueError):
+        Peec.from_deck_text(deck_text("dipole_full.peec"), lib)
+    assert "skineffect = 1 is not available" in _c_stdout(capfd)
+
+
+@pytest.mark.parametrize("bad,msg", [
+    ("resistor = 1 2 -5", "*** invalid resistor data"),
+    ("wire = 0 0 0 0 0 0 1e-3 5.8e7 4", "*** invalid wire data"),
+    ("plate = 0 0 0 1 0 0 0.5 0.5 0 1e-4 5.8e7 4 4", "*** plate : the two edge vectors must be perpendicular"),
+    ("quad = 0 0 0 1 0 0 1 1 0 0 1 0.5 1e-4 5.8e7 2 2", "*** quad : vertices must be coplanar"),
+    ("quad = 0 0 0 1 0 0 0 1 0 1 1 0 1e-4 5.8e7 2 2", "*** invalid quad data"),
+    ("quad = 0 0 0 2 0 0 1 0.5 0 1 2 0 1e-4 5.8e7 2 2", "*** quad : must be convex (vertices in cyclic order)"),
+    ("disk = 0 0 0 0 0 1 0.05 1e-4 5.8e7 4 33", "*** invalid disk data"),
+    ("mutual = LA LB 0.5", "*** mutual : unknown or same inductor : LA LB"),
+    ("port = 1 1 50", "*** invalid port data"),
+    ("planewave = 90 0 3 1.0", "*** invalid planewave data"),
+    ("node = -1 0 0 0", "*** invalid node data"),
+])
+def test_deck_errors_match_reference_messages(lib, capfd, bad, msg):
+    """Rejected decks print the reference's message (input_data.c) and return 1; each case's
+    message was checked against the reference build's input_data on the same text."""
+    text = "OpenPEEC 1 0\nresistor = 1 0 50\n" + bad + "\nport = 1 0 50\nfrequency = 1e6 1e7 1\nend\n"
+    with pytest.raises(ValueError):
+        Peec.from_deck_text(text, lib)
+    assert msg in _c_stdout(capfd)
+
+
+def test_deck_reader_line_rules(lib):
+    """Comments, blank lines, CR-LF endings, malformed lines and unknown keys are passed over;
+    lines after \"end\" are not read; optional fields take their defaults."""
+    text = ("# comment\r\nOpenPEEC 1 0\r\n\r\nnot a statement\r\nfuture_key = 3\r\n"
+            "title =   two resistors \r\nresistor = 1 2 50\r\nresistor = 2 0 25\r\nvsource = 1 0 2.0\r\n"
+            "port = 1 0 50\r\nfrequency = 1e6 1e7 4 log\r\nend\r\nresistor = 7 8 9\r\n")
+    m = Peec.from_deck_text(text, lib)
+    assert m.p.title.decode() == "two resistors "
+    assert (m.p.nres, m.p.nsrc, m.p.nport, m.p.nfreq, m.p.flog, m.p.maxid) == (2, 1, 1, 5, 1, 2)
+    assert m.p.src[0].phase == 0 and m.p.src[0].isvsrc == 1 and m.p.src[0].ibr == -1
+    assert (m.p.nodetol, m.p.tranatt, m.p.ctol) == (1e-8, 40, 1e-6)
+    m.close()
+
+
+def _netlist(p):
+    """every field input_data fills, as bytes: scalars of peec_t up to the cell arrays and the
+    element arrays"""
+    arrays = [("res", p.nres, abi.rc_t), ("cap", p.ncap, abi.rc_t), ("ind", p.nind, abi.ind_t), ("mut", p.nmut, abi.mut_t),
+              ("src", p.nsrc, abi.src_t), ("port", p.nport, abi.port_t), ("wire", p.nwire, abi.wire_t),
+              ("plate", p.nplate, abi.plate_t), ("panel", p.npanel, abi.panel_t), ("diel", p.ndiel, abi.diel_t),
+              ("ncid", p.nnodexyz, C.c_int), ("ncxyz", 3 * p.nnodexyz, C.c_double)]
+    out = {name: C.string_at(getattr(p, name), n * C.sizeof(t)) if n else b"" for name, n, t in arrays}
+    skip = {name for name, _, _ in arrays} | {"seg", "chg", "lp", "chgnode", "cellid", "cellof", "carea", "csoff", "csidx",
+                                               "cpt", "cmat", "gid", "gxyz", "nodemap", "hlp", "zin", "zmat", "smat", "segi",
+                                               "cellq", "voc", "segipw"}
+    for name, _ in abi.peec_t._fields_:
+        if name not in skip:
+            v = getattr(p, name)
+            out[name] = bytes(v) if isinstance(v, (bytes, C.Array)) else v
+    return out
+
+
+@pytest.mark.skipif(not ref_available(), reason="reference build not present")
+@pytest.mark.parametrize("deck", sorted(p.name for p in DECKS.glob("*.peec")))
+def test_netlist_equals_reference_live(lib, deck):
+    """The parsed netlist of every sample deck, field by field, against the reference's
+    input_data on the same text (skin effect switched off: refused here)."""
+    text = without_skin(deck_text(deck))
+    mine = Peec.from_deck_text(text, lib)
+    ref = RefPeec(text)
+    a, b = _netlist(mine.p), _netlist(ref.p)
+    assert a.keys() == b.keys()
+    for k in a:
+        assert a[k] == b[k], (deck, k)
+    mine.close()
+    ref.close()
