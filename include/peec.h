@@ -243,8 +243,9 @@ d_complex_t pot_entry(const peec_t *p, int i, int j, double kw);                
 /* ---- H-matrix compression (reference peec.h:277-296); device resident ---- */
 struct hmat_t;
 typedef d_complex_t (*hmat_entry_fn)(void *ctx, int i, int j, double kw);
-/* Entries are evaluated on the GPU: fn must be the library's own lp_entry / pot_entry kernels
- * (hmat_build / hmat_build_pot pass them); any other callback is refused (NULL returned). */
+/* ACA and the near blocks run on the GPU.  The library's own lp_entry / pot_entry are evaluated
+ * there too; any other (re-entrant) callback is evaluated once per entry on the host, in
+ * parallel, and its n^2 entries uploaded (NULL when they exceed PEEC_HM_GEN_GB GiB, default 16). */
 struct hmat_t *hmat_build_gen(int n, const double *pts, hmat_entry_fn fn, void *ctx, double kw,
 	double tol, const char *label, FILE *fp_log);                                /* hmatrix.c:403 */
 struct hmat_t *hmat_build(peec_t *p, double f, FILE *fp_log);                   /* hmatrix.c:616 */

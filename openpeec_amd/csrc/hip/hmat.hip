@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 #include <vector>
 #include "elements.hpp"
 #include "fill_entries.hpp"
@@ -646,7 +647,7 @@ void dense_dispatch(int km, hipStream_t s, const pe::hent &E, const int *perm, c
 // Build the compressed kernel matrix of a cell set (kind 0 Lp over the current cells, 1 P over
 // the capacitive cells) at wavenumber kw.  pts: representative points in original numbering.
 struct hmat_t *hm_build_dev(pgpu_t *g, int kind, int n, const double *pts, double kw, double tol, const char *label,
-                            FILE *fp_log)
+                            FILE *fp_log, const double2 *host_full)
 {
 	if (n <= 0) return nullptr;
 	hmat_t *h = new hmat_t();
@@ -655,28 +656,44 @@ struct hmat_t *hm_build_dev(pgpu_t *g, int kind, int n, const double *pts, doubl
 	h->kw = kw;
 	h->tol = tol;
 	h->g = g;
-	h->s = pgpu_hip_stream(g);
+	if (g != nullptr) {
+		h->s = pgpu_hip_stream(g);
+	}
+	else {
+		HIPCHK(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
+		h->own_stream = true;
+	}
 	hipStream_t s = h->s;
 	if (hm_tree_build(&h->t, n, pts)) {
 		if (fp_log) fprintf(fp_log, "*** compression : leaf clusters do not tile [0, %d)\n", n);
+		if (h->own_stream) HIPCHK(hipStreamDestroy(h->s));
 		delete h;
 		return nullptr;
 	}
 	const hm_tree_t &t = h->t;
-	int km = 0;
-	const int *csoff = nullptr, *csidx = nullptr;
+	int km = 15;
 	pe::hent E;
-	E.f = pgpu_set_args(g, kind, kw, &km, &csoff, &csidx);
+	memset(&E, 0, sizeof(E));
+	if (g != nullptr) {
+		const int *csoff = nullptr, *csidx = nullptr;
+		E.f = pgpu_set_args(g, kind, kw, &km, &csoff, &csidx);
+		E.csoff = csoff;
+		E.csidx = csidx;
+		E.rec = pgpu_kf_records(g, kind, kw, &E.nslots, &E.T);
+	}
 	E.kind = kind;
-	E.csoff = csoff;
-	E.csidx = csidx;
-	E.rec = pgpu_kf_records(g, kind, kw, &E.nslots, &E.T);
 	E.full = nullptr;
 	E.nfull = n;
 	h->km = km;
 	h->d_perm = up(s, std::vector<int>(t.perm, t.perm + n));
 	double2 *d_full = nullptr;
-	if (hm_full_wanted(n) && (hipMalloc((void **)&d_full, (size_t)n * n * sizeof(double2)) == hipSuccess)) {
+	if (host_full != nullptr) {
+		// the caller's entries: ACA and the dense blocks read them instead of the element code
+		d_full = dal<double2>((size_t)n * n);
+		HIPCHK(hipMemcpyAsync(d_full, host_full, (size_t)n * n * sizeof(double2), hipMemcpyHostToDevice, s));
+		E.full = d_full;
+	}
+	else if (hm_full_wanted(n) && (hipMalloc((void **)&d_full, (size_t)n * n * sizeof(double2)) == hipSuccess)) {
 		if ((E.rec == nullptr) || (pgpu_kf_fill_kind(g, kind, kw, (d_complex_t *)d_full) != 0)) full_dispatch(km, s, E, n, d_full);
 		E.full = d_full;
 	}
@@ -876,18 +893,41 @@ void hm_near_norms(const hmat_t *h, double *off, double *diag)
 
 extern "C" {
 
+// hmatrix.c:403-614 for any kernel: the cluster tree and partition of the n points, then ACA
+// per far block and the dense near blocks on the device.  The caller's fn is a host function,
+// so its n^2 entries are evaluated once on the host (in parallel: the reference requires fn to
+// be re-entrant, hmatrix.c:427-454), uploaded, and read by the device ACA exactly where the
+// library's own kernels read their bulk entries.  The library's own lp_entry / pot_entry are
+// recognised and evaluated on the device instead.  NULL when n^2 entries exceed
+// PEEC_HM_GEN_GB (default 16) GiB of host memory.
 struct hmat_t *hmat_build_gen(int n, const double *pts, hmat_entry_fn fn, void *ctx, double kw, double tol,
                               const char *label, FILE *fp_log)
 {
-	(void)n;
-	(void)pts;
-	(void)fn;
-	(void)ctx;
-	(void)kw;
-	(void)tol;
-	if (fp_log) fprintf(fp_log, "*** compression %s : entries are evaluated on the device; only hmat_build / hmat_build_pot kernels are available\n",
-	                    label ? label : "");
-	return nullptr;
+	if ((n <= 0) || (fn == nullptr)) return nullptr;
+	const char *lab = label ? label : "";
+	const peec_t *p = (const peec_t *)ctx;
+	if ((fn == (hmat_entry_fn)lp_entry) && (n == p->nseg)) return hm_build_dev(pgpu_get(p), 0, n, pts, kw, tol, lab, fp_log);
+	if ((fn == (hmat_entry_fn)pot_entry) && (n == p->ncell)) return hm_build_dev(pgpu_get(p), 1, n, pts, kw, tol, lab, fp_log);
+	const char *env = getenv("PEEC_HM_GEN_GB");
+	const double cap_gb = env ? atof(env) : 16.0;
+	if ((double)n * n * sizeof(double2) > cap_gb * 1073741824.0) {
+		if (fp_log) fprintf(fp_log, "*** compression %s : %d x %d caller entries exceed %.0f GiB of host memory\n", lab, n, n, cap_gb);
+		return nullptr;
+	}
+	if (!pgpu_device_ok()) pgpu_fatal("no HIP device");
+	std::vector<double2> full((size_t)n * n);
+	const int nth = std::max(1, std::min((int)std::thread::hardware_concurrency(), 64));
+	std::vector<std::thread> pool;
+	for (int w = 0; w < nth; w++)
+		pool.emplace_back([&, w]() {
+			for (int i = w; i < n; i += nth)
+				for (int j = 0; j < n; j++) {
+					const d_complex_t v = fn(ctx, i, j, kw);
+					full[(size_t)i * n + j] = make_double2(v.r, v.i);
+				}
+		});
+	for (std::thread &t : pool) t.join();
+	return hm_build_dev(nullptr, 0, n, pts, kw, tol, lab, fp_log, full.data());
 }
 
 struct hmat_t *hmat_build(peec_t *p, double f, FILE *fp_log)
@@ -917,6 +957,7 @@ void hmat_free(struct hmat_t *h)
 	                      h->L.rnoff, h->L.rnidx, h->d_ft, h->d_xp, h->d_x, h->d_y, h->d_part, h->d_item};
 	for (const void *b : bufs)
 		if (b) HIPCHK(hipFree(const_cast<void *>(b)));
+	if (h->own_stream) HIPCHK(hipStreamDestroy(h->s));
 	hm_tree_free(&h->t);
 	delete h;
 }

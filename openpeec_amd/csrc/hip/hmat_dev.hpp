@@ -60,6 +60,7 @@ struct hmat_t {
 	double kw = 0, tol = 0;
 	pgpu_t *g = nullptr;
 	hipStream_t s = nullptr;
+	bool own_stream = false;     // built from caller entries (hmat_build_gen): its own stream
 	hm_tree_t t{};
 	int *d_perm = nullptr;
 	pe::hfar *d_far = nullptr;
@@ -78,8 +79,10 @@ struct hmat_t {
 	std::vector<int> leafhas;
 };
 
+/* g: the device mirror whose cells give the entries (kind 0 Lp, 1 P); or NULL with host_full,
+ * the n x n entries in original numbering evaluated by the caller (hmat_build_gen) */
 struct hmat_t *hm_build_dev(pgpu_t *g, int kind, int n, const double *pts, double kw, double tol, const char *label,
-                            FILE *fp_log);
+                            FILE *fp_log, const double2 *host_full = nullptr);
 void hm_matvec_dev(const hmat_t *h, const double2 *x, double2 *y);
 void hm_near_off_dev(const hmat_t *h, const double2 *x, double2 *y);
 void hm_near_norms(const hmat_t *h, double *off, double *diag);

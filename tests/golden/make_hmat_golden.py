@@ -7,6 +7,8 @@ functions (oracle/_ref/libpeec_ref.so, compiled in place from the reference's C 
   hm/<deck>/<k>/x, y                     y = H x for a fixed seeded x
   hm/<deck>/pc/b, z                      z = M^-1 b (pc_apply) for a fixed seeded b
   hm/<deck>/zmat                         Z(f) of the reference's solve() (compressed path)
+  hm/gen/...                             the same records of hmat_build_gen with a caller's kernel
+                                         (tests/gen_kernel.c: a helix of 400 points, exp(-jkr) / (r + 0.01))
 
 Run in this container (the reference sources are read from /root/reference at build time of
 oracle/_ref only):  make -C oracle ref && python tests/golden/make_hmat_golden.py
@@ -100,10 +102,54 @@ def record(out: dict, key: str, lib, hp: int, seed: int) -> None:
     out[f"{key}/y"] = y
 
 
+# hmat_build_gen with a caller's kernel: points on a helix, A(i, j) = exp(-j k r) / (r + d0)
+GEN_N, GEN_KW, GEN_TOL = 400, 12.0, 1e-6
+
+
+def gen_points() -> np.ndarray:
+    t = np.linspace(0, 6 * np.pi, GEN_N)
+    return np.ascontiguousarray(np.stack([0.05 * np.cos(t), 0.05 * np.sin(t), 0.02 * t], axis=1).ravel())
+
+
+def gen_kernel_lib() -> C.CDLL:
+    """tests/gen_kernel.c (helix_kernel, an hmat_entry_fn) compiled into a temporary library"""
+    import subprocess
+    import tempfile
+    out = Path(tempfile.mkdtemp()) / "libgenkernel.so"
+    subprocess.run(["gcc", "-O2", "-fPIC", "-shared", str(HERE.parent / "gen_kernel.c"), "-o", str(out), "-lm"], check=True)
+    return C.CDLL(str(out))
+
+
+def gen_dense(pts: np.ndarray) -> np.ndarray:
+    xyz = pts.reshape(-1, 3)
+    d = np.linalg.norm(xyz[:, None, :] - xyz[None, :, :], axis=2)
+    return np.exp(-1j * GEN_KW * d) / (d + 0.01)
+
+
+def build_gen(lib, pts: np.ndarray) -> int:
+    klib = gen_kernel_lib()
+    lib.hmat_build_gen.restype = C.c_void_p
+    lib.hmat_build_gen.argtypes = [C.c_int, P(C.c_double), C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_char_p,
+                                   C.c_void_p]
+    fn = C.cast(klib.helix_kernel, C.c_void_p)
+    return lib.hmat_build_gen(GEN_N, pts.ctypes.data_as(P(C.c_double)), fn, pts.ctypes.data_as(C.c_void_p), GEN_KW, GEN_TOL,
+                              b"gen", None)
+
+
+def record_gen(out: dict, lib) -> None:
+    pts = gen_points()
+    h = build_gen(lib, pts)
+    assert h
+    record(out, "hm/gen", lib, h, 17)
+    lib.hmat_free(h)
+
+
 def main() -> None:
     if not ref_available():
         raise SystemExit("reference build missing: run `make -C oracle ref` first")
     out: dict[str, np.ndarray] = {}
+    from refharness import ref_lib
+    record_gen(out, ref_lib())
     for name, text in decks().items():
         r = RefPeec(text)
         r.setup_like_main()

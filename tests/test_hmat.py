@@ -248,3 +248,31 @@ def test_leaf_blocks(lib, name):
         assert sorted(seen) == list(range(len(gperm)))
         lib.hmat_free(h)
     m.close()
+
+
+@pytest.mark.gpu
+def test_build_gen_with_a_caller_kernel(lib):
+    """hmat_build_gen (hmatrix.c:403-614) with a kernel the caller supplies (tests/gen_kernel.c):
+    the block partition integer-identical to the reference's, ACA ranks as close as for the
+    library's kernels, and H x within 2e-6 of the reference's H x and of the dense product."""
+    from make_hmat_golden import GEN_TOL, build_gen, gen_dense, gen_points
+    pts = gen_points()
+    h = build_gen(lib, pts)
+    assert h
+    cnt = (C.c_int * 4)()
+    lib.hmat_info(h, cnt, None, None)
+    perm = np.zeros(cnt[3], dtype=np.int32)
+    ranks = np.zeros(max(1, cnt[0]), dtype=np.int32)
+    lib.hmat_info(h, cnt, perm.ctypes.data_as(P(C.c_int)), ranks.ctypes.data_as(P(C.c_int)))
+    assert np.array_equal(perm, GOLD["hm/gen/perm"])
+    assert cnt[0] == len(GOLD["hm/gen/far"]) and cnt[1] == len(GOLD["hm/gen/near"])
+    gr = GOLD["hm/gen/rank"]
+    ours = np.where(ranks[:cnt[0]] < 0, 0, ranks[:cnt[0]])
+    assert np.mean(ours == gr) >= 0.9 and np.max(np.abs(ours - gr)) <= 3
+    x = GOLD["hm/gen/x"]
+    y = np.zeros_like(x)
+    lib.hmat_matvec(h, x.ctypes.data_as(P(abi.d_complex_t)), y.ctypes.data_as(P(abi.d_complex_t)))
+    lib.hmat_free(h)
+    assert _rel(y, GOLD["hm/gen/y"]) <= 2e-6
+    dense = gen_dense(pts) @ x
+    assert np.linalg.norm(y - dense) <= 10 * GEN_TOL * np.linalg.norm(dense)
