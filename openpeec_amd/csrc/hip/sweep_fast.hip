@@ -228,7 +228,7 @@ __device__ __forceinline__ double row16_sum(double v)
 	return v;
 }
 
-template <int F, int RU>
+template <int F, int RU, bool ROWS>
 __global__ __launch_bounds__(256) void k_mvsym(slots S, int nact, int offS, int nseg, int offQ, int nc, int js,
                                                const double2 *__restrict__ X, double2 *__restrict__ partN,
                                                double2 *__restrict__ partT, int n)
@@ -236,7 +236,11 @@ __global__ __launch_bounds__(256) void k_mvsym(slots S, int nact, int offS, int 
 	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
 	const int cg = lane & 15, ph = lane >> 4;
 	const int ntS = (nseg + 63) / 64;
-	const int ib = blockIdx.x * 4 + wave, s = blockIdx.y;
+	// tile order: ROWS -- the four waves of a workgroup take four consecutive runs of one row
+	// block (a row is read 4 x 1 KB contiguously at once, and consecutive workgroups continue
+	// along it); otherwise four row blocks at one run offset
+	const int ib = ROWS ? (int)blockIdx.y : (int)blockIdx.x * 4 + wave;
+	const int s = ROWS ? (int)blockIdx.x * 4 + wave : (int)blockIdx.y;
 	const bool lblk = ib < ntS;
 	const int nb = lblk ? nseg : nc, o = lblk ? offS : offQ;
 	const int nt = (nb + 63) / 64;
@@ -699,22 +703,26 @@ int mv_cs()
 }
 
 // dense blocks of A x read as symmetric, upper triangle only (PEEC_MV_SYM=0: both triangles,
-// k_mvdense); PEEC_MV_JS 64 x 64 tiles per item (1 by default: the most items)
+// k_mvdense); PEEC_MV_JS 64 x 64 tiles per item (2 by default)
 bool mv_sym()
 {
 	const char *e = getenv("PEEC_MV_SYM");   // read per product (a few per GMRES iteration)
 	return !(e && (e[0] == '0'));
 }
 
+// k_mvsym's tile order: the waves of a workgroup along one row block (default; PEEC_MV_ROWS=0:
+// four row blocks at one run offset, 4.69 against 4.35 ms per product on the full stack)
+bool mv_rows()
+{
+	const char *e = getenv("PEEC_MV_ROWS");
+	return !(e && (e[0] == '0'));
+}
+
 int mv_js()
 {
-	static int v = -1;
-	if (v < 0) {
-		const char *e = getenv("PEEC_MV_JS");
-		const int j = e ? atoi(e) : 1;
-		v = ((j >= 1) && (j <= 64)) ? j : 1;
-	}
-	return v;
+	const char *e = getenv("PEEC_MV_JS");   // read per product, as PEEC_MV_SYM
+	const int j = e ? atoi(e) : 2;
+	return ((j >= 1) && (j <= 64)) ? j : 2;
 }
 
 // doubles of partial sums the products need for order n and F columns
@@ -871,18 +879,29 @@ void matvec(fast &c, hipStream_t s, const double2 *x, double2 *y, int F, int nac
 	if (mv_sym()) {
 		const int js = mv_js();
 		const int ntS = (c.nseg + 63) / 64, ntQ = (c.nc + 63) / 64, ntm = std::max(ntS, ntQ);
-		const dim3 grid((unsigned)((ntS + ntQ + 3) / 4), (unsigned)((ntm + js - 1) / js));
+		const int nseg_runs = (ntm + js - 1) / js;
+		const bool rows = mv_rows();
+		const dim3 grid = rows ? dim3((unsigned)((nseg_runs + 3) / 4), (unsigned)(ntS + ntQ))
+		                       : dim3((unsigned)((ntS + ntQ + 3) / 4), (unsigned)nseg_runs);
 		if ((ntS + ntQ) > 0) {
+#define PEEC_MVSYM_CASE(FF)                                                                                                      \
+	case FF:                                                                                                                     \
+		if (rows) hipLaunchKernelGGL((k_mvsym<FF, 2, true>), grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, js, x,  \
+		                             c.mvp, c.mvt, c.n);                                                                            \
+		else hipLaunchKernelGGL((k_mvsym<FF, 2, false>), grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, js, x,      \
+		                        c.mvp, c.mvt, c.n);                                                                                 \
+		break;
 			switch (F) {
-			case 1: hipLaunchKernelGGL((k_mvsym<1, 2>), grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, js, x, c.mvp, c.mvt, c.n); break;
-			case 2: hipLaunchKernelGGL((k_mvsym<2, 2>), grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, js, x, c.mvp, c.mvt, c.n); break;
-			case 3: hipLaunchKernelGGL((k_mvsym<3, 2>), grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, js, x, c.mvp, c.mvt, c.n); break;
-			case 4: hipLaunchKernelGGL((k_mvsym<4, 2>), grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, js, x, c.mvp, c.mvt, c.n); break;
-			case 5: hipLaunchKernelGGL((k_mvsym<5, 2>), grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, js, x, c.mvp, c.mvt, c.n); break;
-			case 6: hipLaunchKernelGGL((k_mvsym<6, 2>), grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, js, x, c.mvp, c.mvt, c.n); break;
-			case 7: hipLaunchKernelGGL((k_mvsym<7, 2>), grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, js, x, c.mvp, c.mvt, c.n); break;
-			default: hipLaunchKernelGGL((k_mvsym<8, 2>), grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, js, x, c.mvp, c.mvt, c.n); break;
+				PEEC_MVSYM_CASE(1)
+				PEEC_MVSYM_CASE(2)
+				PEEC_MVSYM_CASE(3)
+				PEEC_MVSYM_CASE(4)
+				PEEC_MVSYM_CASE(5)
+				PEEC_MVSYM_CASE(6)
+				PEEC_MVSYM_CASE(7)
+			default: PEEC_MVSYM_CASE(8)
 			}
+#undef PEEC_MVSYM_CASE
 		}
 		if (g_mvt.on) {
 			// the upper triangle's tiles of the nact systems (16 B per entry), x once, the item and
