@@ -344,7 +344,7 @@ LPC_TRAIN_PAIRS = [(fr, fr * s) for fr in (4e8, 1e9, 2e9) for s in (1.0, 1.3, 1.
 
 
 def bench_learned(lib, points: int = 101, aca_tol: float = 1e-12) -> dict:
-    """The learned-preconditioner workload: disk 48 x 32 under a 32 x 16 quad, compressed at ACA
+    """The learned-preconditioner workload: disk 48 x 96 under a 32 x 16 quad, compressed at ACA
     tolerance aca_tol, swept over `points` of 100 MHz - 4 GHz three ways: the reference's rule
     (precond.c's preconditioner rebuilt at every frequency, reference GMRES termination), the
     same preconditioner reused across frequencies without the network (theta = 0), and with the
@@ -384,7 +384,7 @@ def bench_learned(lib, points: int = 101, aca_tol: float = 1e-12) -> dict:
     text = decks.disk_quad_deck(ndiv=points - 1, ctol=aca_tol)
     m = Peec.from_deck_text(text, lib)
     m.setup()
-    out = {"workload": f"disk r = 50 mm, 48 rings x 32 sectors, under a 80 x 40 mm quad (32 x 16) 5 mm above, "
+    out = {"workload": f"disk r = 50 mm, 48 rings x 96 sectors, under a 80 x 40 mm quad (32 x 16) 5 mm above, "
                        f"compressed (ACA {aca_tol:g}), retarded + capacitive, {points} points 100 MHz-4 GHz",
            "unknowns": m.p.nunknown, "branches": m.p.nseg, "charge_cells": m.p.ncell,
            "training": {"samples": S, "geometries": len(LPC_TRAIN_GEOMS), "sample_s": t_samples, "fit_s": t_fit,

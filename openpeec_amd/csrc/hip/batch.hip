@@ -907,8 +907,8 @@ struct batch {
 	{
 		if (s) HIPCHK(hipStreamSynchronize(s));
 		for (auto &kv : graphs) {
-			hipGraphExecDestroy(kv.second.first);
-			hipGraphExecDestroy(kv.second.second);
+			HIPCHK(hipGraphExecDestroy(kv.second.first));
+			HIPCHK(hipGraphExecDestroy(kv.second.second));
 		}
 		if (s) HIPCHK(hipStreamDestroy(s));
 	}

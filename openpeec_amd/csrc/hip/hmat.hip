@@ -5,7 +5,9 @@
 // matrix entries runs here:
 //   * ACA, one 256-thread workgroup per far block (k_aca): the reference's partial-pivot
 //     adaptive cross approximation with its restarts from unrepresented rows (aca_block,
-//     hmatrix.c:250-395), step for step; the entry evaluations of a residual row/column and
+//     hmatrix.c:250-395), step for step, plus a column check where it stops (column_check:
+//     it only acts on blocks left far outside the tolerance, so elsewhere the ranks and
+//     factors are the reference's); the entry evaluations of a residual row/column and
 //     the norms are spread over the workgroup, reductions run in a fixed tree order, pivot
 //     searches keep the reference's first-extremum rule;
 //   * near blocks and far blocks that did not compress are evaluated densely (k_hm_dense);
@@ -195,6 +197,44 @@ __device__ __forceinline__ void sub_uv(double &ar, double &ai, double2 u, double
 	ai = ai - pi;
 }
 
+// Column check where the reference's ACA stops (not part of the reference): the unused column
+// least represented in the approximation (smallest max_l |v_l[j]|, first index among equals) is
+// marked and its residual A(:, jc) - sum_l u_l v_l[jc] formed in the free factor slot U[k].  A
+// block whose rows fall into groups that are exact multiples of each other can stop with a
+// residual row of zero while the pivots never reached the columns that tell the groups apart;
+// the reference then keeps an O(1) error.  Such a column's residual exceeds HM_COLCHECK times
+// the tolerance (a converged block stays within a few tol ||A_k||_F, so it never fires there
+// and the result is the reference's); the ACA then resumes at the unused row of largest
+// residual in that column.  Returns that row, or -1 to stop.  k = 0 and k = lim are left to
+// the reference's rule (stored dense).
+template <int KM>
+__device__ int column_check(const aca_args &a, const hfar &B, double2 *U, const double2 *V, int k, int lim,
+                            const int *rused, const double *vrep, int *cused, double fro2, double *rv, int *ri)
+{
+	if ((k <= 0) || (k >= lim)) return -1;
+	const int m = B.m, nn = B.nn;
+	const int jc = unrepresented(cused, vrep, nn, INFINITY, rv, ri);
+	if (jc < 0) return -1;
+	if (threadIdx.x == 0) cused[jc] = 1;
+	double r2 = 0, rmax = 0;
+	int imax = -1;
+	for (int i = threadIdx.x; i < m; i += AT) {
+		const cplx e = hm_pos<KM>(a.E, a.perm, B.r0 + i, B.c0 + jc);
+		double ar = e.r, ai = e.i;
+		for (int l = 0; l < k; l++) sub_uv(ar, ai, U[(long)l * m + i], V[(long)l * nn + jc]);
+		U[(long)k * m + i] = make_double2(ar, ai);
+		const double t = sqrt((ar * ar) + (ai * ai));
+		r2 += t * t;
+		if (!rused[i] && (t > rmax)) {
+			rmax = t;
+			imax = i;
+		}
+	}
+	r2 = wg_sum(r2, rv);
+	wg_argmax(rmax, imax, rv, ri);
+	return ((imax >= 0) && (sqrt(r2) > HM_COLCHECK * a.tol * sqrt(fro2))) ? imax : -1;
+}
+
 template <int KM>
 __global__ __launch_bounds__(AT) void k_aca(aca_args a)
 {
@@ -210,9 +250,9 @@ __global__ __launch_bounds__(AT) void k_aca(aca_args a)
 	const int lim = (kmax < HM_MAXIT) ? kmax : HM_MAXIT;
 	double2 *U = a.scr + a.soff[b];         // [lim][m]
 	double2 *V = U + (long)m * lim;         // [lim][nn]
-	double *urep = a.urep + a.moff[b];
-	int *rused = a.rused + a.moff[b];
-	for (int i = tid; i < m; i += AT) {
+	double *urep = a.urep + a.moff[b], *vrep = urep + m;
+	int *rused = a.rused + a.moff[b], *cused = rused + m;
+	for (int i = tid; i < m + nn; i += AT) {
 		urep[i] = 0;
 		rused[i] = 0;
 	}
@@ -240,9 +280,12 @@ __global__ __launch_bounds__(AT) void k_aca(aca_args a)
 		if (tid == 0) rused[irow] = 1;
 		__syncthreads();
 		if ((jmax < 0) || (amax <= 0) || (sqrt(rn2) <= a.tol * sqrt(fro2))) {
-			if (++probe > HM_PROBE) break;
-			const int nxt = unrepresented(rused, urep, m, a.tol * urmax, rv, ri);
-			if (nxt < 0) break;
+			int nxt = (++probe > HM_PROBE) ? -1 : unrepresented(rused, urep, m, a.tol * urmax, rv, ri);
+			if (nxt < 0) {
+				nxt = column_check<KM>(a, B, U, V, k, lim, rused, vrep, cused, fro2, rv, ri);
+				if (nxt < 0) break;
+				probe = 0;
+			}
 			irow = nxt;
 			continue;
 		}
@@ -253,10 +296,14 @@ __global__ __launch_bounds__(AT) void k_aca(aca_args a)
 		const double den = (piv.x * piv.x) + (piv.y * piv.y);
 		for (int j = tid; j < nn; j += AT) {
 			const double2 x = V[(long)k * nn + j];
-			V[(long)k * nn + j] = ((fabs(piv.x) <= 0) && (fabs(piv.y) <= 0))
-			                          ? make_double2(0, 0)
-			                          : make_double2(((x.x * piv.x) + (x.y * piv.y)) / den, ((x.y * piv.x) - (x.x * piv.y)) / den);
+			const double2 y = ((fabs(piv.x) <= 0) && (fabs(piv.y) <= 0))
+			                      ? make_double2(0, 0)
+			                      : make_double2(((x.x * piv.x) + (x.y * piv.y)) / den, ((x.y * piv.x) - (x.x * piv.y)) / den);
+			V[(long)k * nn + j] = y;
+			const double t = sqrt((y.x * y.x) + (y.y * y.y));
+			if (t > vrep[j]) vrep[j] = t;
 		}
+		if (tid == 0) cused[jmax] = 1;
 		__syncthreads();
 		// residual column  u = A(:, jmax) - sum_l u_l v_l[jmax]
 		for (int i = tid; i < m; i += AT) {
@@ -340,9 +387,12 @@ __global__ __launch_bounds__(AT) void k_aca(aca_args a)
 			wg_argmax(umax, inext, rv, ri);
 		}
 		if (inext < 0) {
-			if (++probe > HM_PROBE) break;
-			inext = unrepresented(rused, urep, m, a.tol * urmax, rv, ri);
-			if (inext < 0) break;
+			if (++probe <= HM_PROBE) inext = unrepresented(rused, urep, m, a.tol * urmax, rv, ri);
+			if (inext < 0) {
+				inext = column_check<KM>(a, B, U, V, k, lim, rused, vrep, cused, fro2, rv, ri);
+				if (inext < 0) break;
+				probe = 0;
+			}
 		}
 		irow = inext;
 	}
@@ -714,7 +764,7 @@ struct hmat_t *hm_build_dev(pgpu_t *g, int kind, int n, const double *pts, doubl
 		B.uo = B.vo = B.fo = 0;
 		const int kmax = std::min(B.m, B.nn), lim = std::min(kmax, HM_MAXIT);
 		soff[b + 1] = soff[b] + (long)(B.m + B.nn) * lim;
-		moff[b + 1] = moff[b] + B.m;
+		moff[b + 1] = moff[b] + B.m + B.nn;
 	}
 	std::vector<int> rank(t.nfar, 0);
 	pe::hfar *d_far = up(s, far);

@@ -45,10 +45,10 @@ struct aca_args {
 	hent E;
 	const int *perm;
 	const hfar *far;
-	const long *soff, *moff;       // scratch offsets: factors (double2) and row flags
+	const long *soff, *moff;       // scratch offsets: factors (double2); row then column flags
 	double2 *scr;
-	double *urep;
-	int *rused;
+	double *urep;                  // per block: max |u_l[i]| over rows, then max |v_l[j]| over columns
+	int *rused;                    // per block: rows visited, then columns pivoted or checked
 	int *rank;
 	double tol;
 };

@@ -394,8 +394,10 @@ static int kw_quad(peec_t *p, deck_t *d, cursor_t *c, size_t arg)
 	return KW_OK;
 }
 
-/* disk = centre normal radius thick sigma nrings nsectors; the centre cell is a 2 nsec-gon,
- * so nsec is at most POLY_MAX / 2 */
+/* disk = centre normal radius thick sigma nrings nsectors.  The reference stops at nsec =
+ * POLY_MAX / 2 (its centre cell is a 2 nsec-gon in one cell record); here larger disks, up to
+ * DISK_NSEC_MAX sectors, take the centre ring as several sector cells (geometry.c) */
+#define DISK_NSEC_MAX 4096
 static int kw_disk(peec_t *p, deck_t *d, cursor_t *c, size_t arg)
 {
 	(void)arg;
@@ -410,7 +412,7 @@ static int kw_disk(peec_t *p, deck_t *d, cursor_t *c, size_t arg)
 	e->ndiva = int_field(c);
 	e->ndivb = int_field(c);
 	require(c, (norm3(e->nrm) > 0) && (e->radius > 0) && (e->thick > 0) && (e->sigma >= 0) && (e->ndiva >= 1) &&
-	               (e->ndivb >= 3) && (e->ndivb <= POLY_MAX / 2));
+	               (e->ndivb >= 3) && (e->ndivb <= DISK_NSEC_MAX));
 	if (c->bad) return KW_BAD;
 	p->npanel++;
 	return KW_OK;

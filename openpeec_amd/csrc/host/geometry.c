@@ -399,6 +399,21 @@ static void panel_branch(peec_t *p, const panel_t *pa, int n1, int n2, int i1, i
 	p->nseg++;
 }
 
+/* sector cells the disk's centre ring is emitted as: 1 while its 2 nsec vertices fit a cell,
+ * else the fewest equal groups of nsec / k sectors with 2 nsec / k + 2 <= POLY_MAX vertices.
+ * PEEC_DISK_CENTRE_CELLS=k asks for k groups where k divides nsec and they fit (tests compare
+ * a split centre with the reference's single cell). */
+static int centre_split(int nsec)
+{
+	const char *e = getenv("PEEC_DISK_CENTRE_CELLS");
+	const int want = e ? atoi(e) : 0;
+	if ((want >= 1) && (nsec % want == 0) && ((want == 1 ? 2 * nsec : 2 * (nsec / want) + 2) <= POLY_MAX)) return want;
+	if (2 * nsec <= POLY_MAX) return 1;
+	for (int k = 2; k <= nsec; k++)
+		if ((nsec % k == 0) && (2 * (nsec / k) + 2 <= POLY_MAX)) return k;
+	return nsec;
+}
+
 static void poly_charge_finish(peec_t *p, seg_t *c, const panel_t *pa, const double *v0, int node)
 {
 	memcpy(c->papex, v0, 3 * sizeof(double));
@@ -516,20 +531,34 @@ static int build_panels(peec_t *p, nodes_t *nt)
 			}
 		}
 		{
-			/* centre charge cell: ring of spoke midpoints and first-ring cell centres */
-			seg_t *c = &p->chg[p->nchg];
-			memset(c, 0, sizeof(seg_t));
-			c->shape = SHAPE_POLY;
+			/* centre charge cell: ring of spoke midpoints and first-ring cell centres, fanned
+			 * from the centre.  A ring of more than POLY_MAX vertices (nsec > 32, beyond the
+			 * reference) is emitted as k congruent sector cells on the centre node, each the
+			 * fan of nsec / k sectors closed through the centre: their fan triangles are the
+			 * ring's own, and with equal areas the per-cell normalisation (len = sqrt(area),
+			 * cell area = sum of len) gives the whole cell's potential coefficients exactly */
 			double v0[3], q[3];
 			panel_point(pa, 0, 0, v0);
-			int n = 0;
-			for (int j = 0; j < nb; j++) {
-				panel_point(pa, 1, j, q);
-				vmid(v0, q, &c->pv[3 * n++]);
-				panel_centre(pa, 0, j, &c->pv[3 * n++]);
+			const int k = centre_split(nb);
+			const int per = nb / k;
+			for (int part = 0; part < k; part++) {
+				seg_t *c = &p->chg[p->nchg];
+				memset(c, 0, sizeof(seg_t));
+				c->shape = SHAPE_POLY;
+				int n = 0;
+				if (k > 1) memcpy(&c->pv[3 * n++], v0, 3 * sizeof(double));
+				for (int j = part * per; j < (part + 1) * per; j++) {
+					panel_point(pa, 1, j, q);
+					vmid(v0, q, &c->pv[3 * n++]);
+					panel_centre(pa, 0, j, &c->pv[3 * n++]);
+				}
+				if (k > 1) {
+					panel_point(pa, 1, ((part + 1) * per) % nb, q);
+					vmid(v0, q, &c->pv[3 * n++]);
+				}
+				c->npv = n;
+				poly_charge_finish(p, c, pa, v0, nid[0]);
 			}
-			c->npv = n;
-			poly_charge_finish(p, c, pa, v0, nid[0]);
 		}
 		for (int i = 1; i <= na; i++)
 			for (int j = 0; j < nb; j++)
@@ -741,7 +770,7 @@ int wire_build(peec_t *p, FILE *fp_log)
 		else {
 			npoint += 1 + (na * nb);
 			nseg += 2 * na * nb;
-			nchg += 1 + (na * nb);
+			nchg += centre_split(nb) + (na * nb);
 		}
 	}
 	for (int i = 0; i < p->ndiel; i++) {

@@ -71,6 +71,9 @@ int cells_pack(const seg_t *src, int n, int mirror, double gpz,
 #define HM_ETA   2.0
 #define HM_MAXIT 200
 #define HM_PROBE 4
+/* column check at ACA termination (not in the reference): the block is taken as not converged
+ * when the least-represented unused column's residual exceeds HM_COLCHECK tol ||A_k||_F */
+#define HM_COLCHECK 1e3
 typedef struct {
 	int n;
 	int *perm;                       /* cluster order -> cell index */

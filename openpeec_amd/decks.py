@@ -8,9 +8,10 @@ Each function returns .peec text in the reference's deck grammar (OpenPEEC input
                    10 MHz - 3 GHz in 201 points (n = 64: the precision workload)
   random_wire_decks  seeded random polylines (batching workload), 20-400 segments each
   disk_quad_deck   disk panel under a quad panel, compressed (learned-preconditioner workload:
-                   nr = 48 rings, 32 x 16 quad, 100 MHz - 4 GHz in 101 points; nsec = 32 sectors,
-                   the most a disk takes: its centre cell is a 2 nsec-gon and a cell record
-                   holds at most 64 polygon vertices, peec.h seg_t.pv)
+                   48 rings x 96 sectors, 32 x 16 quad, 100 MHz - 4 GHz in 101 points; the disk's
+                   centre ring of 192 vertices is built as 4 sector cells, since a cell record
+                   holds at most 64 polygon vertices, peec.h seg_t.pv; the reference itself stops
+                   at 32 sectors)
 """
 from __future__ import annotations
 
@@ -108,7 +109,7 @@ def random_wire_deck(seed: int, nmin: int = 20, nmax: int = 400, box: float = 0.
     return "\n".join(lines)
 
 
-def disk_quad_deck(nr: int = 48, nsec: int = 32, qx: int = 32, qy: int = 16, radius: float = 0.05,
+def disk_quad_deck(nr: int = 48, nsec: int = 96, qx: int = 32, qy: int = 16, radius: float = 0.05,
                    qw: float = 0.08, qh: float = 0.04, gap: float = 0.005, f0: float = 1e8, f1: float = 4e9,
                    ndiv: int = 100, ctol: float = 1e-6, thick: float = 3.5e-5) -> str:
     """Disk panel of radius `radius` (nr rings x nsec sectors) at z = 0 and a qw x qh quad panel

@@ -9,6 +9,14 @@ functions (oracle/_ref/libpeec_ref.so, compiled in place from the reference's C 
   hm/<deck>/zmat                         Z(f) of the reference's solve() (compressed path)
   hm/gen/...                             the same records of hmat_build_gen with a caller's kernel
                                          (tests/gen_kernel.c: a helix of 400 points, exp(-jkr) / (r + 0.01))
+  hm/disk96/lp/...                       the reference's hmat_build (ACA 1e-12) on a 12 x 96 disk under a
+                                         16 x 8 quad, plus ydense = Lp x from its lp_fill: a geometry on
+                                         which the reference's ACA keeps blocks with errors up to 5e-5
+                                         (H x 7e-7 from the dense product).  Its parser stops at 32
+                                         sectors, so the cells come from this library's wire_build (the
+                                         disk's branches are built by the reference's rules for any
+                                         sector count; only the centre charge cell differs, and Lp does
+                                         not involve it)
 
 Run in this container (the reference sources are read from /root/reference at build time of
 oracle/_ref only):  make -C oracle ref && python tests/golden/make_hmat_golden.py
@@ -144,12 +152,37 @@ def record_gen(out: dict, lib) -> None:
     lib.hmat_free(h)
 
 
+DISK96 = dict(nr=12, nsec=96, qx=16, qy=8, ctol=1e-12, f0=1.27e9, f1=1.27e9, ndiv=0)
+
+
+def disk96_text() -> str:
+    from openpeec_amd import decks as dk
+    return dk.disk_quad_deck(**DISK96)
+
+
+def record_disk96(out: dict, lib) -> None:
+    from openpeec_amd import Peec, load_library
+    from refharness import devnull
+    m = Peec.from_deck_text(disk96_text(), load_library(build_if_missing=False))
+    m.setup()
+    f = DISK96["f0"]
+    h = lib.hmat_build(C.byref(m.p), f, None)
+    assert h
+    record(out, "hm/disk96/lp", lib, h, 23)
+    lib.hmat_free(h)
+    lib.lp_fill(C.byref(m.p), f, devnull())
+    n = m.p.nseg
+    out["hm/disk96/lp/ydense"] = cview(m.p.lp, (n, n)) @ out["hm/disk96/lp/x"]
+    m.close()
+
+
 def main() -> None:
     if not ref_available():
         raise SystemExit("reference build missing: run `make -C oracle ref` first")
     out: dict[str, np.ndarray] = {}
     from refharness import ref_lib
     record_gen(out, ref_lib())
+    record_disk96(out, ref_lib())
     for name, text in decks().items():
         r = RefPeec(text)
         r.setup_like_main()

@@ -81,9 +81,12 @@ def test_deck_errors_match_reference_rules(lib, capfd):
         Peec.from_deck_text(bad, lib)
     C.CDLL(None).fflush(None)   # the library's stdout is a buffered C stream
     assert "no port data" in capfd.readouterr().out
+    # beyond the reference: a 96-sector disk (its centre ring split into 4 sector cells) is read
     disk96 = "OpenPEEC 1 0\ndisk = 0 0 0 0 0 1 0.05 3.5e-5 5.8e7 4 96\nport = 1 2 50\nfrequency = 1e6 1e6 0\nend\n"
-    with pytest.raises(ValueError):
-        Peec.from_deck_text(disk96, lib)
+    m = Peec.from_deck_text(disk96, lib)
+    m.wire_build()
+    assert m.p.nchg == 4 + 4 * 96 and max(m.p.chg[i].npv for i in range(m.p.nchg)) <= 64
+    m.close()
 
 
 def _c_stdout(capfd):
@@ -106,7 +109,7 @@ def test_skin_effect_is_refused(lib, capfd):
     ("quad = 0 0 0 1 0 0 1 1 0 0 1 0.5 1e-4 5.8e7 2 2", "*** quad : vertices must be coplanar"),
     ("quad = 0 0 0 1 0 0 0 1 0 1 1 0 1e-4 5.8e7 2 2", "*** invalid quad data"),
     ("quad = 0 0 0 2 0 0 1 0.5 0 1 2 0 1e-4 5.8e7 2 2", "*** quad : must be convex (vertices in cyclic order)"),
-    ("disk = 0 0 0 0 0 1 0.05 1e-4 5.8e7 4 33", "*** invalid disk data"),
+    ("disk = 0 0 0 0 0 1 0.05 1e-4 5.8e7 4 2", "*** invalid disk data"),
     ("mutual = LA LB 0.5", "*** mutual : unknown or same inductor : LA LB"),
     ("port = 1 1 50", "*** invalid port data"),
     ("planewave = 90 0 3 1.0", "*** invalid planewave data"),

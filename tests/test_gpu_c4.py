@@ -86,3 +86,19 @@ def test_compressed_default_aca_is_truncation_limited_like_the_reference(lib):
     print("device vs dense", np.array2string(e_dev, precision=1))
     print("reference vs dense", np.array2string(e_ref, precision=1))
     assert np.all(e_dev <= 10 * e_ref + 1e-7)
+
+
+def test_split_centre_disk_compressed_and_dense_match_the_oracle(lib):
+    """A disk beyond the reference's 32 sectors (12 rings x 40 sectors: the centre ring of 80
+    vertices built as 2 sector cells) under the 12 x 6 quad, 3 points: the dense plain form
+    within 1e-9 of the oracle on the same cells, the compressed form at ACA 1e-10 within 1e-6."""
+    from oracle_py import oracle_zmat
+    kw = dict(C4_SMALL, nsec=40, ndiv=2, f0=1e9, f1=4e9)
+    comp = decks.disk_quad_deck(ctol=1e-10, **kw)
+    dense = comp.replace("compression = 1 1e-10", "compression = 0")
+    print("[c4] split centre disk", flush=True)
+    zo = oracle_zmat(dense)
+    zd, _ = _sweep(lib, dense, PEEC_FORM_PLAIN)
+    assert rel_err(zd, zo) <= 1e-9, rel_err(zd, zo)
+    zc, _ = _sweep(lib, comp, PEEC_FORM_PLAIN)
+    assert rel_err(zc, zo) <= 1e-6, rel_err(zc, zo)

@@ -237,3 +237,49 @@ def test_full_stack_deck_fix():
     assert z.shape == (2, 1, 1) and np.all(np.isfinite(z))
     # a capacitive input at 20 MHz (plate over ground through the brick): Im Z < 0
     assert z[0, 0, 0].imag < 0
+
+
+def _disk_deck(nr, ns):
+    return ("OpenPEEC 1 0\ntitle = disk\ncapacitance = 1\nnode = 1 0 0 0\nnode = 2 0 -0.2 0\n"
+            f"disk = 0 0 0  0 0 1  0.1 1e-4 5.8e7 {nr} {ns}\nport = 1 2 50\nfrequency = 1e6 1e6 0\nend\n")
+
+
+def _disk_pc(text):
+    m = oracle_peec(text)
+    nc = m.p.ncell
+    P = np.zeros((nc, nc), dtype=np.complex128)
+    Cm = np.zeros_like(P)
+    assert ORC.orc_pot_fill(C.byref(m.p), 0.0, P.ctypes.data_as(C.POINTER(abi.d_complex_t)),
+                            Cm.ctypes.data_as(C.POINTER(abi.d_complex_t))) == 0
+    out = (m.p.nchg, nc, P, Cm.real.sum())
+    m.close()
+    return out
+
+
+@pytest.mark.parametrize("k", [2, 3])
+def test_disk_centre_ring_split_equals_single_cell(monkeypatch, k):
+    """A disk centre ring of more than POLY_MAX vertices (nsec > 32, beyond the reference) is
+    built as k congruent sector cells on the centre node.  Forced on a disk the reference can
+    build (nsec = 24), the split must reproduce the single cell: same capacitive cells, the centre
+    row of P to 1e-5 (its far entries differ only by which quadrature rule each sub-cell pair
+    takes), the centre diagonal and the total capacitance to 1e-7.  A normalisation by the
+    whole ring's area instead of the sum of the sub-cells' sqrt(area) would be off by sqrt(k)."""
+    text = _disk_deck(4, 24)
+    monkeypatch.setenv("PEEC_DISK_CENTRE_CELLS", "1")
+    n1, nc1, P1, C1 = _disk_pc(text)
+    monkeypatch.setenv("PEEC_DISK_CENTRE_CELLS", str(k))
+    nk, nck, Pk, Ck = _disk_pc(text)
+    assert (nk, nck) == (n1 + k - 1, nc1)
+    assert np.max(np.abs(Pk[0, 1:] - P1[0, 1:]) / np.abs(P1[0, 1:])) <= 1e-5
+    assert abs(Pk[0, 0] - P1[0, 0]) <= 1e-7 * abs(P1[0, 0])
+    assert abs(Ck / C1 - 1) <= 1e-7
+
+
+def test_disk_beyond_the_reference_converges_to_the_analytic_capacitance():
+    """The 8 x 96 disk (centre ring split in 4): total capacitance within 1 % of the thin disk's
+    8 eps0 a (7.08335e-12 F for a = 0.1 m, data/sample/peec_check.sh), closer than 8 x 32."""
+    _, _, _, c96 = _disk_pc(_disk_deck(8, 96))
+    _, _, _, c32 = _disk_pc(_disk_deck(8, 32))
+    ref = 7.08335e-12
+    assert abs(c96 / ref - 1) <= 0.01
+    assert abs(c96 / ref - 1) < abs(c32 / ref - 1)
