@@ -9,10 +9,10 @@ functions (oracle/_ref/libpeec_ref.so, compiled in place from the reference's C 
   hm/<deck>/zmat                         Z(f) of the reference's solve() (compressed path)
   hm/gen/...                             the same records of hmat_build_gen with a caller's kernel
                                          (tests/gen_kernel.c: a helix of 400 points, exp(-jkr) / (r + 0.01))
-  hm/disk96/lp/...                       the reference's hmat_build (ACA 1e-12) on a 12 x 96 disk under a
-                                         16 x 8 quad, plus ydense = Lp x from its lp_fill: a geometry on
-                                         which the reference's ACA keeps blocks with errors up to 5e-5
-                                         (H x 7e-7 from the dense product).  Its parser stops at 32
+  hm/disk96/lp/...                       the reference's hmat_build (ACA 1e-12) on the learned-preconditioner
+                                         geometry (48 x 96 disk under a 32 x 16 quad) at 1.27 GHz, plus
+                                         ydense = Lp x from its lp_fill: its ACA keeps blocks with O(1)
+                                         errors there (H x 1.2e-3 from the dense product).  Its parser stops at 32
                                          sectors, so the cells come from this library's wire_build (the
                                          disk's branches are built by the reference's rules for any
                                          sector count; only the centre charge cell differs, and Lp does
@@ -152,7 +152,7 @@ def record_gen(out: dict, lib) -> None:
     lib.hmat_free(h)
 
 
-DISK96 = dict(nr=12, nsec=96, qx=16, qy=8, ctol=1e-12, f0=1.27e9, f1=1.27e9, ndiv=0)
+DISK96 = dict(nr=48, nsec=96, qx=32, qy=16, ctol=1e-12, f0=1.27e9, f1=1.27e9, ndiv=0)
 
 
 def disk96_text() -> str:
