@@ -101,32 +101,68 @@ __device__ __forceinline__ cplx hm_pos(const hent &E, const int *perm, int a, in
 
 constexpr int AT = 256;
 
+// Workgroup reductions: a butterfly inside each wave (every lane ends with the same bits: each
+// step adds the same two partial values, a + b = b + a), then the four wave results combined
+// in a fixed order by every thread: two barriers per reduction instead of a 256-wide tree.
+constexpr int NW = AT / 64;
+static_assert(NW == 4, "the wave results are combined as four");
+
 __device__ __forceinline__ double wg_sum(double x, double *red)
 {
-	const int t = threadIdx.x;
-	red[t] = x;
+	for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+	if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
 	__syncthreads();
-	for (int s = AT / 2; s > 0; s >>= 1) {
-		if (t < s) red[t] += red[t + s];
-		__syncthreads();
-	}
-	const double r = red[0];
+	const double r = (red[0] + red[1]) + (red[2] + red[3]);
 	__syncthreads();
 	return r;
 }
 
 __device__ __forceinline__ double wg_max(double x, double *red)
 {
-	const int t = threadIdx.x;
-	red[t] = x;
+	for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off));
+	if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
 	__syncthreads();
-	for (int s = AT / 2; s > 0; s >>= 1) {
-		if ((t < s) && (red[t + s] > red[t])) red[t] = red[t + s];
-		__syncthreads();
-	}
-	const double r = red[0];
+	const double r = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
 	__syncthreads();
 	return r;
+}
+
+// (v, i) beats (w, j): an index beats none; a larger value (a smaller one when MIN); equal
+// values go to the smaller index.  A total order, so every combination order gives the same
+// winner.
+template <bool MIN>
+__device__ __forceinline__ bool beats(double v, int i, double w, int j)
+{
+	if (i < 0) return false;
+	if (j < 0) return true;
+	if (v != w) return MIN ? (v < w) : (v > w);
+	return i < j;
+}
+
+template <bool MIN>
+__device__ __forceinline__ void wg_argext(double &v, int &idx, double *rv, int *ri)
+{
+	for (int off = 32; off > 0; off >>= 1) {
+		const double ov = __shfl_xor(v, off);
+		const int oi = __shfl_xor(idx, off);
+		if (beats<MIN>(ov, oi, v, idx)) {
+			v = ov;
+			idx = oi;
+		}
+	}
+	if ((threadIdx.x & 63) == 0) {
+		rv[threadIdx.x >> 6] = v;
+		ri[threadIdx.x >> 6] = idx;
+	}
+	__syncthreads();
+	v = rv[0];
+	idx = ri[0];
+	for (int w = 1; w < NW; w++)
+		if (beats<MIN>(rv[w], ri[w], v, idx)) {
+			v = rv[w];
+			idx = ri[w];
+		}
+	__syncthreads();
 }
 
 // largest value, ties to the smaller index; (0, -1) when nothing positive was seen.  Each thread
@@ -134,24 +170,7 @@ __device__ __forceinline__ double wg_max(double x, double *red)
 // is the first index of the maximum, as a serial scan with ">" would find.
 __device__ __forceinline__ void wg_argmax(double &v, int &idx, double *rv, int *ri)
 {
-	const int t = threadIdx.x;
-	rv[t] = v;
-	ri[t] = idx;
-	__syncthreads();
-	for (int s = AT / 2; s > 0; s >>= 1) {
-		if (t < s) {
-			const double ov = rv[t + s];
-			const int oi = ri[t + s];
-			if ((oi >= 0) && ((ri[t] < 0) || (ov > rv[t]) || ((ov == rv[t]) && (oi < ri[t])))) {
-				rv[t] = ov;
-				ri[t] = oi;
-			}
-		}
-		__syncthreads();
-	}
-	v = rv[0];
-	idx = ri[0];
-	__syncthreads();
+	wg_argext<false>(v, idx, rv, ri);
 }
 
 // unused row with the smallest representation (first index among equals), if at most thr
@@ -167,24 +186,7 @@ __device__ int unrepresented(const int *rused, const double *urep, int m, double
 			nxt = i;
 		}
 	}
-	const int t = threadIdx.x;
-	rv[t] = best;
-	ri[t] = nxt;
-	__syncthreads();
-	for (int s = AT / 2; s > 0; s >>= 1) {
-		if (t < s) {
-			const double ov = rv[t + s];
-			const int oi = ri[t + s];
-			if ((oi >= 0) && ((ri[t] < 0) || (ov < rv[t]) || ((ov == rv[t]) && (oi < ri[t])))) {
-				rv[t] = ov;
-				ri[t] = oi;
-			}
-		}
-		__syncthreads();
-	}
-	best = rv[0];
-	nxt = ri[0];
-	__syncthreads();
+	wg_argext<true>(best, nxt, rv, ri);
 	return ((nxt >= 0) && (best <= thr)) ? nxt : -1;
 }
 
@@ -242,7 +244,7 @@ __global__ __launch_bounds__(AT) void k_aca(aca_args a)
 	__shared__ int ri[AT];
 	__shared__ double2 cdu[HM_MAXIT], cdv[HM_MAXIT];
 	__shared__ double sh_cross;
-	const int b = blockIdx.x, tid = threadIdx.x;
+	const int b = a.order[blockIdx.x], tid = threadIdx.x;
 	const int wave = tid >> 6, lane = tid & 63;
 	const hfar B = a.far[b];
 	const int r0 = B.r0, m = B.m, c0 = B.c0, nn = B.nn;
@@ -769,6 +771,12 @@ struct hmat_t *hm_build_dev(pgpu_t *g, int kind, int n, const double *pts, doubl
 	std::vector<int> rank(t.nfar, 0);
 	pe::hfar *d_far = up(s, far);
 	if (t.nfar > 0) {
+		// launch the blocks with the most work per rank step first, so the longest sequential
+		// rank loops do not start last (scheduling only: each block's result is its own)
+		std::vector<int> order(t.nfar);
+		for (int b = 0; b < t.nfar; b++) order[b] = b;
+		std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return soff[x + 1] - soff[x] > soff[y + 1] - soff[y]; });
+		int *d_order = up(s, order);
 		long *d_soff = up(s, soff), *d_moff = up(s, moff);
 		double2 *scr = dal<double2>(soff[t.nfar]);
 		double *urep = dal<double>(moff[t.nfar]);
@@ -778,6 +786,7 @@ struct hmat_t *hm_build_dev(pgpu_t *g, int kind, int n, const double *pts, doubl
 		a.E = E;
 		a.perm = h->d_perm;
 		a.far = d_far;
+		a.order = d_order;
 		a.soff = d_soff;
 		a.moff = d_moff;
 		a.scr = scr;
@@ -817,6 +826,7 @@ struct hmat_t *hm_build_dev(pgpu_t *g, int kind, int n, const double *pts, doubl
 		HIPCHK(hipFree(d_rank));
 		HIPCHK(hipFree(d_soff));
 		HIPCHK(hipFree(d_moff));
+		HIPCHK(hipFree(d_order));
 		(void)nd;
 	}
 	// dense storage: far blocks that did not compress, then the near blocks

@@ -45,6 +45,7 @@ struct aca_args {
 	hent E;
 	const int *perm;
 	const hfar *far;
+	const int *order;              // far blocks in launch order (largest first)
 	const long *soff, *moff;       // scratch offsets: factors (double2); row then column flags
 	double2 *scr;
 	double *urep;                  // per block: max |u_l[i]| over rows, then max |v_l[j]| over columns
