@@ -127,7 +127,7 @@ def cpu_baseline(m, f: float, label: str) -> dict:
     parts["p_inverse_solves_s"] = t_sv * nc * (nc / ns) ** 2
     parts["port_solve_s"] = t_sv * (n / ns) ** 2
     t_total = sum(parts.values())
-    return {
+    out = {
         "value": 1.0 / t_total, "unit": "freq points/s", "cores": threads, "kind": "port",
         "sample": (f"oracle (reference arithmetic, bitwise-equal on the sample decks) at {f / 1e9:g} GHz: "
                    f"3 x 24 rows of the Lp and charge-cell triangles, LU and 16 solves of a {ns}^2 "
@@ -135,6 +135,15 @@ def cpu_baseline(m, f: float, label: str) -> dict:
                    f"{label} (n = {n}, ncell = {nc})"),
         "seconds_per_point": t_total, "parts_s": {k: round(v, 3) for k, v in parts.items()},
     }
+    # the oracle's wall time against the reference's own C build on the same inputs, measured
+    # where the reference sources are (tests/golden/time_oracle_vs_reference.py)
+    tj = ROOT / "tests" / "golden" / "oracle_timing.json"
+    if tj.exists():
+        t = json.loads(tj.read_text())
+        r = [c[k] for c in t["cases"].values() for k in ("ratio_lp", "ratio_pot")] + [t["lu_2048"]["ratio"]]
+        out["oracle_over_reference_time"] = {"min": min(r), "max": max(r), "threads": t["threads"],
+                                             "cases": sorted(t["cases"]) + ["lu_2048"]}
+    return out
 
 
 def small_check(lib) -> dict:
