@@ -14,10 +14,11 @@ cached preconditioner and records are dropped before timing.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu] ...
     python bench.py --only batch | learned | plate     (one side workload alone)
 
-Form: the accelerated sweep -- k-factored Lp/P fill, charge-unknown system, an LU of A(f_ref) on
-FP32 storage (FP64 panels and pivoting, FP32 MFMA trailing updates) whose factors are inverted in
-FP32, FP64 flexible GMRES to a 1e-10 relative residual in lock-step batches of B frequencies, a
-fresh factorisation when the extra iterations have cost about as much as one.  Zin of the
+Form: the accelerated sweep -- k-factored Lp/P fill, charge-unknown system, A(f_ref) factored
+as the preconditioner through its Schur complement on the charges (C = P^-1 and an LU of the
+plain system on FP32 storage, FP64 panels and pivoting, FP32 MFMA trailing updates, factors
+inverted in FP32), FP64 flexible GMRES to a 1e-10 relative residual in lock-step batches of B
+frequencies, a fresh factorisation when the extra iterations have cost as much as one.  Zin of the
 first, middle and last points is compared with the plain FP64 form (the reference's algorithm:
 element fill, P^-1, dense FP64 LU); the run fails when the error exceeds 1e-6.  The precision
 form (FP32-stored LU with FP64 refinement) runs on the same points.  roofline: the kernel with
@@ -211,7 +212,7 @@ def precision_points(m, idx, zp):
 
 
 def lu_roofline(achieved: float, workload: str) -> dict:
-    """Roofline of the trailing update of the accelerated form's LU of A(f_ref): FP32 storage by
+    """Roofline of the trailing update of the accelerated form's LU (the Schur complement of A(f_ref)): FP32 storage by
     default (rocBLAS cgemm on FP32 MFMA; the hand-written k_cgemm32 with PEEC_GEMM=hip), FP64 with
     PEEC_MIXED_LU64=1 (rocBLAS zgemm / k_zgemm on FP64 MFMA).  Traffic from the workload's profile
     (profiles/<kernel>_<workload>.md) when one was taken."""
@@ -220,12 +221,12 @@ def lu_roofline(achieved: float, workload: str) -> dict:
         peak, alg = PEAK_FP64_MFMA, "16 (2 M N + (M + N) K) B"
         name = "k_zgemm" if hip else "rocBLAS zgemm"
         prof = f"{'k_zgemm' if hip else 'zgemm_lu_update'}_{workload}.md"
-        name += " (LU trailing update of A(f_ref), FP64 MFMA)"
+        name += " (LU trailing update of the preconditioner, FP64 MFMA)"
     else:
         peak, alg = PEAK_FP32_MFMA, "8 (2 M N + (M + N) K) B"
         name = "k_cgemm32" if hip else "rocBLAS cgemm"
         prof = f"{'k_cgemm32' if hip else 'cgemm_lu_update'}_{workload}.md"
-        name += " (LU trailing update of A(f_ref), FP32 MFMA)"
+        name += " (LU trailing update of the preconditioner, FP32 MFMA)"
     traffic = lu_traffic(prof)
     return {"kernel": name, "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
             "frac": achieved / peak, "traffic": traffic,

@@ -7,11 +7,14 @@
 //     block-wise as the reference's mna_apply does (mna.c:338-364): sparse stamps (CSR) plus
 //     -jw Lp on the branch-current block plus P on the charge block, so each GMRES product
 //     streams Lp and P once (1.4 GB on the 64 x 64 plate) instead of a dense n^2 matrix;
-//   * one LU of A(f_ref) serves a run of frequencies as the right preconditioner of FP64
-//     flexible GMRES (the reference's acceleration = 1 scheme, solve.c:47-64 / iterative.c:
-//     230-252).  The factors are inverted once (recursive blocking on MFMA GEMMs) and the two
-//     inverses stored in FP32, so applying the preconditioner is two bandwidth-bound triangular
-//     matrix-vector products instead of two chains of dependent substitution steps.  Flexible
+//   * one factorisation of A(f_ref) serves a run of frequencies as the right preconditioner of
+//     FP64 flexible GMRES (the reference's acceleration = 1 scheme, solve.c:47-64 / iterative.c:
+//     230-252).  It is factored through its Schur complement on the charge unknowns: C = P^-1
+//     at f_ref, and the LU of S = A11 + jw E^T C E, the plain system, of order nunknown instead
+//     of nunknown + ncell.  The factors of S are inverted once (recursive blocking on MFMA
+//     GEMMs) and the two inverses and C stored in FP32, so applying the preconditioner is two
+//     bandwidth-bound triangular matrix-vector products and two products with C instead of
+//     chains of dependent substitution steps.  Flexible
 //     GMRES keeps the preconditioned directions, so the preconditioner may be applied inexactly
 //     (FP32 data, any summation order) without limiting the final residual;
 //   * frequencies advance in lock-step batches of F systems sharing the preconditioner, so each
@@ -60,9 +63,16 @@ __device__ __forceinline__ double2 zinv(double2 z)
 
 // ---------------------------------------------------------------- assembly
 
-// dense blocks of the charge-unknown system: -jw Lp at [offS, offS+nseg)^2, P at [offQ, offQ+nc)^2
-__global__ void k_assemble_aug(double2 *A, int n, int offS, int nseg, const double2 *lp, double omega, int offQ, int nc,
-                               const double2 *P)
+// The preconditioner of the charge-unknown system A = [[A11, A12], [A21, P]] at f_ref is
+// factored through its Schur complement: with A21 = -E (charge rows: -v of the cell's node) and
+// A12 = jw E^T (node rows: jw q), S = A11 - A12 P^-1 A21 = A11 + jw E^T C E, C = P^-1, which is
+// the plain MNA system of order nunknown.  Exactly A^-1 in exact arithmetic, at 0.65x the
+// factorisation work of the full order on the full stack.
+
+// S: -jw Lp on the branch block, jw C between the node rows of capacitive cells (rowcell[r] =
+// the cell on node row r, or -1), 0 elsewhere; the sparse stamps follow (k_stamp_lead)
+__global__ void k_assemble_schur(double2 *A, int n, int offS, int nseg, const double2 *lp, double omega,
+                                 const int *rowcell, const double2 *C, int nc)
 {
 	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (e >= (long)n * n) return;
@@ -70,21 +80,71 @@ __global__ void k_assemble_aug(double2 *A, int n, int offS, int nseg, const doub
 	double2 v = make_double2(0, 0);
 	if ((r >= offS) && (r < offS + nseg) && (c >= offS) && (c < offS + nseg)) {
 		const double2 x = lp[(long)(r - offS) * nseg + (c - offS)];
-		v = make_double2((0 * x.x) - ((-omega) * x.y), (0 * x.y) + ((-omega) * x.x));
+		v = make_double2(omega * x.y, -omega * x.x);
 	}
-	else if ((r >= offQ) && (r < offQ + nc) && (c >= offQ) && (c < offQ + nc)) {
-		v = P[(long)(r - offQ) * nc + (c - offQ)];
+	else if (nc > 0) {
+		const int cr = rowcell[r], cc = rowcell[c];
+		if ((cr >= 0) && (cc >= 0)) {
+			const double2 x = C[(long)cr * nc + cc];
+			v = make_double2(-omega * x.y, omega * x.x);
+		}
 	}
-	A[e] = make_double2(0 + v.x, 0 + v.y);
+	A[e] = v;
 }
 
-__global__ void k_stamp(double2 *A, long lda, const int *ri, const int *ci, const double2 *val, int nst)
+// the charge-unknown system's stamps that fall inside the leading n x n block (its couplings
+// to the charge unknowns are the jw C block of S)
+__global__ void k_stamp_lead(double2 *A, long lda, int n, const int *ri, const int *ci, const double2 *val, int nst)
 {
 	const int e = blockIdx.x * blockDim.x + threadIdx.x;
 	if (e >= nst) return;
-	double2 *a = &A[(long)ri[e] * lda + ci[e]];
+	const int r = ri[e], c = ci[e];
+	if ((r >= n) || (c >= n)) return;
+	double2 *a = &A[(long)r * lda + c];
 	const double2 x = *a, v = val[e];
 	*a = make_double2(v.x + x.x, v.y + x.y);
+}
+
+// zt[k][f] = r[perm[k]][f] with r = v1 - jw E^T w (w = C v2 per cell): the right-hand side of S,
+// permuted for its factors
+__global__ void k_permute_schur(const double2 *v, const double2 *w, const int *perm, const int *rowcell, double omega,
+                                double2 *zt, int n, int F)
+{
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= (long)n * F) return;
+	const int k = (int)(e / F), f = (int)(e - (long)k * F);
+	const int r = perm[k];
+	double2 x = v[(long)r * F + f];
+	const int cr = (rowcell != nullptr) ? rowcell[r] : -1;
+	if (cr >= 0) {
+		const double2 wv = w[(long)cr * F + f];
+		x = make_double2(x.x + omega * wv.y, x.y - omega * wv.x);
+	}
+	zt[e] = x;
+}
+
+// t[i][f] = v2[i][f] + x1[cellrow[i]][f]: the charge rows' right-hand side once the node
+// potentials x1 are known (cellrow[i] = the node row of cell i, or -1 on the reference node)
+__global__ void k_charge_rhs(const double2 *v2, const double2 *x1, const int *cellrow, double2 *t, int nc, int F)
+{
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= (long)nc * F) return;
+	const int i = (int)(e / F), f = (int)(e - (long)i * F);
+	double2 a = v2[e];
+	const int r = cellrow[i];
+	if (r >= 0) {
+		const double2 x = x1[(long)r * F + f];
+		a = make_double2(a.x + x.x, a.y + x.y);
+	}
+	t[e] = a;
+}
+
+__global__ void k_eye(double2 *B, int n)
+{
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= (long)n * n) return;
+	const int r = (int)(e / n), c = (int)(e - (long)r * n);
+	B[e] = make_double2((r == c) ? 1.0 : 0.0, 0);
 }
 
 __global__ void k_demote(const double2 *src, float2 *dst, long count)
@@ -585,7 +645,7 @@ __global__ void k_permute(const double2 *src, double2 *dst, const int *perm, int
 // increasing s (deterministic).
 constexpr int TG_RB = 64;
 
-template <bool LOWER, int F>
+template <bool LOWER, int F, bool FULL = false>
 __global__ __launch_bounds__(256) void k_tgemv(const float2 *__restrict__ Tc, int n, int cs,
                                                const double2 *__restrict__ X, double2 *__restrict__ part)
 {
@@ -596,7 +656,12 @@ __global__ __launch_bounds__(256) void k_tgemv(const float2 *__restrict__ Tc, in
 	if (rlo >= n) return;
 	const int rhi = min(n, rlo + TG_RB);
 	int cb, ce;
-	if (LOWER) {
+	if (FULL) {
+		// a full matrix (the Schur preconditioner's C, symmetric, so its rows are its columns)
+		cb = s * cs;
+		ce = min(n, cb + cs);
+	}
+	else if (LOWER) {
 		cb = s * cs;
 		ce = min(rhi, cb + cs);
 	}
@@ -657,15 +722,15 @@ __global__ __launch_bounds__(256) void k_tgemv(const float2 *__restrict__ Tc, in
 
 // Y[row][f] = sum over the row's segments of part[s][row][f] (the items k_tgemv ran for the
 // row's 64-row block), in increasing s
-template <bool LOWER>
+template <bool LOWER, bool FULL = false>
 __global__ void k_tgsum(const double2 *__restrict__ part, int n, int F, int cs, double2 *__restrict__ Y)
 {
 	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (e >= (long)n * F) return;
 	const int row = (int)(e / F);
 	const int rlo = (row / TG_RB) * TG_RB, rhi = min(n, rlo + TG_RB);
-	const int s0 = LOWER ? 0 : rlo / cs;
-	const int s1 = LOWER ? (rhi - 1) / cs : (n - 1) / cs;
+	const int s0 = (LOWER || FULL) ? 0 : rlo / cs;
+	const int s1 = FULL ? (n - 1) / cs : LOWER ? (rhi - 1) / cs : (n - 1) / cs;
 	const long stride = (long)n * F;
 	double2 a = part[(long)s0 * stride + e];
 	for (int s = s0 + 1; s <= s1; s++) {
@@ -731,28 +796,29 @@ size_t tg_part_count(int n, int F)
 	return (size_t)((n + tg_cs() - 1) / tg_cs()) * n * F;
 }
 
-template <bool LOWER, int F>
+template <bool LOWER, int F, bool FULL>
 void tgemv_f(hipStream_t s, const float2 *Mt, int n, const double2 *X, double2 *Y, double2 *part)
 {
 	const int cs = tg_cs();
 	const dim3 grid((unsigned)((n + 4 * TG_RB - 1) / (4 * TG_RB)), (unsigned)((n + cs - 1) / cs));
-	hipLaunchKernelGGL((k_tgemv<LOWER, F>), grid, dim3(256), 0, s, Mt, n, cs, X, part);
-	hipLaunchKernelGGL(k_tgsum<LOWER>, dim3((unsigned)(((long)n * F + 255) / 256)), dim3(256), 0, s,
+	hipLaunchKernelGGL((k_tgemv<LOWER, F, FULL>), grid, dim3(256), 0, s, Mt, n, cs, X, part);
+	hipLaunchKernelGGL((k_tgsum<LOWER, FULL>), dim3((unsigned)(((long)n * F + 255) / 256)), dim3(256), 0, s,
 	                   (const double2 *)part, n, F, cs, Y);
 }
 
-template <bool LOWER>
+// Y = T X for a triangle (LOWER or upper) or, with FULL, a full matrix stored by columns
+template <bool LOWER, bool FULL = false>
 void tgemv(hipStream_t s, const float2 *Mt, int n, const double2 *X, double2 *Y, int F, double2 *part)
 {
 	switch (F) {
-	case 1: tgemv_f<LOWER, 1>(s, Mt, n, X, Y, part); break;
-	case 2: tgemv_f<LOWER, 2>(s, Mt, n, X, Y, part); break;
-	case 3: tgemv_f<LOWER, 3>(s, Mt, n, X, Y, part); break;
-	case 4: tgemv_f<LOWER, 4>(s, Mt, n, X, Y, part); break;
-	case 5: tgemv_f<LOWER, 5>(s, Mt, n, X, Y, part); break;
-	case 6: tgemv_f<LOWER, 6>(s, Mt, n, X, Y, part); break;
-	case 7: tgemv_f<LOWER, 7>(s, Mt, n, X, Y, part); break;
-	default: tgemv_f<LOWER, 8>(s, Mt, n, X, Y, part); break;
+	case 1: tgemv_f<LOWER, 1, FULL>(s, Mt, n, X, Y, part); break;
+	case 2: tgemv_f<LOWER, 2, FULL>(s, Mt, n, X, Y, part); break;
+	case 3: tgemv_f<LOWER, 3, FULL>(s, Mt, n, X, Y, part); break;
+	case 4: tgemv_f<LOWER, 4, FULL>(s, Mt, n, X, Y, part); break;
+	case 5: tgemv_f<LOWER, 5, FULL>(s, Mt, n, X, Y, part); break;
+	case 6: tgemv_f<LOWER, 6, FULL>(s, Mt, n, X, Y, part); break;
+	case 7: tgemv_f<LOWER, 7, FULL>(s, Mt, n, X, Y, part); break;
+	default: tgemv_f<LOWER, 8, FULL>(s, Mt, n, X, Y, part); break;
 	}
 }
 
@@ -781,6 +847,7 @@ struct slotbuf {
 
 struct fast {
 	int n = 0, nc = 0, F = 0, nseg = 0;
+	int ns = 0;               // order of the factored Schur complement (the plain system)
 	slotbuf sl[FMAX];
 	double2 *luw = nullptr;   // FP64 factorisation workspace
 	double2 *inv = nullptr;   // FP64 inverse of one factor (built, then demoted)
@@ -816,12 +883,19 @@ struct fast {
 	double2 *h2 = nullptr, *nn = nullptr;   // second Gram-Schmidt pass and norms, device side
 	double2 *hp = nullptr;    // pinned host copy of h, h2, nn (one transfer per Arnoldi step)
 	double *scale = nullptr;
+	// the Schur preconditioner's charge part: C = P(f_ref)^-1 (FP64, then FP32 by columns),
+	// the node row of each cell and the cell of each node row, two n_c x F work vectors
+	double2 *Pw = nullptr, *Cd = nullptr, *cwork = nullptr;
+	float2 *Cc = nullptr;
+	int *ppiv = nullptr, *pperm = nullptr, *rowcell = nullptr, *cellrow = nullptr;
+	double2 *Wc = nullptr, *Wt = nullptr;
+	double omega_ref = 0;
 };
 
 // both FP32 inverses from the FP64 factors in c.luw, stored by columns
 void build_inverses(fast &c, hipStream_t s)
 {
-	const int n = c.n;
+	const int n = c.ns;
 	for (int pass = 0; pass < 2; pass++) {
 		const bool lower = (pass == 0);
 		dlu_tri_inverse(s, n, (const d_complex_t *)c.luw, n, lower ? 1 : 0, (d_complex_t *)c.inv, (d_complex_t *)c.scr,
@@ -832,12 +906,24 @@ void build_inverses(fast &c, hipStream_t s)
 	}
 }
 
-// z = M^-1 v = Uinv (Linv (P v)), v and z n x F
+// z = M^-1 v, v and z n x F (n = ns + nc): block elimination through the Schur complement,
+//   w = C v2,  x1 = S^-1 (v1 - jw E^T w) = Uinv (Linv (P (...))),  x2 = C (v2 + E x1)
+// (without capacitive cells, nc = 0: z = Uinv (Linv (P v)))
 void precond(fast &c, hipStream_t s, const double2 *v, double2 *z, int F)
 {
-	hipLaunchKernelGGL(k_permute, dim3(blocks((long)c.n * F)), dim3(256), 0, s, v, z, c.perm, c.n, F);
-	tgemv<true>(s, (const float2 *)c.ML, c.n, (const double2 *)z, c.T1, F, c.tgp);
-	tgemv<false>(s, (const float2 *)c.MU, c.n, (const double2 *)c.T1, z, F, c.tgp);
+	const int ns = c.ns, nc = c.nc;
+	const double2 *v2 = v + (long)ns * F;
+	double2 *x2 = z + (long)ns * F;
+	if (nc > 0) tgemv<false, true>(s, (const float2 *)c.Cc, nc, v2, c.Wc, F, c.tgp);
+	hipLaunchKernelGGL(k_permute_schur, dim3(blocks((long)ns * F)), dim3(256), 0, s, v, (const double2 *)c.Wc, c.perm,
+	                   (nc > 0) ? (const int *)c.rowcell : nullptr, c.omega_ref, z, ns, F);
+	tgemv<true>(s, (const float2 *)c.ML, ns, (const double2 *)z, c.T1, F, c.tgp);
+	tgemv<false>(s, (const float2 *)c.MU, ns, (const double2 *)c.T1, z, F, c.tgp);
+	if (nc > 0) {
+		hipLaunchKernelGGL(k_charge_rhs, dim3(blocks((long)nc * F)), dim3(256), 0, s, v2, (const double2 *)z, c.cellrow, c.Wt,
+		                   nc, F);
+		tgemv<false, true>(s, (const float2 *)c.Cc, nc, (const double2 *)c.Wt, x2, F, c.tgp);
+	}
 }
 
 int g_offS = 0, g_offQ = 0;
@@ -1130,6 +1216,11 @@ void free_fast(fast &c)
 	if (c.h2) HIPCHK(hipFree(c.h2));
 	if (c.nn) HIPCHK(hipFree(c.nn));
 	if (c.hp) HIPCHK(hipHostFree(c.hp));
+	{
+		void *bb[] = {c.Pw, c.Cd, c.cwork, c.Cc, c.ppiv, c.pperm, c.rowcell, c.cellrow, c.Wc, c.Wt};
+		for (void *p : bb)
+			if (p) HIPCHK(hipFree(p));
+	}
 	c = fast();
 }
 
@@ -1200,15 +1291,15 @@ static int warm_start()
 }
 
 // refactor rule: PEEC_REFACTOR_ALPHA > 0 prices extra iterations against the factorisation time
-// (0.75 by default: iteration counts grow about linearly with the distance from the factorisation's
-// frequency, so refactoring somewhat before the break-even point pays); 0 selects the
-// iteration-ratio rule (more than twice the first batch's)
+// (1 by default, the break-even point: on the full stack 3 factorisations and 15.3 s against 4
+// and 16.5 s at 0.75, and 18.2 s at 1.5, measured on one GPU); 0 selects the iteration-ratio
+// rule (more than twice the first batch's)
 static double refactor_alpha()
 {
 	static double a = -1;
 	if (a < 0) {
 		const char *e = getenv("PEEC_REFACTOR_ALPHA");
-		a = e ? std::max(0.0, atof(e)) : 0.75;
+		a = e ? std::max(0.0, atof(e)) : 1.0;
 	}
 	return a;
 }
@@ -1239,11 +1330,13 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 	const int nc = usecap ? p->ncell : 0;
 	const int n = p->nunknown + nc;
 	const int offQ = p->nunknown;
+	const int ns = p->nunknown;   // order of the factored Schur complement
 	const int F = std::max(1, std::min(batch, FMAX));
-	if ((c.n != n) || (c.F < F) || (c.nseg != p->nseg)) {
+	if ((c.n != n) || (c.F < F) || (c.nseg != p->nseg) || (c.nc != nc)) {
 		free_fast(c);
 		c.n = n;
 		c.nc = nc;
+		c.ns = ns;
 		c.F = F;
 		c.nseg = p->nseg;
 		for (int f = 0; f < F; f++) {
@@ -1253,21 +1346,34 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 			hipLaunchKernelGGL(k_zero_u32, dim3(blocks(n + 1, 64)), dim3(64), 0, s, (unsigned *)c.sl[f].rp, n + 1);
 		}
 		c.ahead.rp = dal<int>(n + 1);
-		const int nblk = (n + TB - 1) / TB;
-		c.luw = dal<double2>((size_t)n * n);
-		c.inv = dal<double2>((size_t)n * n);
-		c.scr = dal<double2>((size_t)((n + 1) / 2 + TB) * ((n + 1) / 2 + TB));
-		c.scr2 = dal<double2>((size_t)((n + 1) / 2 + TB) * ((n + 1) / 2 + TB));
+		const int nblk = (ns + TB - 1) / TB;
+		c.luw = dal<double2>((size_t)ns * ns);
+		// the inverse workspace also holds a look-ahead point's Lp and P until its factorisation
+		c.inv = dal<double2>(std::max((size_t)ns * ns, (size_t)p->nseg * p->nseg + (size_t)nc * nc));
+		c.scr = dal<double2>((size_t)((ns + 1) / 2 + TB) * ((ns + 1) / 2 + TB));
+		c.scr2 = dal<double2>((size_t)((ns + 1) / 2 + TB) * ((ns + 1) / 2 + TB));
 		pgpu_at_exit(fast_release_at_exit);
 		HIPCHK(hipStreamCreate(&c.s2));
 		HIPCHK(hipEventCreateWithFlags(&c.ev_f, hipEventDisableTiming));
 		HIPCHK(hipEventCreateWithFlags(&c.ev_u, hipEventDisableTiming));
-		c.ML = dal<float2>((size_t)n * n);
-		c.MU = dal<float2>((size_t)n * n);
+		c.ML = dal<float2>((size_t)ns * ns);
+		c.MU = dal<float2>((size_t)ns * ns);
 		c.dinvL = dal<double2>((size_t)nblk * TB * TB);
 		c.dinvU = dal<double2>((size_t)nblk * TB * TB);
-		c.piv = dal<int>(n);
-		c.perm = dal<int>(n);
+		c.piv = dal<int>(ns);
+		c.perm = dal<int>(ns);
+		c.rowcell = dal<int>(ns);
+		if (nc > 0) {
+			c.Pw = dal<double2>((size_t)nc * nc);
+			c.Cd = dal<double2>((size_t)nc * nc);
+			c.cwork = dal<double2>((size_t)nc * nc);
+			c.Cc = dal<float2>((size_t)nc * nc);
+			c.ppiv = dal<int>(nc);
+			c.pperm = dal<int>(nc);
+			c.cellrow = dal<int>(nc);
+			c.Wc = dal<double2>((size_t)nc * FMAX);
+			c.Wt = dal<double2>((size_t)nc * FMAX);
+		}
 		const long nf = (long)n * FMAX;
 		c.V = dal<double2>((size_t)(MKRYLOV + 1) * nf);
 		c.Z = dal<double2>((size_t)MKRYLOV * nf);
@@ -1294,6 +1400,21 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 	}
 	g_offS = p->offS;
 	g_offQ = offQ;
+	// the cell of each node row and the node row of each cell (the incidence E of the charge
+	// rows), for the Schur preconditioner; the deck's, so set at every call
+	{
+		std::vector<int> rc(ns, -1), cr(nc, -1);
+		for (int i = 0; i < nc; i++) {
+			const int mi = p->nodemap[p->cellid[i]];
+			if ((mi >= 0) && (mi < ns)) {
+				cr[i] = mi;
+				rc[mi] = i;
+			}
+		}
+		HIPCHK(hipMemcpyAsync(c.rowcell, rc.data(), (size_t)ns * sizeof(int), hipMemcpyHostToDevice, s));
+		if (nc > 0) HIPCHK(hipMemcpyAsync(c.cellrow, cr.data(), (size_t)nc * sizeof(int), hipMemcpyHostToDevice, s));
+		HIPCHK(hipStreamSynchronize(s));
+	}
 	std::vector<int> i1(np), i2(np);
 	for (int j = 0; j < np; j++) {
 		i1[j] = p->nodemap[p->port[j].n1];
@@ -1366,10 +1487,22 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 		// (re)factor the preconditioner on the system in sb, that of sweep point k
 		auto refactor_on = [&](const slotbuf &sb, int k) -> int {
 			const auto ta = std::chrono::steady_clock::now();
-			hipLaunchKernelGGL(k_assemble_aug, dim3(blocks((long)n * n)), dim3(256), 0, s, c.luw, n, p->offS, p->nseg,
-			                   (const double2 *)sb.lp, sb.omega, offQ, nc, (const double2 *)sb.P);
+			const int n = ns;   // the factored order from here on: the Schur complement's
+			if (nc > 0) {
+				// C = P(f_ref)^-1 in FP64 (the plain form's P LU and unit solves), then FP32 by columns
+				HIPCHK(hipMemcpyAsync(c.Pw, sb.P, (size_t)nc * nc * sizeof(double2), hipMemcpyDeviceToDevice, s));
+				if (dlu_factor(s, nc, (d_complex_t *)c.Pw, nc, c.ppiv, c.pperm, nullptr) >= 0) return 2;
+				hipLaunchKernelGGL(k_eye, dim3(blocks((long)nc * nc)), dim3(256), 0, s, c.Cd, nc);
+				dlu_solve_perm(s, nc, (const d_complex_t *)c.Pw, nc, c.pperm, (d_complex_t *)c.Cd, nc, nc,
+				               (d_complex_t *)c.cwork);
+				const unsigned ntc = (unsigned)((nc + 31) / 32);
+				hipLaunchKernelGGL(k_transpose32<double2>, dim3(ntc, ntc), dim3(256), 0, s, (const double2 *)c.Cd, c.Cc, nc);
+			}
+			c.omega_ref = sb.omega;
+			hipLaunchKernelGGL(k_assemble_schur, dim3(blocks((long)n * n)), dim3(256), 0, s, c.luw, n, p->offS, p->nseg,
+			                   (const double2 *)sb.lp, sb.omega, (const int *)c.rowcell, (const double2 *)c.Cd, nc);
 			if (sb.nnz > 0)
-				hipLaunchKernelGGL(k_stamp, dim3(blocks(sb.nnz)), dim3(256), 0, s, c.luw, (long)n, sb.ri, sb.ci,
+				hipLaunchKernelGGL(k_stamp_lead, dim3(blocks(sb.nnz)), dim3(256), 0, s, c.luw, (long)n, n, sb.ri, sb.ci,
 				                   (const double2 *)sb.val, sb.nnz);
 			if (lu64()) {
 				const int info = dlu_factor(s, n, (d_complex_t *)c.luw, n, c.piv, c.perm, st);
