@@ -259,153 +259,155 @@ __global__ __launch_bounds__(256) void k_mvdense(slots S, int nact, int offS, in
 	}
 }
 
-// k_mvsym: the same product reading only the upper triangle of each symmetric block.  Items are
-// (64-row block I, run of js 64 x 64 tiles J >= I), the systems of the batch one after another.
-// Lane l takes columns 16 g + (l mod 16), g = 0..3, of rows 4 q + (l / 16), q = 0..15: four
-// rows per step, each read whole by four back-to-back loads.  A tile above the diagonal serves
-// both triangles: B[r][c] x[c] summed over the lane's four columns and then the 16 lanes of a
-// row (four DPP steps inside the row, no LDS) goes to row r, kept by the lane that owns r;
-// B[r][c] x[r] accumulates per lane for its columns c and the four row phases are added once
-// per tile, written per (I, c) to partT.  A diagonal tile is read whole and used once.  Row sums per item go to
-// partN[s][row][f].  Half the bytes of k_mvdense.
-template <int CTRL>
-__device__ __forceinline__ double dpp_move(double v)
+// k_mvsym: the same product reading only the upper triangle of each symmetric block (whole
+// 64 x 64 diagonal tiles), half the bytes of k_mvdense.  A workgroup takes row block I of one
+// block and a run of MV_JS tiles J >= I (J0 = I + MV_JS blockIdx.x); wave w takes rows
+// 16 w .. 16 w + 15 of the row block and lane l column l of each tile, so one buffer load reads
+// 1 KB of a stored row.  A tile above the diagonal serves both triangles:
+//   * B[r][c] x[c] accumulates per lane and row over the run; the 16 row sums are reduced across
+//     the wave once per run (partN[f][run][row]);
+//   * B[r][c] x[r] accumulates per lane and column over the wave's 16 rows; the four waves' sums
+//     are added in LDS, in wave order, at the end of the run (partT[f][I][column]).
+// The next tile's loads are issued before the current tile's arithmetic.  Partial sums are stored
+// system-major, so each store is a contiguous run of rows (the system-interleaved layout cost
+// 12 % of the product's time in partial-line writes).  Reads past a block's last column wrap into
+// the next stored row and meet a zero x; rows past the last one meet a zero x[r] and land in sums
+// that are never written; the buffer descriptor's range ends at the end of the matrix, so nothing
+// past it is read (such loads return zero).
+constexpr int MV_JS = 8;     // tiles per run
+
+__device__ __forceinline__ void cfma(double2 &acc, double2 b, double2 x)
 {
-	const long long b = __double_as_longlong(v);
-	const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
-	const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
-	return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+	acc.x = fma(b.x, x.x, acc.x);
+	acc.x = fma(-b.y, x.y, acc.x);
+	acc.y = fma(b.x, x.y, acc.y);
+	acc.y = fma(b.y, x.x, acc.y);
 }
 
-// sum over the 16 lanes of each row of the wave, in every lane of the row: quad_perm [1,0,3,2],
-// quad_perm [2,3,0,1], row_half_mirror, row_mirror
-__device__ __forceinline__ double row16_sum(double v)
+__device__ __forceinline__ double2 add2(double2 a, double2 b)
 {
-	v += dpp_move<0xB1>(v);
-	v += dpp_move<0x4E>(v);
-	v += dpp_move<0x141>(v);
-	v += dpp_move<0x140>(v);
-	return v;
+	return make_double2(a.x + b.x, a.y + b.y);
 }
 
-template <int F, int RU, bool ROWS>
-__global__ __launch_bounds__(256) void k_mvsym(slots S, int nact, int offS, int nseg, int offQ, int nc, int js,
-                                               const double2 *__restrict__ X, double2 *__restrict__ partN,
+__device__ __forceinline__ double2 shx(double2 v, int m)
+{
+	return make_double2(__shfl_xor(v.x, m), __shfl_xor(v.y, m));
+}
+
+// v[0..2H) held by lane groups ST lanes apart: each step halves the values a lane keeps (by lane
+// bit H ST), adding its partner's; the lane ends with index (lane / ST) % 2H in v[0]
+template <int H, int ST>
+__device__ __forceinline__ void halve_sums(double2 *v, int lane)
+{
+	if constexpr (H >= 1) {
+		constexpr int m = H * ST;
+		const bool up = lane & m;
+#pragma unroll
+		for (int k = 0; k < H; k++) {
+			const double2 snd = up ? v[k] : v[k + H], kp = up ? v[k + H] : v[k];
+			v[k] = add2(kp, shx(snd, m));
+		}
+		halve_sums<H / 2, ST>(v, lane);
+	}
+}
+
+__global__ __launch_bounds__(256) void k_mvsym(slots S, int nact, int F, int offS, int nseg, int offQ, int nc, int NR,
+                                               int NI, const double2 *__restrict__ X, double2 *__restrict__ partN,
                                                double2 *__restrict__ partT, int n)
 {
+	constexpr int RW = 16;
+	__shared__ double2 tT[MV_JS][4][64];
 	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-	const int cg = lane & 15, ph = lane >> 4;
 	const int ntS = (nseg + 63) / 64;
-	// tile order: ROWS -- the four waves of a workgroup take four consecutive runs of one row
-	// block (a row is read 4 x 1 KB contiguously at once, and consecutive workgroups continue
-	// along it); otherwise four row blocks at one run offset
-	const int ib = ROWS ? (int)blockIdx.y : (int)blockIdx.x * 4 + wave;
-	const int s = ROWS ? (int)blockIdx.x * 4 + wave : (int)blockIdx.y;
+	const int ib = blockIdx.y;
 	const bool lblk = ib < ntS;
 	const int nb = lblk ? nseg : nc, o = lblk ? offS : offQ;
 	const int nt = (nb + 63) / 64;
 	const int I = lblk ? ib : ib - ntS;
 	if (I >= nt) return;
-	const int J0 = I + s * js;
-	if (J0 >= nt) return;
-	const int J1 = min(nt, J0 + js);
+	const int sI = blockIdx.x;
+	const int J0 = I + sI * MV_JS;
+	if (J0 >= nt) return;                       // uniform over the workgroup
+	const int J1 = min(nt, J0 + MV_JS);
 	const int rlo = I * 64, nr = min(64, nb - rlo);
-	// lane l keeps row rlo + 4 (l mod 16) + l / 16
-	double nr_[F], ni_[F];
+	const int r0 = RW * wave;
+	const long mat_bytes = (long)nb * nb * 16;
+	for (int f = 0; f < nact; f++) {
+		const double2 *Bf = nullptr;
 #pragma unroll
-	for (int f = 0; f < F; f++) nr_[f] = ni_[f] = 0;
-	for (int J = J0; J < J1; J++) {
-		const bool diag = (J == I);
-		// one system at a time: its tile rows are read whole (four 16-column groups per lane)
+		for (int h = 0; h < FMAX; h++)
+			if (h == f) Bf = lblk ? S.s[h].lp : S.s[h].P;
+		double2 xr[RW];                          // x at the wave's rows (uniform)
 #pragma unroll
-		for (int f = 0; f < F; f++) {
-			if (f >= nact) continue;
-			const double2 *Bf = lblk ? S.s[f].lp : S.s[f].P;
-			int cl[4];
-			bool cact[4];
-			double2 xl[4];
-			double tr_[4], ti_[4];
+		for (int i = 0; i < RW; i++) {
+			const int r = r0 + i;
+			xr[i] = (r < nr) ? X[(long)(o + rlo + r) * F + f] : make_double2(0, 0);
+		}
+		double2 acc[RW];
 #pragma unroll
-			for (int g = 0; g < 4; g++) {
-				cl[g] = J * 64 + g * 16 + cg;
-				cact[g] = cl[g] < nb;
-				xl[g] = cact[g] ? X[(long)(o + cl[g]) * F + f] : make_double2(0, 0);
-				tr_[g] = ti_[g] = 0;
+		for (int i = 0; i < RW; i++) acc[i] = make_double2(0, 0);
+		double2 bA[RW], bB[RW], xjA, xjB;
+		auto load = [&](int J, double2 *b, double2 &xj) {
+			const int c = J * 64 + lane;
+			xj = (c < nb) ? X[(long)(o + c) * F + f] : make_double2(0, 0);
+			const long tofs = ((long)(rlo + r0) * nb + J * 64) * 16;
+			const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+				(void *)((const char *)Bf + tofs), (short)0, (int)max(0L, min(mat_bytes - tofs, 0x7fffffffL)), 0x00020000);
+#pragma unroll
+			for (int u = 0; u < RW; u++) {
+				const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, lane * 16, u * nb * 16, 0);
+				b[u] = *(const double2 *)&v;
 			}
-			for (int q0 = 0; q0 < 16; q0 += RU) {
-				if (4 * q0 >= nr) break;
-				double2 b[RU][4];
+		};
+		auto tile = [&](int J, const double2 *b, double2 xj) {
+			const bool diag = (J == I);
+			double2 t0 = make_double2(0, 0), t1 = make_double2(0, 0);
 #pragma unroll
-				for (int u = 0; u < RU; u++) {
-					const int rr = 4 * (q0 + u) + ph;
-#pragma unroll
-					for (int g = 0; g < 4; g++)
-						b[u][g] = (cact[g] && (rr < nr)) ? Bf[(long)(rlo + rr) * nb + cl[g]] : make_double2(0, 0);
-				}
-#pragma unroll
-				for (int u = 0; u < RU; u++) {
-					const int q = q0 + u;
-					if (4 * q >= nr) continue;
-					const int rr = 4 * q + ph;
-					// row rlo + rr: sum over the tile's 64 columns (4 per lane, then the 16 lanes)
-					double pr = 0, pi = 0;
-#pragma unroll
-					for (int g = 0; g < 4; g++) {
-						pr = fma(b[u][g].x, xl[g].x, pr);
-						pr = fma(-b[u][g].y, xl[g].y, pr);
-						pi = fma(b[u][g].x, xl[g].y, pi);
-						pi = fma(b[u][g].y, xl[g].x, pi);
-					}
-					pr = row16_sum(pr);
-					pi = row16_sum(pi);
-					if (cg == q) {
-						nr_[f] += pr;
-						ni_[f] += pi;
-					}
-					// the mirror entries B[c][r] = B[r][c]: each column c takes B[r][c] x[r]
-					if (!diag) {
-						const double2 x = X[(long)(o + rlo + min(rr, nr - 1)) * F + f];
-#pragma unroll
-						for (int g = 0; g < 4; g++) {
-							tr_[g] = fma(b[u][g].x, x.x, tr_[g]);
-							tr_[g] = fma(-b[u][g].y, x.y, tr_[g]);
-							ti_[g] = fma(b[u][g].x, x.y, ti_[g]);
-							ti_[g] = fma(b[u][g].y, x.x, ti_[g]);
-						}
-					}
-				}
+			for (int u = 0; u < RW; u++) {
+				cfma(acc[u], b[u], xj);
+				if (!diag) cfma((u & 1) ? t1 : t0, b[u], xr[u]);
 			}
-			if (!diag) {
-				// the four row phases of each column
-#pragma unroll
-				for (int g = 0; g < 4; g++) {
-					tr_[g] += __shfl_xor(tr_[g], 16);
-					ti_[g] += __shfl_xor(ti_[g], 16);
-					tr_[g] += __shfl_xor(tr_[g], 32);
-					ti_[g] += __shfl_xor(ti_[g], 32);
-					if (cact[g] && (ph == 0)) partT[((long)I * n + o + cl[g]) * F + f] = make_double2(tr_[g], ti_[g]);
-				}
+			tT[J - J0][wave][lane] = add2(t0, t1);
+		};
+		load(J0, bA, xjA);
+#pragma unroll 1
+		for (int J = J0; J < J1; J += 2) {
+			if (J + 1 < J1) load(J + 1, bB, xjB);
+			tile(J, bA, xjA);
+			if (J + 1 < J1) {
+				if (J + 2 < J1) load(J + 2, bA, xjA);
+				tile(J + 1, bB, xjB);
 			}
 		}
-	}
-	const int myrow = 4 * cg + ph;
-	if (myrow < nr) {
-#pragma unroll
-		for (int f = 0; f < F; f++) partN[((long)s * n + o + rlo + myrow) * F + f] = make_double2(nr_[f], ni_[f]);
+		// mirror sums of the run's tiles: the four waves' partials added in wave order
+		__syncthreads();
+		for (int j = wave; j < J1 - J0; j += 4) {
+			if (J0 + j == I) continue;
+			const double2 v = add2(add2(add2(tT[j][0][lane], tT[j][1][lane]), tT[j][2][lane]), tT[j][3][lane]);
+			const int c = (J0 + j) * 64 + lane;
+			if (c < nb) partT[((long)f * NI + I) * n + o + c] = v;
+		}
+		__syncthreads();
+		// the 16 row sums over the wave: halved over lane bits 5..2, then the four lanes of a row
+		halve_sums<RW / 2, 4>(acc, lane);
+		acc[0] = add2(acc[0], shx(acc[0], 2));
+		acc[0] = add2(acc[0], shx(acc[0], 1));
+		const int row = r0 + (lane >> 2);
+		if (((lane & 3) == 0) && (row < nr)) partN[((long)f * NR + sI) * n + o + rlo + row] = acc[0];
 	}
 }
 
-// per (row, f) of the symmetric product: the row's item sums (partN, increasing s), then the
-// mirror contributions of the tiles above it in its column (partT, increasing I), the -jw scale
-// on the Lp rows, and the row's sparse stamps
+// per (system, row) of the symmetric product: the row's run sums (partN, increasing run), then
+// the mirror contributions of the tiles above it in its column (partT, increasing I), the -jw
+// scale on the Lp rows, and the row's sparse stamps
 __global__ __launch_bounds__(256) void k_mvfin_sym(slots S, int n, int F, int nact, int offS, int nseg, int offQ, int nc,
-                                                   int js, const double2 *__restrict__ partN,
+                                                   int NR, int NI, const double2 *__restrict__ partN,
                                                    const double2 *__restrict__ partT, const double2 *__restrict__ X,
                                                    double2 *__restrict__ Y)
 {
 	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (e >= (long)n * F) return;
-	const int row = (int)(e / F), f = (int)(e - (long)row * F);
+	const int f = (int)(e / n), row = (int)(e - (long)f * n);
 	double2 y = make_double2(0, 0);
 	if (f < nact) {
 		const slot &sl = S.s[f];
@@ -414,15 +416,15 @@ __global__ __launch_bounds__(256) void k_mvfin_sym(slots S, int n, int F, int na
 		if (l || q) {
 			const int nb = l ? nseg : nc, o = l ? offS : offQ;
 			const int nt = (nb + 63) / 64, Ib = (row - o) / 64;
-			const int nsg = (nt - Ib + js - 1) / js;
+			const int nsg = (nt - Ib + MV_JS - 1) / MV_JS;
 			double tr = 0, ti = 0;
 			for (int sg = 0; sg < nsg; sg++) {
-				const double2 p = partN[((long)sg * n + row) * F + f];
+				const double2 p = partN[((long)f * NR + sg) * n + row];
 				tr += p.x;
 				ti += p.y;
 			}
 			for (int I = 0; I < Ib; I++) {
-				const double2 p = partT[((long)I * n + row) * F + f];
+				const double2 p = partT[((long)f * NI + I) * n + row];
 				tr += p.x;
 				ti += p.y;
 			}
@@ -434,7 +436,7 @@ __global__ __launch_bounds__(256) void k_mvfin_sym(slots S, int n, int F, int na
 			y.y += p.y;
 		}
 	}
-	Y[e] = y;
+	Y[(long)row * F + f] = y;
 }
 
 __global__ __launch_bounds__(256) void k_mvfin(slots S, int n, int F, int nact, int offS, int nseg, int offQ, int nc, int cs,
@@ -768,26 +770,11 @@ int mv_cs()
 }
 
 // dense blocks of A x read as symmetric, upper triangle only (PEEC_MV_SYM=0: both triangles,
-// k_mvdense); PEEC_MV_JS 64 x 64 tiles per item (2 by default)
+// k_mvdense)
 bool mv_sym()
 {
 	const char *e = getenv("PEEC_MV_SYM");   // read per product (a few per GMRES iteration)
 	return !(e && (e[0] == '0'));
-}
-
-// k_mvsym's tile order: the waves of a workgroup along one row block (default; PEEC_MV_ROWS=0:
-// four row blocks at one run offset, 4.69 against 4.35 ms per product on the full stack)
-bool mv_rows()
-{
-	const char *e = getenv("PEEC_MV_ROWS");
-	return !(e && (e[0] == '0'));
-}
-
-int mv_js()
-{
-	const char *e = getenv("PEEC_MV_JS");   // read per product, as PEEC_MV_SYM
-	const int j = e ? atoi(e) : 2;
-	return ((j >= 1) && (j <= 64)) ? j : 2;
 }
 
 // doubles of partial sums the products need for order n and F columns
@@ -963,34 +950,13 @@ void matvec(fast &c, hipStream_t s, const double2 *x, double2 *y, int F, int nac
 		S.s[f] = slot{b.rp, b.ci, b.val, b.lp, b.P, b.omega};
 	}
 	if (mv_sym()) {
-		const int js = mv_js();
 		const int ntS = (c.nseg + 63) / 64, ntQ = (c.nc + 63) / 64, ntm = std::max(ntS, ntQ);
-		const int nseg_runs = (ntm + js - 1) / js;
-		const bool rows = mv_rows();
-		const dim3 grid = rows ? dim3((unsigned)((nseg_runs + 3) / 4), (unsigned)(ntS + ntQ))
-		                       : dim3((unsigned)((ntS + ntQ + 3) / 4), (unsigned)nseg_runs);
-		if ((ntS + ntQ) > 0) {
-#define PEEC_MVSYM_CASE(FF)                                                                                                      \
-	case FF:                                                                                                                     \
-		if (rows) hipLaunchKernelGGL((k_mvsym<FF, 2, true>), grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, js, x,  \
-		                             c.mvp, c.mvt, c.n);                                                                            \
-		else hipLaunchKernelGGL((k_mvsym<FF, 2, false>), grid, dim3(256), 0, s, S, nact, g_offS, c.nseg, g_offQ, c.nc, js, x,      \
-		                        c.mvp, c.mvt, c.n);                                                                                 \
-		break;
-			switch (F) {
-				PEEC_MVSYM_CASE(1)
-				PEEC_MVSYM_CASE(2)
-				PEEC_MVSYM_CASE(3)
-				PEEC_MVSYM_CASE(4)
-				PEEC_MVSYM_CASE(5)
-				PEEC_MVSYM_CASE(6)
-				PEEC_MVSYM_CASE(7)
-			default: PEEC_MVSYM_CASE(8)
-			}
-#undef PEEC_MVSYM_CASE
-		}
+		const int NR = (ntm + MV_JS - 1) / MV_JS;   // runs per row block at most (partN rows per system)
+		if ((ntS + ntQ) > 0)
+			hipLaunchKernelGGL(k_mvsym, dim3((unsigned)NR, (unsigned)(ntS + ntQ)), dim3(256), 0, s, S, nact, F, g_offS, c.nseg,
+			                   g_offQ, c.nc, NR, ntm, x, c.mvp, c.mvt, c.n);
 		if (g_mvt.on) {
-			// the upper triangle's tiles of the nact systems (16 B per entry), x once, the item and
+			// the upper triangle's tiles of the nact systems (16 B per entry), x once, the run and
 			// mirror sums written
 			HIPCHK(hipEventRecord(g_mvt.ev[g_mvt.used + 1], s));
 			g_mvt.used += 2;
@@ -1000,13 +966,13 @@ void matvec(fast &c, hipStream_t s, const double2 *x, double2 *y, int F, int nac
 				for (int I = 0; I < nt; I++) {
 					const int rows = std::min(64, nb - 64 * I);
 					ent += (double)rows * (nb - 64 * I);
-					sums += (double)((nt - I + js - 1) / js) * rows + (double)(nb - 64 * (I + 1) > 0 ? nb - 64 * (I + 1) : 0);
+					sums += (double)((nt - I + MV_JS - 1) / MV_JS) * rows + (double)(nb - 64 * (I + 1) > 0 ? nb - 64 * (I + 1) : 0);
 				}
 			}
-			g_mvt.bytes += 16.0 * nact * ent + 16.0 * c.n * F + 16.0 * F * sums;
+			g_mvt.bytes += 16.0 * nact * ent + 16.0 * c.n * F + 16.0 * nact * sums;
 		}
 		hipLaunchKernelGGL(k_mvfin_sym, dim3(blocks((long)c.n * F)), dim3(256), 0, s, S, c.n, F, nact, g_offS, c.nseg, g_offQ,
-		                   c.nc, js, (const double2 *)c.mvp, (const double2 *)c.mvt, x, y);
+		                   c.nc, NR, ntm, (const double2 *)c.mvp, (const double2 *)c.mvt, x, y);
 		return;
 	}
 	const int cs = mv_cs();
@@ -1386,7 +1352,7 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 		{
 			const int ntm = (std::max(p->nseg, nc) + 63) / 64;
 			const size_t segs = std::max((size_t)((std::max(p->nseg, nc) + mv_cs() - 1) / mv_cs()),
-			                             (size_t)((ntm + mv_js() - 1) / mv_js()));
+			                             (size_t)((ntm + MV_JS - 1) / MV_JS));
 			c.mvp = dal<double2>(segs * n * FMAX);
 			c.mvt = dal<double2>((size_t)std::max(1, ntm) * n * FMAX);
 		}
