@@ -671,71 +671,107 @@ void launch_gemm(hipStream_t s, int M, int N, int K, const double2 *A, long lda,
 	hipLaunchKernelGGL(k_zgemm, grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc);
 }
 
-// C[M x N] -= A[M x K] B[K x N] with complex FP32 storage, FP32 MFMA products accumulated in
-// FP32 (v_mfma_f32_16x16x4f32: A row = lane%16, k = lane/16; B col = lane%16, k = lane/16;
-// D register v of lane l holds row 4 (l/16) + v, col l%16).  Wave tile 32 x 32 (2 x 2 blocks),
-// workgroup 2 x 2 waves = 64 x 64; complex = 4 real MFMAs into (re, im) accumulators.
-typedef float f32x4 __attribute__((ext_vector_type(4)));
+// C[M x N] -= A[M x K] B[K x N] with complex FP32 storage, FP32 MFMA products accumulated in FP32.
+// Workgroup tile 128 x 128 (four waves of 64 x 64), K in steps of 8 staged through LDS in two
+// stages: the next step's global loads are issued before the current step's products.  A's rows
+// and B's columns are stored in LDS as real and imaginary planes, k-contiguous with a 12-float
+// stride (16-byte reads without bank conflicts).  Products on v_mfma_f32_32x32x2f32: lane l feeds
+// row (A) / column (B) l % 32 and, in step t of a K step, k = 4 (l / 32) + t, so one 16-byte LDS
+// read serves four products; D register v of lane l holds row 8 (v / 4) + 4 (l / 32) + v % 4,
+// column l % 32.  Complex = four real products into (re, im) accumulators.  The epilogue reads a
+// 32 x 32 block's C entries before writing any of them.
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+constexpr int CG_BM = 128, CG_BN = 128, CG_KT = 8, CG_LS = 12;
 
-__global__ __launch_bounds__(256) void k_cgemm32(int M, int N, int K, const float2 *__restrict__ A, long lda,
-                                                 const float2 *__restrict__ B, long ldb, float2 *__restrict__ C, long ldc)
+__global__ __launch_bounds__(256, 2) void k_cgemm32(int M, int N, int K, const float2 *__restrict__ A, long lda,
+                                                    const float2 *__restrict__ B, long ldb, float2 *__restrict__ C, long ldc)
 {
-	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	__shared__ __attribute__((aligned(16))) float sA[2][2][CG_BM * CG_LS];
+	__shared__ __attribute__((aligned(16))) float sB[2][2][CG_BN * CG_LS];
+	const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
 	int tx, ty;
 	xcd_tile(tx, ty);
-	const int row0 = ty * 64 + (wave >> 1) * 32, col0 = tx * 64 + (wave & 1) * 32;
-	const int li = lane & 15, lk = lane >> 4;
-	f32x4 acr[2][2], aci[2][2];
+	const int bm0 = ty * CG_BM, bn0 = tx * CG_BN;
+	const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+	// global loads: thread t takes row t / 2 of A's tile (k half t % 2) and column t % 128 of B's
+	// (k half t / 128), four k each
+	const int ar = tid >> 1, akh = (tid & 1) * 4;
+	const int bj = tid & 127, bkh = (tid >> 7) * 4;
+	const int agr = bm0 + ar, bgc = bn0 + bj;
+	const bool aok = agr < M, bok = bgc < N;
+	const float2 *ap = A + (long)(aok ? agr : 0) * lda;
+	const float2 *bp = B + (bok ? bgc : 0);
+	float2 ra[4], rb[4];
+	auto gload = [&](int k0) {
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			const int ka = k0 + akh + u, kb = k0 + bkh + u;
+			ra[u] = (aok && (ka < K)) ? ap[ka] : make_float2(0, 0);
+			rb[u] = (bok && (kb < K)) ? bp[(long)kb * ldb] : make_float2(0, 0);
+		}
+	};
+	auto sstore = [&](int st) {
+		*(f32x4v *)&sA[st][0][ar * CG_LS + akh] = f32x4v{ra[0].x, ra[1].x, ra[2].x, ra[3].x};
+		*(f32x4v *)&sA[st][1][ar * CG_LS + akh] = f32x4v{ra[0].y, ra[1].y, ra[2].y, ra[3].y};
+		*(f32x4v *)&sB[st][0][bj * CG_LS + bkh] = f32x4v{rb[0].x, rb[1].x, rb[2].x, rb[3].x};
+		*(f32x4v *)&sB[st][1][bj * CG_LS + bkh] = f32x4v{rb[0].y, rb[1].y, rb[2].y, rb[3].y};
+	};
+	f32x16 cr[2][2], ci[2][2];
+#pragma unroll
+	for (int a = 0; a < 2; a++)
+#pragma unroll
+		for (int b = 0; b < 2; b++)
+#pragma unroll
+			for (int v = 0; v < 16; v++) cr[a][b][v] = ci[a][b][v] = 0;
+	const int li = lane & 31, lh = (lane >> 5) * 4;
+	const int nkt = (K + CG_KT - 1) / CG_KT;
+	gload(0);
+	sstore(0);
+	__syncthreads();
+	for (int kt = 0; kt < nkt; kt++) {
+		const int st = kt & 1;
+		if (kt + 1 < nkt) gload((kt + 1) * CG_KT);
+		f32x4v xr[2], xi[2], xn[2], yr[2], yi[2];
+#pragma unroll
+		for (int b = 0; b < 2; b++) {
+			xr[b] = *(const f32x4v *)&sA[st][0][(wm + b * 32 + li) * CG_LS + lh];
+			xi[b] = *(const f32x4v *)&sA[st][1][(wm + b * 32 + li) * CG_LS + lh];
+			xn[b] = -xi[b];
+			yr[b] = *(const f32x4v *)&sB[st][0][(wn + b * 32 + li) * CG_LS + lh];
+			yi[b] = *(const f32x4v *)&sB[st][1][(wn + b * 32 + li) * CG_LS + lh];
+		}
+#pragma unroll
+		for (int t = 0; t < 4; t++)
+#pragma unroll
+			for (int a = 0; a < 2; a++)
+#pragma unroll
+				for (int b = 0; b < 2; b++) {
+					cr[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(xr[a][t], yr[b][t], cr[a][b], 0, 0, 0);
+					cr[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(xn[a][t], yi[b][t], cr[a][b], 0, 0, 0);
+					ci[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(xr[a][t], yi[b][t], ci[a][b], 0, 0, 0);
+					ci[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(xi[a][t], yr[b][t], ci[a][b], 0, 0, 0);
+				}
+		if (kt + 1 < nkt) sstore(st ^ 1);
+		__syncthreads();
+	}
 #pragma unroll
 	for (int a = 0; a < 2; a++)
 #pragma unroll
 		for (int b = 0; b < 2; b++) {
-			acr[a][b] = f32x4{0, 0, 0, 0};
-			aci[a][b] = f32x4{0, 0, 0, 0};
+			const int c = bn0 + wn + b * 32 + li;
+			float2 cv[16];
+#pragma unroll
+			for (int v = 0; v < 16; v++) {
+				const int r = bm0 + wm + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+				cv[v] = ((r < M) && (c < N)) ? C[(long)r * ldc + c] : make_float2(0, 0);
+			}
+#pragma unroll
+			for (int v = 0; v < 16; v++) {
+				const int r = bm0 + wm + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+				if ((r < M) && (c < N)) C[(long)r * ldc + c] = make_float2(cv[v].x - cr[a][b][v], cv[v].y - ci[a][b][v]);
+			}
 		}
-	const float2 *arow[2], *bcol[2];
-	bool aok[2], bok[2];
-#pragma unroll
-	for (int rb = 0; rb < 2; rb++) {
-		const int r = row0 + rb * 16 + li;
-		aok[rb] = r < M;
-		arow[rb] = A + (long)(aok[rb] ? r : 0) * lda;
-	}
-#pragma unroll
-	for (int cb = 0; cb < 2; cb++) {
-		const int c = col0 + cb * 16 + li;
-		bok[cb] = c < N;
-		bcol[cb] = B + (bok[cb] ? c : 0);
-	}
-	for (int k = 0; k < K; k += 4) {
-		const int kk = k + lk;
-		float2 fa[2], fb[2];
-#pragma unroll
-		for (int rb = 0; rb < 2; rb++) fa[rb] = (aok[rb] && (kk < K)) ? arow[rb][kk] : make_float2(0, 0);
-#pragma unroll
-		for (int cb = 0; cb < 2; cb++) fb[cb] = (bok[cb] && (kk < K)) ? bcol[cb][(long)kk * ldb] : make_float2(0, 0);
-#pragma unroll
-		for (int rb = 0; rb < 2; rb++)
-#pragma unroll
-			for (int cb = 0; cb < 2; cb++) {
-				acr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[rb].x, fb[cb].x, acr[rb][cb], 0, 0, 0);
-				acr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(-fa[rb].y, fb[cb].y, acr[rb][cb], 0, 0, 0);
-				aci[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[rb].x, fb[cb].y, aci[rb][cb], 0, 0, 0);
-				aci[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[rb].y, fb[cb].x, aci[rb][cb], 0, 0, 0);
-			}
-	}
-#pragma unroll
-	for (int rb = 0; rb < 2; rb++)
-#pragma unroll
-		for (int cb = 0; cb < 2; cb++)
-#pragma unroll
-			for (int v = 0; v < 4; v++) {
-				const int r = row0 + rb * 16 + 4 * lk + v, c = col0 + cb * 16 + li;
-				if ((r < M) && (c < N)) {
-					float2 &x = C[(long)r * ldc + c];
-					x = make_float2(x.x - acr[rb][cb][v], x.y - aci[rb][cb][v]);
-				}
-			}
 }
 
 void launch_update(hipStream_t s, int M, int N, int K, const double2 *A, long lda, const double2 *B, long ldb, double2 *C,
@@ -763,7 +799,7 @@ void launch_update(hipStream_t s, int M, int N, int K, const float2 *A, long lda
 		           "rocblas_cgemm");
 		return;
 	}
-	dim3 grid((N + 63) / 64, (M + 63) / 64);
+	dim3 grid((N + CG_BN - 1) / CG_BN, (M + CG_BM - 1) / CG_BM);
 	hipLaunchKernelGGL(k_cgemm32, grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc);
 }
 
