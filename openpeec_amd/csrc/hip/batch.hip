@@ -673,15 +673,17 @@ __device__ void lu_subst(const TS *A, int N, const int *piv, double2 *v)
 	}
 }
 
-constexpr int BRF_MAX = 10;   // refinement sweeps of the FP32 batched factorisation
+constexpr int BRF_MAX = 10;          // refinement sweeps of the FP32 batched factorisation
+constexpr double BRF_TARGET = 1e-10;  // componentwise backward error the FP32 form guarantees
+constexpr double BRF_STOP = 1e-12;    // refinement ends at this backward error
 
 // persistent: factor and solve jobs from the queue until it is empty.  Dynamic LDS holds the
 // right-hand side (N complex) and the pivot rows (N ints) of the current job.  FP64 storage
 // (arena, TS = double2) or FP32 storage (TS = float2) with every port solution refined against
-// the FP64 system A64 (r = b - A64 x in FP64, x += LU^-1 r) while the componentwise backward
-// error max_i |r_i| / (|A64| |x| + |b|)_i falls by half and is above 2 eps64; a job whose error
-// stays above 1e-10 is marked (info -2) for an FP64 factorisation.  rscr holds one residual
-// vector per workgroup.
+// the FP64 system A64 (r = b - A64 x in FP64, x += LU^-1 r) until the componentwise backward
+// error max_i |r_i| / (|A64| |x| + |b|)_i is at most BRF_STOP, or stops falling by half, or
+// reaches 2 eps64; a job whose error stays above BRF_TARGET is marked (info -2) for an FP64
+// factorisation.  rscr holds one residual vector per workgroup.
 template <int NB, int CB, typename TS>
 __global__ __launch_bounds__(256) void k_b_lu(const bdesc *D, const bjob *J, const int *order, int nj, unsigned *counter,
                                               TS *arena, const double2 *arena64, double2 *rscr, long rstride,
@@ -757,7 +759,7 @@ __global__ __launch_bounds__(256) void k_b_lu(const bdesc *D, const bjob *J, con
 					be = s_rv[0];
 					for (int w = 1; w < 4; w++) be = (s_rv[w] > be) ? s_rv[w] : be;
 					__syncthreads();
-					if ((be <= 4.5e-16) || (be > 0.5 * last) || (it == BRF_MAX)) break;
+					if ((be <= BRF_STOP) || (be <= 4.5e-16) || (be > 0.5 * last) || (it == BRF_MAX)) break;
 					last = be;
 					lu_subst<TS>(A, N, piv, r);
 					for (int i = tid; i < N; i += 256) b[i] = make_double2(b[i].x + r[i].x, b[i].y + r[i].y);
@@ -774,7 +776,7 @@ __global__ __launch_bounds__(256) void k_b_lu(const bdesc *D, const bjob *J, con
 			__syncthreads();
 		}
 		if (tid == 0) {
-			info_out[jid] = (worst > 1e-10) ? -2 : -1;
+			info_out[jid] = (worst > BRF_TARGET) ? -2 : -1;
 			if (berr_out) berr_out[jid] = worst;
 			if (sweeps_out) sweeps_out[jid] = sweeps;
 		}

@@ -1015,11 +1015,13 @@ int pgpu_sweep(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq, d_complex_t *z
 			               (d_complex_t *)g->Bw);
 		}
 		else {
-			// x = LU32^-1 b, then r = b - A x in FP64 and x += LU32^-1 r while the componentwise
-			// backward error max_i |r_i| / (|A| |x| + |b|)_i falls by half per sweep and is above
-			// 2 eps64 (the stopping rule of LAPACK's zgerfs, at most RF_MAX sweeps); a point whose
-			// error stays above 1e-10 is factored again in FP64
+			// x = LU32^-1 b, then r = b - A x in FP64 and x += LU32^-1 r until the componentwise
+			// backward error max_i |r_i| / (|A| |x| + |b|)_i is at most RF_STOP, or stops falling
+			// by half per sweep, or reaches 2 eps64 (LAPACK zgerfs's rule; at most RF_MAX sweeps);
+			// a point whose error stays above RF_TARGET (the form's guarantee) is factored again
+			// in FP64.  One sweep usually takes the FP32 solve's 1e-7 to about 1e-14.
 			constexpr int RF_MAX = 10;
+			constexpr double RF_TARGET = 1e-10, RF_STOP = 1e-12;
 			HIPCHK(hipMemcpyAsync(g->Brhs, g->B, (size_t)n * np * sizeof(double2), hipMemcpyDeviceToDevice, g->s));
 			slu_solve_perm(g->s, n, g->A32, n, g->aperm, (d_complex_t *)g->B, np, np, (d_complex_t *)g->Bw);
 			std::vector<double> h(np);
@@ -1034,14 +1036,14 @@ int pgpu_sweep(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq, d_complex_t *z
 				berr = 0;
 				for (int j = 0; j < np; j++)
 					if (h[j] > berr) berr = h[j];
-				if ((berr <= 4.5e-16) || (berr > 0.5 * last) || (its == RF_MAX)) break;
+				if ((berr <= RF_STOP) || (berr <= 4.5e-16) || (berr > 0.5 * last) || (its == RF_MAX)) break;
 				last = berr;
 				slu_solve_perm(g->s, n, g->A32, n, g->aperm, (d_complex_t *)g->R, np, np, (d_complex_t *)g->Bw);
 				hipLaunchKernelGGL(k_addto, dim3((unsigned)(((long)n * np + 255) / 256)), dim3(256), 0, g->s, g->B,
 				                   (const double2 *)g->R, (long)n * np);
 				its++;
 			}
-			if (berr > 1e-10) {
+			if (berr > RF_TARGET) {
 				if (st) st->fp64_fallbacks++;
 				HIPCHK(hipMemcpyAsync(g->B, g->Brhs, (size_t)n * np * sizeof(double2), hipMemcpyDeviceToDevice, g->s));
 				if (dlu_factor(g->s, n, (d_complex_t *)g->A, n, g->apiv, g->aperm, st) >= 0) {
