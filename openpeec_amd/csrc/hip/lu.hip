@@ -144,7 +144,8 @@ __global__ __launch_bounds__(PANEL_T) void k_panel(TS *A, long lda, int n, int k
 			if (lane == 0) ws.pubi[par * P + w] = bi;
 			if ((bv >= 0) && (lane < nb)) ws.pubrow[((long)par * P + w) * NB + lane] = S[(bi - r0) * SROW + lane];
 			if ((kk >= r0) && (kk < r0 + rc) && (lane < nb)) ws.diagrow[par * NB + lane] = S[(kk - r0) * SROW + lane];
-			__threadfence();
+			// release only (L2 write-back): this side reads nothing the other workgroups wrote
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
 			if (lane == 0) __hip_atomic_store(&ws.slot[ring * PANEL_PMAX + w], bv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 			// wait until every workgroup's slot of this column holds a value (NaN: not yet),
 			// reading the values themselves; bounded, so a fault in the exchange ends the
@@ -153,16 +154,18 @@ __global__ __launch_bounds__(PANEL_T) void k_panel(TS *A, long lda, int n, int k
 			unsigned spins = 0;
 			bool timeout = false;
 			for (;;) {
-				bool ok = true;
+				// the lane's slots read together (one round trip), then tested
 #pragma unroll
 				for (int i = 0; i < PANEL_PMAX / 64; i++) {
 					const int q = lane + 64 * i;
-					qv[i] = -1;
-					if (q < P) {
-						qv[i] = __hip_atomic_load(&ws.slot[ring * PANEL_PMAX + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-						if (qv[i] != qv[i]) ok = false;
-					}
+					qv[i] = (q < P) ? __hip_atomic_load(&ws.slot[ring * PANEL_PMAX + min(q, P - 1)], __ATOMIC_RELAXED,
+					                                    __HIP_MEMORY_SCOPE_AGENT)
+					                : -1.0;
 				}
+				bool ok = true;
+#pragma unroll
+				for (int i = 0; i < PANEL_PMAX / 64; i++)
+					if (qv[i] != qv[i]) ok = false;
 				if (__all(ok)) break;
 				if (++spins > PANEL_SPIN_LIMIT) {
 					timeout = true;
@@ -170,7 +173,8 @@ __global__ __launch_bounds__(PANEL_T) void k_panel(TS *A, long lda, int n, int k
 				}
 				__builtin_amdgcn_s_sleep(1);
 			}
-			__threadfence();
+			// acquire only (L2 invalidate): the candidates' rows are read after their slots
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 			// the winner: largest value, ties to the smaller workgroup (whose rows come first)
 			double gv = -1;
 			int gw = INT_MAX;
