@@ -48,6 +48,7 @@ struct pgpu {
 	int *apiv = nullptr, *aperm = nullptr;
 	double2 *B = nullptr, *Bw = nullptr;   // n x nport right-hand sides and workspace
 	float2 *A32 = nullptr;                 // precision form: FP32 copy that is factored
+	double2 *dinv32 = nullptr;             // its factors' 64 x 64 diagonal-block inverses (L, then U)
 	double2 *Brhs = nullptr, *R = nullptr; // precision form: right-hand sides, residuals
 	unsigned long long *cberr = nullptr;  // precision form: componentwise backward error per port
 	int *cellrow = nullptr;      // [n] capacitive cell of node row r, or -1
@@ -474,6 +475,7 @@ void release_mirror(pgpu *g)
 		dfree(g->B);
 		dfree(g->Bw);
 		dfree(g->A32);
+		dfree(g->dinv32);
 		dfree(g->Brhs);
 		dfree(g->R);
 		dfree(g->cberr);
@@ -982,6 +984,7 @@ int pgpu_sweep(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq, d_complex_t *z
 		if (precision) {
 			if (g->A32 == nullptr) {
 				g->A32 = dalloc<float2>((size_t)n * n);
+				g->dinv32 = dalloc<double2>((size_t)2 * ((n + 63) / 64) * 64 * 64);
 				const int npa = (np > 0) ? np : 1;
 				g->Brhs = dalloc<double2>((size_t)n * npa);
 				g->R = dalloc<double2>((size_t)n * npa);
@@ -1022,8 +1025,16 @@ int pgpu_sweep(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq, d_complex_t *z
 			// in FP64.  One sweep usually takes the FP32 solve's 1e-7 to about 1e-14.
 			constexpr int RF_MAX = 10;
 			constexpr double RF_TARGET = 1e-10, RF_STOP = 1e-12;
+			// the solves with the factors: each 64-row step through its diagonal block's inverse
+			// (up to four ports at once), else by substitution
+			d_complex_t *dL = (d_complex_t *)g->dinv32, *dU = dL + (size_t)((n + 63) / 64) * 64 * 64;
+			if (np <= 4) slu_diag_inverse(g->s, n, g->A32, n, dL, dU);
+			auto solve32 = [&](double2 *X) {
+				if (np <= 4) slu_solve_perm_di(g->s, n, g->A32, n, g->aperm, (d_complex_t *)X, np, np, (d_complex_t *)g->Bw, dL, dU);
+				else slu_solve_perm(g->s, n, g->A32, n, g->aperm, (d_complex_t *)X, np, np, (d_complex_t *)g->Bw);
+			};
 			HIPCHK(hipMemcpyAsync(g->Brhs, g->B, (size_t)n * np * sizeof(double2), hipMemcpyDeviceToDevice, g->s));
-			slu_solve_perm(g->s, n, g->A32, n, g->aperm, (d_complex_t *)g->B, np, np, (d_complex_t *)g->Bw);
+			solve32(g->B);
 			std::vector<double> h(np);
 			double berr = 1, last = 2;
 			int its = 0;
@@ -1038,7 +1049,7 @@ int pgpu_sweep(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq, d_complex_t *z
 					if (h[j] > berr) berr = h[j];
 				if ((berr <= RF_STOP) || (berr <= 4.5e-16) || (berr > 0.5 * last) || (its == RF_MAX)) break;
 				last = berr;
-				slu_solve_perm(g->s, n, g->A32, n, g->aperm, (d_complex_t *)g->R, np, np, (d_complex_t *)g->Bw);
+				solve32(g->R);
 				hipLaunchKernelGGL(k_addto, dim3((unsigned)(((long)n * np + 255) / 256)), dim3(256), 0, g->s, g->B,
 				                   (const double2 *)g->R, (long)n * np);
 				its++;

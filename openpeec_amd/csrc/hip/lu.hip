@@ -838,8 +838,8 @@ unsigned tiblocks(long n)
 // Inverses of the TI x TI diagonal blocks of the packed LU factors (unit lower when LOWER,
 // else upper), FP64 arithmetic on either storage.  One workgroup per block; thread c solves
 // T y = e_c in LDS.  Rows past the end of a short last block act as identity.
-template <bool LOWER, typename TS>
-__global__ __launch_bounds__(64) void k_diaginv(const TS *A, long lda, int n, TS *dinv)
+template <bool LOWER, typename TS, typename TD = TS>
+__global__ __launch_bounds__(64) void k_diaginv(const TS *A, long lda, int n, TD *dinv)
 {
 	__shared__ double2 T[TI * TI];
 	__shared__ double2 Y[TI * TI];
@@ -1115,6 +1115,28 @@ extern "C" int slu_factor(void *stream, int n, void *dA, int lda, int *dpiv, int
 	return lu_factor_t<float2>((hipStream_t)stream, n, (float2 *)dA, lda, dpiv, dperm, st);
 }
 
+// X[0..nb) <- D X[0..nb) for one 64 x 64 block D (row-major, FP64) and nrhs <= 4 columns of X:
+// the block is staged in LDS, thread t forms row t % 64 of column t / 64
+__global__ __launch_bounds__(256) void k_dmv(const double2 *__restrict__ D, double2 *X, long ldx, int nb, int nrhs)
+{
+	__shared__ double2 xs[TI * 4];
+	const int t = threadIdx.x, r = t & 63, j = t >> 6;
+	if ((r < nb) && (j < nrhs)) xs[r * 4 + j] = X[(long)r * ldx + j];
+	__syncthreads();
+	if ((r < nb) && (j < nrhs)) {
+		double ar = 0, ai = 0;
+		const double2 *d = D + (long)r * TI;
+		for (int c = 0; c < nb; c++) {
+			const double2 a = d[c], x = xs[c * 4 + j];
+			ar = fma(a.x, x.x, ar);
+			ar = fma(-a.y, x.y, ar);
+			ai = fma(a.x, x.y, ai);
+			ai = fma(a.y, x.x, ai);
+		}
+		X[(long)r * ldx + j] = make_double2(ar, ai);
+	}
+}
+
 // B <- A^-1 B (FP64 right-hand sides) with FP32-stored factors from slu_factor, nrhs <= 4 per
 // pass; dwork holds n * nrhs entries.
 extern "C" void slu_solve_perm(void *stream, int n, const void *dA_, int lda, const int *dperm, d_complex_t *dB_, int ldb,
@@ -1152,6 +1174,53 @@ extern "C" void slu_solve_perm(void *stream, int n, const void *dA_, int lda, co
 				hipLaunchKernelGGL(k_gemv<float2>, dim3((r0 + 15) / 16), dim3(256), 0, s, r0, nb, nr, A + r0, (long)lda,
 				                   (const double2 *)(B + (long)r0 * ldb), (long)ldb, B, (long)ldb);
 		}
+	}
+}
+
+// The inverses of the 64 x 64 diagonal blocks of slu_factor's factors (unit lower, upper), in
+// FP64: dinvL, dinvU hold ceil(n / 64) blocks each.
+extern "C" void slu_diag_inverse(void *stream, int n, const void *dA_, int lda, d_complex_t *dinvL, d_complex_t *dinvU)
+{
+	hipStream_t s = (hipStream_t)stream;
+	const float2 *A = (const float2 *)dA_;
+	const int nblk = (n + TI - 1) / TI;
+	hipLaunchKernelGGL((k_diaginv<true, float2, double2>), dim3(nblk), dim3(64), 0, s, A, (long)lda, n, (double2 *)dinvL);
+	hipLaunchKernelGGL((k_diaginv<false, float2, double2>), dim3(nblk), dim3(64), 0, s, A, (long)lda, n, (double2 *)dinvU);
+}
+
+// slu_solve_perm with each 64-row step's triangle solved by its block inverse (slu_diag_inverse)
+// instead of a substitution: a step is then two parallel products, where the substitution's
+// 64 dependent rows made it the chain's longest link.  nrhs <= 4.
+extern "C" void slu_solve_perm_di(void *stream, int n, const void *dA_, int lda, const int *dperm, d_complex_t *dB_,
+                                  int ldb, int nrhs, d_complex_t *dwork, const d_complex_t *dinvL, const d_complex_t *dinvU)
+{
+	hipStream_t s = (hipStream_t)stream;
+	const float2 *A = (const float2 *)dA_;
+	double2 *B = (double2 *)dB_;
+	double2 *W = (double2 *)dwork;
+	const double2 *DL = (const double2 *)dinvL, *DU = (const double2 *)dinvU;
+	const int nr = nrhs;
+	const long tot = (long)n * nr;
+	hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, (const double2 *)B, (long)ldb, W,
+	                   (long)nr, dperm, n, nr);
+	hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, (const double2 *)W, (long)nr, B,
+	                   (long)ldb, n, nr);
+	for (int r0 = 0; r0 < n; r0 += NB) {
+		const int nb = (n - r0 < NB) ? (n - r0) : NB;
+		hipLaunchKernelGGL(k_dmv, dim3(1), dim3(256), 0, s, DL + (long)(r0 / TI) * TI * TI, B + (long)r0 * ldb, (long)ldb, nb, nr);
+		const int rest = n - r0 - nb;
+		if (rest > 0)
+			hipLaunchKernelGGL(k_gemv<float2>, dim3((rest + 15) / 16), dim3(256), 0, s, rest, nb, nr, A + (long)(r0 + nb) * lda + r0,
+			                   (long)lda, (const double2 *)(B + (long)r0 * ldb), (long)ldb, B + (long)(r0 + nb) * ldb, (long)ldb);
+	}
+	const int nblk = (n + NB - 1) / NB;
+	for (int b = nblk - 1; b >= 0; b--) {
+		const int r0 = b * NB;
+		const int nb = (n - r0 < NB) ? (n - r0) : NB;
+		hipLaunchKernelGGL(k_dmv, dim3(1), dim3(256), 0, s, DU + (long)b * TI * TI, B + (long)r0 * ldb, (long)ldb, nb, nr);
+		if (r0 > 0)
+			hipLaunchKernelGGL(k_gemv<float2>, dim3((r0 + 15) / 16), dim3(256), 0, s, r0, nb, nr, A + r0, (long)lda,
+			                   (const double2 *)(B + (long)r0 * ldb), (long)ldb, B, (long)ldb);
 	}
 }
 

@@ -188,6 +188,11 @@ void slu_tri_inverse(void *stream, int n, const void *LU, int lda, int lower, vo
 int  slu_factor(void *stream, int n, void *dA, int lda, int *dpiv, int *dperm, pgpu_stats_t *st);
 void slu_solve_perm(void *stream, int n, const void *dA, int lda, const int *dperm,
 	d_complex_t *dB, int ldb, int nrhs, d_complex_t *dwork);
+/* inverses of the 64 x 64 diagonal blocks of slu_factor's factors (FP64, ceil(n/64) blocks each),
+ * and the solve that uses them for each block step (nrhs <= 4) */
+void slu_diag_inverse(void *stream, int n, const void *dA, int lda, d_complex_t *dinvL, d_complex_t *dinvU);
+void slu_solve_perm_di(void *stream, int n, const void *dA, int lda, const int *dperm, d_complex_t *dB, int ldb, int nrhs,
+	d_complex_t *dwork, const d_complex_t *dinvL, const d_complex_t *dinvU);
 /* one right-preconditioned GMRES(60) cycle on a dense device matrix A with the LU factors LU /
  * perm (dlu_factor's) of a nearby matrix as preconditioner (dgmres.hip); b (device) becomes the
  * solution when the cycle reaches tol |b|.  Returns iterations or -1.  ws: dgmres_ws_create(n). */
