@@ -376,22 +376,31 @@ __device__ __forceinline__ void trailing_update(double2 *A, long ld, int k0, int
 #pragma unroll
 			for (int q = 0; q < CB; q++) fb[q] = nb4[q];
 		}
+		// the tile's C entries, every load issued before the first store
+		double2 cv[2][CB][4];
 #pragma unroll
 		for (int rb = 0; rb < 2; rb++)
 #pragma unroll
-			for (int cb = 0; cb < CB; cb++) {
-				const int c = c0 + cb * 16 + li;
-				if (c >= m) continue;
+			for (int cb = 0; cb < CB; cb++)
 #pragma unroll
 				for (int v = 0; v < 4; v++) {
-					const int r = r0 + rb * 16 + lk + 4 * v;
-					if (r >= m) continue;
-					double2 x = C[(long)r * ld + c];
-					x.x -= ar[rb][cb][v];
-					x.y -= ai[rb][cb][v];
-					C[(long)r * ld + c] = x;
+					const int r = r0 + rb * 16 + lk + 4 * v, c = c0 + cb * 16 + li;
+					cv[rb][cb][v] = ((r < m) && (c < m)) ? C[(long)r * ld + c] : make_double2(0, 0);
 				}
-			}
+#pragma unroll
+		for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+			for (int cb = 0; cb < CB; cb++)
+#pragma unroll
+				for (int v = 0; v < 4; v++) {
+					const int r = r0 + rb * 16 + lk + 4 * v, c = c0 + cb * 16 + li;
+					if ((r < m) && (c < m)) {
+						double2 x = cv[rb][cb][v];
+						x.x -= ar[rb][cb][v];
+						x.y -= ai[rb][cb][v];
+						C[(long)r * ld + c] = x;
+					}
+				}
 	}
 }
 
@@ -439,6 +448,16 @@ __device__ __forceinline__ void trailing_update(float2 *A, long ld, int k0, int 
 			bok[cb] = c < m;
 			bcol[cb] = U12 + (bok[cb] ? c : 0);
 		}
+		float2 cv[2][CB][4];
+#pragma unroll
+		for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+			for (int cb = 0; cb < CB; cb++)
+#pragma unroll
+				for (int v = 0; v < 4; v++) {
+					const int r = r0 + rb * 16 + 4 * lk + v, c = c0 + cb * 16 + li;
+					cv[rb][cb][v] = ((r < m) && (c < m)) ? C[(long)r * ld + c] : make_float2(0, 0);
+				}
 #pragma unroll 1
 		for (int k = 0; k < nb; k += 4) {
 			const int kk = k + lk;
@@ -465,16 +484,159 @@ __device__ __forceinline__ void trailing_update(float2 *A, long ld, int k0, int 
 				for (int v = 0; v < 4; v++) {
 					const int r = r0 + rb * 16 + 4 * lk + v, c = c0 + cb * 16 + li;
 					if ((r < m) && (c < m)) {
-						float2 &x = C[(long)r * ld + c];
-						x = make_float2(x.x - ar[rb][cb][v], x.y - ai[rb][cb][v]);
+						const float2 x = cv[rb][cb][v];
+						C[(long)r * ld + c] = make_float2(x.x - ar[rb][cb][v], x.y - ai[rb][cb][v]);
 					}
 				}
 	}
 }
 
+// a value as the storage type holds it
+template <typename TS>
+__device__ __forceinline__ double2 rnd(double2 v)
+{
+	return v;
+}
+template <>
+__device__ __forceinline__ double2 rnd<float2>(double2 v)
+{
+	return make_double2((double)(float)v.x, (double)(float)v.y);
+}
+
+constexpr int BREG = 5;        // sub-panel rows held per thread: systems up to 256 BREG in registers
+
+// The unblocked steps of one sub-panel (columns c0 .. ce-1, ce - c0 <= BSW) with its rows in
+// registers: thread t holds rows t + 256 i (i < R).  Pivot search, row interchange, scaling and
+// rank-1 update touch no global memory; the pivot row reaches every thread through s_row[0, BSW)
+// and the displaced row its new owner through s_row[BSW, 2 BSW).  Values are rounded to the
+// storage type after every operation, so the factors are those of the column-by-column form.
+// Nothing else writes the columns outside the sub-panel meanwhile, so their interchanges are
+// applied after the last step, in step order, each thread down its own columns.
+template <int R, typename TS>
+__device__ __forceinline__ void subpanel_reg(TS *A, int N, int c0, int ce, int *piv, double2 *s_row, double *s_rv, int *s_ri,
+                                             int &info)
+{
+	const int tid = threadIdx.x;
+	double2 *s_y = s_row, *s_x = s_row + BSW;
+	double2 sub[R][BSW];
+	int swp[BSW];
+#pragma unroll
+	for (int i = 0; i < R; i++) {
+		const int r = tid + 256 * i;
+#pragma unroll
+		for (int j = 0; j < BSW; j++)
+			sub[i][j] = ((r >= c0) && (r < N) && (c0 + j < ce)) ? ld2(A[(long)r * N + c0 + j]) : make_double2(0, 0);
+	}
+	int done = 0;
+#pragma unroll
+	for (int jc = 0; jc < BSW; jc++) {
+		const int c = c0 + jc;
+		swp[jc] = c;
+		if ((c >= ce) || (info >= 0)) continue;
+		double best = -1;
+		int bi = N;
+#pragma unroll
+		for (int i = 0; i < R; i++) {
+			const int r = tid + 256 * i;
+			if ((r >= c) && (r < N)) {
+				const double a = fabs(sub[i][jc].x) + fabs(sub[i][jc].y);
+				if (a > best) {
+					best = a;
+					bi = r;
+				}
+			}
+		}
+		wg_argmax(best, bi, s_rv, s_ri);
+		if (best < 1e-30) {
+			info = c;
+			continue;
+		}
+		if (tid == 0) piv[c] = bi;
+		swp[jc] = bi;
+		done = jc + 1;
+		const int oc = c & 255, ic = c >> 8, ob = bi & 255, ib = bi >> 8;
+		if (tid == ob) {
+#pragma unroll
+			for (int i = 0; i < R; i++)
+				if (i == ib)
+#pragma unroll
+					for (int j = 0; j < BSW; j++) s_y[j] = sub[i][j];
+		}
+		if ((tid == oc) && (bi != c)) {
+#pragma unroll
+			for (int i = 0; i < R; i++)
+				if (i == ic)
+#pragma unroll
+					for (int j = 0; j < BSW; j++) s_x[j] = sub[i][j];
+		}
+		__syncthreads();
+		if (bi != c) {
+			if (tid == oc) {
+#pragma unroll
+				for (int i = 0; i < R; i++)
+					if (i == ic)
+#pragma unroll
+						for (int j = 0; j < BSW; j++) sub[i][j] = s_y[j];
+			}
+			if (tid == ob) {
+#pragma unroll
+				for (int i = 0; i < R; i++)
+					if (i == ib)
+#pragma unroll
+						for (int j = 0; j < BSW; j++) sub[i][j] = s_x[j];
+			}
+		}
+		const double2 inv = crecip(s_y[jc]);
+		double2 u[BSW];
+#pragma unroll
+		for (int t = jc + 1; t < BSW; t++) u[t] = s_y[t];
+#pragma unroll
+		for (int i = 0; i < R; i++) {
+			const int r = tid + 256 * i;
+			if ((r > c) && (r < N)) {
+				const double2 l = rnd<TS>(cmul(sub[i][jc], inv));
+				sub[i][jc] = l;
+#pragma unroll
+				for (int t = jc + 1; t < BSW; t++) {
+					if (c0 + t >= ce) break;
+					double2 x = sub[i][t];
+					x.x -= (l.x * u[t].x) - (l.y * u[t].y);
+					x.y -= (l.x * u[t].y) + (l.y * u[t].x);
+					sub[i][t] = rnd<TS>(x);
+				}
+			}
+		}
+		__syncthreads();
+	}
+#pragma unroll
+	for (int i = 0; i < R; i++) {
+		const int r = tid + 256 * i;
+		if ((r >= c0) && (r < N))
+#pragma unroll
+			for (int j = 0; j < BSW; j++)
+				if (c0 + j < ce) st2(A[(long)r * N + c0 + j], sub[i][j]);
+	}
+	for (int col = tid; col < N; col += 256) {
+		if ((col >= c0) && (col < ce)) continue;
+#pragma unroll
+		for (int jc = 0; jc < BSW; jc++) {
+			if (jc >= done) break;
+			const int c = c0 + jc, p = swp[jc];
+			if (p != c) {
+				const TS t = A[(long)c * N + col];
+				A[(long)c * N + col] = A[(long)p * N + col];
+				A[(long)p * N + col] = t;
+			}
+		}
+	}
+	__syncthreads();
+}
+
 // in-place LU of the N x N row-major system A: right-looking, panel width NB, row pivoting by
 // the first maximum of |re| + |im| (lu.c:19-30), full-row interchanges recorded in piv (any
 // address space); returns -1, or the column whose pivot is below 1e-30.  All 256 threads call.
+// Systems up to 256 BREG factor each sub-panel in registers (subpanel_reg); larger ones step
+// column by column through memory.
 template <int NB, int CB, typename TS>
 __device__ __forceinline__ int blu_factor(TS *A, int N, int *piv, double2 *s_row, double2 *s_L, double *s_rv, int *s_ri)
 {
@@ -489,50 +651,53 @@ __device__ __forceinline__ int blu_factor(TS *A, int N, int *piv, double2 *s_row
 		// the panel is swept once per sub-panel instead of once per column
 		for (int c0 = k0; (c0 < kend) && (info < 0); c0 += BSW) {
 			const int ce = min(c0 + BSW, kend);
-			for (int c = c0; c < ce; c++) {
-				double best = -1;
-				int bi = N;
-				for (int r = c + tid; r < N; r += 256) {
-					const double2 v = ld2(A[(long)r * N + c]);
-					const double a = fabs(v.x) + fabs(v.y);
-					if (a > best) {
-						best = a;
-						bi = r;
-					}
-				}
-				wg_argmax(best, bi, s_rv, s_ri);
-				if (best < 1e-30) {
-					info = c;
-					break;
-				}
-				if (tid == 0) piv[c] = bi;
-				if (bi != c)
-					for (int col = tid; col < N; col += 256) {
-						const TS t = A[(long)c * N + col];
-						A[(long)c * N + col] = A[(long)bi * N + col];
-						A[(long)bi * N + col] = t;
-					}
-				__syncthreads();
-				const double2 inv = crecip(ld2(A[(long)c * N + c]));
-				for (int r = c + 1 + tid; r < N; r += 256) st2(A[(long)r * N + c], cmul(ld2(A[(long)r * N + c]), inv));
-				const int w = ce - c - 1;
-				if (tid < w) s_row[tid] = ld2(A[(long)c * N + c + 1 + tid]);
-				__syncthreads();
-				if (w > 0) {
-					// rank-1 update of the sub-panel's remaining columns, every row below c
-					for (int r = c + 1 + tid; r < N; r += 256) {
-						TS *ar = A + (long)r * N;
-						const double2 l = ld2(ar[c]);
-						for (int t = 0; t < w; t++) {
-							const double2 u = s_row[t];
-							double2 x = ld2(ar[c + 1 + t]);
-							x.x -= (l.x * u.x) - (l.y * u.y);
-							x.y -= (l.x * u.y) + (l.y * u.x);
-							st2(ar[c + 1 + t], x);
+			if (N <= 256 * BREG) subpanel_reg<BREG, TS>(A, N, c0, ce, piv, s_row, s_rv, s_ri, info);
+			else {
+				for (int c = c0; c < ce; c++) {
+					double best = -1;
+					int bi = N;
+					for (int r = c + tid; r < N; r += 256) {
+						const double2 v = ld2(A[(long)r * N + c]);
+						const double a = fabs(v.x) + fabs(v.y);
+						if (a > best) {
+							best = a;
+							bi = r;
 						}
 					}
+					wg_argmax(best, bi, s_rv, s_ri);
+					if (best < 1e-30) {
+						info = c;
+						break;
+					}
+					if (tid == 0) piv[c] = bi;
+					if (bi != c)
+						for (int col = tid; col < N; col += 256) {
+							const TS t = A[(long)c * N + col];
+							A[(long)c * N + col] = A[(long)bi * N + col];
+							A[(long)bi * N + col] = t;
+						}
+					__syncthreads();
+					const double2 inv = crecip(ld2(A[(long)c * N + c]));
+					for (int r = c + 1 + tid; r < N; r += 256) st2(A[(long)r * N + c], cmul(ld2(A[(long)r * N + c]), inv));
+					const int w = ce - c - 1;
+					if (tid < w) s_row[tid] = ld2(A[(long)c * N + c + 1 + tid]);
+					__syncthreads();
+					if (w > 0) {
+						// rank-1 update of the sub-panel's remaining columns, every row below c
+						for (int r = c + 1 + tid; r < N; r += 256) {
+							TS *ar = A + (long)r * N;
+							const double2 l = ld2(ar[c]);
+							for (int t = 0; t < w; t++) {
+								const double2 u = s_row[t];
+								double2 x = ld2(ar[c + 1 + t]);
+								x.x -= (l.x * u.x) - (l.y * u.y);
+								x.y -= (l.x * u.y) + (l.y * u.x);
+								st2(ar[c + 1 + t], x);
+							}
+						}
+					}
+					__syncthreads();
 				}
-				__syncthreads();
 			}
 			if (info >= 0) break;
 			const int wr = kend - ce, sw = ce - c0;
