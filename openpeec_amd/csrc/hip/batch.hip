@@ -277,6 +277,16 @@ __device__ __forceinline__ double2 crecip(double2 a)
 	return make_double2(a.x / d, -a.y / d);
 }
 
+// x -= l u in two fused multiply-adds per component (the library builds without contraction,
+// so the LU's update loops state their fused form)
+__device__ __forceinline__ void cfms(double2 &x, double2 l, double2 u)
+{
+	x.x = fma(-l.x, u.x, x.x);
+	x.x = fma(l.y, u.y, x.x);
+	x.y = fma(-l.x, u.y, x.y);
+	x.y = fma(-l.y, u.x, x.y);
+}
+
 // (value, index) maximum over the workgroup; ties keep the smaller index
 __device__ __forceinline__ void wg_argmax(double &v, int &ix, double *rv, int *ri)
 {
@@ -600,8 +610,7 @@ __device__ __forceinline__ void subpanel_reg(TS *A, int N, int c0, int ce, int *
 				for (int t = jc + 1; t < BSW; t++) {
 					if (c0 + t >= ce) break;
 					double2 x = sub[i][t];
-					x.x -= (l.x * u[t].x) - (l.y * u[t].y);
-					x.y -= (l.x * u[t].y) + (l.y * u[t].x);
+					cfms(x, l, u[t]);
 					sub[i][t] = rnd<TS>(x);
 				}
 			}
@@ -720,9 +729,7 @@ __device__ __forceinline__ int blu_factor(TS *A, int N, int *piv, double2 *s_row
 #pragma unroll
 						for (int j = 0; j < BSW; j++) {
 							if (j >= i) break;
-							const double2 l = s_L[i * BSW + j];
-							x.x -= (l.x * u[j].x) - (l.y * u[j].y);
-							x.y -= (l.x * u[j].y) + (l.y * u[j].x);
+							cfms(x, s_L[i * BSW + j], u[j]);
 						}
 						u[i] = x;
 						st2(A[(long)(c0 + i) * N + col], x);
@@ -741,9 +748,7 @@ __device__ __forceinline__ int blu_factor(TS *A, int N, int *piv, double2 *s_row
 #pragma unroll
 						for (int j = 0; j < BSW; j++) {
 							if (j >= sw) break;
-							const double2 u = s_U[j * NB + t];
-							x.x -= (l[j].x * u.x) - (l[j].y * u.y);
-							x.y -= (l[j].x * u.y) + (l[j].y * u.x);
+							cfms(x, l[j], s_U[j * NB + t]);
 						}
 						st2(ar[ce + t], x);
 					}
@@ -774,20 +779,12 @@ __device__ __forceinline__ int blu_factor(TS *A, int N, int *piv, double2 *s_row
 				for (int c = 0; c < i0; c++) {
 					const double2 x = ld2(A[(long)(k0 + c) * N + col]);
 #pragma unroll
-					for (int q = 0; q < 8; q++) {
-						const double2 l = s_L[(i0 + q) * NB + c];
-						u[q].x -= (l.x * x.x) - (l.y * x.y);
-						u[q].y -= (l.x * x.y) + (l.y * x.x);
-					}
+					for (int q = 0; q < 8; q++) cfms(u[q], s_L[(i0 + q) * NB + c], x);
 				}
 #pragma unroll
 				for (int c = 0; c < 7; c++)
 #pragma unroll
-					for (int q = c + 1; q < 8; q++) {
-						const double2 l = s_L[(i0 + q) * NB + i0 + c];
-						u[q].x -= (l.x * u[c].x) - (l.y * u[c].y);
-						u[q].y -= (l.x * u[c].y) + (l.y * u[c].x);
-					}
+					for (int q = c + 1; q < 8; q++) cfms(u[q], s_L[(i0 + q) * NB + i0 + c], u[c]);
 #pragma unroll
 				for (int q = 0; q < 8; q++)
 					if (i0 + q < nb) st2(A[(long)(k0 + i0 + q) * N + col], u[q]);
@@ -819,9 +816,7 @@ __device__ void lu_subst(const TS *A, int N, const int *piv, double2 *v)
 		const double2 xc = v[c];
 		if ((xc.x != 0) || (xc.y != 0))
 			for (int r = c + 1 + tid; r < N; r += 256) {
-				const double2 l = ld2(A[(long)r * N + c]);
-				v[r].x -= (l.x * xc.x) - (l.y * xc.y);
-				v[r].y -= (l.x * xc.y) + (l.y * xc.x);
+				cfms(v[r], ld2(A[(long)r * N + c]), xc);
 			}
 		__syncthreads();
 	}
@@ -830,9 +825,7 @@ __device__ void lu_subst(const TS *A, int N, const int *piv, double2 *v)
 		__syncthreads();
 		const double2 xc = v[c];
 		for (int r = tid; r < c; r += 256) {
-			const double2 u = ld2(A[(long)r * N + c]);
-			v[r].x -= (u.x * xc.x) - (u.y * xc.y);
-			v[r].y -= (u.x * xc.y) + (u.y * xc.x);
+			cfms(v[r], ld2(A[(long)r * N + c]), xc);
 		}
 		__syncthreads();
 	}
