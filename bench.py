@@ -495,6 +495,12 @@ def headline_roofline(st, workload: str = "fullstack_sweep", what: str = "of 4 s
     return (mv, lu) if st.mv_ms >= st.lu_gemm_ms else (lu, mv)
 
 
+def device_sync() -> None:
+    """Waits for every stream of the device, the library's own included, so the timed region
+    ends after all the work it started."""
+    C.CDLL("libamdhip64.so").hipDeviceSynchronize()
+
+
 def bench_headline(lib, args) -> dict:
     """The full-stack workload swept cold in K steps (see the module docstring)."""
     from openpeec_amd import PEEC_FORM_MIXED, Peec, decks
@@ -519,6 +525,7 @@ def bench_headline(lib, args) -> dict:
         step_ms.append(1e3 * (time.perf_counter() - ts))
         z[i0:i0 + c] = zc
         tot = st if tot is None else add_stats(tot, st)
+    device_sync()
     elapsed = time.perf_counter() - t0
     log(f"full stack: cold sweep of {npts} points in {len(chunks)} steps, {elapsed:.1f} s ({tot.n_lu} factorisations, "
         f"{tot.gmres_its} GMRES iterations)")
