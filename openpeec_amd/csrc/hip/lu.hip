@@ -1018,6 +1018,21 @@ __global__ void k_fill_ones(unsigned *p, int n)
 // The blocked factorisation on a given workspace: panel exchange buffers ws, permutation
 // scratch perm_ws (n ints, used when dperm is NULL); own_gemm selects the hand-written trailing
 // update (no library call, no allocation) whatever PEEC_GEMM says.
+// Columns per trailing update of the FP32 factorisation (PEEC_LU_KB: 64, 128, 192 or 256;
+// 256 by default): a block of that many columns is factored panel by panel, each panel updating
+// only the rest of the block, and the columns beyond it take one update with K = the block's
+// width, where a library GEMM runs nearer its peak than at K = 64.
+int lu_kb()
+{
+	static int kb = -1;
+	if (kb < 0) {
+		const char *e = getenv("PEEC_LU_KB");
+		const int v = e ? atoi(e) : 256;
+		kb = ((v >= NB) && (v <= 4 * NB) && (v % NB == 0)) ? v : 256;
+	}
+	return kb;
+}
+
 template <typename TS>
 int lu_factor_ws(hipStream_t s, int n, TS *A, int lda, int *dpiv, int *dperm, pgpu_stats_t *st, const panel_ws &wsp,
                  int *perm_ws, bool own_gemm)
@@ -1034,6 +1049,9 @@ int lu_factor_ws(hipStream_t s, int n, TS *A, int lda, int *dpiv, int *dperm, pg
 	}
 	const int pmax = (g_lw.ncu < PANEL_PMAX) ? g_lw.ncu : PANEL_PMAX;
 	const long ldal = lda;
+	// columns per trailing update: FP32 storage takes blocks of lu_kb() columns (panels of NB
+	// inside), FP64 one panel
+	const int kb = (sizeof(TS) == sizeof(float2)) ? lu_kb() : NB;
 	for (int k0 = 0; k0 < n; k0 += NB) {
 		const int nb = (n - k0 < NB) ? (n - k0) : NB;
 		const int m = n - k0;
@@ -1060,22 +1078,46 @@ int lu_factor_ws(hipStream_t s, int n, TS *A, int lda, int *dpiv, int *dperm, pg
 		const int nout = n - nb;
 		hipLaunchKernelGGL(k_laswp<TS>, dim3((nout + 1 + 255) / 256), dim3(256), 0, s, A, ldal, n, k0, nb, dpiv, perm,
 		                   (const int *)wsp.info);
-		const int nr = n - k0 - nb;
-		if (nr > 0) {
+		// the block's columns right of this panel: their U rows, and their rows below updated
+		const int b0 = (k0 / kb) * kb;                 // the block of this panel: [b0, be)
+		const int be = (b0 + kb < n) ? (b0 + kb) : n;
+		const int nbr = be - k0 - nb;
+		if (nbr > 0) {
 			TS *A11 = A + (long)k0 * lda + k0;
 			TS *A12 = A + (long)k0 * lda + k0 + nb;
-			TS *A21 = A + (long)(k0 + nb) * lda + k0;
-			TS *A22 = A + (long)(k0 + nb) * lda + k0 + nb;
-			hipLaunchKernelGGL((k_trsm<true, TS>), dim3((nr + 63) / 64), dim3(64), 0, s, A11, ldal, A12, ldal, nb, nr);
+			hipLaunchKernelGGL((k_trsm<true, TS>), dim3((nbr + 63) / 64), dim3(64), 0, s, A11, ldal, A12, ldal, nb, nbr);
+			launch_update(s, n - k0 - nb, nbr, nb, (const TS *)(A + (long)(k0 + nb) * lda + k0), ldal, (const TS *)A12, ldal,
+			              A + (long)(k0 + nb) * lda + k0 + nb, ldal, own_gemm);
+		}
+		if (k0 + nb < be) continue;
+		// the block is factored: its U rows of the columns beyond it (a 64-row triangular solve
+		// per panel, each followed by the update of the block's later rows), then the trailing
+		// update with K = the block's width
+		const int nr = n - be;
+		if (nr > 0) {
+			for (int j0 = b0; j0 < be; j0 += NB) {
+				const int jb = (be - j0 < NB) ? (be - j0) : NB;
+				TS *X = A + (long)j0 * lda + be;
+				hipLaunchKernelGGL((k_trsm<true, TS>), dim3((nr + 63) / 64), dim3(64), 0, s, A + (long)j0 * lda + j0, ldal, X,
+				                   ldal, jb, nr);
+				const int mr = be - j0 - jb;
+				if (mr > 0)
+					launch_update(s, mr, nr, jb, (const TS *)(A + (long)(j0 + jb) * lda + j0), ldal, (const TS *)X, ldal,
+					              A + (long)(j0 + jb) * lda + be, ldal, own_gemm);
+			}
+			const int kw = be - b0;
+			TS *A21 = A + (long)be * lda + b0;
+			TS *A12 = A + (long)b0 * lda + be;
+			TS *A22 = A + (long)be * lda + be;
 			if (st) HIPCHK(hipEventRecord(e0, s));
-			launch_update(s, nr, nr, nb, (const TS *)A21, ldal, (const TS *)A12, ldal, A22, ldal, own_gemm);
+			launch_update(s, nr, nr, kw, (const TS *)A21, ldal, (const TS *)A12, ldal, A22, ldal, own_gemm);
 			if (st) {
 				HIPCHK(hipEventRecord(e1, s));
 				HIPCHK(hipEventSynchronize(e1));
 				float ms = 0;
 				HIPCHK(hipEventElapsedTime(&ms, e0, e1));
 				st->lu_gemm_ms += ms;
-				st->lu_gemm_flop += 8.0 * nr * (double)nr * nb;
+				st->lu_gemm_flop += 8.0 * nr * (double)nr * kw;
 				st->lu_gemm_launches++;
 			}
 		}

@@ -1256,6 +1256,18 @@ static int warm_start()
 	return w;
 }
 
+// the cost rule counts the batch just solved once more, as the next batch's expected excess
+// (PEEC_REFACTOR_LOOKAHEAD=0: only the excess already spent)
+static bool refactor_lookahead()
+{
+	static int on = -1;
+	if (on < 0) {
+		const char *e = getenv("PEEC_REFACTOR_LOOKAHEAD");
+		on = !(e && (e[0] == '0'));
+	}
+	return on != 0;
+}
+
 // refactor rule: PEEC_REFACTOR_ALPHA > 0 prices extra iterations against the factorisation time
 // (1 by default, the break-even point: on the full stack 3 factorisations and 15.3 s against 4
 // and 16.5 s at 0.75, and 18.2 s at 1.5, measured on one GPU); 0 selects the iteration-ratio
@@ -1637,7 +1649,10 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 			c.hist.push_back(std::make_pair(its_lock, batch_ms / its_lock));
 			c.excess_ms = 0;
 			for (const auto &h : c.hist) c.excess_ms += (double)(h.first - c.lock_ref) * h.second;
-			if (c.excess_ms >= refactor_alpha() * c.lu_ms) c.stale = true;
+			// with look-ahead, the next batch is assumed to cost at least this one's excess, so
+			// the factorisation comes a batch earlier where the iteration counts climb fast
+			const double next = refactor_lookahead() ? (double)(its_lock - c.lock_ref) * (batch_ms / its_lock) : 0;
+			if (c.excess_ms + next >= refactor_alpha() * c.lu_ms) c.stale = true;
 		}
 		if (st) st->total_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 	}
