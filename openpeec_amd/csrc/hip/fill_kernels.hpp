@@ -47,7 +47,7 @@ void kf_eval_launch(hipStream_t s, int which, const pe::fill_args &f, const doub
 // leading dimension f.ld; f.kw and f.out unused)
 constexpr int KF_FMAX = 8;
 void kf_eval_multi_launch(hipStream_t s, int which, const pe::fill_args &f, const double *rec, int T, int nk,
-                          const double *kw, void *const *out);
+                          const double *kw, void *const *out, bool upper);
 
 // cell set of a device mirror as fill arguments (gpu_runtime.hip); which 0 current cells, 1
 // charge cells (carea set, sub-cell CSR returned)

@@ -698,19 +698,21 @@ int pgpu_kf_fill_kind(pgpu_t *g, int which, double kw, d_complex_t *out)
 	return 0;
 }
 
-// the same for nk wavenumbers (a batch of frequencies): the records are read once for all of them
-void pgpu_kf_fill_multi(pgpu_t *g, int nk, const double *kw, d_complex_t *const *dlp, d_complex_t *const *dP)
+// the same for nk wavenumbers (a batch of frequencies): the records are read once for all of them.
+// With upper set, the lower triangles outside the diagonal 64 x 64 blocks are left as they were
+// wherever the records serve (the sub-cell path for P still writes whole matrices).
+void pgpu_kf_fill_multi(pgpu_t *g, int nk, const double *kw, d_complex_t *const *dlp, d_complex_t *const *dP, int upper)
 {
 	const peec_t *p = g->p;
 	const int ns = g->seg.n, nh = g->chg.n, nc = p->ncell;
 	if (nk > KF_FMAX) {
 		for (int k0 = 0; k0 < nk; k0 += KF_FMAX)
-			pgpu_kf_fill_multi(g, std::min(KF_FMAX, nk - k0), kw + k0, dlp + k0, dP ? dP + k0 : nullptr);
+			pgpu_kf_fill_multi(g, std::min(KF_FMAX, nk - k0), kw + k0, dlp + k0, dP ? dP + k0 : nullptr, upper);
 		return;
 	}
 	if (ns > 0) {
 		pe::fill_args a = base_args(g->seg, ns, kw[0], p->gp, dlp[0], ns);
-		kf_eval_multi_launch(g->s, 0, a, g->kf_rec_lp, g->kf_T, nk, kw, (void *const *)dlp);
+		kf_eval_multi_launch(g->s, 0, a, g->kf_rec_lp, g->kf_T, nk, kw, (void *const *)dlp, upper != 0);
 		for (int k = 0; k < nk; k++) {
 			a.kw = kw[k];
 			a.out = dlp[k];
@@ -721,7 +723,7 @@ void pgpu_kf_fill_multi(pgpu_t *g, int nk, const double *kw, d_complex_t *const 
 		if (g->cells_identity) {
 			pe::fill_args a = base_args(g->chg, nh, kw[0], p->gp, dP[0], nc);
 			a.carea = g->carea;
-			kf_eval_multi_launch(g->s, 1, a, g->kf_rec_ig, g->kf_T, nk, kw, (void *const *)dP);
+			kf_eval_multi_launch(g->s, 1, a, g->kf_rec_ig, g->kf_T, nk, kw, (void *const *)dP, upper != 0);
 			for (int k = 0; k < nk; k++) {
 				a.kw = kw[k];
 				a.out = dP[k];

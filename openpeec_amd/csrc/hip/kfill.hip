@@ -481,10 +481,14 @@ struct kf_multi {
 	double kw[KF_FMAX];
 	void *out[KF_FMAX];
 	int nk;
+	int upper;   // mirrors only inside the diagonal 64 x 64 blocks (the rest of the lower triangle untouched)
 };
 
 // k_kf_eval for nk wavenumbers at once: each record read once, nk entries (and their mirrors)
-// written, one output matrix per wavenumber (same ld)
+// written, one output matrix per wavenumber (same ld).  With mk.upper the mirror (j, i) is
+// written only when i and j share a 64-row block: a symmetric product that reads the upper
+// 64 x 64 tiles, whole diagonal tiles included, needs no more, and the column-strided mirror
+// stores are most of the kernel's write traffic.
 template <int T>
 __global__ __launch_bounds__(256) void k_kf_eval_multi(kf_args a, int which, kf_multi mk)
 {
@@ -510,6 +514,7 @@ __global__ __launch_bounds__(256) void k_kf_eval_multi(kf_args a, int which, kf_
 		sij = 1 / (f.k4pe * ai * aj);
 		sji = 1 / (f.k4pe * aj * ai);
 	}
+	const bool mirror = (i != j) && (!mk.upper || ((i >> 6) == (j >> 6)));
 	for (int k = 0; k < mk.nk; k++) {
 		double vr = 0, vi = 0;
 #pragma unroll
@@ -517,18 +522,18 @@ __global__ __launch_bounds__(256) void k_kf_eval_multi(kf_args a, int which, kf_
 		double2 *out = (double2 *)mk.out[k];
 		if (scaled) {
 			out[(long)i * f.ld + j] = make_double2(sij * (0 + vr), sij * (0 + vi));
-			if (i != j) out[(long)j * f.ld + i] = make_double2(sji * (0 + vr), sji * (0 + vi));
+			if (mirror) out[(long)j * f.ld + i] = make_double2(sji * (0 + vr), sji * (0 + vi));
 		}
 		else {
 			out[(long)i * f.ld + j] = make_double2(vr, vi);
-			if (i != j) out[(long)j * f.ld + i] = make_double2(vr, vi);
+			if (mirror) out[(long)j * f.ld + i] = make_double2(vr, vi);
 		}
 	}
 }
 }  // namespace pe
 
 void kf_eval_multi_launch(hipStream_t s, int which, const pe::fill_args &f, const double *rec, int T, int nk,
-                          const double *kw, void *const *out)
+                          const double *kw, void *const *out, bool upper)
 {
 	pe::kf_args a;
 	a.f = f;
@@ -541,6 +546,7 @@ void kf_eval_multi_launch(hipStream_t s, int which, const pe::fill_args &f, cons
 	a.nlist = nullptr;
 	pe::kf_multi mk{};
 	mk.nk = nk;
+	mk.upper = upper ? 1 : 0;
 	for (int k = 0; k < nk; k++) {
 		mk.kw[k] = kw[k];
 		mk.out[k] = out[k];

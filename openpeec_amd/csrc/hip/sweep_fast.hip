@@ -180,6 +180,32 @@ __global__ __launch_bounds__(256) void k_transpose32(const TS *__restrict__ src,
 	}
 }
 
+// the lower triangle of a symmetric n x n matrix from its upper triangle (32 x 32 tiles through
+// LDS; a workgroup per tile above the diagonal)
+__global__ __launch_bounds__(256) void k_mirror_lower(double2 *M, int n)
+{
+	__shared__ double2 t[32][33];
+	const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+	if (c0 < r0) return;
+	const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+	for (int k = 0; k < 32; k += 8) {
+		const int r = r0 + ty + k, c = c0 + tx;
+		if ((r < n) && (c < n)) t[ty + k][tx] = M[(long)r * n + c];
+	}
+	__syncthreads();
+	for (int k = 0; k < 32; k += 8) {
+		const int c = c0 + ty + k, r = r0 + tx;
+		if ((r < n) && (c < n) && (c > r)) M[(long)c * n + r] = t[tx][ty + k];
+	}
+}
+
+void mirror_lower(hipStream_t s, double2 *M, int n)
+{
+	if (n <= 1) return;
+	const unsigned nt = (unsigned)((n + 31) / 32);
+	hipLaunchKernelGGL(k_mirror_lower, dim3(nt, nt), dim3(256), 0, s, M, n);
+}
+
 // ---------------------------------------------------------------- batched dense products
 
 // One system of the batch: CSR stamps and its dense kernel blocks.
@@ -830,6 +856,7 @@ struct slotbuf {
 	double2 *val = nullptr;
 	int cap = 0, nnz = 0;
 	double omega = 0;
+	bool upper = false;   // lp and P hold their upper triangles and diagonal 64 x 64 blocks only
 };
 
 struct fast {
@@ -1320,6 +1347,10 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 		for (int f = 0; f < F; f++) {
 			c.sl[f].lp = dal<double2>((size_t)p->nseg * p->nseg);
 			c.sl[f].P = dal<double2>((size_t)nc * nc);
+			// zeros once: the batch fill leaves most of the lower triangles as they were, and the
+			// symmetric product's last partial tile column multiplies a few of them by zero
+			hipLaunchKernelGGL(k_zero, dim3(blocks((long)p->nseg * p->nseg)), dim3(256), 0, s, c.sl[f].lp, (long)p->nseg * p->nseg);
+			if (nc > 0) hipLaunchKernelGGL(k_zero, dim3(blocks((long)nc * nc)), dim3(256), 0, s, c.sl[f].P, (long)nc * nc);
 			c.sl[f].rp = dal<int>(n + 1);
 			hipLaunchKernelGGL(k_zero_u32, dim3(blocks(n + 1, 64)), dim3(64), 0, s, (unsigned *)c.sl[f].rp, n + 1);
 		}
@@ -1433,9 +1464,11 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 				// dense blocks already filled (batch fill)
 			}
 			else if (kf) {
+				sb.upper = false;
 				pgpu_kf_fill(g, kw, (d_complex_t *)sb.lp, nc ? (d_complex_t *)sb.P : nullptr);
 			}
 			else {
+				sb.upper = false;
 				pgpu_fill_lp(g, kw);
 				if (usecap) pgpu_fill_pot(g, kw);
 				HIPCHK(hipMemcpyAsync(sb.lp, pgpu_lp_dev(g), (size_t)p->nseg * p->nseg * sizeof(double2), hipMemcpyDeviceToDevice, s));
@@ -1457,7 +1490,11 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 				lps[b] = (d_complex_t *)c.sl[b].lp;
 				Ps[b] = (d_complex_t *)c.sl[b].P;
 			}
-			pgpu_kf_fill_multi(g, nb, kws, lps, nc ? Ps : nullptr);
+			// the symmetric product reads the upper triangles: the batch fill leaves the rest of
+			// the lower ones to the factorisation, which mirrors them when it takes a system
+			const bool up = mv_sym();
+			pgpu_kf_fill_multi(g, nb, kws, lps, nc ? Ps : nullptr, up ? 1 : 0);
+			for (int b = 0; b < nb; b++) c.sl[b].upper = up;
 		}
 		for (int b = 0; b < nb; b++) fill_system(c.sl[b], f0 + b, !kf);
 		const auto t1 = std::chrono::steady_clock::now();
@@ -1466,6 +1503,10 @@ extern "C" int pgpu_sweep_fast(pgpu_t *g, const peec_t *p, int ifreq0, int nfreq
 		auto refactor_on = [&](const slotbuf &sb, int k) -> int {
 			const auto ta = std::chrono::steady_clock::now();
 			const int n = ns;   // the factored order from here on: the Schur complement's
+			if (sb.upper) {
+				mirror_lower(s, sb.lp, p->nseg);
+				if (nc > 0) mirror_lower(s, sb.P, nc);
+			}
 			if (nc > 0) {
 				// C = P(f_ref)^-1 in FP64 (the plain form's P LU and unit solves), then FP32 by columns
 				HIPCHK(hipMemcpyAsync(c.Pw, sb.P, (size_t)nc * nc * sizeof(double2), hipMemcpyDeviceToDevice, s));

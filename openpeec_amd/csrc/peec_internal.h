@@ -123,7 +123,7 @@ void pgpu_kf_fill(pgpu_t *g, double kw, d_complex_t *dlp, d_complex_t *dP);
  * not cover kw or P's sub-cell matrix would not fit in a quarter of free memory */
 int  pgpu_kf_fill_kind(pgpu_t *g, int which, double kw, d_complex_t *out);
 /* the same for nk frequencies at once (the records read once): dlp[k], dP[k] at kw[k] */
-void pgpu_kf_fill_multi(pgpu_t *g, int nk, const double *kw, d_complex_t *const *dlp, d_complex_t *const *dP);
+void pgpu_kf_fill_multi(pgpu_t *g, int nk, const double *kw, d_complex_t *const *dlp, d_complex_t *const *dP, int upper);
 int  pgpu_kf_direct_pairs(pgpu_t *g);
 void pgpu_kf_reset_all(void);     /* every device mirror prepares its records afresh */
 /* Lp and P at kw (mode 0 element code, mode 1 k-factored, set up beforehand) to host buffers */
