@@ -337,8 +337,8 @@ __device__ __forceinline__ void halve_sums(double2 *v, int lane)
 	}
 }
 
-__global__ __launch_bounds__(256) void k_mvsym(slots S, int nact, int F, int offS, int nseg, int offQ, int nc, int NR,
-                                               int NI, const double2 *__restrict__ X, double2 *__restrict__ partN,
+__global__ __launch_bounds__(256) void k_mvsym(slots S, int nact, unsigned am, int F, int offS, int nseg, int offQ, int nc,
+                                               int NR, int NI, const double2 *__restrict__ X, double2 *__restrict__ partN,
                                                double2 *__restrict__ partT, int n)
 {
 	constexpr int RW = 16;
@@ -359,6 +359,7 @@ __global__ __launch_bounds__(256) void k_mvsym(slots S, int nact, int F, int off
 	const int r0 = RW * wave;
 	const long mat_bytes = (long)nb * nb * 16;
 	for (int f = 0; f < nact; f++) {
+		if (!((am >> f) & 1u)) continue;   // a system outside the mask: its sums are not read
 		const double2 *Bf = nullptr;
 #pragma unroll
 		for (int h = 0; h < FMAX; h++)
@@ -426,8 +427,8 @@ __global__ __launch_bounds__(256) void k_mvsym(slots S, int nact, int F, int off
 // per (system, row) of the symmetric product: the row's run sums (partN, increasing run), then
 // the mirror contributions of the tiles above it in its column (partT, increasing I), the -jw
 // scale on the Lp rows, and the row's sparse stamps
-__global__ __launch_bounds__(256) void k_mvfin_sym(slots S, int n, int F, int nact, int offS, int nseg, int offQ, int nc,
-                                                   int NR, int NI, const double2 *__restrict__ partN,
+__global__ __launch_bounds__(256) void k_mvfin_sym(slots S, int n, int F, int nact, unsigned am, int offS, int nseg, int offQ,
+                                                   int nc, int NR, int NI, const double2 *__restrict__ partN,
                                                    const double2 *__restrict__ partT, const double2 *__restrict__ X,
                                                    double2 *__restrict__ Y)
 {
@@ -435,7 +436,7 @@ __global__ __launch_bounds__(256) void k_mvfin_sym(slots S, int n, int F, int na
 	if (e >= (long)n * F) return;
 	const int f = (int)(e / n), row = (int)(e - (long)f * n);
 	double2 y = make_double2(0, 0);
-	if (f < nact) {
+	if ((f < nact) && ((am >> f) & 1u)) {
 		const slot &sl = S.s[f];
 		const bool l = (row >= offS) && (row < offS + nseg);
 		const bool q = (row >= offQ) && (row < offQ + nc);
@@ -956,8 +957,10 @@ struct mvtimer {
 	}
 } g_mvt;
 
-// y = A_f x for the first nact systems of the batch (columns f >= nact are set to zero)
-void matvec(fast &c, hipStream_t s, const double2 *x, double2 *y, int F, int nact)
+// y = A_f x for the first nact systems of the batch (columns f >= nact are set to zero).  With
+// the symmetric product, systems outside the mask am are skipped and their columns set to zero
+// as well (their matrices are not read); the two-triangle product ignores the mask.
+void matvec(fast &c, hipStream_t s, const double2 *x, double2 *y, int F, int nact, unsigned am = ~0u)
 {
 	if (g_mvt.on) {
 		if ((int)g_mvt.ev.size() < g_mvt.used + 2) {
@@ -980,8 +983,8 @@ void matvec(fast &c, hipStream_t s, const double2 *x, double2 *y, int F, int nac
 		const int ntS = (c.nseg + 63) / 64, ntQ = (c.nc + 63) / 64, ntm = std::max(ntS, ntQ);
 		const int NR = (ntm + MV_JS - 1) / MV_JS;   // runs per row block at most (partN rows per system)
 		if ((ntS + ntQ) > 0)
-			hipLaunchKernelGGL(k_mvsym, dim3((unsigned)NR, (unsigned)(ntS + ntQ)), dim3(256), 0, s, S, nact, F, g_offS, c.nseg,
-			                   g_offQ, c.nc, NR, ntm, x, c.mvp, c.mvt, c.n);
+			hipLaunchKernelGGL(k_mvsym, dim3((unsigned)NR, (unsigned)(ntS + ntQ)), dim3(256), 0, s, S, nact, am, F, g_offS,
+			                   c.nseg, g_offQ, c.nc, NR, ntm, x, c.mvp, c.mvt, c.n);
 		if (g_mvt.on) {
 			// the upper triangle's tiles of the nact systems (16 B per entry), x once, the run and
 			// mirror sums written
@@ -996,10 +999,12 @@ void matvec(fast &c, hipStream_t s, const double2 *x, double2 *y, int F, int nac
 					sums += (double)((nt - I + MV_JS - 1) / MV_JS) * rows + (double)(nb - 64 * (I + 1) > 0 ? nb - 64 * (I + 1) : 0);
 				}
 			}
-			g_mvt.bytes += 16.0 * nact * ent + 16.0 * c.n * F + 16.0 * nact * sums;
+			int nin = 0;
+			for (int f = 0; f < nact; f++) nin += (am >> f) & 1u;
+			g_mvt.bytes += 16.0 * nin * ent + 16.0 * c.n * F + 16.0 * nin * sums;
 		}
-		hipLaunchKernelGGL(k_mvfin_sym, dim3(blocks((long)c.n * F)), dim3(256), 0, s, S, c.n, F, nact, g_offS, c.nseg, g_offQ,
-		                   c.nc, NR, ntm, (const double2 *)c.mvp, (const double2 *)c.mvt, x, y);
+		hipLaunchKernelGGL(k_mvfin_sym, dim3(blocks((long)c.n * F)), dim3(256), 0, s, S, c.n, F, nact, am, g_offS, c.nseg,
+		                   g_offQ, c.nc, NR, ntm, (const double2 *)c.mvp, (const double2 *)c.mvt, x, y);
 		return;
 	}
 	const int cs = mv_cs();
@@ -1077,9 +1082,17 @@ std::vector<int> gmres_batch(fast &c, hipStream_t s, int F, int nactive, bool wa
 	dots(c, s, c.Bv, 0, 1, c.Bv, F, hh);
 	for (int f = 0; f < F; f++) bnorm[f] = std::sqrt(hh[f].x);
 	if (!warm) hipLaunchKernelGGL(k_zero, dim3(blocks(nf)), dim3(256), 0, s, c.Xv, nf);
+	// systems whose products are still needed: not done (residual), and not converged in this
+	// cycle (Arnoldi steps); the others' columns come out zero
+	auto mask = [&](const std::vector<int> *kd) {
+		unsigned am = 0;
+		for (int f = 0; f < F; f++)
+			if (!done[f] && (!kd || ((*kd)[f] < 0))) am |= 1u << f;
+		return am;
+	};
 	for (int cyc = 0; cyc < MCYCLES; cyc++) {
 		// r = b - A x
-		matvec(c, s, c.Xv, c.R, F, nactive);
+		matvec(c, s, c.Xv, c.R, F, nactive, mask(nullptr));
 		hipLaunchKernelGGL(k_resid, dim3(blocks(nf)), dim3(256), 0, s, c.Bv, c.R, nf);
 		dots(c, s, c.R, 0, 1, c.R, F, hh);
 		std::vector<double> beta(F);
@@ -1105,7 +1118,7 @@ std::vector<int> gmres_batch(fast &c, hipStream_t s, int F, int nactive, bool wa
 			double2 *vj = c.V + (long)j * vstride;
 			double2 *zj = c.Z + (long)j * vstride;
 			precond(c, s, vj, zj, F);
-			matvec(c, s, zj, c.W, F, nactive);
+			matvec(c, s, zj, c.W, F, nactive, mask(&kdone));
 			// classical Gram-Schmidt, two passes, and the new vector's norm, all on the device;
 			// the coefficients reach the host in one transfer for the Hessenberg column
 			dots_dev(c, s, c.V, vstride, j + 1, c.W, F, c.h);
