@@ -41,8 +41,8 @@
 namespace {
 
 constexpr int FMAX = 8;        // systems per batch
-constexpr int MKRYLOV = 40;    // GMRES restart length
-constexpr int MCYCLES = 6;     // restarts before refactorising
+constexpr int MKRYLOV = 120;   // largest GMRES restart length (buffers are sized for it)
+constexpr int MITS = 240;      // iterations of one solve before refactorising
 constexpr int TB = 64;         // triangular-solve block
 constexpr double TOL = 1e-10;  // relative residual
 constexpr int REFACTOR_MIN_ITS = 24;   // batches below this many iterations never trigger a refactor
@@ -1069,8 +1069,22 @@ void dots_dev(fast &c, hipStream_t s, const double2 *V, long vstride, int nk, co
 // batched restarted flexible GMRES on F systems A_f x_f = b_f (b in c.Bv, x in c.Xv); returns
 // per-system iteration counts, -1 for a system that did not converge
 // warm: c.Xv already holds the initial guesses, else they are zero
+// GMRES restart length (PEEC_GMRES_M, 10 .. MKRYLOV; 40 by default); the cycles before a
+// refactorisation make up MITS iterations
+int gmres_m()
+{
+	static int m = -1;
+	if (m < 0) {
+		const char *e = getenv("PEEC_GMRES_M");
+		const int v = e ? atoi(e) : 40;
+		m = ((v >= 10) && (v <= MKRYLOV)) ? v : 40;
+	}
+	return m;
+}
+
 std::vector<int> gmres_batch(fast &c, hipStream_t s, int F, int nactive, bool warm)
 {
+	const int mk = gmres_m(), ncyc = std::max(1, MITS / mk);
 	const int n = c.n;
 	const long nf = (long)n * F;
 	const long vstride = nf;
@@ -1090,7 +1104,7 @@ std::vector<int> gmres_batch(fast &c, hipStream_t s, int F, int nactive, bool wa
 			if (!done[f] && (!kd || ((*kd)[f] < 0))) am |= 1u << f;
 		return am;
 	};
-	for (int cyc = 0; cyc < MCYCLES; cyc++) {
+	for (int cyc = 0; cyc < ncyc; cyc++) {
 		// r = b - A x
 		matvec(c, s, c.Xv, c.R, F, nactive, mask(nullptr));
 		hipLaunchKernelGGL(k_resid, dim3(blocks(nf)), dim3(256), 0, s, c.Bv, c.R, nf);
@@ -1114,7 +1128,7 @@ std::vector<int> gmres_batch(fast &c, hipStream_t s, int F, int nactive, bool wa
 		std::vector<int> kdone(F, -1);
 		for (int f = 0; f < F; f++) g[f][0] = beta[f];
 		int j;
-		for (j = 0; j < MKRYLOV; j++) {
+		for (j = 0; j < mk; j++) {
 			double2 *vj = c.V + (long)j * vstride;
 			double2 *zj = c.Z + (long)j * vstride;
 			precond(c, s, vj, zj, F);
@@ -1167,7 +1181,7 @@ std::vector<int> gmres_batch(fast &c, hipStream_t s, int F, int nactive, bool wa
 			}
 		}
 		// y = H^-1 g per system; x += Z y
-		const int kmax = std::min(j, MKRYLOV);
+		const int kmax = std::min(j, mk);
 		std::vector<double2> yv((size_t)kmax * F, make_double2(0, 0));
 		for (int f = 0; f < F; f++) {
 			if (done[f]) continue;
