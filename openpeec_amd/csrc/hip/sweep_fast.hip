@@ -444,17 +444,23 @@ __global__ __launch_bounds__(256) void k_mvfin_sym(slots S, int n, int F, int na
 			const int nb = l ? nseg : nc, o = l ? offS : offQ;
 			const int nt = (nb + 63) / 64, Ib = (row - o) / 64;
 			const int nsg = (nt - Ib + MV_JS - 1) / MV_JS;
+			// the partial sums loaded eight at a time, then added one by one in the same order
 			double tr = 0, ti = 0;
-			for (int sg = 0; sg < nsg; sg++) {
-				const double2 p = partN[((long)f * NR + sg) * n + row];
-				tr += p.x;
-				ti += p.y;
-			}
-			for (int I = 0; I < Ib; I++) {
-				const double2 p = partT[((long)f * NI + I) * n + row];
-				tr += p.x;
-				ti += p.y;
-			}
+			auto sum = [&](const double2 *base, int cnt) {
+				for (int q0 = 0; q0 < cnt; q0 += 8) {
+					double2 p[8];
+#pragma unroll
+					for (int q = 0; q < 8; q++) p[q] = (q0 + q < cnt) ? base[(long)(q0 + q) * n] : make_double2(0, 0);
+#pragma unroll
+					for (int q = 0; q < 8; q++)
+						if (q0 + q < cnt) {
+							tr += p[q].x;
+							ti += p[q].y;
+						}
+				}
+			};
+			sum(partN + (long)f * NR * n + row, nsg);
+			sum(partT + (long)f * NI * n + row, Ib);
 			y = l ? make_double2(sl.omega * ti, -sl.omega * tr) : make_double2(tr, ti);   // Lp rows: -jw * sum
 		}
 		for (int k = sl.rp[row]; k < sl.rp[row + 1]; k++) {
@@ -762,10 +768,17 @@ __global__ void k_tgsum(const double2 *__restrict__ part, int n, int F, int cs, 
 	const int s1 = FULL ? (n - 1) / cs : LOWER ? (rhi - 1) / cs : (n - 1) / cs;
 	const long stride = (long)n * F;
 	double2 a = part[(long)s0 * stride + e];
-	for (int s = s0 + 1; s <= s1; s++) {
-		const double2 p = part[(long)s * stride + e];
-		a.x += p.x;
-		a.y += p.y;
+	// the segments loaded eight at a time, then added one by one in increasing s
+	for (int q0 = s0 + 1; q0 <= s1; q0 += 8) {
+		double2 p[8];
+#pragma unroll
+		for (int q = 0; q < 8; q++) p[q] = (q0 + q <= s1) ? part[(long)(q0 + q) * stride + e] : make_double2(0, 0);
+#pragma unroll
+		for (int q = 0; q < 8; q++)
+			if (q0 + q <= s1) {
+				a.x += p[q].x;
+				a.y += p[q].y;
+			}
 	}
 	Y[e] = a;
 }
