@@ -549,11 +549,19 @@ __global__ void k_dots_final(const double2 *partial, int nblk, int nk, int F, do
 {
 	const int e = blockIdx.x * blockDim.x + threadIdx.x;
 	if (e >= nk * F) return;
+	// the block partials loaded eight at a time, then added one by one in block order
 	double sr = 0, si = 0;
-	for (int b = 0; b < nblk; b++) {
-		const double2 p = partial[(long)b * nk * F + e];
-		sr += p.x;
-		si += p.y;
+	const long st = (long)nk * F;
+	for (int b0 = 0; b0 < nblk; b0 += 8) {
+		double2 p[8];
+#pragma unroll
+		for (int q = 0; q < 8; q++) p[q] = (b0 + q < nblk) ? partial[(long)(b0 + q) * st + e] : make_double2(0, 0);
+#pragma unroll
+		for (int q = 0; q < 8; q++)
+			if (b0 + q < nblk) {
+				sr += p[q].x;
+				si += p[q].y;
+			}
 	}
 	h[e] = make_double2(sr, si);
 }
