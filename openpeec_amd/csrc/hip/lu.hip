@@ -889,14 +889,16 @@ __global__ void k_scale_block(TS *X, long ld, int rows, int cols, double sc)
 	st2(x, (sc == 0) ? make_double2(0, 0) : make_double2(sc * v.x, sc * v.y));
 }
 
-// Out[r][c] = D[r][c] for an n x n corner of a TI x TI block
+// every diagonal TI x TI block of the n x n inverse at once (blockIdx.y: the block; a short last
+// block has n - TI y rows and columns)
 template <typename TS>
-__global__ void k_put_block(const TS *D, TS *Out, long ld, int n)
+__global__ void k_put_blocks(const TS *D, TS *Out, long ld, int n)
 {
+	const int b = blockIdx.y, i0 = b * TI, nb = min(TI, n - i0);
 	const int e = blockIdx.x * blockDim.x + threadIdx.x;
-	if (e >= n * n) return;
-	const int r = e / n, c = e - r * n;
-	Out[(long)r * ld + c] = D[r * TI + c];
+	if (e >= nb * nb) return;
+	const int r = e / nb, c = e - r * nb;
+	Out[(long)(i0 + r) * ld + i0 + c] = D[(long)b * TI * TI + r * TI + c];
 }
 
 template <typename TS>
@@ -944,11 +946,7 @@ void trmm(hipStream_t s, bool right, bool upper, int m, int n, double alpha, con
 template <typename TS>
 void tri_inv(hipStream_t s, bool lower, const TS *T, long ld, TS *Out, int i0, int n, const TS *dinv, TS *scr)
 {
-	if (n <= TI) {
-		hipLaunchKernelGGL((k_put_block<TS>), dim3(tiblocks((long)n * n)), dim3(256), 0, s, dinv + (long)(i0 / TI) * TI * TI,
-		                   Out + (long)i0 * ld + i0, ld, n);
-		return;
-	}
+	if (n <= TI) return;   // the diagonal blocks are in place already (k_put_blocks)
 	const int n1 = ((n / 2 + TI - 1) / TI) * TI, n2 = n - n1;
 	tri_inv(s, lower, T, ld, Out, i0, n1, dinv, scr);
 	tri_inv(s, lower, T, ld, Out, i0 + n1, n2, dinv, scr);
@@ -994,6 +992,8 @@ void tri_inverse_t(hipStream_t s, int n, const TS *LU, int lda, int lower, TS *O
 	if (lower) hipLaunchKernelGGL((k_diaginv<true, TS>), dim3(nblk), dim3(64), 0, s, LU, (long)lda, n, dinv);
 	else hipLaunchKernelGGL((k_diaginv<false, TS>), dim3(nblk), dim3(64), 0, s, LU, (long)lda, n, dinv);
 	hipLaunchKernelGGL((k_ti_zero<TS>), dim3(tiblocks((long)n * n)), dim3(256), 0, s, Out, (long)n * n);
+	// the diagonal blocks in one launch; the recursion's leaves then have nothing to do
+	hipLaunchKernelGGL((k_put_blocks<TS>), dim3((TI * TI + 255) / 256, nblk), dim3(256), 0, s, (const TS *)dinv, Out, (long)lda, n);
 	tri_inv(s, lower != 0, LU, (long)lda, Out, 0, n, (const TS *)dinv, scr);
 }
 
