@@ -97,3 +97,27 @@ def test_mixed_symmetric_product_matches_full(lib, monkeypatch, text_fn):
         print(f"  sym={sym}: {st.gmres_its} iterations", flush=True)
     assert _rel(zs["1"], zs["0"]) <= 1e-8, _rel(zs["1"], zs["0"])
     m.close()
+
+
+def test_upper_fill_preconditioner(lib, monkeypatch):
+    """With the symmetric product the batch fill writes only the upper triangles (and diagonal
+    64 x 64 blocks) of Lp and P, and a factorisation mirrors the lower triangle of the system it
+    takes.  A wrong mirror leaves GMRES with the preconditioner of another matrix: the answer
+    still converges, but the iterations do not match those of the two-triangle product (whose
+    fill writes whole matrices)."""
+    text = decks.plate_deck(n=24, ndiv=40)
+    its = {}
+    for sym in ("1", "0"):
+        monkeypatch.setenv("PEEC_MV_SYM", sym)
+        print(f"[mixed] plate 24x24, symmetric product {sym}", flush=True)
+        lib.peec_sweep_reset()
+        m = Peec.from_deck_text(text, lib)
+        m.setup()
+        zp, _ = m.sweep(0, 12, PEEC_FORM_PLAIN)
+        zm, st = m.sweep(0, 12, PEEC_FORM_MIXED, 4)
+        assert _rel(zm, zp) <= 1e-6
+        its[sym] = (st.gmres_its, st.n_lu)
+        m.close()
+    (i1, n1), (i0, n0) = its["1"], its["0"]
+    assert n1 == n0, its
+    assert abs(i1 - i0) <= max(2, 0.05 * i0), its
