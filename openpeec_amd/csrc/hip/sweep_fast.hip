@@ -41,7 +41,7 @@
 namespace {
 
 constexpr int FMAX = 8;        // systems per batch
-constexpr int MKRYLOV = 120;   // largest GMRES restart length (buffers are sized for it)
+constexpr int MKRYLOV = 80;    // largest GMRES restart length (buffers are sized for it)
 constexpr int MITS = 240;      // iterations of one solve before refactorising
 constexpr int TB = 64;         // triangular-solve block
 constexpr double TOL = 1e-10;  // relative residual
