@@ -656,8 +656,8 @@ def main():
         copy_probe(args.copy_probe)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU frequency sharding is "
-                                   "described in DESIGN.md but not part of this build", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this library runs on one GPU; sharding the frequencies over several "
+                                   "GPUs is not part of it", "n_gpus": args.gpus}))
         sys.exit(2)
     from openpeec_amd import load_library
 
