@@ -1018,7 +1018,7 @@ __global__ void k_fill_ones(unsigned *p, int n)
 // The blocked factorisation on a given workspace: panel exchange buffers ws, permutation
 // scratch perm_ws (n ints, used when dperm is NULL); own_gemm selects the hand-written trailing
 // update (no library call, no allocation) whatever PEEC_GEMM says.
-// Columns per trailing update of the FP32 factorisation (PEEC_LU_KB: 64, 128, 192 or 256;
+// Columns per trailing update of the FP32 factorisation (PEEC_LU_KB: a multiple of 64 up to 512;
 // 256 by default): a block of that many columns is factored panel by panel, each panel updating
 // only the rest of the block, and the columns beyond it take one update with K = the block's
 // width, where a library GEMM runs nearer its peak than at K = 64.
@@ -1028,7 +1028,7 @@ int lu_kb()
 	if (kb < 0) {
 		const char *e = getenv("PEEC_LU_KB");
 		const int v = e ? atoi(e) : 256;
-		kb = ((v >= NB) && (v <= 4 * NB) && (v % NB == 0)) ? v : 256;
+		kb = ((v >= NB) && (v <= 8 * NB) && (v % NB == 0)) ? v : 256;
 	}
 	return kb;
 }
